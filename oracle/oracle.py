@@ -1,0 +1,313 @@
+"""ctypes view of liboracle.so, the CPU restatement of PRESTO's accelsearch
+path.  Test infrastructure only: imported by tests/, by
+__graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
+product package."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+_d, _i, _f, _ll, _p = C.c_double, C.c_int, C.c_float, C.c_longlong, C.c_void_p
+
+
+class OObs(C.Structure):
+    _fields_ = [
+        ("N", _d), ("dt", _d), ("numbins", _ll),
+        ("numharm", _i), ("zmax", _i), ("wmax", _i), ("wmaxP", _i),
+        ("sigma", _f),
+        ("flo", _d), ("fhi", _d), ("rlo_arg", _d), ("rhi_arg", _d),
+        ("floP", _i), ("fhiP", _i), ("rloP", _i), ("rhiP", _i),
+        ("lobin_arg", _ll), ("inmemP", _i), ("force_standard", _i), ("dat_input", _i),
+        ("lobin", _ll), ("highestbin", _ll),
+        ("maxkernlen", _i), ("corr_uselen", _i), ("fftlen", _i), ("numharmstages", _i),
+        ("numz", _i), ("numw", _i), ("numbetween", _i), ("inmem", _i),
+        ("T", _d), ("rlo", _d), ("rhi", _d), ("dr", _d), ("zlo", _d), ("zhi", _d),
+        ("dz", _d), ("wlo", _d), ("whi", _d), ("dw", _d),
+        ("powcut", _f * 6), ("numindep", _ll * 6),
+    ]
+
+
+class OHit(C.Structure):
+    _fields_ = [("block", _i), ("stage", _i), ("wi", _i), ("zi", _i), ("ri", _i),
+                ("pow", _f), ("sigma", _f)]
+
+
+class OResult(C.Structure):
+    _fields_ = [("hits", C.POINTER(OHit)), ("nhits", _ll), ("caphits", _ll),
+                ("cands", _p), ("nblocks", _i)]
+
+
+class OFfdot(C.Structure):
+    _fields_ = [("rlo", _ll), ("zlo", _i), ("wlo", _i), ("numrs", _i), ("numzs", _i),
+                ("numws", _i), ("powers", C.POINTER(_f)), ("dpowers", C.POINTER(_d)),
+                ("rinds", C.POINTER(C.c_ushort)), ("zinds", C.POINTER(C.c_ushort))]
+
+
+def _sig(name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_cf = np.ctypeslib.ndpointer(dtype=np.complex64, flags="C_CONTIGUOUS")
+_cd = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+
+_sig("o_fresnl", _i, _d, C.POINTER(_d), C.POINTER(_d))
+_sig("o_r_resp_halfwidth", _i, _i)
+_sig("o_z_resp_halfwidth", _i, _d, _i)
+_sig("o_w_resp_halfwidth", _i, _d, _d, _i)
+_sig("o_gen_r_response", None, _d, _i, _i, _cd)
+_sig("o_gen_z_response", None, _d, _i, _d, _i, _cd)
+_sig("o_gen_w_response", None, _d, _i, _d, _d, _i, _p, _p)
+_sig("o_next2_to_n", _ll, _ll)
+_sig("o_next_good_fftlen", _i, _i)
+_sig("o_fftlen_from_kernwidth", _i, _i)
+for _n in ("o_gauss_Q", "o_gauss_Qinv", "o_equivalent_gaussian_sigma"):
+    _sig(_n, _d, _d)
+for _n in ("o_chisq_Q_even", "o_chisq_Qinv_even", "o_chi2_logp"):
+    _sig(_n, _d, _d, _d)
+_sig("o_candidate_sigma", _d, _d, _i, _d)
+_sig("o_power_for_sigma", _d, _d, _i, _d)
+_sig("o_median", _f, _fp, _i)
+_sig("o_invsqrtf", _f, _f)
+_sig("o_cfft_f", None, _cf, C.c_long, _i)
+_sig("o_cfft_d", None, _cd, C.c_long, _i)
+_sig("o_realfft_f", None, _fp, C.c_long, _i)
+_sig("o_deredden", C.c_long, _cf, C.c_long, _i, _i, _d, _d)
+_sig("o_create_accelobs", _i, C.POINTER(OObs))
+_sig("o_accelsearch", _i, C.POINTER(OObs), _cf, _i, _i, C.POINTER(OResult))
+_sig("o_free_result", None, C.POINTER(OResult))
+_sig("o_result_ncands", _i, C.POINTER(OResult))
+_sig("o_result_cands", None, C.POINTER(OResult), _dp)
+_sig("o_prepare_dat", _ll, _fp, _ll, _d)
+_sig("o_create_subharminfos", _p, C.POINTER(OObs), _i)
+_sig("o_free_subharminfos", None, C.POINTER(OObs), _p)
+_sig("o_subharm_fderivs_vol", C.POINTER(OFfdot), _i, _i, _d, _d, _p, C.POINTER(OObs), _cf, _i)
+_sig("o_free_ffdot", None, C.POINTER(OFfdot))
+_sig("o_block_norm", _d, C.POINTER(OObs), _cf, _ll, _i)
+
+
+# ---------------------------------------------------------------- helpers
+def fresnl(x):
+    s, c = _d(), _d()
+    lib.o_fresnl(float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def gen_r_response(roffset, numbetween, numkern):
+    out = np.zeros(numkern, np.complex128)
+    lib.o_gen_r_response(roffset, numbetween, numkern, out)
+    return out
+
+
+def gen_z_response(roffset, numbetween, z, numkern):
+    out = np.zeros(numkern, np.complex128)
+    lib.o_gen_z_response(roffset, numbetween, z, numkern, out)
+    return out
+
+
+def gen_w_response(roffset, numbetween, z, w, numkern, fp64=False):
+    if fp64:
+        out = np.zeros(numkern, np.complex128)
+        lib.o_gen_w_response(roffset, numbetween, z, w, numkern, None,
+                             out.ctypes.data_as(_p))
+    else:
+        out = np.zeros(numkern, np.complex64)
+        lib.o_gen_w_response(roffset, numbetween, z, w, numkern,
+                             out.ctypes.data_as(_p), None)
+    return out
+
+
+def cfft(x, isign=-1):
+    x = np.ascontiguousarray(x)
+    y = x.copy()
+    if y.dtype == np.complex64:
+        lib.o_cfft_f(y, len(y), isign)
+    else:
+        y = y.astype(np.complex128)
+        lib.o_cfft_d(y, len(y), isign)
+    return y
+
+
+def realfft(x, isign=-1):
+    y = np.ascontiguousarray(x, dtype=np.float32).copy()
+    lib.o_realfft_f(y, len(y), isign)
+    return y
+
+
+def median(x):
+    y = np.ascontiguousarray(x, dtype=np.float32).copy()
+    return float(lib.o_median(y, len(y)))
+
+
+def deredden(fft, startwidth, endwidth, endfreq, T):
+    y = np.ascontiguousarray(fft, dtype=np.complex64).copy()
+    n = lib.o_deredden(y, len(y), startwidth, endwidth, endfreq, T)
+    return y, n
+
+
+def prepare_dat(series, dt):
+    y = np.ascontiguousarray(series, dtype=np.float32).copy()
+    nb = lib.o_prepare_dat(y, len(y), dt)
+    return y.view(np.complex64)[:nb].copy()
+
+
+def make_obs(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0,
+             fhi=10000.0, inmem=False, force_standard=False, lobin=0, dat_input=True,
+             rlo=None, rhi=None):
+    """Fill an o_obs the way accelsearch's command line does (flo/fhi default
+    to 'given', clig/accelsearch_cmd.cli) and run o_create_accelobs."""
+    o = OObs()
+    o.N, o.dt, o.numbins = float(N), float(dt), int(numbins)
+    o.numharm, o.zmax, o.sigma = int(numharm), int(zmax), float(sigma)
+    if wmax is not None:
+        o.wmax, o.wmaxP = int(wmax), 1
+    if rlo is not None:
+        o.floP, o.rloP, o.rlo_arg = 0, 1, float(rlo)
+    else:
+        o.floP, o.flo = 1, float(flo)
+    if rhi is not None:
+        o.fhiP, o.rhiP, o.rhi_arg = 0, 1, float(rhi)
+    else:
+        o.fhiP, o.fhi = 1, float(fhi)
+    o.inmemP = int(inmem)
+    o.force_standard = int(force_standard)
+    o.lobin_arg = int(lobin)
+    o.dat_input = int(dat_input)
+    rc = lib.o_create_accelobs(C.byref(o))
+    if rc:
+        raise ValueError(f"o_create_accelobs failed ({rc})")
+    return o
+
+
+def accelsearch(obs, fft, fp64=False, max_blocks=-1):
+    """Run the reference block loop; returns (hits structured array, cands)."""
+    fft = np.ascontiguousarray(fft, dtype=np.complex64)
+    res = OResult()
+    lib.o_accelsearch(C.byref(obs), fft, int(fp64), int(max_blocks), C.byref(res))
+    n = res.nhits
+    hits = np.zeros(n, dtype=[("block", "i4"), ("stage", "i4"), ("wi", "i4"),
+                              ("zi", "i4"), ("ri", "i4"), ("pow", "f4"), ("sigma", "f4")])
+    if n:
+        raw = np.ctypeslib.as_array(C.cast(res.hits, C.POINTER(C.c_byte)),
+                                    shape=(n * C.sizeof(OHit),))
+        hits = np.frombuffer(raw.tobytes(), dtype=hits.dtype).copy()
+    nc = lib.o_result_ncands(C.byref(res))
+    cands = np.zeros((nc, 7))
+    if nc:
+        lib.o_result_cands(C.byref(res), cands)
+    nblocks = res.nblocks
+    lib.o_free_result(C.byref(res))
+    return hits, cands[:, :6], nblocks
+
+
+class SubharmBanks:
+    """The reference's template banks (create_subharminfos) for one obs."""
+
+    def __init__(self, obs, fp64=False):
+        self.obs = obs
+        self.ptr = lib.o_create_subharminfos(C.byref(obs), int(fp64))
+        self.fp64 = fp64
+        self.shi_size = 7 * 4 + 4 + 3 * 8  # int fields + pad + 3 pointers
+
+    def shi(self, stage, harm):
+        """Pointer to shis[stage][harm-1]."""
+        arr = C.cast(self.ptr, C.POINTER(C.c_void_p))
+        base = arr[stage]
+        return C.c_void_p(base + (harm - 1) * self.shi_size)
+
+    def close(self):
+        if self.ptr:
+            lib.o_free_subharminfos(C.byref(self.obs), self.ptr)
+            self.ptr = None
+
+
+def subharm_fderivs_vol(numharm, harmnum, fullrlo, fullrhi, banks, obs, fft, fp64=False):
+    """One block's plane [numws][numzs][numrs] plus its rlo/zlo/wlo."""
+    fft = np.ascontiguousarray(fft, dtype=np.complex64)
+    stage = int(np.log2(numharm))
+    p = lib.o_subharm_fderivs_vol(numharm, harmnum, fullrlo, fullrhi,
+                                  banks.shi(stage, harmnum), C.byref(obs), fft, int(fp64))
+    f = p.contents
+    n = f.numws * f.numzs * f.numrs
+    src = f.dpowers if fp64 else f.powers
+    arr = np.ctypeslib.as_array(src, shape=(n,)).copy().reshape(f.numws, f.numzs, f.numrs)
+    meta = dict(rlo=f.rlo, zlo=f.zlo, wlo=f.wlo, numrs=f.numrs, numzs=f.numzs, numws=f.numws)
+    lib.o_free_ffdot(p)
+    return arr, meta
+
+
+def block_norm(obs, fft, lobin, numdata):
+    fft = np.ascontiguousarray(fft, dtype=np.complex64)
+    return lib.o_block_norm(C.byref(obs), fft, int(lobin), int(numdata))
+
+
+class OCand(C.Structure):
+    pass
+
+
+OCand._fields_ = [("power", _f), ("sigma", _f), ("numharm", _i), ("r", _d), ("z", _d),
+                  ("w", _d), ("next", C.POINTER(OCand))]
+_sig("o_insert_new_accelcand", C.POINTER(OCand), C.POINTER(OCand), _f, _f, _i, _d, _d, _d,
+     C.POINTER(_i))
+_sig("o_sort_accelcands", C.POINTER(OCand), C.POINTER(OCand))
+_sig("o_eliminate_harmonics", C.POINTER(OCand), C.POINTER(OCand), C.POINTER(_i))
+_sig("o_free_cands", None, C.POINTER(OCand))
+
+
+class CandList:
+    """The reference's candidate list (GSList of accelcand) in the oracle."""
+
+    def __init__(self):
+        self.head = C.POINTER(OCand)()
+
+    def insert(self, power, sigma, numharm, r, z=0.0, w=0.0):
+        added = _i(0)
+        self.head = lib.o_insert_new_accelcand(self.head, power, sigma, numharm, r, z, w,
+                                               C.byref(added))
+        return added.value
+
+    def _items(self):
+        out, p = [], self.head
+        while p:
+            out.append(p.contents)
+            p = p.contents.next
+        return out
+
+    def rs(self):
+        return [c.r for c in self._items()]
+
+    def sigmas(self):
+        return [c.sigma for c in self._items()]
+
+    def sort(self):
+        self.head = lib.o_sort_accelcands(self.head)
+
+    def eliminate(self):
+        n = _i(len(self._items()))
+        self.head = lib.o_eliminate_harmonics(self.head, C.byref(n))
+        return n.value
+
+    def free(self):
+        lib.o_free_cands(self.head)
+        self.head = C.POINTER(OCand)()

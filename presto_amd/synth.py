@@ -1,0 +1,48 @@
+"""Seeded synthetic time series for the accelsearch workloads: Gaussian
+noise plus pulsars with a Gaussian pulse profile and constant f-dot /
+f-dotdot.  PRESTO's Fourier derivatives are z = fdot*T^2 and
+w = fdotdot*T^3 (src/accel_utils.c:1431-1433 works on the same grid)."""
+import numpy as np
+
+
+def pulsar_series(N, dt, f0, z=0.0, w=0.0, duty=0.05, amp=0.05):
+    """Pulse train with average Fourier frequency f0*T + z/2 (+ w/6) bins.
+
+    phase(t) = f0 t + fdot t^2/2 + fdotdot t^3/6, Gaussian pulse of FWHM
+    duty/f0 in phase units, peak amplitude amp per sample."""
+    T = N * dt
+    t = np.arange(N, dtype=np.float64) * dt
+    fdot = z / T**2
+    fdd = w / T**3
+    ph = f0 * t + 0.5 * fdot * t * t + fdd * t * t * t / 6.0
+    d = ph - np.floor(ph + 0.5)          # distance to the nearest pulse centre
+    sig = duty / (2.0 * np.sqrt(2.0 * np.log(2.0)))
+    return amp * np.exp(-0.5 * (d / sig) ** 2)
+
+
+def make_series(N, dt, seed, pulsars=()):
+    """float32 series: N(0,1) noise from PCG64(seed) plus the given pulsars,
+    each a dict of pulsar_series keyword arguments."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = rng.standard_normal(N)
+    for p in pulsars:
+        x = x + pulsar_series(N, dt, **p)
+    return x.astype(np.float32)
+
+
+def config1_series():
+    """One 2^20-sample series, dt 64 us, one pulsar at 123.456 Hz, z=20."""
+    return make_series(1 << 20, 64e-6, 1,
+                       [dict(f0=123.456, z=20.0, duty=0.05, amp=0.05)])
+
+
+def config2_series(N=1 << 23, dt=64e-6, seed=2):
+    """2^23 samples, dt 64 us, 3 pulsars with f in [50,700] Hz, z in
+    [-150,150], duty 2-10 %, amplitude drawn in [0.01,0.05] from the seed."""
+    rng = np.random.Generator(np.random.PCG64(seed + 1000))
+    ps = []
+    for _ in range(3):
+        ps.append(dict(f0=float(rng.uniform(50, 700)), z=float(rng.uniform(-150, 150)),
+                       duty=float(rng.uniform(0.02, 0.10)),
+                       amp=float(rng.uniform(0.01, 0.05))))
+    return make_series(N, dt, seed, ps), ps

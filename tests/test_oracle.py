@@ -1,0 +1,358 @@
+"""Pins the oracle (oracle/liboracle.so) before anything is compared with it.
+
+Each check is chosen so that a plausible restatement mistake (a dropped
+term, a sign, an index or offset error) fails: the reference's own golden
+values, closed forms, library routines for the same mathematics, and
+brute force on tiny inputs."""
+import numpy as np
+import pytest
+import scipy.special as sp
+import scipy.stats as st
+from scipy.integrate import quad
+
+from oracle import oracle as O
+from presto_amd import synth
+
+
+# ---------------------------------------------------------------- Fresnel
+def test_fresnel_matches_scipy_cephes():
+    xs = np.concatenate([np.linspace(-40.0, 40.0, 4001), np.linspace(100, 36000, 301),
+                         [1e-9, 1.6007, 1.6008]])
+    for x in xs:
+        s, c = O.fresnl(x)
+        S, C = sp.fresnel(x)
+        assert abs(s - S) < 5e-12 and abs(c - C) < 5e-12, x
+
+
+def test_fresnel_large_argument_cutoff():
+    # src/fresnl.c:385-389: x > 36974 returns exactly 1/2
+    assert O.fresnl(40000.0) == (0.5, 0.5)
+    assert O.fresnl(-40000.0) == (-0.5, -0.5)
+
+
+# ---------------------------------------------------------------- responses
+def _resp_golden_setup():
+    nb, z, w = 16, 5.0, 40.0
+    bps = max(O.lib.o_r_resp_halfwidth(0), O.lib.o_z_resp_halfwidth(z, 0),
+              O.lib.o_w_resp_halfwidth(z, w, 0))
+    return nb, z, w, nb * bps * 2
+
+
+def test_response_golden_values():
+    # tests/test_presto_python.py:117-147 of the reference
+    nb, z, w, nn = _resp_golden_setup()
+    pr = np.abs(O.gen_r_response(0.0, nb, nn).astype(np.complex64)) ** 2
+    pz = np.abs(O.gen_z_response(0.0, nb, z, nn).astype(np.complex64)) ** 2
+    pw = np.abs(O.gen_w_response(0.0, nb, z, w, nn)) ** 2
+    assert pr[nn // 2] == 1.0
+    assert round(float(pz[nn // 2]) - 0.227675, 6) == 0
+    assert round(float(pw[nn // 2]) - 0.019462, 6) == 0
+
+
+def _chirp_coeff(f, z, w=0.0):
+    """Fourier coefficient at offset f (bins) of a unit chirp whose average
+    frequency is 0: r(t) = -z/2 + z t (+ w (t^2/2 - t/2 + 1/12))."""
+    def ph(t):
+        return 2 * np.pi * ((-z / 2 + w / 12) * t + (z - w / 2) / 2 * t * t
+                            + w / 6 * t ** 3 - f * t)
+    re = quad(lambda t: np.cos(ph(t)), 0, 1, limit=800)[0]
+    im = quad(lambda t: np.sin(ph(t)), 0, 1, limit=800)[0]
+    return re + 1j * im
+
+
+@pytest.mark.parametrize("z", [7.0, -12.0, 0.5])
+def test_z_response_is_chirp_fourier_coefficient(z):
+    nb = 2
+    nk = 2 * nb * O.lib.o_z_resp_halfwidth(z, 0)
+    resp = O.gen_z_response(0.0, nb, z, nk)
+    for k in range(0, nk, 7):
+        off = k / nb - nk / (2 * nb)
+        assert abs(resp[k] - _chirp_coeff(off, z)) < 1e-7, k
+
+
+def test_r_response_is_sinc():
+    # z = 0: the coefficient of a constant tone, exp(-i pi f) sinc(f)
+    nb, nk, roff = 4, 64, 0.3
+    resp = O.gen_r_response(roff, nb, nk)
+    for k in range(nk):
+        f = k / nb - nk / (2 * nb) - roff
+        assert abs(resp[k] - np.exp(-1j * np.pi * f) * np.sinc(f)) < 1e-12, k
+
+
+def test_w_response_properties():
+    nb, z = 2, 10.0
+    # w -> 0 reduces to the z response (src/responses.c:374-377)
+    nk = 2 * nb * O.lib.o_w_resp_halfwidth(z, 0.0, 0)
+    assert np.allclose(O.gen_w_response(0.0, nb, z, 0.0, nk, fp64=True),
+                       O.gen_z_response(0.0, nb, z, nk))
+    w = 60.0
+    nk = 2 * nb * O.lib.o_w_resp_halfwidth(z, w, 0)
+    r32 = O.gen_w_response(0.0, nb, z, w, nk)
+    r64 = O.gen_w_response(0.0, nb, z, w, nk, fp64=True)
+    # FP32 construction vs FP64 DFT of the same FP32 samples
+    assert np.max(np.abs(r32 - r64)) < 2e-6 * np.max(np.abs(r64))
+    # the sampled chirp approximates the analytic cubic-phase coefficient
+    # (the reference's own accuracy table, src/responses.c:319-331)
+    for k in range(nk // 4, 3 * nk // 4, 5):
+        off = k / nb - nk / (2 * nb)
+        assert abs(r64[k] - _chirp_coeff(off, z, w)) < 0.02 * np.max(np.abs(r64)), k
+
+
+# ---------------------------------------------------------------- FFTs
+@pytest.mark.parametrize("n", [8, 12, 20, 192, 384, 768, 1280, 2048, 4096, 25600, 63])
+def test_cfft_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.allclose(O.cfft(x, -1), np.fft.fft(x), atol=1e-10 * np.sqrt(n))
+    assert np.allclose(O.cfft(x, +1), np.fft.ifft(x) * n, atol=1e-10 * np.sqrt(n))
+    xf = O.cfft(x.astype(np.complex64), -1)
+    assert np.max(np.abs(xf - np.fft.fft(x))) < 2e-6 * np.sqrt(n) * np.log2(n)
+
+
+def test_realfft_packing_and_roundtrip():
+    # src/fastffts.c:163-243; round trip as tests/test_presto_python.py:9-14
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(1 << 12).astype(np.float32)
+    y = O.realfft(x).view(np.complex64)
+    ref = np.fft.rfft(x.astype(np.float64))
+    assert np.max(np.abs(y[1:] - ref[1:len(y)])) < 1e-4
+    assert abs(y[0].real - ref[0].real) < 1e-3 and abs(y[0].imag - ref[-1].real) < 1e-3
+    assert np.allclose(O.realfft(O.realfft(x), +1), x, atol=1e-5)
+
+
+# ---------------------------------------------------------------- statistics
+@pytest.mark.parametrize("n", [1, 2, 7, 8, 96, 1024, 1001])
+def test_median_is_exact_order_statistic(n):
+    rng = np.random.default_rng(n)
+    x = (rng.exponential(size=n)).astype(np.float32)
+    s = np.sort(x)
+    expect = s[n // 2] if n % 2 else np.float32(0.5 * (float(s[n // 2 - 1]) + float(s[n // 2])))
+    assert O.median(x) == expect
+
+
+def test_invsqrtf_error_bound():
+    # one Newton step after the 0x5f3759df seed: max relative error 1.752e-3
+    # (Lomont 2003); invsqrtf(1) = 0.998307...
+    xs = np.logspace(-10, 10, 20001).astype(np.float32)
+    ys = np.array([O.lib.o_invsqrtf(float(v)) for v in xs])
+    rel = np.abs(ys * np.sqrt(xs.astype(np.float64)) - 1.0)
+    assert rel.max() < 1.76e-3
+    assert abs(O.lib.o_invsqrtf(1.0) - 0.998307) < 1e-6
+
+
+@pytest.mark.parametrize("sig,ns,nt", [(2.0, 1, 2.4e8), (2.0, 16, 1.5e7), (5.0, 8, 1e10),
+                                       (2.0, 32, 3e6), (1.0, 2, 10.0)])
+def test_power_for_sigma_matches_scipy(sig, ns, nt):
+    p = O.lib.o_power_for_sigma(sig, ns, nt)
+    ref = 0.5 * st.chi2.isf(st.norm.sf(sig) / nt, 2 * ns)
+    assert abs(p - ref) < 1e-11 * ref
+
+
+@pytest.mark.parametrize("pw,ns,nt", [(30.0, 1, 2e8), (60.0, 8, 1e7), (200.0, 16, 1e6),
+                                      (25.0, 4, 1e3), (45.0, 16, 1e6)])
+def test_candidate_sigma_matches_scipy(pw, ns, nt):
+    s = O.lib.o_candidate_sigma(pw, ns, nt)
+    lp = st.chi2.logsf(2 * pw, 2 * ns) + np.log(nt)
+    if 2 * pw > 1200 or (2 * ns > 30 and 2 * pw > 100):
+        # the reference switches to A&S 26.4.19 with Stirling's log-gamma
+        assert abs(s - st.norm.isf(np.exp(lp))) < 5e-3
+    else:
+        assert abs(s - st.norm.isf(np.exp(lp))) < 1e-9
+
+
+def test_extended_sigma_branch():
+    # logp < -600 uses A&S 26.2.23 (|error| < 4.5e-4 in p-space); compare with
+    # the exact inverse through log space
+    for lp in (-700.0, -1500.0, -5000.0):
+        s = O.lib.o_equivalent_gaussian_sigma(lp)
+        exact = -sp.ndtri_exp(lp)
+        assert abs(s - exact) < 2e-3 * exact
+
+
+# ---------------------------------------------------------------- geometry
+@pytest.mark.parametrize("n,dt,zmax,numharm,wmax,expect", [
+    # numbins, rlo, highestbin, numz, numw, maxkernlen,
+    # fftlen, corr_uselen, blocks, in-memory
+    (2**20, 64e-6, 50, 8, None, (524288, 67, 524287, 51, 0, 172, 2048, 1876, 558, 1)),
+    (2**23, 64e-6, 200, 16, None, (4194304, 536, 4194303, 201, 0, 504, 2048, 1544, 5432, 0)),
+    (2**21, 256e-6, 200, 16, None, (1048576, 536, 1048575, 201, 0, 504, 2048, 1544, 1357, 1)),
+    (2**22, 128e-6, 200, 16, None, (2097152, 536, 2097151, 201, 0, 504, 2048, 1544, 2715, 0)),
+    (2**22, 64e-6, 200, 8, 300, (2097152, 268, 2097151, 201, 31, 612, 2048, 1436, 2920, 0)),
+    (2**24, 64e-6, 200, 16, 100, (8388608, 1073, 8388607, 201, 11, 540, 2048, 1508, 11124, 0)),
+    (2**20, 64e-6, 0, 8, None, (524288, 67, 524287, 1, 0, 64, 2048, 1984, 528, 1)),
+])
+def test_geometry_known_answers(n, dt, zmax, numharm, wmax, expect):
+    o = O.make_obs(n // 2, n, dt, zmax=zmax, numharm=numharm, wmax=wmax)
+    rstep = o.corr_uselen // 2
+    nblk, s = 0, o.rlo
+    while s + rstep < o.highestbin:
+        s += rstep
+        nblk += 1
+    got = (o.numbins, int(o.rlo), o.highestbin, o.numz, o.numw, o.maxkernlen, o.fftlen,
+           o.corr_uselen, nblk, o.inmem)
+    assert got == expect
+
+
+# ---------------------------------------------------------------- the plane
+def _raw_power(fft, numbins, r_half, z, nb=2):
+    """Brute-force |sum_m d_sp[2r+m] conj(resp_z[nk/2+m])|^2 over the spread
+    (interbinned) spectrum, zero outside [0, numbins)."""
+    nk = 2 * nb * O.lib.o_z_resp_halfwidth(z, 0)
+    resp = O.gen_z_response(0.0, nb, z, nk)
+    acc = 0j
+    for m in range(-nk // 2, nk // 2):
+        j = r_half + m
+        if j % 2 == 0 and 0 <= j // 2 < numbins:
+            acc += complex(fft[j // 2]) * np.conj(resp[nk // 2 + m])
+    return abs(acc) ** 2
+
+
+@pytest.fixture(scope="module")
+def small_case():
+    N, dt = 8192, 1e-3
+    x = synth.make_series(N, dt, 11, [dict(f0=40.3, z=6.0, duty=0.1, amp=0.4)])
+    fft = O.prepare_dat(x, dt)
+    obs = O.make_obs(len(fft), N, dt, zmax=20, numharm=4, sigma=2.0, force_standard=True)
+    return fft, obs
+
+
+def test_fundamental_plane_is_normalised_direct_correlation(small_case):
+    fft, obs = small_case
+    banks = O.SubharmBanks(obs, fp64=True)
+    startr = obs.rlo + 3 * (obs.corr_uselen // 2)
+    lastr = startr + obs.corr_uselen // 2 - 0.5
+    plane, meta = O.subharm_fderivs_vol(1, 1, startr, lastr, banks, obs, fft, fp64=True)
+    hw = O.lib.o_z_resp_halfwidth(obs.zhi, 0)
+    norm = O.block_norm(obs, fft, meta["rlo"] - hw, obs.fftlen // 2)
+    rng = np.random.default_rng(0)
+    for _ in range(40):
+        zi = int(rng.integers(meta["numzs"]))
+        ri = int(rng.integers(meta["numrs"]))
+        z = meta["zlo"] + 2 * zi
+        expect = norm ** 2 * _raw_power(fft, obs.numbins, 2 * meta["rlo"] + ri, z)
+        assert abs(plane[0, zi, ri] - expect) <= 1e-9 * max(expect, 1.0), (zi, ri)
+    banks.close()
+
+
+@pytest.mark.parametrize("numharm,harm", [(2, 1), (4, 1), (4, 3)])
+def test_subharmonic_plane_is_normalised_direct_correlation(small_case, numharm, harm):
+    """The standard path's sub-harmonic block equals norm_h^2 times the raw
+    correlation at the same (r, z): the fact the HIP path builds on."""
+    fft, obs = small_case
+    banks = O.SubharmBanks(obs, fp64=True)
+    startr = obs.rlo + 2 * (obs.corr_uselen // 2)
+    lastr = startr + obs.corr_uselen // 2 - 0.5
+    plane, meta = O.subharm_fderivs_vol(numharm, harm, startr, lastr, banks, obs, fft,
+                                        fp64=True)
+    hf = harm / numharm
+    zmax_h = int((0.5 * obs.zhi * hf + 0.5)) * 2
+    hw = O.lib.o_z_resp_halfwidth(zmax_h, 0)
+    numkern = 4 * hw
+    bins_needed = int(np.ceil(obs.corr_uselen * hf)) + 2
+    fftlen = O.lib.o_next_good_fftlen(bins_needed + numkern)
+    norm = O.block_norm(obs, fft, meta["rlo"] - hw, fftlen // 2)
+    rng = np.random.default_rng(1)
+    for _ in range(30):
+        zi = int(rng.integers(meta["numzs"]))
+        ri = int(rng.integers(int(obs.corr_uselen * hf)))
+        z = meta["zlo"] + 2 * zi
+        expect = norm ** 2 * _raw_power(fft, obs.numbins, 2 * meta["rlo"] + ri, z)
+        assert abs(plane[0, zi, ri] - expect) <= 1e-9 * max(expect, 1.0), (zi, ri)
+    banks.close()
+
+
+def test_fp32_plane_close_to_fp64(small_case):
+    fft, obs = small_case
+    b32, b64 = O.SubharmBanks(obs), O.SubharmBanks(obs, fp64=True)
+    startr = obs.rlo
+    lastr = startr + obs.corr_uselen // 2 - 0.5
+    p32, _ = O.subharm_fderivs_vol(1, 1, startr, lastr, b32, obs, fft)
+    p64, _ = O.subharm_fderivs_vol(1, 1, startr, lastr, b64, obs, fft, fp64=True)
+    err = np.abs(p32 - p64) / np.maximum(p64, 1.0)
+    assert err.max() < 1e-5
+    b32.close()
+    b64.close()
+
+
+# ---------------------------------------------------------------- list logic
+def test_insert_new_accelcand_cases():
+    # src/accel_utils.c:387-469: one candidate per 15 bins, higher sigma wins
+    L = O.CandList()
+    assert L.insert(10.0, 3.0, 1, 100.0) == 1
+    assert L.insert(11.0, 2.0, 1, 110.0) == 0          # weaker, within 15 bins
+    assert L.insert(12.0, 4.0, 2, 105.0) == 1          # stronger: overwrites
+    assert L.rs() == [105.0]
+    assert L.insert(9.0, 3.5, 1, 200.0) == 1           # new, appended
+    assert L.insert(9.0, 3.5, 1, 50.0) == 1            # new, at the head
+    assert L.rs() == [50.0, 105.0, 200.0]
+    assert L.insert(9.0, 3.6, 1, 160.0) == 1           # new, between
+    assert L.rs() == [50.0, 105.0, 160.0, 200.0]
+    assert L.insert(9.0, 3.6, 1, 185.0) == 1           # 25 and 15 away: new
+    assert L.rs() == [50.0, 105.0, 160.0, 185.0, 200.0]
+    L.free()
+    # within 15 of both neighbours and stronger than both: prev overwritten,
+    # next removed
+    L = O.CandList()
+    L.insert(5.0, 3.0, 1, 100.0)
+    L.insert(5.0, 3.0, 1, 120.0)
+    assert L.insert(20.0, 9.0, 4, 110.0) == 1
+    assert L.rs() == [110.0] and L.sigmas() == [9.0]
+    L.free()
+    # stronger than the next only: the next is overwritten
+    L = O.CandList()
+    L.insert(5.0, 10.0, 1, 100.0)
+    L.insert(5.0, 3.0, 1, 120.0)
+    assert L.insert(20.0, 5.0, 2, 110.0) == 1
+    assert L.rs() == [100.0, 110.0] and L.sigmas() == [10.0, 5.0]
+    L.free()
+
+
+def test_sort_and_eliminate_harmonics():
+    L = O.CandList()
+    for pw, sg, r in [(30, 5.0, 1000.0), (40, 8.0, 2000.4), (35, 8.0, 3000.0),
+                      (20, 3.0, 500.2), (25, 4.0, 1500.9), (50, 6.0, 7777.0)]:
+        L.insert(pw, sg, 1, r)
+    L.sort()
+    assert L.sigmas() == [8.0, 8.0, 6.0, 5.0, 4.0, 3.0]
+    assert L.rs()[:2] == [2000.4, 3000.0]               # tie: higher power first
+    n = L.eliminate()
+    # 1000 = 2000.4/2, 500.2 = 2000.4/4, 3000 = 2000.4*3/2, 1500.9 = 2000.4*3/4
+    assert L.rs() == [2000.4, 7777.0] and n == 2
+    L.free()
+
+
+def test_search_recovers_injected_pulsar(small_case):
+    fft, obs = small_case
+    hits, cands, nblk = O.accelsearch(obs, fft)
+    assert len(cands) > 0
+    T = obs.T
+    # strongest candidate: fundamental near f0*T + z/2 = 40.3*8.192 + 3
+    best = cands[0]
+    r_expect = 40.3 * T + 3.0
+    assert abs(best[3] - r_expect) < 1.0 and abs(best[4] - 6.0) < 2.5
+    # every hit is above its stage's cut and the list is sorted by sigma
+    for h in hits:
+        assert h["pow"] > obs.powcut[h["stage"]]
+    assert np.all(np.diff(cands[:, 1]) <= 0)
+
+
+def test_inmem_and_standard_agree_on_strong_signal():
+    N, dt = 8192, 1e-3
+    x = synth.make_series(N, dt, 12, [dict(f0=52.1, z=-4.0, duty=0.08, amp=0.5)])
+    fft = O.prepare_dat(x, dt)
+    o_in = O.make_obs(len(fft), N, dt, zmax=10, numharm=4)
+    o_st = O.make_obs(len(fft), N, dt, zmax=10, numharm=4, force_standard=True)
+    assert o_in.inmem == 1 and o_st.inmem == 0
+    _, c_in, _ = O.accelsearch(o_in, fft)
+    _, c_st, _ = O.accelsearch(o_st, fft)
+    assert abs(c_in[0][3] - c_st[0][3]) < 0.6 and c_in[0][2] == c_st[0][2]
+
+
+def test_deredden_whitens_noise():
+    rng = np.random.default_rng(3)
+    n = 1 << 16
+    x = (rng.standard_normal(n) * 7.0).astype(np.float32)
+    fft = O.prepare_dat(x, 1e-3)
+    pw = np.abs(fft[1000:]) ** 2
+    assert abs(np.mean(pw) - 1.0) < 0.05
+    assert fft[0] == np.complex64(1 + 1j)
