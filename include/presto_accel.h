@@ -1,0 +1,155 @@
+/*
+ * presto_accel.h -- C ABI of the MI355X accelsearch search path
+ * (libpresto_accel.so).
+ *
+ * The reference path is PRESTO's accelsearch: src/accelsearch.c:161-246
+ * drives the functions of include/accel.h:134-164 over overlap-save blocks
+ * of the spectrum.  This library keeps that behaviour -- block geometry,
+ * normalisation, harmonic index rules, thresholds, the order-dependent
+ * candidate merge, sorting and harmonic removal -- and computes the planes
+ * and harmonic sums on the GPU.  Each entry point below names the
+ * reference interface it replaces.
+ *
+ * Conventions: plain C types, host pointers unless a name says "device",
+ * return 0 on success and a negative code on failure (accel_last_error()
+ * describes it), where the reference printed and called exit().
+ */
+#ifndef PRESTO_ACCEL_H
+#define PRESTO_ACCEL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/ransomfft.h:13-18 */
+typedef struct {
+    float r, i;
+} accel_fcomplex;
+
+/* The search inputs: the accelsearch command line
+ * (clig/accelsearch_cmd.cli:13-48) and the .inf values that
+ * create_accelobs reads (src/accel_utils.c:1497, 1616, 1676). */
+typedef struct {
+    double N;            /* samples in the time series (.inf N) */
+    double dt;           /* sample time (s) (.inf dt) */
+    long long numbins;   /* complex bins in the spectrum */
+    long long lobin;     /* -lobin: bin number of the first spectrum element */
+    int numharm;         /* -numharm: 1, 2, 4, 8, 16 or 32 */
+    int zmax;            /* -zmax */
+    int wmax, wmaxP;     /* -wmax (jerk search when wmaxP) */
+    float sigma;         /* -sigma */
+    double flo, fhi;     /* -flo / -fhi (Hz) */
+    double rlo, rhi;     /* -rlo / -rhi (bins) */
+    int floP, fhiP, rloP, rhiP;
+    int inmem;           /* -inmem */
+    int force_standard;  /* never pick the in-memory path (testing) */
+    int dat_input;       /* the spectrum came from a .dat time series */
+} accel_params;
+
+/* Fills the command-line defaults: numharm 8, zmax 200, sigma 2.0,
+ * flo 1 Hz and fhi 10 kHz counted as given (src/accelsearch_cmd.c:30-74). */
+void accel_params_default(accel_params *p);
+
+/* The numeric part of accelobs (include/accel.h:37-81) as computed by
+ * create_accelobs (src/accel_utils.c:1596-1804). */
+typedef struct {
+    long long N, numbins, lobin, highestbin;
+    int maxkernlen, corr_uselen, fftlen, numharmstages, numz, numw, numbetween;
+    int inmem;
+    double dt, T, rlo, rhi, dr, zlo, zhi, dz, wlo, whi, dw;
+    float sigma;
+    float powcut[6];
+    long long numindep[6];
+    int nblocks;         /* blocks of the search loop (src/accelsearch.c:189) */
+} accel_geom;
+
+/* Replaces the numeric part of create_accelobs (src/accel_utils.c:1455). */
+int accel_geom_create(const accel_params *p, accel_geom *g);
+
+/* One entry of the final candidate list (accelcand, include/accel.h:83-95,
+ * without the refinement fields). */
+typedef struct {
+    float power, sigma;
+    int numharm;
+    int pad;
+    double r, z, w;
+} accel_cand;
+
+/* One threshold crossing, in the reference's search order
+ * (src/accel_utils.c:1419-1450). */
+typedef struct {
+    int block, stage, wi, zi, ii;
+    float pow;
+} accel_hit;
+
+typedef struct accel_search accel_search;
+
+/* Creates the device state of one search: geometry, FP64-generated
+ * templates resident in HBM, normalisation windows, index maps, the raw
+ * plane and the hit buffer.  Replaces create_subharminfos
+ * (src/accel_utils.c:240) and the allocations of create_accelobs.
+ * device: HIP device ordinal.  Returns NULL on failure. */
+accel_search *accel_search_create(const accel_params *p, int device);
+void accel_search_destroy(accel_search *s);   /* free_subharminfos / free_accelobs */
+const accel_geom *accel_search_geom(const accel_search *s);
+
+/* Copies the spectrum (numbins complex values, first one at bin lobin) to
+ * HBM.  Replaces the mmap / .dat read of create_accelobs. */
+int accel_search_set_spectrum(accel_search *s, const accel_fcomplex *fft);
+/* Uses a spectrum already in device memory (not copied, not owned). */
+int accel_search_set_device_spectrum(accel_search *s, const void *d_fft);
+
+/* Runs the block loop of src/accelsearch.c:161-228: normalisation of every
+ * block (subharm_fderivs_vol's median, src/accel_utils.c:1087-1099), the
+ * fused correlation (subharm_fderivs_vol), the harmonic sums
+ * (add_ffdotpows / inmem_add_ffdotpows) and the thresholds
+ * (search_ffdotpows), then replays the hits through the reference's
+ * candidate merge, sort_accelcands and eliminate_harmonics
+ * (src/accelsearch.c:232-241). */
+int accel_search_run(accel_search *s);
+/* Only the device part (norms, correlation, harmonic sums); no hit replay.
+ * Leaves the hit count on the device; accel_search_collect() finishes. */
+int accel_search_launch(accel_search *s);
+int accel_search_collect(accel_search *s);
+
+int accel_search_ncands(const accel_search *s);
+int accel_search_cands(const accel_search *s, accel_cand *out, int n);
+long long accel_search_nhits(const accel_search *s);
+int accel_search_hits(const accel_search *s, accel_hit *out, long long n);
+
+/* Inspection (tests): raw plane values for row (wi, zi), half-bins
+ * [hb0, hb0+n); normalisation factors (FP64) of every window. */
+int accel_search_plane_row(const accel_search *s, int wi, int zi, long long hb0, int n,
+                           float *out);
+int accel_search_nwindows(const accel_search *s);
+int accel_search_norms(const accel_search *s, double *out, int n);
+/* Milliseconds of the last run: norms, correlation, harmonic sum, total. */
+int accel_search_timings(const accel_search *s, float *ms4);
+/* Launch shape of the correlation kernel: segment length, used half-bins
+ * per segment, template groups, segments. */
+int accel_search_corr_shape(const accel_search *s, int *L, int *uselen, int *groups,
+                            long long *nseg);
+/* Raw plane half-bins written per run (templates x half-bins). */
+long long accel_search_plane_cells(const accel_search *s);
+
+/* Normalisation windows of the last plan: bins [lobin, lobin+numdata). */
+int accel_search_windows(const accel_search *s, long long *lobin, int *numdata, int n);
+
+/* Running-median de-reddening of a spectrum in place; replaces deredden
+ * (src/misc_utils.c:1054).  Returns the bins written or -1. */
+long accel_deredden(accel_fcomplex *fft, long numbins, int startwidth, int endwidth,
+                    double endfreq, double T);
+/* The .dat preparation after the real FFT: accel_deredden(fft, numbins, 6,
+ * 200, 0.5/dt, N*dt) then bin 0 := 1+1i (src/accel_utils.c:1561,1621-1624). */
+long accel_prepare_dat_spectrum(accel_fcomplex *fft, long numbins, double N, double dt);
+
+/* Significance (src/characteristics.c:526-559), for callers and tests. */
+double accel_candidate_sigma(double power, int numsum, double numtrials);
+double accel_power_for_sigma(double sigma, int numsum, double numtrials);
+
+const char *accel_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

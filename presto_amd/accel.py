@@ -1,0 +1,249 @@
+"""ctypes binding of libpresto_accel.so (include/presto_accel.h).
+
+The binding only loads the built HIP library; there is no CPU fallback.  A
+missing library raises at import, and every entry point reports the C
+library's error text on failure."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpresto_accel.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                      "or python presto_amd/build.py")
+lib = C.CDLL(LIB_PATH)
+
+_d, _i, _f, _ll, _p = C.c_double, C.c_int, C.c_float, C.c_longlong, C.c_void_p
+
+
+class Params(C.Structure):
+    _fields_ = [("N", _d), ("dt", _d), ("numbins", _ll), ("lobin", _ll),
+                ("numharm", _i), ("zmax", _i), ("wmax", _i), ("wmaxP", _i),
+                ("sigma", _f), ("flo", _d), ("fhi", _d), ("rlo", _d), ("rhi", _d),
+                ("floP", _i), ("fhiP", _i), ("rloP", _i), ("rhiP", _i),
+                ("inmem", _i), ("force_standard", _i), ("dat_input", _i)]
+
+
+class Geom(C.Structure):
+    _fields_ = [("N", _ll), ("numbins", _ll), ("lobin", _ll), ("highestbin", _ll),
+                ("maxkernlen", _i), ("corr_uselen", _i), ("fftlen", _i),
+                ("numharmstages", _i), ("numz", _i), ("numw", _i), ("numbetween", _i),
+                ("inmem", _i),
+                ("dt", _d), ("T", _d), ("rlo", _d), ("rhi", _d), ("dr", _d), ("zlo", _d),
+                ("zhi", _d), ("dz", _d), ("wlo", _d), ("whi", _d), ("dw", _d),
+                ("sigma", _f), ("powcut", _f * 6), ("numindep", _ll * 6), ("nblocks", _i)]
+
+
+class Cand(C.Structure):
+    _fields_ = [("power", _f), ("sigma", _f), ("numharm", _i), ("pad", _i),
+                ("r", _d), ("z", _d), ("w", _d)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("block", _i), ("stage", _i), ("wi", _i), ("zi", _i), ("ii", _i),
+                ("pow", _f)]
+
+
+HIT_DTYPE = np.dtype([("block", "i4"), ("stage", "i4"), ("wi", "i4"), ("zi", "i4"),
+                      ("ii", "i4"), ("pow", "f4")])
+CAND_DTYPE = np.dtype([("power", "f4"), ("sigma", "f4"), ("numharm", "i4"), ("pad", "i4"),
+                       ("r", "f8"), ("z", "f8"), ("w", "f8")])
+
+
+def _sig(name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+
+
+_PP, _GP = C.POINTER(Params), C.POINTER(Geom)
+_sig("accel_params_default", None, _PP)
+_sig("accel_geom_create", _i, _PP, _GP)
+_sig("accel_search_create", _p, _PP, _i)
+_sig("accel_search_destroy", None, _p)
+_sig("accel_search_geom", _GP, _p)
+_sig("accel_search_set_spectrum", _i, _p, _p)
+_sig("accel_search_set_device_spectrum", _i, _p, _p)
+_sig("accel_search_run", _i, _p)
+_sig("accel_search_launch", _i, _p)
+_sig("accel_search_collect", _i, _p)
+_sig("accel_search_ncands", _i, _p)
+_sig("accel_search_cands", _i, _p, _p, _i)
+_sig("accel_search_nhits", _ll, _p)
+_sig("accel_search_hits", _i, _p, _p, _ll)
+_sig("accel_search_plane_row", _i, _p, _i, _i, _ll, _i, _p)
+_sig("accel_search_nwindows", _i, _p)
+_sig("accel_search_norms", _i, _p, _p, _i)
+_sig("accel_search_timings", _i, _p, _p)
+_sig("accel_search_corr_shape", _i, _p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
+     C.POINTER(_ll))
+_sig("accel_search_plane_cells", _ll, _p)
+_sig("accel_search_windows", _i, _p, _p, _p, _i)
+_sig("accel_deredden", C.c_long, _p, C.c_long, _i, _i, _d, _d)
+_sig("accel_prepare_dat_spectrum", C.c_long, _p, C.c_long, _d, _d)
+_sig("accel_candidate_sigma", _d, _d, _i, _d)
+_sig("accel_power_for_sigma", _d, _d, _i, _d)
+_sig("accel_last_error", C.c_char_p)
+
+EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
+            "accel_search_destroy", "accel_search_geom", "accel_search_set_spectrum",
+            "accel_search_set_device_spectrum", "accel_search_run", "accel_search_launch",
+            "accel_search_collect", "accel_search_ncands", "accel_search_cands",
+            "accel_search_nhits", "accel_search_hits", "accel_search_plane_row",
+            "accel_search_nwindows", "accel_search_norms", "accel_search_timings",
+            "accel_search_corr_shape", "accel_search_plane_cells",
+            "accel_search_windows", "accel_deredden", "accel_prepare_dat_spectrum",
+            "accel_candidate_sigma", "accel_power_for_sigma", "accel_last_error"]
+
+
+class AccelError(RuntimeError):
+    pass
+
+
+def _err():
+    return AccelError(lib.accel_last_error().decode())
+
+
+def make_params(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0,
+                fhi=10000.0, inmem=False, force_standard=False, lobin=0, dat_input=True,
+                rlo=None, rhi=None):
+    """accelsearch command-line semantics (flo/fhi default to given)."""
+    p = Params()
+    lib.accel_params_default(C.byref(p))
+    p.N, p.dt, p.numbins, p.lobin = float(N), float(dt), int(numbins), int(lobin)
+    p.numharm, p.zmax, p.sigma = int(numharm), int(zmax), float(sigma)
+    if wmax is not None:
+        p.wmax, p.wmaxP = int(wmax), 1
+    if rlo is not None:
+        p.floP, p.rloP, p.rlo = 0, 1, float(rlo)
+    else:
+        p.floP, p.flo = 1, float(flo)
+    if rhi is not None:
+        p.fhiP, p.rhiP, p.rhi = 0, 1, float(rhi)
+    else:
+        p.fhiP, p.fhi = 1, float(fhi)
+    p.inmem, p.force_standard, p.dat_input = int(inmem), int(force_standard), int(dat_input)
+    return p
+
+
+def geometry(params):
+    g = Geom()
+    if lib.accel_geom_create(C.byref(params), C.byref(g)):
+        raise _err()
+    return g
+
+
+class Search:
+    """One accelsearch on the GPU: create (templates resident in HBM), set a
+    spectrum, run, read candidates."""
+
+    def __init__(self, params, device=0):
+        self.params = params
+        self.h = lib.accel_search_create(C.byref(params), int(device))
+        if not self.h:
+            raise _err()
+        self.geom = lib.accel_search_geom(self.h).contents
+
+    def close(self):
+        if self.h:
+            lib.accel_search_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def set_spectrum(self, fft):
+        fft = np.ascontiguousarray(fft, dtype=np.complex64)
+        if len(fft) != self.params.numbins:
+            raise ValueError("spectrum length differs from params.numbins")
+        if lib.accel_search_set_spectrum(self.h, fft.ctypes.data):
+            raise _err()
+
+    def set_device_spectrum(self, dptr):
+        if lib.accel_search_set_device_spectrum(self.h, C.c_void_p(int(dptr))):
+            raise _err()
+
+    def run(self):
+        if lib.accel_search_run(self.h):
+            raise _err()
+
+    def launch(self):
+        if lib.accel_search_launch(self.h):
+            raise _err()
+
+    def collect(self):
+        if lib.accel_search_collect(self.h):
+            raise _err()
+
+    def candidates(self):
+        n = lib.accel_search_ncands(self.h)
+        out = np.zeros(n, dtype=CAND_DTYPE)
+        if n:
+            lib.accel_search_cands(self.h, out.ctypes.data, n)
+        return out
+
+    def hits(self):
+        n = lib.accel_search_nhits(self.h)
+        out = np.zeros(n, dtype=HIT_DTYPE)
+        if n:
+            lib.accel_search_hits(self.h, out.ctypes.data, n)
+        return out
+
+    def plane_row(self, wi, zi, hb0, n):
+        out = np.zeros(n, dtype=np.float32)
+        if lib.accel_search_plane_row(self.h, wi, zi, int(hb0), int(n), out.ctypes.data):
+            raise _err()
+        return out
+
+    def norms(self):
+        n = lib.accel_search_nwindows(self.h)
+        out = np.zeros(max(n, 1), dtype=np.float64)
+        lib.accel_search_norms(self.h, out.ctypes.data, n)
+        return out[:n]
+
+    def windows(self):
+        n = lib.accel_search_nwindows(self.h)
+        lo = np.zeros(max(n, 1), dtype=np.int64)
+        nd = np.zeros(max(n, 1), dtype=np.int32)
+        if lib.accel_search_windows(self.h, lo.ctypes.data, nd.ctypes.data, n) < 0:
+            raise _err()
+        return lo[:n], nd[:n]
+
+    def timings(self):
+        ms = np.zeros(4, dtype=np.float32)
+        lib.accel_search_timings(self.h, ms.ctypes.data)
+        return dict(norms=float(ms[0]), corr=float(ms[1]), harmsum=float(ms[2]),
+                    total=float(ms[3]))
+
+    def corr_shape(self):
+        L, u, g, ns = _i(), _i(), _i(), _ll()
+        lib.accel_search_corr_shape(self.h, C.byref(L), C.byref(u), C.byref(g), C.byref(ns))
+        return dict(L=L.value, uselen=u.value, groups=g.value, nseg=ns.value)
+
+    def plane_cells(self):
+        return lib.accel_search_plane_cells(self.h)
+
+
+def spectrum_from_series(x, dt):
+    """Spectrum of a float32 time series the way accelsearch prepares a .dat:
+    forward real FFT in PRESTO's packing (bin 0 holds DC and Nyquist), then
+    accel_prepare_dat_spectrum.  The real FFT here is numpy's (FP64, rounded
+    to FP32)."""
+    x = np.asarray(x, dtype=np.float32)
+    X = np.fft.rfft(x.astype(np.float64))
+    n2 = len(x) // 2
+    out = np.empty(n2, dtype=np.complex64)
+    out[1:] = X[1:n2]
+    out[0] = complex(X[0].real, X[n2].real)
+    if lib.accel_prepare_dat_spectrum(out.ctypes.data, n2, float(len(x)), float(dt)) < 0:
+        raise AccelError("deredden failed (spectrum too short)")
+    return out

@@ -1,0 +1,880 @@
+/*
+ * accel_host.c -- the C host of the MI355X accelsearch path.
+ *
+ * Computes the reference's geometry and index rules on the host, lays out
+ * device state (templates, raw plane, normalisation windows, index maps,
+ * hit buffer), launches the HIP kernels through agpu.h and replays the
+ * threshold crossings through the reference's candidate logic.
+ *
+ * Device layout (one search):
+ *   spectrum   float2[numbins]                       (bin lobin first)
+ *   templates  float2[numw'][numz][L]                L = 4096, FP32, 1/L folded in
+ *   raw plane  float [numw'][numz][stride]           column = absolute half-bin,
+ *                                                    un-normalised |corr|^2
+ *   windows    {lobin, numdata}[nwin] -> norm (FP64), nsq = norm^2 (FP32)
+ *   zmap/wmap  int[nfrac][numz] / int[nfrac][numw']  sub-harmonic rows
+ *   hits       accel_hit[cap] + counter
+ * Correlation is linear, so the reference's normalised plane of any block
+ * and any harmonic fraction is norm^2 times the raw plane at the same
+ * (r, z, w); the harmonic-sum kernel applies the norm of the window that
+ * the reference would have used for that block and fraction.
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "presto_accel.h"
+#include "agpu.h"
+
+#define ACCEL_NUMBETWEEN 2
+#define ACCEL_DR 0.5
+#define ACCEL_RDR 2
+#define ACCEL_DZ 2
+#define ACCEL_RDZ 0.5
+#define ACCEL_DW 20
+#define ACCEL_RDW 0.05
+#define ACCEL_CLOSEST_R 15.0
+#define NUMFINTBINS 16
+#define MAXRAMUSE 2000000000LL
+#define CORR_L 4096
+#define HIT_CAP0 (1u << 20)
+
+static char g_err[1024];
+
+static void set_err(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+const char *accel_last_error(void) { return g_err; }
+
+#define GPU(call)                                                                \
+    do {                                                                         \
+        if ((call) != 0) {                                                       \
+            set_err("%s: %s", #call, agpu_last_error());                         \
+            return -1;                                                           \
+        }                                                                        \
+    } while (0)
+
+/* ---- index rules: src/accel_utils.c:36, 54-102 ---- */
+static int nearest_int(double x) { return (int) (x < 0 ? x - 0.5 : x + 0.5); }
+static double calc_required_r(double hf, double rfull) { return rint(ACCEL_RDR * rfull * hf) * ACCEL_DR; }
+static int calc_required_z(double hf, double zfull) { return nearest_int(ACCEL_RDZ * zfull * hf) * ACCEL_DZ; }
+static int calc_required_w(double hf, double wfull) { return nearest_int(ACCEL_RDW * wfull * hf) * ACCEL_DW; }
+
+/* ---- kernel widths and FFT lengths: src/responses.c:27-84, src/corr_prep.c:3-35 ---- */
+static int z_resp_halfwidth(double z) { return (int) (0.5 * 1.1 * fabs(z)) + NUMFINTBINS; }
+
+static int w_resp_halfwidth(double z, double w)
+{
+    if (fabs(w) < 1.0e-7)
+        return z_resp_halfwidth(z);
+    double r0 = 0.5 * (w / 6.0 - z), r1 = 0.5 * (w / 6.0 + z);
+    double maxdev = fabs(r0) > fabs(r1) ? fabs(r0) : fabs(r1);
+    double u_ext = 0.5 - z / w;
+    if (u_ext > 0.0 && u_ext < 1.0) {
+        double r_ext = 0.5 * w * u_ext * u_ext + (z - w / 2.0) * u_ext + r0;
+        if (fabs(r_ext) > maxdev)
+            maxdev = fabs(r_ext);
+    }
+    return (int) (1.1 * maxdev) + NUMFINTBINS;
+}
+
+static long long next2(long long x)
+{
+    long long t = 1;
+    while (t < x)
+        t <<= 1;
+    return t;
+}
+
+static int next_good_fftlen(int n)
+{
+    static const int lens[17] = {128, 192, 256, 384, 512, 768, 1024, 1280, 2048,
+                                 4096, 5120, 7680, 10240, 12288, 15360, 16384, 25600};
+    if (n <= lens[0])
+        return lens[0];
+    if (n > lens[16])
+        return (int) next2(n);
+    int i = 0;
+    while (n > lens[i])
+        i++;
+    return lens[i];
+}
+
+static int fftlen_from_kernwidth(int kw)
+{
+    static const int lim[9] = {6, 52, 67, 378, 664, 1672, 3015, 3554, 6000};
+    static const int len[9] = {128, 256, 512, 1024, 2048, 4096, 10240, 15360, 25600};
+    for (int i = 0; i < 9; i++)
+        if (kw < lim[i])
+            return len[i];
+    return (int) next2((long long) kw * 5);
+}
+
+static int calc_fftlen(int numharm, int harmnum, int zmax, int wmax, int corr_uselen)
+{
+    double hf = (double) harmnum / (double) numharm;
+    int bins_needed = (int) ceil(corr_uselen * hf) + 2;
+    int end_effects = 2 * ACCEL_NUMBETWEEN *
+        w_resp_halfwidth(calc_required_z(hf, zmax), calc_required_w(hf, wmax));
+    return next_good_fftlen(bins_needed + end_effects);
+}
+
+void accel_params_default(accel_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->numharm = 8;
+    p->zmax = 200;
+    p->sigma = 2.0f;
+    p->flo = 1.0;
+    p->fhi = 10000.0;
+    p->floP = 1;
+    p->fhiP = 1;
+}
+
+static int log2i(int n)
+{
+    int x = 0;
+    while (n > 1) {
+        n >>= 1;
+        x++;
+    }
+    return x;
+}
+
+/* src/accel_utils.c:1596-1804 */
+int accel_geom_create(const accel_params *p, accel_geom *g)
+{
+    memset(g, 0, sizeof(*g));
+    int zmax = p->zmax, wmax = p->wmax;
+    if (zmax < 0) {
+        set_err("zmax must be >= 0");
+        return -1;
+    }
+    if (zmax % ACCEL_DZ)
+        zmax = (zmax / ACCEL_DZ + 1) * ACCEL_DZ;
+    g->N = (long long) p->N;
+    g->numbins = p->numbins;
+    g->lobin = p->lobin;
+    if (g->lobin > 0 && g->lobin > g->numbins - 1) {
+        set_err("'lobin' is greater than the total number of frequencies in the data set");
+        return -1;
+    }
+    if (p->numharm != 1 && p->numharm != 2 && p->numharm != 4 && p->numharm != 8 &&
+        p->numharm != 16 && p->numharm != 32) {
+        set_err("'numharm' = %d must be a power-of-two", p->numharm);
+        return -1;
+    }
+    g->numharmstages = log2i(p->numharm) + 1;
+    g->dz = ACCEL_DZ;
+    g->numz = (zmax / ACCEL_DZ) * 2 + 1;
+    if (p->wmaxP) {
+        if (wmax % ACCEL_DW)
+            wmax = (wmax / ACCEL_DW + 1) * ACCEL_DW;
+        g->whi = wmax;
+        g->wlo = -wmax;
+        g->dw = ACCEL_DW;
+        g->numw = (wmax / ACCEL_DW) * 2 + 1;
+        if (wmax == 0)
+            g->numw = 0;
+    }
+    g->numbetween = ACCEL_NUMBETWEEN;
+    g->dt = p->dt;
+    g->T = p->dt * p->N;
+    if (p->floP) {
+        g->rlo = floor(p->flo * g->T);
+    } else {
+        g->rlo = p->rloP ? p->rlo : 1.0;
+    }
+    if (g->rlo < g->lobin)
+        g->rlo = (double) g->lobin;
+    if (g->rlo > g->numbins - 1) {
+        set_err("low frequency to search is above the highest available frequency");
+        return -1;
+    }
+    g->highestbin = g->numbins - 1;
+    g->rhi = (double) g->highestbin;
+    if (p->fhiP || p->rhiP) {
+        g->highestbin = p->fhiP ? (long long) ceil(p->fhi * g->T) : (long long) p->rhi;
+        if (g->highestbin > g->numbins - 1)
+            g->highestbin = g->numbins - 1;
+        g->rhi = (double) g->highestbin;
+        if (g->highestbin < g->rlo) {
+            set_err("high frequency to search is below the low frequency");
+            return -1;
+        }
+    }
+    g->dr = ACCEL_DR;
+    g->zhi = zmax;
+    g->zlo = -zmax;
+    g->sigma = p->sigma;
+    for (int ii = 0; ii < g->numharmstages; ii++) {
+        if (g->numz == 1 && g->numw == 0)
+            g->numindep[ii] = (long long) ((g->rhi - g->rlo) / (1 << ii));
+        else if (g->numz > 1 && g->numw == 0)
+            g->numindep[ii] = (long long) ((g->rhi - g->rlo) * (g->numz + 1) *
+                                           (g->dz / 6.95) / (1 << ii));
+        else
+            g->numindep[ii] = (long long) ((g->rhi - g->rlo) * (g->numz + 1) *
+                                           (g->dz / 6.95) * (g->numw + 1) *
+                                           (g->dw / 44.2) / (1 << ii));
+        g->powcut[ii] = (float) accel_power_for_sigma(g->sigma, 1 << ii,
+                                                      (double) g->numindep[ii]);
+    }
+    g->maxkernlen = 2 * ACCEL_NUMBETWEEN * w_resp_halfwidth(g->zhi, g->whi);
+    g->fftlen = fftlen_from_kernwidth(g->maxkernlen);
+    if (g->fftlen < 2048)
+        g->fftlen = 2048;
+    g->corr_uselen = g->fftlen - g->maxkernlen;
+    if (g->corr_uselen % ACCEL_RDR)
+        g->corr_uselen = g->corr_uselen / ACCEL_RDR * ACCEL_RDR;
+    long long memuse = 4LL * (g->highestbin + g->corr_uselen) * g->numbetween * g->numz *
+                       (p->wmaxP ? g->numw : 1);
+    g->inmem = (!p->wmaxP && (memuse < MAXRAMUSE || p->inmem)) ? 1 : 0;
+    if (p->force_standard)
+        g->inmem = 0;
+    /* src/accelsearch.c:186-227 */
+    int rstep = (int) (g->corr_uselen * ACCEL_DR);
+    double startr = g->rlo;
+    int nb = 0;
+    while (startr + rstep < g->highestbin) {
+        startr += rstep;
+        nb++;
+    }
+    g->nblocks = nb;
+    return 0;
+}
+
+/* ---- candidate list: src/accel_utils.c:330-549 ---- */
+typedef struct cnode {
+    accel_cand c;
+    struct cnode *next;
+} cnode;
+
+static cnode *cnew(float power, float sigma, int numharm, double r, double z, double w)
+{
+    cnode *n = (cnode *) calloc(1, sizeof(cnode));
+    n->c.power = power;
+    n->c.sigma = sigma;
+    n->c.numharm = numharm;
+    n->c.r = r;
+    n->c.z = z;
+    n->c.w = w;
+    return n;
+}
+
+static void cset(cnode *n, float power, float sigma, int numharm, double r, double z, double w)
+{
+    n->c.power = power;
+    n->c.sigma = sigma;
+    n->c.numharm = numharm;
+    n->c.r = r;
+    n->c.z = z;
+    n->c.w = w;
+}
+
+/* insert_new_accelcand, src/accel_utils.c:387-469 */
+static cnode *insert_cand(cnode *list, float power, float sigma, int numharm, double rr,
+                          double zz, double ww, int *added)
+{
+    cnode *tmp = list, *prev = NULL;
+    double prev_diff = ACCEL_CLOSEST_R + 1.0, next_diff;
+    *added = 0;
+    if (!list) {
+        *added = 1;
+        return cnew(power, sigma, numharm, rr, zz, ww);
+    }
+    while (tmp->next && tmp->c.r < rr) {
+        prev = tmp;
+        tmp = tmp->next;
+    }
+    next_diff = fabs(rr - tmp->c.r);
+    if (prev)
+        prev_diff = fabs(rr - prev->c.r);
+    if (prev_diff < ACCEL_CLOSEST_R) {
+        if (prev->c.sigma < sigma) {
+            cset(prev, power, sigma, numharm, rr, zz, ww);
+            *added = 1;
+        }
+        if (next_diff < ACCEL_CLOSEST_R && tmp->c.sigma < sigma) {
+            if (*added) {
+                prev->next = tmp->next;
+                free(tmp);
+            } else {
+                cset(tmp, power, sigma, numharm, rr, zz, ww);
+                *added = 1;
+            }
+        }
+    } else if (next_diff < ACCEL_CLOSEST_R) {
+        if (tmp->c.sigma < sigma) {
+            cset(tmp, power, sigma, numharm, rr, zz, ww);
+            *added = 1;
+        }
+    } else {
+        cnode *n = cnew(power, sigma, numharm, rr, zz, ww);
+        *added = 1;
+        if (!tmp->next && tmp->c.r < (rr - ACCEL_CLOSEST_R)) {
+            tmp->next = n;
+            return list;
+        }
+        if (prev) {
+            prev->next = n;
+            n->next = tmp;
+        } else {
+            n->next = list;
+            return n;
+        }
+    }
+    return list;
+}
+
+/* compare_accelcand_sigma, src/accel_utils.c:364-377 */
+static int cmp_sigma(const accel_cand *a, const accel_cand *b)
+{
+    int r = (a->sigma < b->sigma) - (a->sigma > b->sigma);
+    return r ? r : (a->power < b->power) - (a->power > b->power);
+}
+
+/* sort_accelcands: a stable merge sort like g_slist_sort */
+static cnode *sort_cands(cnode *l)
+{
+    if (!l || !l->next)
+        return l;
+    cnode *slow = l, *fast = l->next;
+    while (fast && fast->next) {
+        slow = slow->next;
+        fast = fast->next->next;
+    }
+    cnode *b = slow->next;
+    slow->next = NULL;
+    cnode *a = sort_cands(l);
+    b = sort_cands(b);
+    cnode head, *t = &head;
+    while (a && b) {
+        if (cmp_sigma(&a->c, &b->c) <= 0) {
+            t->next = a;
+            a = a->next;
+        } else {
+            t->next = b;
+            b = b->next;
+        }
+        t = t->next;
+    }
+    t->next = a ? a : b;
+    return head.next;
+}
+
+/* eliminate_harmonics, src/accel_utils.c:472-549 */
+static const double HARM_RATIOS[23][2] = {
+    {3, 2}, {5, 2}, {2, 3}, {4, 3}, {5, 3}, {3, 4}, {5, 4}, {2, 5}, {3, 5}, {4, 5},
+    {5, 6}, {2, 7}, {3, 7}, {4, 7}, {3, 8}, {5, 8}, {2, 9}, {3, 10}, {2, 11}, {3, 11},
+    {2, 13}, {3, 13}, {2, 15}};
+
+static int harmonically_related(double cr, double orr)
+{
+    const double tooclose = 1.5;
+    for (int ii = 1; ii <= 16; ii++)
+        if (fabs(cr / ii - orr) < tooclose)
+            return 1;
+    for (int ii = 1; ii <= 16; ii++)
+        if (fabs(cr * ii - orr) < tooclose)
+            return 1;
+    for (int k = 0; k < 23; k++)
+        if (fabs(cr * HARM_RATIOS[k][0] / HARM_RATIOS[k][1] - orr) < tooclose)
+            return 1;
+    return 0;
+}
+
+static cnode *eliminate_harmonics(cnode *cands, int *numcands)
+{
+    cnode *cur = cands;
+    if (!cur)
+        return cands;
+    while (cur->next) {
+        cnode *before = cur, *other = cur->next;
+        do {
+            if (harmonically_related(cur->c.r, other->c.r)) {
+                cnode *gone = other;
+                other = other->next;
+                before->next = other;
+                free(gone);
+                (*numcands)--;
+            } else {
+                before = other;
+                other = other->next;
+            }
+        } while (other);
+        if (cur->next)
+            cur = cur->next;
+    }
+    return cands;
+}
+
+static void free_cands(cnode *l)
+{
+    while (l) {
+        cnode *n = l->next;
+        free(l);
+        l = n;
+    }
+}
+
+/* ---- the search ---- */
+struct accel_search {
+    accel_params p;
+    accel_geom g;
+    int numw1;            /* max(numw, 1) */
+    int nt;               /* templates = numw1 * numz */
+    int hw;               /* largest kernel half width (bins) */
+    int L, uselen, groups;
+    long long nseg;
+    long long hb_end, stride;
+    int nfrac;
+    int npre;             /* in-memory pre-population blocks */
+    int nwin, maxn;
+    void *stream;
+    void *ev[4];
+    void *d_spec;
+    int own_spec;
+    void *d_T;
+    float *d_plane;
+    agpu_window *d_win;
+    double *d_norm;
+    float *d_nsq;
+    int *d_zmap, *d_wmap;
+    agpu_hit *d_hits;
+    unsigned *d_nhits;
+    unsigned cap;
+    /* results */
+    accel_hit *hits;
+    long long nhits;
+    cnode *cands;
+    int ncands;
+    float ms[4];
+};
+
+const accel_geom *accel_search_geom(const accel_search *s) { return &s->g; }
+
+static int build_tables(accel_search *s, agpu_window **win_out, int **zmap_out,
+                        int **wmap_out)
+{
+    const accel_geom *g = &s->g;
+    const int numz = g->numz, numw1 = s->numw1;
+    const int rstep = g->corr_uselen / 2;
+    const int zmax = (int) g->zhi, wmax = (int) g->whi;
+    s->nfrac = (1 << (g->numharmstages - 1)) - 1;
+    int *zmap = (int *) calloc((size_t) (s->nfrac > 0 ? s->nfrac : 1) * numz, sizeof(int));
+    int *wmap = (int *) calloc((size_t) (s->nfrac > 0 ? s->nfrac : 1) * numw1, sizeof(int));
+    int f = 0;
+    for (int st = 1; st < g->numharmstages; st++) {
+        int H = 1 << st;
+        for (int k = 1; k < H; k += 2, f++) {
+            double hf = (double) k / (double) H;
+            for (int zi = 0; zi < numz; zi++)
+                zmap[f * numz + zi] = (calc_required_z(hf, g->zlo + zi * ACCEL_DZ) + zmax) / ACCEL_DZ;
+            for (int wi = 0; wi < numw1; wi++)
+                wmap[f * numw1 + wi] =
+                    g->numw ? (calc_required_w(hf, g->wlo + wi * ACCEL_DW) + wmax) / ACCEL_DW : 0;
+        }
+    }
+    /* normalisation windows */
+    agpu_window *win;
+    int fund_hw = w_resp_halfwidth(-g->zhi, -g->whi);
+    int fund_nd = g->fftlen / ACCEL_NUMBETWEEN;
+    s->maxn = fund_nd;
+    if (g->inmem) {
+        s->npre = 0;
+        for (double sr = 8; sr < g->rlo; sr += rstep)
+            s->npre++;
+        s->nwin = s->npre + g->nblocks;
+        win = (agpu_window *) calloc(s->nwin > 0 ? s->nwin : 1, sizeof(agpu_window));
+        int i = 0;
+        for (double sr = 8; sr < g->rlo; sr += rstep, i++) {
+            win[i].lobin = (long long) floor(sr) - fund_hw;
+            win[i].numdata = fund_nd;
+        }
+        for (int b = 0; b < g->nblocks; b++, i++) {
+            win[i].lobin = (long long) (g->rlo + (double) b * rstep) - fund_hw;
+            win[i].numdata = fund_nd;
+        }
+    } else {
+        s->npre = 0;
+        s->nwin = g->nblocks * (s->nfrac + 1);
+        win = (agpu_window *) calloc(s->nwin > 0 ? s->nwin : 1, sizeof(agpu_window));
+        int hwf[64], ndf[64];
+        f = 0;
+        for (int st = 1; st < g->numharmstages; st++) {
+            int H = 1 << st;
+            for (int k = 1; k < H; k += 2, f++) {
+                double hf = (double) k / (double) H;
+                int zmax_h = calc_required_z(hf, zmax), wmax_h = calc_required_w(hf, wmax);
+                hwf[f] = w_resp_halfwidth(-zmax_h, -wmax_h);
+                ndf[f] = calc_fftlen(H, k, zmax, wmax, g->corr_uselen) / ACCEL_NUMBETWEEN;
+                if (ndf[f] > s->maxn)
+                    s->maxn = ndf[f];
+            }
+        }
+        for (int b = 0; b < g->nblocks; b++) {
+            double startr = g->rlo + (double) b * rstep;
+            agpu_window *wb = win + (size_t) b * (s->nfrac + 1);
+            wb[0].lobin = (long long) startr - fund_hw;
+            wb[0].numdata = fund_nd;
+            f = 0;
+            for (int st = 1; st < g->numharmstages; st++) {
+                int H = 1 << st;
+                for (int k = 1; k < H; k += 2, f++) {
+                    double hf = (double) k / (double) H;
+                    long long rlo_h = (long long) floor(calc_required_r(hf, startr));
+                    wb[1 + f].lobin = rlo_h - hwf[f];
+                    wb[1 + f].numdata = ndf[f];
+                }
+            }
+        }
+    }
+    *win_out = win;
+    *zmap_out = zmap;
+    *wmap_out = wmap;
+    return 0;
+}
+
+static int gen_templates(accel_search *s)
+{
+    const accel_geom *g = &s->g;
+    if (g->numw) {
+        set_err("jerk (w) templates are not built on this path yet");
+        return -1;
+    }
+    int *zs = (int *) malloc(sizeof(int) * s->nt);
+    int *nk = (int *) malloc(sizeof(int) * s->nt);
+    for (int zi = 0; zi < g->numz; zi++) {
+        int z = (int) g->zlo + zi * ACCEL_DZ;
+        zs[zi] = z;
+        nk[zi] = 2 * ACCEL_NUMBETWEEN * z_resp_halfwidth(z);
+    }
+    int rc = agpu_gen_templates_z(zs, nk, s->nt, s->L, s->d_T, s->stream);
+    free(zs);
+    free(nk);
+    if (rc) {
+        set_err("template generation: %s", agpu_last_error());
+        return -1;
+    }
+    return 0;
+}
+
+static int search_init(accel_search *s, int device)
+{
+    accel_geom *g = &s->g;
+    GPU(agpu_init(device));
+    GPU(agpu_stream_create(&s->stream));
+    for (int i = 0; i < 4; i++)
+        GPU(agpu_event_create(&s->ev[i]));
+    s->numw1 = g->numw ? g->numw : 1;
+    s->nt = s->numw1 * g->numz;
+    s->hw = g->maxkernlen / (2 * ACCEL_NUMBETWEEN);
+    s->L = CORR_L;
+    if (4 * s->hw + 64 > s->L) {
+        set_err("kernel width %d too large for %d-point segments", 4 * s->hw, s->L);
+        return -1;
+    }
+    s->uselen = (s->L - 4 * s->hw) & ~1;
+    s->hb_end = (long long) (2.0 * g->rlo) + (long long) g->nblocks * g->corr_uselen;
+    s->stride = (s->hb_end + 63) / 64 * 64;
+    s->groups = 16;
+    s->nseg = (s->hb_end + s->uselen - 1) / s->uselen;
+    agpu_window *win = NULL;
+    int *zmap = NULL, *wmap = NULL;
+    build_tables(s, &win, &zmap, &wmap);
+    size_t plane_bytes = sizeof(float) * (size_t) s->nt * (size_t) s->stride;
+    GPU(agpu_malloc(&s->d_spec, sizeof(accel_fcomplex) * (size_t) (g->numbins > 0 ? g->numbins : 1)));
+    s->own_spec = 1;
+    GPU(agpu_malloc(&s->d_T, sizeof(accel_fcomplex) * (size_t) s->nt * s->L));
+    GPU(agpu_malloc((void **) &s->d_plane, plane_bytes > 0 ? plane_bytes : 4));
+    GPU(agpu_malloc((void **) &s->d_win, sizeof(agpu_window) * (size_t) (s->nwin > 0 ? s->nwin : 1)));
+    GPU(agpu_malloc((void **) &s->d_norm, sizeof(double) * (size_t) (s->nwin > 0 ? s->nwin : 1)));
+    GPU(agpu_malloc((void **) &s->d_nsq, sizeof(float) * (size_t) (s->nwin > 0 ? s->nwin : 1)));
+    int nf = s->nfrac > 0 ? s->nfrac : 1;
+    GPU(agpu_malloc((void **) &s->d_zmap, sizeof(int) * (size_t) nf * g->numz));
+    GPU(agpu_malloc((void **) &s->d_wmap, sizeof(int) * (size_t) nf * s->numw1));
+    s->cap = HIT_CAP0;
+    GPU(agpu_malloc((void **) &s->d_hits, sizeof(agpu_hit) * s->cap));
+    GPU(agpu_malloc((void **) &s->d_nhits, sizeof(unsigned)));
+    GPU(agpu_h2d(s->d_win, win, sizeof(agpu_window) * (size_t) (s->nwin > 0 ? s->nwin : 1), s->stream));
+    GPU(agpu_h2d(s->d_zmap, zmap, sizeof(int) * (size_t) nf * g->numz, s->stream));
+    GPU(agpu_h2d(s->d_wmap, wmap, sizeof(int) * (size_t) nf * s->numw1, s->stream));
+    free(win);
+    free(zmap);
+    free(wmap);
+    if (gen_templates(s))
+        return -1;
+    return 0;
+}
+
+accel_search *accel_search_create(const accel_params *p, int device)
+{
+    accel_search *s = (accel_search *) calloc(1, sizeof(accel_search));
+    s->p = *p;
+    if (accel_geom_create(p, &s->g) || search_init(s, device)) {
+        char keep[sizeof(g_err)];
+        memcpy(keep, g_err, sizeof(keep));
+        accel_search_destroy(s);
+        memcpy(g_err, keep, sizeof(keep));
+        return NULL;
+    }
+    return s;
+}
+
+void accel_search_destroy(accel_search *s)
+{
+    if (!s)
+        return;
+    if (s->stream)
+        agpu_stream_sync(s->stream);
+    if (s->own_spec)
+        agpu_free(s->d_spec);
+    agpu_free(s->d_T);
+    agpu_free(s->d_plane);
+    agpu_free(s->d_win);
+    agpu_free(s->d_norm);
+    agpu_free(s->d_nsq);
+    agpu_free(s->d_zmap);
+    agpu_free(s->d_wmap);
+    agpu_free(s->d_hits);
+    agpu_free(s->d_nhits);
+    for (int i = 0; i < 4; i++)
+        if (s->ev[i])
+            agpu_event_destroy(s->ev[i]);
+    if (s->stream)
+        agpu_stream_destroy(s->stream);
+    free(s->hits);
+    free_cands(s->cands);
+    free(s);
+}
+
+int accel_search_set_spectrum(accel_search *s, const accel_fcomplex *fft)
+{
+    if (!s->own_spec) {
+        GPU(agpu_malloc(&s->d_spec, sizeof(accel_fcomplex) * (size_t) s->g.numbins));
+        s->own_spec = 1;
+    }
+    GPU(agpu_h2d(s->d_spec, fft, sizeof(accel_fcomplex) * (size_t) s->g.numbins, s->stream));
+    return 0;
+}
+
+int accel_search_set_device_spectrum(accel_search *s, const void *d_fft)
+{
+    if (s->own_spec)
+        agpu_free(s->d_spec);
+    s->own_spec = 0;
+    s->d_spec = (void *) d_fft;
+    return 0;
+}
+
+static void fill_harmsum_args(const accel_search *s, agpu_harmsum_args *a)
+{
+    const accel_geom *g = &s->g;
+    memset(a, 0, sizeof(*a));
+    a->plane = s->d_plane;
+    a->stride = s->stride;
+    a->numz = g->numz;
+    a->numw = s->numw1;
+    a->rh0 = (long long) (2.0 * g->rlo);
+    a->corr_uselen = g->corr_uselen;
+    a->nblocks = g->nblocks;
+    a->nstages = g->numharmstages;
+    a->inmem = g->inmem;
+    for (int i = 0; i < g->numharmstages; i++)
+        a->powcut[i] = g->powcut[i];
+    a->nfrac = s->nfrac;
+    a->zmap = s->d_zmap;
+    a->wmap = s->d_wmap;
+    a->nsq = s->d_nsq;
+    a->npre = s->npre;
+    a->hits = s->d_hits;
+    a->nhits = s->d_nhits;
+    a->cap = s->cap;
+}
+
+int accel_search_launch(accel_search *s)
+{
+    const accel_geom *g = &s->g;
+    agpu_harmsum_args a;
+    GPU(agpu_memset(s->d_nhits, 0, sizeof(unsigned), s->stream));
+    GPU(agpu_event_record(s->ev[0], s->stream));
+    GPU(agpu_block_norms(s->d_spec, g->numbins, g->lobin, s->d_win, s->nwin, s->maxn,
+                         s->d_norm, s->d_nsq, s->stream));
+    GPU(agpu_event_record(s->ev[1], s->stream));
+    GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw, s->uselen,
+                       s->hb_end, s->d_plane, s->stride, s->groups, s->stream));
+    GPU(agpu_event_record(s->ev[2], s->stream));
+    fill_harmsum_args(s, &a);
+    GPU(agpu_harmsum(&a, s->stream));
+    GPU(agpu_event_record(s->ev[3], s->stream));
+    return 0;
+}
+
+static int cmp_hit(const void *pa, const void *pb)
+{
+    const accel_hit *a = (const accel_hit *) pa, *b = (const accel_hit *) pb;
+    if (a->block != b->block) return a->block < b->block ? -1 : 1;
+    if (a->stage != b->stage) return a->stage < b->stage ? -1 : 1;
+    if (a->wi != b->wi) return a->wi < b->wi ? -1 : 1;
+    if (a->zi != b->zi) return a->zi < b->zi ? -1 : 1;
+    if (a->ii != b->ii) return a->ii < b->ii ? -1 : 1;
+    return 0;
+}
+
+/* Replays the hits in the reference's order: block, then stage, then
+ * w, z, r (src/accelsearch.c:189-227, src/accel_utils.c:1419-1450). */
+static int replay(accel_search *s)
+{
+    const accel_geom *g = &s->g;
+    const int rstep = g->corr_uselen / 2;
+    qsort(s->hits, (size_t) s->nhits, sizeof(accel_hit), cmp_hit);
+    free_cands(s->cands);
+    s->cands = NULL;
+    for (long long i = 0; i < s->nhits; i++) {
+        const accel_hit *h = &s->hits[i];
+        int numharm = 1 << h->stage;
+        double startr = g->rlo + (double) h->block * rstep;
+        float sig = (float) accel_candidate_sigma(h->pow, numharm,
+                                                  (double) g->numindep[h->stage]);
+        double rr = ((long long) startr + h->ii * (double) ACCEL_DR) / (double) numharm;
+        double zz = (g->zlo + h->zi * (double) ACCEL_DZ) / (double) numharm;
+        double ww = (g->wlo + h->wi * (double) ACCEL_DW) / (double) numharm;
+        int added;
+        s->cands = insert_cand(s->cands, h->pow, sig, numharm, rr, zz, ww, &added);
+    }
+    int n = 0;
+    for (cnode *c = s->cands; c; c = c->next)
+        n++;
+    if (n) {
+        s->cands = sort_cands(s->cands);
+        if (s->p.numharm > 1)
+            s->cands = eliminate_harmonics(s->cands, &n);
+    }
+    s->ncands = n;
+    return 0;
+}
+
+int accel_search_collect(accel_search *s)
+{
+    unsigned nh = 0;
+    GPU(agpu_d2h(&nh, s->d_nhits, sizeof(unsigned), s->stream));
+    if (nh > s->cap) {
+        /* too many crossings for the buffer: grow it and sum again */
+        GPU(agpu_free(s->d_hits));
+        s->cap = nh + nh / 4 + 1024;
+        GPU(agpu_malloc((void **) &s->d_hits, sizeof(agpu_hit) * s->cap));
+        agpu_harmsum_args a;
+        GPU(agpu_memset(s->d_nhits, 0, sizeof(unsigned), s->stream));
+        fill_harmsum_args(s, &a);
+        GPU(agpu_harmsum(&a, s->stream));
+        GPU(agpu_d2h(&nh, s->d_nhits, sizeof(unsigned), s->stream));
+        if (nh > s->cap) {
+            set_err("hit buffer overflow after regrowth (%u > %u)", nh, s->cap);
+            return -1;
+        }
+    }
+    for (int i = 0; i < 3; i++)
+        GPU(agpu_event_elapsed(&s->ms[i], s->ev[i], s->ev[i + 1]));
+    GPU(agpu_event_elapsed(&s->ms[3], s->ev[0], s->ev[3]));
+    free(s->hits);
+    s->nhits = nh;
+    s->hits = (accel_hit *) malloc(sizeof(accel_hit) * (nh ? nh : 1));
+    if (nh)
+        GPU(agpu_d2h(s->hits, s->d_hits, sizeof(agpu_hit) * nh, s->stream));
+    return replay(s);
+}
+
+int accel_search_run(accel_search *s)
+{
+    if (accel_search_launch(s))
+        return -1;
+    return accel_search_collect(s);
+}
+
+int accel_search_ncands(const accel_search *s) { return s->ncands; }
+
+int accel_search_cands(const accel_search *s, accel_cand *out, int n)
+{
+    int i = 0;
+    for (cnode *c = s->cands; c && i < n; c = c->next, i++)
+        out[i] = c->c;
+    return i;
+}
+
+long long accel_search_nhits(const accel_search *s) { return s->nhits; }
+
+int accel_search_hits(const accel_search *s, accel_hit *out, long long n)
+{
+    long long m = n < s->nhits ? n : s->nhits;
+    memcpy(out, s->hits, sizeof(accel_hit) * (size_t) m);
+    return (int) m;
+}
+
+int accel_search_plane_row(const accel_search *s, int wi, int zi, long long hb0, int n,
+                           float *out)
+{
+    if (wi < 0 || wi >= s->numw1 || zi < 0 || zi >= s->g.numz || hb0 < 0 || n < 0 ||
+        hb0 + n > s->stride) {
+        set_err("plane_row: out of range");
+        return -1;
+    }
+    const float *src = s->d_plane + ((size_t) wi * s->g.numz + zi) * s->stride + hb0;
+    GPU(agpu_d2h(out, src, sizeof(float) * (size_t) n, s->stream));
+    return 0;
+}
+
+int accel_search_nwindows(const accel_search *s) { return s->nwin; }
+
+int accel_search_windows(const accel_search *s, long long *lobin, int *numdata, int n)
+{
+    if (n > s->nwin)
+        n = s->nwin;
+    agpu_window *w = (agpu_window *) malloc(sizeof(agpu_window) * (n > 0 ? n : 1));
+    if (agpu_d2h(w, s->d_win, sizeof(agpu_window) * (size_t) n, s->stream)) {
+        free(w);
+        set_err("windows: %s", agpu_last_error());
+        return -1;
+    }
+    for (int i = 0; i < n; i++) {
+        lobin[i] = w[i].lobin;
+        numdata[i] = w[i].numdata;
+    }
+    free(w);
+    return n;
+}
+
+int accel_search_norms(const accel_search *s, double *out, int n)
+{
+    if (n > s->nwin)
+        n = s->nwin;
+    GPU(agpu_d2h(out, s->d_norm, sizeof(double) * (size_t) n, s->stream));
+    return n;
+}
+
+int accel_search_timings(const accel_search *s, float *ms4)
+{
+    memcpy(ms4, s->ms, sizeof(s->ms));
+    return 0;
+}
+
+int accel_search_corr_shape(const accel_search *s, int *L, int *uselen, int *groups,
+                            long long *nseg)
+{
+    *L = s->L;
+    *uselen = s->uselen;
+    *groups = s->groups;
+    *nseg = s->nseg;
+    return 0;
+}
+
+long long accel_search_plane_cells(const accel_search *s)
+{
+    return (long long) s->nt * s->hb_end;
+}

@@ -1,0 +1,100 @@
+/*
+ * agpu.h -- the thin C ABI between the C host (accel_host.c) and the HIP
+ * kernels of accel_gpu.hip.  Plain pointers, sizes and opaque stream/event
+ * handles; no HIP types cross it.  All functions return 0 on success and a
+ * negative code on failure; agpu_last_error() describes the failure.
+ */
+#ifndef PRESTO_AGPU_H
+#define PRESTO_AGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One block-median normalisation window: bins [lobin, lobin+numdata) of
+ * the spectrum (absolute bin numbers), zero outside [lobin_file, numbins). */
+typedef struct {
+    long long lobin;
+    int numdata;
+    int pad;
+} agpu_window;
+
+/* One threshold crossing of the harmonic-sum kernel. */
+typedef struct {
+    int block;   /* fundamental block of the reference's block loop */
+    int stage;   /* log2(number of harmonics summed) */
+    int wi, zi;  /* w and z indices on the fundamental grid */
+    int ii;      /* half-bin index inside the block */
+    float pow;   /* summed normalised power */
+} agpu_hit;
+
+#define AGPU_MAX_FRACS 32
+#define AGPU_MAX_STAGES 6
+
+/* Harmonic-sum / threshold launch description. */
+typedef struct {
+    const float *plane;      /* raw plane [numw][numz][stride] */
+    long long stride;        /* floats per (w,z) row; column = absolute half-bin */
+    int numz, numw;          /* numw >= 1 */
+    long long rh0;           /* first fundamental half-bin (2*rlo) */
+    int corr_uselen;         /* half-bins per block */
+    int nblocks;
+    int nstages;
+    int inmem;               /* 1: in-memory index rules and norms */
+    float powcut[AGPU_MAX_STAGES];
+    int nfrac;               /* sub-harmonic fractions, stage order */
+    int frac_k[AGPU_MAX_FRACS], frac_log2h[AGPU_MAX_FRACS];
+    int stage_nfrac[AGPU_MAX_STAGES];  /* fractions summed through stage s */
+    const int *zmap;         /* [nfrac][numz]: plane z row of the sub-harmonic */
+    const int *wmap;         /* [nfrac][numw] */
+    const float *nsq;        /* standard: [nblocks][nfrac+1]; in-memory: [npre+nblocks] */
+    int npre;                /* in-memory pre-population blocks (start at half-bin 16) */
+    agpu_hit *hits;
+    unsigned int *nhits;
+    unsigned int cap;
+} agpu_harmsum_args;
+
+int agpu_init(int device);
+const char *agpu_last_error(void);
+int agpu_device_name(char *buf, int len);
+
+int agpu_malloc(void **p, size_t bytes);
+int agpu_free(void *p);
+int agpu_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int agpu_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int agpu_memset(void *p, int v, size_t bytes, void *stream);
+int agpu_stream_create(void **s);
+int agpu_stream_destroy(void *s);
+int agpu_stream_sync(void *s);
+int agpu_event_create(void **e);
+int agpu_event_destroy(void *e);
+int agpu_event_record(void *e, void *stream);
+int agpu_event_elapsed(float *ms, void *start, void *stop);
+
+/* FP64 z-response templates for one segment length L (power of two):
+ * T[t][f] = conj(FFT_L(placed kernel of z[t]))[f] / L, rounded to FP32.
+ * zs/numkern are host arrays of nt entries. */
+int agpu_gen_templates_z(const int *zs, const int *numkern, int nt, int L, void *d_T,
+                         void *stream);
+
+/* Exact block medians of |x|^2 (FP32 powers) and the reference's
+ * normalisation: norm = 1/sqrt(median/ln 2) (FP64), nsq = (float)(norm^2).
+ * maxn bounds every window's numdata (at most 15360). */
+int agpu_block_norms(const void *d_spec, long long numbins, long long lobin_file,
+                     const agpu_window *d_win, int nwin, int maxn, double *d_norm,
+                     float *d_nsq, void *stream);
+
+/* Fused overlap-save correlation: raw |corr|^2 plane rows for templates
+ * [0, nt), half-bins [0, hb_end), segment length L = 4096. */
+int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
+                   const void *d_T, int nt, int L, int hw, int uselen, long long hb_end,
+                   float *d_plane, long long stride, int groups, void *stream);
+
+int agpu_harmsum(const agpu_harmsum_args *a, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

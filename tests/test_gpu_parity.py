@@ -1,0 +1,253 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Bars (from the task statement for this path):
+  * plane powers within 1e-4 of the FP64 oracle, measured as
+    |gpu - fp64| / max(fp64, 1) (powers are normalised so that noise has
+    mean 1; the error of each case is printed);
+  * normalisation factors equal to the oracle's to FP64 rounding;
+  * hit lists equal except for cells within 1e-5 of a threshold;
+  * identical candidate lists: same numharm, r within 1e-3 bins, z within
+    0.05, sigma within 1e-3.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from presto_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+A = pytest.importorskip("presto_amd.accel")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _case(N, dt, seed, pulsars, **kw):
+    x = synth.make_series(N, dt, seed, pulsars)
+    fft = A.spectrum_from_series(x, dt)
+    p = A.make_params(len(fft), N, dt, **kw)
+    o = O.make_obs(len(fft), N, dt, **kw)
+    return fft, p, o
+
+
+def _run_gpu(fft, p):
+    s = A.Search(p, device=0)
+    s.set_spectrum(fft)
+    s.run()
+    return s
+
+
+def _compare_hits(gh, oh, powcut):
+    gk = {(h["block"], h["stage"], h["wi"], h["zi"], h["ii"]): float(h["pow"]) for h in gh}
+    ok = {(h["block"], h["stage"], h["wi"], h["zi"], h["ri"]): float(h["pow"]) for h in oh}
+    worst = 0.0
+    for k in set(gk) | set(ok):
+        if k in gk and k in ok:
+            worst = max(worst, abs(gk[k] - ok[k]) / ok[k])
+        else:
+            v = gk.get(k, ok.get(k))
+            cut = powcut[k[1]]
+            assert abs(v - cut) < 1e-5 * cut, ("hit only on one side", k, v, cut)
+    return worst
+
+
+def _compare_cands(gc, oc):
+    assert len(gc) == len(oc), (len(gc), len(oc))
+    for g, w in zip(gc, oc):
+        assert g["numharm"] == int(w[2])
+        assert abs(g["r"] - w[3]) < 1e-3
+        assert abs(g["z"] - w[4]) < 0.05
+        assert abs(g["w"] - w[5]) < 0.05
+        assert abs(g["sigma"] - w[1]) < 1e-3
+
+
+def _plane_error(s, o, fft, blocks, fracs):
+    """Max |gpu*norm^2 - fp64| / max(fp64, 1) over the cells the reference
+    reads, for the given blocks and harmonic fractions (numharm, harmnum)."""
+    banks = O.SubharmBanks(o, fp64=True)
+    rstep = o.corr_uselen // 2
+    lo, nd = s.windows()
+    norms = s.norms()
+    nfrac = (1 << (o.numharmstages - 1)) - 1
+    order = [(1, 1)] + [(1 << st, k) for st in range(1, o.numharmstages)
+                        for k in range(1, 1 << st, 2)]
+    worst = 0.0
+    for b in blocks:
+        startr = o.rlo + b * rstep
+        for (H, k) in fracs:
+            plane, meta = O.subharm_fderivs_vol(H, k, startr, startr + rstep - 0.5, banks, o,
+                                                fft, fp64=True)
+            if o.inmem:
+                assert (H, k) == (1, 1)
+                widx = s.geom.nblocks * 0 + (len(norms) - s.geom.nblocks) + b
+            else:
+                widx = b * (nfrac + 1) + order.index((H, k))
+            # window parity: the GPU normalised with the oracle's window
+            bn = O.block_norm(o, fft, int(lo[widx]), int(nd[widx]))
+            assert abs(norms[widx] - bn) <= 1e-15 * bn, (b, H, k, norms[widx], bn)
+            nuse = o.corr_uselen if H == 1 else int(np.ceil(o.corr_uselen * k / H)) + 1
+            nuse = min(nuse, meta["numrs"])
+            hb0 = 2 * meta["rlo"]
+            for zi in range(meta["numzs"]):
+                z = meta["zlo"] + 2 * zi
+                row = (z - int(o.zlo)) // 2
+                raw = s.plane_row(0, row, hb0, nuse).astype(np.float64)
+                gpu = raw * np.float32(norms[widx] ** 2)
+                ref = plane[0, zi, :nuse]
+                worst = max(worst, float(np.max(np.abs(gpu - ref) / np.maximum(ref, 1.0))))
+    banks.close()
+    return worst
+
+
+# ------------------------------------------------------------------ standard path
+@pytest.fixture(scope="module")
+def std_case():
+    _say("case: standard path, N=2^16 dt=64us zmax=200 numharm=16, 2 pulsars")
+    fft, p, o = _case(1 << 16, 64e-6, 21,
+                      [dict(f0=61.3, z=9.0, duty=0.05, amp=0.35),
+                       dict(f0=377.7, z=-7.0, duty=0.08, amp=0.3)],
+                      zmax=200, numharm=16, sigma=2.0, force_standard=True)
+    s = _run_gpu(fft, p)
+    oh, oc, nb = O.accelsearch(o, fft)
+    yield fft, p, o, s, oh, oc
+    s.close()
+
+
+def test_std_geometry_matches(std_case):
+    fft, p, o, s, oh, oc = std_case
+    g = s.geom
+    assert (g.nblocks, g.corr_uselen, g.numz, g.inmem) == (
+        (int(o.highestbin - o.rlo) - 1) // (o.corr_uselen // 2), o.corr_uselen, o.numz, 0)
+
+
+def test_std_plane_within_1e4_of_fp64(std_case):
+    fft, p, o, s, oh, oc = std_case
+    fr = [(1, 1), (2, 1), (4, 3), (8, 5), (16, 1), (16, 15)]
+    err = _plane_error(s, o, fft, [0, 7, s.geom.nblocks - 1], fr)
+    _say(f"standard path plane error vs FP64 oracle: {err:.3e}")
+    assert err < 1e-4
+
+
+def test_std_hits_match_oracle(std_case):
+    fft, p, o, s, oh, oc = std_case
+    gh = s.hits()
+    assert len(gh) > 0
+    worst = _compare_hits(gh, oh, list(o.powcut))
+    _say(f"standard path: {len(gh)} hits, worst power rel. diff {worst:.2e}")
+    assert worst < 1e-4
+
+
+def test_std_candidates_identical(std_case):
+    fft, p, o, s, oh, oc = std_case
+    gc = s.candidates()
+    _compare_cands(gc, oc)
+    assert len(gc) >= 2
+
+
+# ------------------------------------------------------------------ in-memory path
+@pytest.fixture(scope="module")
+def inmem_case():
+    _say("case: in-memory path, c1 geometry (N=2^20 dt=64us zmax=50 numharm=8), amp 0.12")
+    fft, p, o = _case(1 << 20, 64e-6, 1,
+                      [dict(f0=123.456, z=20.0, duty=0.05, amp=0.12),
+                       dict(f0=17.3, z=-3.0, duty=0.1, amp=0.08)],
+                      zmax=50, numharm=8, sigma=2.0)
+    assert o.inmem == 1
+    s = _run_gpu(fft, p)
+    oh, oc, nb = O.accelsearch(o, fft)
+    yield fft, p, o, s, oh, oc
+    s.close()
+
+
+def test_inmem_plane_within_1e4_of_fp64(inmem_case):
+    fft, p, o, s, oh, oc = inmem_case
+    err = _plane_error(s, o, fft, [0, 100, s.geom.nblocks - 1], [(1, 1)])
+    _say(f"in-memory path plane error vs FP64 oracle: {err:.3e}")
+    assert err < 1e-4
+
+
+def test_inmem_hits_and_candidates(inmem_case):
+    fft, p, o, s, oh, oc = inmem_case
+    worst = _compare_hits(s.hits(), oh, list(o.powcut))
+    _say(f"in-memory path: {len(oh)} oracle hits, worst power rel. diff {worst:.2e}")
+    assert worst < 1e-4
+    _compare_cands(s.candidates(), oc)
+    assert len(oc) >= 1
+
+
+def test_config1_workload_identical():
+    """The CPU-reference workload itself (2^20 samples, z=20 pulsar at
+    amplitude 0.05, zmax 50, numharm 8)."""
+    _say("case: config-1 workload, exact parameters")
+    x = synth.config1_series()
+    fft = A.spectrum_from_series(x, 64e-6)
+    p = A.make_params(len(fft), len(x), 64e-6, zmax=50, numharm=8, sigma=2.0)
+    o = O.make_obs(len(fft), len(x), 64e-6, zmax=50, numharm=8, sigma=2.0)
+    with A.Search(p) as s:
+        s.set_spectrum(fft)
+        s.run()
+        gh, gc = s.hits(), s.candidates()
+    oh, oc, _ = O.accelsearch(o, fft)
+    _compare_hits(gh, oh, list(o.powcut))
+    _compare_cands(gc, oc)
+    _say(f"config-1: {len(gh)} hits, {len(gc)} candidates, identical")
+
+
+@pytest.mark.parametrize("zmax,numharm,inmem", [(0, 1, False), (0, 8, True), (2, 2, False),
+                                               (30, 32, False), (100, 4, True)])
+def test_edge_configurations(zmax, numharm, inmem):
+    _say(f"case: zmax={zmax} numharm={numharm} inmem={inmem}")
+    fft, p, o = _case(1 << 15, 5e-4, 33, [dict(f0=77.7, z=0.0, duty=0.05, amp=0.4)],
+                      zmax=zmax, numharm=numharm, sigma=2.0, force_standard=not inmem)
+    assert o.inmem == int(inmem)
+    with _run_gpu(fft, p) as s:
+        gh, gc = s.hits(), s.candidates()
+    oh, oc, _ = O.accelsearch(o, fft)
+    _compare_hits(gh, oh, list(o.powcut))
+    _compare_cands(gc, oc)
+
+
+def test_repeat_runs_are_deterministic(std_case):
+    fft, p, o, s, oh, oc = std_case
+    c1 = s.candidates().copy()
+    h1 = np.sort(s.hits(), order=["block", "stage", "wi", "zi", "ii"])
+    s.run()
+    h2 = np.sort(s.hits(), order=["block", "stage", "wi", "zi", "ii"])
+    assert np.array_equal(h1, h2)
+    assert np.array_equal(c1, s.candidates())
+
+
+# ------------------------------------------------------------------ full size
+def test_config2_full_size_properties():
+    """2^23 samples, zmax 200, numharm 16 (standard path): the 3 injected
+    pulsars are found, and plane spot checks and the first blocks' hits
+    agree with the oracle."""
+    _say("case: config-2 full size")
+    x, ps = synth.config2_series()
+    dt = 64e-6
+    fft = A.spectrum_from_series(x, dt)
+    p = A.make_params(len(fft), len(x), dt, zmax=200, numharm=16, sigma=2.0)
+    o = O.make_obs(len(fft), len(x), dt, zmax=200, numharm=16, sigma=2.0)
+    assert o.inmem == 0
+    with A.Search(p) as s:
+        s.set_spectrum(fft)
+        s.run()
+        gc, gh = s.candidates(), s.hits()
+        T = len(x) * dt
+        _say("injected: " + ", ".join(f"f0={q['f0']:.2f} z={q['z']:.1f} amp={q['amp']:.3f}"
+                                       for q in ps))
+        _say("top candidates: " + ", ".join(f"r={c['r']:.2f} z={c['z']:.1f} h={c['numharm']} "
+                                             f"sig={c['sigma']:.1f}" for c in gc[:6]))
+        for q in ps:
+            r_exp = q["f0"] * T + q["z"] / 2
+            near = [c for c in gc for m in (1, 0.5, 2, 1 / 3, 3, 0.25, 4)
+                    if abs(c["r"] - m * r_exp) < 2.0]
+            assert near, ("pulsar not found", q)
+        err = _plane_error(s, o, fft, [0, 2500, s.geom.nblocks - 1], [(1, 1), (2, 1), (16, 9)])
+        _say(f"config-2 plane spot-check error vs FP64 oracle: {err:.3e}")
+        assert err < 1e-4
+        ohits, _, nb = O.accelsearch(o, fft, max_blocks=12)
+        sub = gh[gh["block"] < 12]
+        _compare_hits(sub, ohits, list(o.powcut))
