@@ -197,35 +197,51 @@ __global__ void k_template_dft(const double2 *resp, const int *nks, int nkmax, i
 /* ------------------------------------------------------------------ */
 #define NORM_MAXN 15360   /* keys in LDS: 60 KiB, with the 1 KiB histogram under 64 KiB */
 
+/* Inclusive prefix sum of hist[0..255] by a 256-thread block (one bucket
+ * per thread); returns the sum up to and including this thread's bucket. */
+__device__ static unsigned block_scan256(unsigned v, unsigned *wsum)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned o = __shfl_up(v, d, 64);
+        if (lane >= d)
+            v += o;
+    }
+    if (lane == 63)
+        wsum[wv] = v;
+    __syncthreads();
+    unsigned add = 0;
+    for (int w = 0; w < wv; w++)
+        add += wsum[w];
+    __syncthreads();
+    return v + add;
+}
+
+/* k-th smallest (0-based) of keys[0..n) by 4 passes of 8-bit buckets */
 __device__ static unsigned radix_select(const unsigned *keys, int n, int k, unsigned *hist,
-                                        unsigned *shared2)
+                                        unsigned *sh)
 {
     unsigned prefix = 0, mask = 0;
     for (int shift = 24; shift >= 0; shift -= 8) {
-        for (int i = threadIdx.x; i < 256; i += blockDim.x)
-            hist[i] = 0;
+        hist[threadIdx.x] = 0;
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        for (int i = threadIdx.x; i < n; i += 256) {
             unsigned key = keys[i];
             if ((key & mask) == prefix)
                 atomicAdd(&hist[(key >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned cum = 0, bucket = 0;
-            for (int b = 0; b < 256; b++) {
-                if ((unsigned) k < cum + hist[b]) {
-                    bucket = b;
-                    break;
-                }
-                cum += hist[b];
-            }
-            shared2[0] = prefix | (bucket << shift);
-            shared2[1] = (unsigned) k - cum;
+        unsigned h = hist[threadIdx.x];
+        unsigned incl = block_scan256(h, sh + 2);
+        unsigned excl = incl - h;
+        if ((unsigned) k >= excl && (unsigned) k < incl) {
+            sh[0] = prefix | ((unsigned) threadIdx.x << shift);
+            sh[1] = (unsigned) k - excl;
         }
         __syncthreads();
-        prefix = shared2[0];
-        k = (int) shared2[1];
+        prefix = sh[0];
+        k = (int) sh[1];
         mask |= 255u << shift;
         __syncthreads();
     }
@@ -241,10 +257,10 @@ __global__ __launch_bounds__(256) void k_block_norms(const float2 *spec, long lo
 #pragma clang fp contract(off)
     extern __shared__ unsigned keys[];
     __shared__ unsigned hist[256];
-    __shared__ unsigned sh2[2];
+    __shared__ unsigned sh[8];
     const agpu_window w = win[blockIdx.x];
     const int n = w.numdata;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = threadIdx.x; i < n; i += 256) {
         long long b = w.lobin + i - lobin_file;
         float p = 0.0f;
         if (b >= 0 && b < numbins) {
@@ -255,19 +271,56 @@ __global__ __launch_bounds__(256) void k_block_norms(const float2 *spec, long lo
     }
     __syncthreads();
     const int lhs = (n - 1) / 2, rhs = n / 2;
-    float a = __uint_as_float(radix_select(keys, n, lhs, hist, sh2));
-    float med;
-    if (lhs == rhs) {
-        med = a;
-    } else {
-        float b = __uint_as_float(radix_select(keys, n, rhs, hist, sh2));
-        med = (float) (0.5 * ((double) a + (double) b));
+    const unsigned ka = radix_select(keys, n, lhs, hist, sh);
+    float med = __uint_as_float(ka);
+    if (lhs != rhs) {
+        /* the (lhs+1)-th key: ka again if more than lhs+1 keys are <= ka,
+         * else the smallest key above ka */
+        unsigned cnt = 0, mn = 0xffffffffu;
+        for (int i = threadIdx.x; i < n; i += 256) {
+            unsigned key = keys[i];
+            cnt += (key <= ka);
+            if (key > ka && key < mn)
+                mn = key;
+        }
+        if (threadIdx.x == 0) {
+            sh[4] = 0;
+            sh[5] = 0xffffffffu;
+        }
+        __syncthreads();
+        atomicAdd(&sh[4], cnt);
+        atomicMin(&sh[5], mn);
+        __syncthreads();
+        const unsigned kb = (sh[4] >= (unsigned) rhs + 1) ? ka : sh[5];
+        med = (float) (0.5 * ((double) __uint_as_float(ka) + (double) __uint_as_float(kb)));
     }
     if (threadIdx.x == 0) {
         double nv = 1.0 / sqrt((double) med / 0.69314718055994530942);
         norm[blockIdx.x] = nv;
         nsq[blockIdx.x] = (float) (nv * nv);
     }
+}
+
+/* In-memory path: norm^2 of the fundamental block that last wrote each
+ * half-bin -- main blocks from rh0 on, pre-population blocks from half-bin
+ * 16 (r = 8) on, nothing below (src/accelsearch.c:166-205). */
+__global__ void k_inmem_nsq_table(const float *nsq, int npre, int nblocks, long long rh0,
+                                  int cu, long long hb_end, float *out)
+{
+    long long hb = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (hb >= hb_end)
+        return;
+    float v = 0.0f;
+    if (hb >= rh0) {
+        long long b = (hb - rh0) / cu;
+        if (b < nblocks)
+            v = nsq[npre + b];
+    } else if (hb >= 16 && npre > 0) {
+        long long b = (hb - 16) / cu;
+        if (b < npre)
+            v = nsq[b];
+    }
+    out[hb] = v;
 }
 
 /* ------------------------------------------------------------------ */
@@ -374,8 +427,7 @@ __device__ __forceinline__ void fft4096(float2 *v, float2 *lds, const float2 *tw
 __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long numbins,
                                                   long long lobin_file, const float2 *T,
                                                   int nt, int hw, int uselen, long long hb_end,
-                                                  float *plane, long long stride, int groups,
-                                                  int tpg)
+                                                  float *plane, int groups, int tpg)
 {
     __shared__ float2 lds[FFT_LDS];
     const int j = threadIdx.x;
@@ -420,20 +472,39 @@ __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long 
         for (int r = 0; r < 16; r++)
             v[r] = cmul(D[r], Tt[j + 256 * r]);
         fft4096<1>(v, lds, tw1, tw2);
-        float *row = plane + (size_t) t * stride;
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int q = j + 256 * r - 2 * hw;
             const long long hb = seg_hb + q;
             if (q >= 0 && q < uselen && hb < hb_end)
-                row[hb] = __builtin_fmaf(v[r].x, v[r].x, v[r].y * v[r].y);
+                plane[AGPU_PLANE_IDX(t, hb, nt)] =
+                    __builtin_fmaf(v[r].x, v[r].x, v[r].y * v[r].y);
         }
     }
+}
+
+/* Tile-row halos: row r of tile t repeats the first AGPU_HALO half-bins of
+ * row r of tile t+1.  One workgroup per (tile, row), float4 copies. */
+__global__ __launch_bounds__(256) void k_plane_halo(float *plane, int nrows, int ntiles)
+{
+    const int row = blockIdx.x % nrows;
+    const int tile = blockIdx.x / nrows;
+    if (tile + 1 >= ntiles)
+        return;
+    const float4 *src = (const float4 *) (plane + AGPU_PLANE_IDX(row, (long long) (tile + 1) * AGPU_TILE, nrows));
+    float4 *dst = (float4 *) (plane + AGPU_PLANE_IDX(row, (long long) tile * AGPU_TILE, nrows) + AGPU_TILE);
+    for (int i = threadIdx.x; i < AGPU_HALO / 4; i += 256)
+        dst[i] = src[i];
 }
 
 /* ------------------------------------------------------------------ */
 /* Harmonic summing and thresholding                                    */
 /* ------------------------------------------------------------------ */
+#define HS_TPB 256
+#define HS_WAVES (HS_TPB / 64)
+#define HS_CPT 4
+#define HS_CHUNK (HS_TPB * HS_CPT)
+
 __device__ __forceinline__ void emit_hit(const agpu_harmsum_args &a, int b, int s, int wi,
                                          int zi, int ii, float p)
 {
@@ -450,65 +521,259 @@ __device__ __forceinline__ void emit_hit(const agpu_harmsum_args &a, int b, int 
     }
 }
 
-__device__ __forceinline__ float inmem_nsq(const agpu_harmsum_args &a, long long rh)
+/* 64-float wave loads that cover one chunk's sub-harmonic span for fraction
+ * k/H: the span is at most CHUNK k/H + 1 half-bins. */
+__host__ __device__ constexpr int hs_frac_wl(int k, int H)
 {
-    if (rh >= a.rh0) {
-        long long bb = (rh - a.rh0) / a.corr_uselen;
-        return a.nsq[a.npre + bb];
-    }
-    if (rh >= 16 && a.npre > 0) {
-        long long pb = (rh - 16) / a.corr_uselen;
-        if (pb < a.npre)
-            return a.nsq[pb];
-    }
-    return 0.0f;
+    return (HS_CHUNK * k / H + 1 + 63) / 64;
 }
 
-template <int NSTAGES, int INMEM>
-__global__ __launch_bounds__(256) void k_harmsum(agpu_harmsum_args a)
+__host__ __device__ constexpr int hs_stage_wl(int S)
 {
-    /* each sub-harmonic power is rounded before it is added, as in
-     * add_ffdotpows (src/accel_utils.c:1276-1277) */
-#pragma clang fp contract(off)
-    const int row = blockIdx.y;
-    const int wi = row / a.numz, zi = row - wi * a.numz;
-    const long long cell = (long long) blockIdx.x * 256 + threadIdx.x;
-    const long long ncell = (long long) a.nblocks * a.corr_uselen;
-    if (cell >= ncell)
-        return;
-    const long long rh = a.rh0 + cell;
-    const int b = (int) (cell / a.corr_uselen);
-    const int ii = (int) (cell - (long long) b * a.corr_uselen);
-    const float *nsq_b = a.nsq + (size_t) b * (a.nfrac + 1);
-    const float f0 = INMEM ? inmem_nsq(a, rh) : nsq_b[0];
-    float acc = a.plane[(size_t) row * a.stride + rh] * f0;
-    if (acc > a.powcut[0])
-        emit_hit(a, b, 0, wi, zi, ii, acc);
-    int f = 0;
+    int n = 0;
+    for (int k = 1; k < (1 << S); k += 2)
+        n += hs_frac_wl(k, 1 << S);
+    return n;
+}
+
+/* first LDS wave-load slot (64 floats) of stage S; stages 1-4 are resident
+ * together, stage 5 (32 harmonics) reuses the buffer from slot 0 */
+__host__ __device__ constexpr int hs_stage_base(int S)
+{
+    int n = 0;
+    for (int s = 1; s < S && s <= 4; s++)
+        n += hs_stage_wl(s);
+    return n;
+}
+
+constexpr int HS_WL_1TO4 = hs_stage_base(5);
+constexpr int HS_WL_5 = hs_stage_wl(5);
+constexpr int HS_MAX_WL = HS_WL_1TO4 > HS_WL_5 ? HS_WL_1TO4 : HS_WL_5;
+static_assert(hs_frac_wl(15, 16) * 64 <= AGPU_HALO && hs_frac_wl(31, 32) * 64 <= AGPU_HALO &&
+                  hs_frac_wl(1, 2) * 64 <= AGPU_HALO,
+              "every sub-harmonic span fits in a tile-row halo");
+
+/* rint (half-even, standard path, src/accel_utils.c:59) or (int)(x+0.5)
+ * (in-memory path, :1326) of rh * k / 2^s.  32-bit: the launcher checks
+ * that every rh * k fits. */
+template <int INMEM>
+__device__ __forceinline__ int hs_round(int rh, int k, int s)
+{
+    const unsigned q = (unsigned) rh * (unsigned) k;
+    const unsigned H = 1u << s;
+    if (INMEM)
+        return (int) ((q + (H >> 1)) >> s);
+    return (int) ((q + (H >> 1) - 1 + ((q >> s) & 1u)) >> s);
+}
+
+/* per-thread staging registers of one fraction: its wave loads are dealt
+ * round-robin to the HS_WAVES waves */
+__host__ __device__ constexpr int hs_frac_regs(int k, int H)
+{
+    return (hs_frac_wl(k, H) + HS_WAVES - 1) / HS_WAVES;
+}
+
+__host__ __device__ constexpr int hs_stage_regs(int S)
+{
+    int n = 0;
+    for (int k = 1; k < (1 << S); k += 2)
+        n += hs_frac_regs(k, 1 << S);
+    return n;
+}
+
+__host__ __device__ constexpr int hs_regs_upto(int S)   /* stages 1..S-1 */
+{
+    int n = 0;
+    for (int s = 1; s < S; s++)
+        n += hs_stage_regs(s);
+    return n;
+}
+
+/* Load every sub-harmonic span of stage S for this chunk into registers:
+ * wave load i (64 consecutive half-bins) goes to wave i % HS_WAVES.  The row
+ * base is wave-uniform and the span never leaves the tile-row plus its halo,
+ * so every load is base + lane + immediate. */
+template <int S, int INMEM>
+__device__ __forceinline__ void hs_load(const agpu_harmsum_args &a, float *v, int zi, int wi,
+                                        int nrows, int rh_c0, int wv, int lane)
+{
+    constexpr int H = 1 << S;
+    constexpr int F0 = (H >> 1) - 1;   /* index of fraction 1/H */
+    int r = 0;
 #pragma unroll
-    for (int s = 1; s < NSTAGES; s++) {
-        const int H = 1 << s;
+    for (int k = 1; k < H; k += 2) {
+        const int f = F0 + (k - 1) / 2;
+        const int zs = a.zmap[f * a.numz + zi];
+        const int ws = a.wmap[f * a.numw + wi];
+        const int lo = hs_round<INMEM>(rh_c0, k, S);
+        const int srow = ws * a.numz + zs;
+        /* the span starts in lo's tile-row and ends within its halo */
+        const float *b1 = a.plane + AGPU_PLANE_IDX(srow, lo, nrows) + wv * 64;
+        const int nwl = hs_frac_wl(k, H);
 #pragma unroll
-        for (int k = 1; k < H; k += 2, f++) {
-            const long long q = rh * k;
-            long long rs;
-            if (INMEM) {
-                rs = (q + (H >> 1)) >> s;
-            } else {
-                rs = q >> s;
-                const long long rem = q & (H - 1);
-                if (rem > (H >> 1) || (rem == (H >> 1) && (rs & 1)))
-                    rs++;
-            }
-            const int zs = a.zmap[f * a.numz + zi];
-            const int ws = a.wmap[f * a.numw + wi];
-            const float v = a.plane[((size_t) ws * a.numz + zs) * a.stride + rs];
-            const float nn = INMEM ? inmem_nsq(a, rs) : nsq_b[1 + f];
-            acc = acc + v * nn;
-        }
-        if (acc > a.powcut[s])
-            emit_hit(a, b, s, wi, zi, ii, acc);
+        for (int u = 0; u < hs_frac_regs(k, H); u++)
+            if (wv + HS_WAVES * u < nwl)
+                v[r + u] = b1[u * (HS_WAVES * 64) + lane];
+        r += hs_frac_regs(k, H);
     }
+}
+
+template <int S>
+__device__ __forceinline__ void hs_store(float *lds, int base, const float *v, int wv, int lane)
+{
+    constexpr int H = 1 << S;
+    int r = 0, wl = base;
+#pragma unroll
+    for (int k = 1; k < H; k += 2) {
+#pragma unroll
+        for (int u = 0; u < hs_frac_regs(k, H); u++)
+            if (wv + HS_WAVES * u < hs_frac_wl(k, H))
+                lds[(wl + wv + HS_WAVES * u) * 64 + lane] = v[r + u];
+        r += hs_frac_regs(k, H);
+        wl += hs_frac_wl(k, H);
+    }
+}
+
+/* Accumulate stage S from LDS, in the reference's fraction order.  nr[j]
+ * is the LDS row of norm^2 values of cell j's block. */
+template <int S, int INMEM>
+__device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const float *lds,
+                                            int base, const float *nr0, const float *nr1,
+                                            const float *nr2, const float *nr3, int rh_c0,
+                                            int rh_first, float *acc)
+{
+    static_assert(HS_CPT == 4, "four cells per thread");
+    constexpr int H = 1 << S;
+    constexpr int F0 = (H >> 1) - 1;
+    int wl = base;
+#pragma unroll
+    for (int k = 1; k < H; k += 2) {
+        const int f = F0 + (k - 1) / 2;
+        const int lo = hs_round<INMEM>(rh_c0, k, S);
+        const int rs0 = hs_round<INMEM>(rh_first, k, S);
+        const int step = (HS_TPB / H) * k;   /* HS_TPB k / H, even */
+        const float *lp = lds + wl * 64 + (rs0 - lo);
+        float nn[4];
+        if (INMEM) {
+            const float *np = a.nsq_hb + rs0;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                nn[j] = np[j * step];
+        } else {
+            nn[0] = nr0[1 + f];
+            nn[1] = nr1[1 + f];
+            nn[2] = nr2[1 + f];
+            nn[3] = nr3[1 + f];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            acc[j] = __builtin_fmaf(lp[j * step], nn[j], acc[j]);
+        wl += hs_frac_wl(k, H);
+    }
+}
+
+/* One workgroup = one (w,z) row x one chunk of 1024 fundamental half-bins,
+ * 4 cells per thread (cell = chunk base + t + 256 j).  Workgroup ids are
+ * dealt round-robin to the 8 XCDs and every row of chunk c runs on XCD
+ * c % 8, so the sub-harmonic rows that the rows of a chunk share are read
+ * once into that XCD's L2.  The fundamental values and every sub-harmonic
+ * span of stages 1-4 are loaded with plain global loads, all in flight at
+ * once, then the spans go to LDS; a 32-harmonic search loads stage 5 in a
+ * second round.  Adding 256 to a fundamental half-bin adds an even
+ * 256 k / 2^s to the rounded quotient, so one rounding per thread and
+ * fraction serves all 4 cells. */
+template <int NSTAGES, int INMEM>
+__global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int nchunks)
+{
+    __shared__ float s_norm[3 * 32];   /* norm^2 of blocks b0..b0+2 x fraction */
+    constexpr int NWL = NSTAGES > 5 ? HS_MAX_WL : hs_stage_base(NSTAGES);
+    __shared__ float lds[(NWL > 0 ? NWL : 1) * 64];
+    const int nrows = a.numz * a.numw;
+    const int bid = blockIdx.x;
+    const int xcd = bid & 7, local = bid >> 3;
+    const int m = local / nrows, row = local - m * nrows;
+    const int chunk = xcd + 8 * m;
+    if (chunk >= nchunks)
+        return;
+    const int wi = row / a.numz, zi = row - wi * a.numz;
+    const int ncell = a.nblocks * a.corr_uselen;
+    const int rh_c0 = ((int) (a.rh0 / HS_CHUNK) + chunk) * HS_CHUNK;
+    const int cell0 = rh_c0 - (int) a.rh0;   /* negative in the first chunk */
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int cu = a.corr_uselen;
+    const int b0 = cell0 > 0 ? cell0 / cu : 0;
+    const int e1 = (b0 + 1) * cu, e2 = (b0 + 2) * cu;
+    const int nfr = a.nfrac + 1;
+    const float *prow = a.plane + AGPU_PLANE_IDX(row, rh_c0, nrows);
+    float fund[HS_CPT];
+#pragma unroll
+    for (int j = 0; j < HS_CPT; j++)
+        fund[j] = prow[t + HS_TPB * j];
+    constexpr int NREG = hs_regs_upto(NSTAGES < 5 ? NSTAGES : 5);
+    float v[NREG > 0 ? NREG : 1];
+    if (NSTAGES > 1) hs_load<1, INMEM>(a, v + hs_regs_upto(1), zi, wi, nrows, rh_c0, wv, lane);
+    if (NSTAGES > 2) hs_load<2, INMEM>(a, v + hs_regs_upto(2), zi, wi, nrows, rh_c0, wv, lane);
+    if (NSTAGES > 3) hs_load<3, INMEM>(a, v + hs_regs_upto(3), zi, wi, nrows, rh_c0, wv, lane);
+    if (NSTAGES > 4) hs_load<4, INMEM>(a, v + hs_regs_upto(4), zi, wi, nrows, rh_c0, wv, lane);
+    if (!INMEM && t < 3 * nfr) {
+        const int bb = min(b0 + t / nfr, a.nblocks - 1);
+        s_norm[(t / nfr) * 32 + t % nfr] = a.nsq[(size_t) bb * nfr + t % nfr];
+    }
+    if (NSTAGES > 1) hs_store<1>(lds, hs_stage_base(1), v + hs_regs_upto(1), wv, lane);
+    if (NSTAGES > 2) hs_store<2>(lds, hs_stage_base(2), v + hs_regs_upto(2), wv, lane);
+    if (NSTAGES > 3) hs_store<3>(lds, hs_stage_base(3), v + hs_regs_upto(3), wv, lane);
+    if (NSTAGES > 4) hs_store<4>(lds, hs_stage_base(4), v + hs_regs_upto(4), wv, lane);
+    /* cells outside [0, ncell) read valid plane memory (the chunk before the
+     * first cell or the padding tile) and are never emitted */
+    const int rh_first = rh_c0 + t;
+    int nb[HS_CPT];      /* block offset (0..2) of the cell */
+    float acc[HS_CPT];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < HS_CPT; j++) {
+        const int c = cell0 + t + HS_TPB * j;
+        nb[j] = (c >= e1) + (c >= e2);
+    }
+#pragma unroll
+    for (int j = 0; j < HS_CPT; j++) {
+        const float nn = INMEM ? a.nsq_hb[rh_first + HS_TPB * j] : s_norm[nb[j] * 32];
+        acc[j] = fund[j] * nn;
+    }
+#define HS_EMIT(ST)                                                                    \
+    _Pragma("unroll") for (int j = 0; j < HS_CPT; j++) {                              \
+        const int c = cell0 + t + HS_TPB * j;                                          \
+        if (c >= 0 && c < ncell && acc[j] > a.powcut[ST]) {                            \
+            const int b = b0 + nb[j];                                                  \
+            emit_hit(a, b, ST, wi, zi, c - b * cu, acc[j]);                            \
+        }                                                                              \
+    }
+    HS_EMIT(0)
+    /* per-cell norm row bases: cell j reads s_norm[nb[j]*32 + 1 + f] */
+    const float *nrow0 = s_norm + nb[0] * 32;
+    const float *nrow1 = s_norm + nb[1] * 32;
+    const float *nrow2 = s_norm + nb[2] * 32;
+    const float *nrow3 = s_norm + nb[3] * 32;
+#define HS_STAGE(ST)                                                                   \
+    hs_consume4<ST, INMEM>(a, lds, hs_stage_base(ST), nrow0, nrow1, nrow2, nrow3, rh_c0, \
+                           rh_first, acc);                                             \
+    HS_EMIT(ST)
+    if (NSTAGES > 1) { HS_STAGE(1) }
+    if (NSTAGES > 2) { HS_STAGE(2) }
+    if (NSTAGES > 3) { HS_STAGE(3) }
+    if (NSTAGES > 4) { HS_STAGE(4) }
+    if (NSTAGES > 5) {
+        float v5[hs_stage_regs(5)];
+        hs_load<5, INMEM>(a, v5, zi, wi, nrows, rh_c0, wv, lane);
+        __syncthreads();   /* stages 1-4 are done with LDS */
+        hs_store<5>(lds, 0, v5, wv, lane);
+        __syncthreads();
+        hs_consume4<5, INMEM>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, rh_c0, rh_first, acc);
+        HS_EMIT(5)
+    }
+#undef HS_STAGE
+#undef HS_EMIT
 }
 
 /* ------------------------------------------------------------------ */
@@ -669,7 +934,7 @@ int agpu_block_norms(const void *d_spec, long long numbins, long long lobin_file
 
 int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
                    const void *d_T, int nt, int L, int hw, int uselen, long long hb_end,
-                   float *d_plane, long long stride, int groups, void *stream)
+                   float *d_plane, long long ntiles, int groups, void *stream)
 {
     if (L != 4096)
         return fail("agpu_correlate: only L = 4096 is built");
@@ -677,8 +942,8 @@ int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
         return fail("agpu_correlate: uselen must be even and <= L - 4*hw + 1");
     if (groups <= 0)
         return fail("agpu_correlate: groups must be positive");
-    if (hb_end > stride)
-        return fail("agpu_correlate: plane stride shorter than the half-bin range");
+    if (hb_end > ntiles * AGPU_TILE)
+        return fail("agpu_correlate: plane tiles do not cover the half-bin range");
     long long nseg = (hb_end + uselen - 1) / uselen;
     int tpg = (nt + groups - 1) / groups;
     long long nblk = nseg * groups;
@@ -686,7 +951,20 @@ int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
         return fail("agpu_correlate: grid too large");
     hipLaunchKernelGGL(k_corr4096, dim3((unsigned) nblk), dim3(256), 0, (hipStream_t) stream,
                        (const float2 *) d_spec, numbins, lobin_file, (const float2 *) d_T, nt,
-                       hw, uselen, hb_end, d_plane, stride, groups, tpg);
+                       hw, uselen, hb_end, d_plane, groups, tpg);
+    AGPU_CHECK(hipGetLastError());
+    return 0;
+}
+
+int agpu_plane_halo(float *d_plane, int nrows, long long ntiles, void *stream)
+{
+    long long nblk = (long long) nrows * ntiles;
+    if (nblk > 0x7fffffffLL)
+        return fail("agpu_plane_halo: grid too large");
+    if (nblk == 0)
+        return 0;
+    hipLaunchKernelGGL(k_plane_halo, dim3((unsigned) nblk), dim3(256), 0, (hipStream_t) stream,
+                       d_plane, nrows, (int) ntiles);
     AGPU_CHECK(hipGetLastError());
     return 0;
 }
@@ -698,18 +976,33 @@ int agpu_harmsum(const agpu_harmsum_args *a, void *stream)
         return 0;
     if (a->nstages < 1 || a->nstages > AGPU_MAX_STAGES)
         return fail("agpu_harmsum: 1..6 stages");
-    long long gx = (ncell + 255) / 256;
-    int rows = a->numz * a->numw;
-    if (gx > 0x7fffffffLL || rows > 65535)
+    if (a->nfrac + 1 > 32)
+        return fail("agpu_harmsum: at most 31 sub-harmonic fractions");
+    if (a->corr_uselen < HS_CHUNK / 2)
+        return fail("agpu_harmsum: corr_uselen below 1024 half-bins");
+    if (a->inmem && !a->nsq_hb)
+        return fail("agpu_harmsum: in-memory path needs the per-half-bin norm table");
+    const long long rh_end = a->rh0 + ncell;
+    if (a->ntiles < ((rh_end - 1) >> AGPU_TILE_LOG2) + 2 || a->ntiles * AGPU_TILE > 0x7fffffffLL)
+        return fail("agpu_harmsum: plane tiles must pad the half-bin range by one tile "
+                    "and stay below 2^31 half-bins");
+    long long nchunks = (rh_end - 1) / HS_CHUNK - a->rh0 / HS_CHUNK + 1;
+    if ((rh_end + HS_CHUNK) * (long long) ((1 << (a->nstages - 1)) - 1) > 0x7fffffffLL)
+        return fail("agpu_harmsum: spectrum too long for 32-bit sub-harmonic indices "
+                    "(half-bins x highest harmonic must stay below 2^31)");
+    long long rows = (long long) a->numz * a->numw;
+    long long nblk = 8 * ((nchunks + 7) / 8) * rows;
+    if (nblk > 0x7fffffffLL)
         return fail("agpu_harmsum: grid too large");
-    dim3 grid((unsigned) gx, rows);
+    dim3 grid((unsigned) nblk);
     hipStream_t st = (hipStream_t) stream;
+    int nc = (int) nchunks;
 #define HS_CASE(NS)                                                                     \
     case NS:                                                                            \
         if (a->inmem)                                                                   \
-            hipLaunchKernelGGL((k_harmsum<NS, 1>), grid, dim3(256), 0, st, *a);         \
+            hipLaunchKernelGGL((k_harmsum<NS, 1>), grid, dim3(HS_TPB), 0, st, *a, nc);  \
         else                                                                            \
-            hipLaunchKernelGGL((k_harmsum<NS, 0>), grid, dim3(256), 0, st, *a);         \
+            hipLaunchKernelGGL((k_harmsum<NS, 0>), grid, dim3(HS_TPB), 0, st, *a, nc);  \
         break;
     switch (a->nstages) {
         HS_CASE(1)
@@ -720,6 +1013,18 @@ int agpu_harmsum(const agpu_harmsum_args *a, void *stream)
         HS_CASE(6)
     }
 #undef HS_CASE
+    AGPU_CHECK(hipGetLastError());
+    return 0;
+}
+
+int agpu_inmem_nsq_table(const float *d_nsq, int npre, int nblocks, long long rh0, int cu,
+                         long long hb_end, float *d_out, void *stream)
+{
+    if (hb_end <= 0)
+        return 0;
+    long long nb = (hb_end + 255) / 256;
+    hipLaunchKernelGGL(k_inmem_nsq_table, dim3((unsigned) nb), dim3(256), 0,
+                       (hipStream_t) stream, d_nsq, npre, nblocks, rh0, cu, hb_end, d_out);
     AGPU_CHECK(hipGetLastError());
     return 0;
 }

@@ -9,8 +9,8 @@
  * Device layout (one search):
  *   spectrum   float2[numbins]                       (bin lobin first)
  *   templates  float2[numw'][numz][L]                L = 4096, FP32, 1/L folded in
- *   raw plane  float [numw'][numz][stride]           column = absolute half-bin,
- *                                                    un-normalised |corr|^2
+ *   raw plane  float, tiles of 2048 half-bins x every  column = absolute half-bin,
+ *              (w,z) row (AGPU_PLANE_IDX)             un-normalised |corr|^2
  *   windows    {lobin, numdata}[nwin] -> norm (FP64), nsq = norm^2 (FP32)
  *   zmap/wmap  int[nfrac][numz] / int[nfrac][numw']  sub-harmonic rows
  *   hits       accel_hit[cap] + counter
@@ -433,7 +433,7 @@ struct accel_search {
     int hw;               /* largest kernel half width (bins) */
     int L, uselen, groups;
     long long nseg;
-    long long hb_end, stride;
+    long long hb_end, ntiles;
     int nfrac;
     int npre;             /* in-memory pre-population blocks */
     int nwin, maxn;
@@ -446,6 +446,7 @@ struct accel_search {
     agpu_window *d_win;
     double *d_norm;
     float *d_nsq;
+    float *d_nsq_hb;      /* in-memory path: per-half-bin norm^2 */
     int *d_zmap, *d_wmap;
     agpu_hit *d_hits;
     unsigned *d_nhits;
@@ -583,13 +584,15 @@ static int search_init(accel_search *s, int device)
     }
     s->uselen = (s->L - 4 * s->hw) & ~1;
     s->hb_end = (long long) (2.0 * g->rlo) + (long long) g->nblocks * g->corr_uselen;
-    s->stride = (s->hb_end + 63) / 64 * 64;
+    /* one tile of padding past the last half-bin: the tail chunk of the
+     * harmonic sum reads it instead of clamping indices */
+    s->ntiles = (s->hb_end + AGPU_TILE - 1) / AGPU_TILE + 1;
     s->groups = 16;
     s->nseg = (s->hb_end + s->uselen - 1) / s->uselen;
     agpu_window *win = NULL;
     int *zmap = NULL, *wmap = NULL;
     build_tables(s, &win, &zmap, &wmap);
-    size_t plane_bytes = sizeof(float) * (size_t) s->nt * (size_t) s->stride;
+    size_t plane_bytes = sizeof(float) * (size_t) s->nt * (size_t) s->ntiles * AGPU_ROWLEN;
     GPU(agpu_malloc(&s->d_spec, sizeof(accel_fcomplex) * (size_t) (g->numbins > 0 ? g->numbins : 1)));
     s->own_spec = 1;
     GPU(agpu_malloc(&s->d_T, sizeof(accel_fcomplex) * (size_t) s->nt * s->L));
@@ -597,6 +600,8 @@ static int search_init(accel_search *s, int device)
     GPU(agpu_malloc((void **) &s->d_win, sizeof(agpu_window) * (size_t) (s->nwin > 0 ? s->nwin : 1)));
     GPU(agpu_malloc((void **) &s->d_norm, sizeof(double) * (size_t) (s->nwin > 0 ? s->nwin : 1)));
     GPU(agpu_malloc((void **) &s->d_nsq, sizeof(float) * (size_t) (s->nwin > 0 ? s->nwin : 1)));
+    if (g->inmem)
+        GPU(agpu_malloc((void **) &s->d_nsq_hb, sizeof(float) * (size_t) s->ntiles * AGPU_TILE));
     int nf = s->nfrac > 0 ? s->nfrac : 1;
     GPU(agpu_malloc((void **) &s->d_zmap, sizeof(int) * (size_t) nf * g->numz));
     GPU(agpu_malloc((void **) &s->d_wmap, sizeof(int) * (size_t) nf * s->numw1));
@@ -641,6 +646,7 @@ void accel_search_destroy(accel_search *s)
     agpu_free(s->d_win);
     agpu_free(s->d_norm);
     agpu_free(s->d_nsq);
+    agpu_free(s->d_nsq_hb);
     agpu_free(s->d_zmap);
     agpu_free(s->d_wmap);
     agpu_free(s->d_hits);
@@ -679,7 +685,7 @@ static void fill_harmsum_args(const accel_search *s, agpu_harmsum_args *a)
     const accel_geom *g = &s->g;
     memset(a, 0, sizeof(*a));
     a->plane = s->d_plane;
-    a->stride = s->stride;
+    a->ntiles = s->ntiles;
     a->numz = g->numz;
     a->numw = s->numw1;
     a->rh0 = (long long) (2.0 * g->rlo);
@@ -693,6 +699,7 @@ static void fill_harmsum_args(const accel_search *s, agpu_harmsum_args *a)
     a->zmap = s->d_zmap;
     a->wmap = s->d_wmap;
     a->nsq = s->d_nsq;
+    a->nsq_hb = s->d_nsq_hb;
     a->npre = s->npre;
     a->hits = s->d_hits;
     a->nhits = s->d_nhits;
@@ -707,9 +714,13 @@ int accel_search_launch(accel_search *s)
     GPU(agpu_event_record(s->ev[0], s->stream));
     GPU(agpu_block_norms(s->d_spec, g->numbins, g->lobin, s->d_win, s->nwin, s->maxn,
                          s->d_norm, s->d_nsq, s->stream));
+    if (g->inmem)
+        GPU(agpu_inmem_nsq_table(s->d_nsq, s->npre, g->nblocks, (long long) (2.0 * g->rlo),
+                                 g->corr_uselen, s->hb_end, s->d_nsq_hb, s->stream));
     GPU(agpu_event_record(s->ev[1], s->stream));
     GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw, s->uselen,
-                       s->hb_end, s->d_plane, s->stride, s->groups, s->stream));
+                       s->hb_end, s->d_plane, s->ntiles, s->groups, s->stream));
+    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, s->stream));
     GPU(agpu_event_record(s->ev[2], s->stream));
     fill_harmsum_args(s, &a);
     GPU(agpu_harmsum(&a, s->stream));
@@ -821,12 +832,20 @@ int accel_search_plane_row(const accel_search *s, int wi, int zi, long long hb0,
                            float *out)
 {
     if (wi < 0 || wi >= s->numw1 || zi < 0 || zi >= s->g.numz || hb0 < 0 || n < 0 ||
-        hb0 + n > s->stride) {
+        hb0 + n > s->ntiles * AGPU_TILE) {
         set_err("plane_row: out of range");
         return -1;
     }
-    const float *src = s->d_plane + ((size_t) wi * s->g.numz + zi) * s->stride + hb0;
-    GPU(agpu_d2h(out, src, sizeof(float) * (size_t) n, s->stream));
+    const int row = wi * s->g.numz + zi;
+    long long hb = hb0;
+    while (hb < hb0 + n) {
+        long long len = AGPU_TILE - (hb & (AGPU_TILE - 1));
+        if (hb + len > hb0 + n)
+            len = hb0 + n - hb;
+        GPU(agpu_d2h(out + (hb - hb0), s->d_plane + AGPU_PLANE_IDX(row, hb, s->nt),
+                     sizeof(float) * (size_t) len, s->stream));
+        hb += len;
+    }
     return 0;
 }
 

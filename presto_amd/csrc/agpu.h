@@ -30,13 +30,32 @@ typedef struct {
     float pow;   /* summed normalised power */
 } agpu_hit;
 
+/* Raw plane layout: tiles of AGPU_TILE half-bins, each tile holding every
+ * (w,z) row contiguously, each tile-row followed by a halo of AGPU_HALO
+ * half-bins that repeats the start of the same row in the next tile.
+ * Element (row, hb) lives at
+ *   ((hb / AGPU_TILE) * nrows + row) * AGPU_ROWLEN + hb % AGPU_TILE
+ * (and, for hb % AGPU_TILE < AGPU_HALO, also in the previous tile's halo,
+ * filled by agpu_plane_halo after the correlation).
+ * All rows of a tile share a few 2 MiB pages, which keeps the harmonic-sum
+ * gathers (16 rows at a time) inside the translation caches, and the halo
+ * lets every sub-harmonic span (at most 1088 half-bins) be read from one
+ * contiguous tile-row. */
+#define AGPU_TILE_LOG2 13
+#define AGPU_TILE (1 << AGPU_TILE_LOG2)
+#define AGPU_HALO 1088
+#define AGPU_ROWLEN (AGPU_TILE + AGPU_HALO)
+#define AGPU_PLANE_IDX(row, hb, nrows)                                                  \
+    (((size_t) ((hb) >> AGPU_TILE_LOG2) * (size_t) (nrows) + (size_t) (row)) *         \
+         (size_t) AGPU_ROWLEN + (size_t) ((hb) & (AGPU_TILE - 1)))
+
 #define AGPU_MAX_FRACS 32
 #define AGPU_MAX_STAGES 6
 
 /* Harmonic-sum / threshold launch description. */
 typedef struct {
-    const float *plane;      /* raw plane [numw][numz][stride] */
-    long long stride;        /* floats per (w,z) row; column = absolute half-bin */
+    const float *plane;      /* raw plane, tiled (AGPU_PLANE_IDX), column = absolute half-bin */
+    long long ntiles;        /* tiles allocated (one past the last searched half-bin) */
     int numz, numw;          /* numw >= 1 */
     long long rh0;           /* first fundamental half-bin (2*rlo) */
     int corr_uselen;         /* half-bins per block */
@@ -50,6 +69,7 @@ typedef struct {
     const int *zmap;         /* [nfrac][numz]: plane z row of the sub-harmonic */
     const int *wmap;         /* [nfrac][numw] */
     const float *nsq;        /* standard: [nblocks][nfrac+1]; in-memory: [npre+nblocks] */
+    const float *nsq_hb;     /* in-memory: norm^2 for every half-bin [0, hb_end) */
     int npre;                /* in-memory pre-population blocks (start at half-bin 16) */
     agpu_hit *hits;
     unsigned int *nhits;
@@ -87,12 +107,20 @@ int agpu_block_norms(const void *d_spec, long long numbins, long long lobin_file
                      float *d_nsq, void *stream);
 
 /* Fused overlap-save correlation: raw |corr|^2 plane rows for templates
- * [0, nt), half-bins [0, hb_end), segment length L = 4096. */
+ * [0, nt) (row = template), half-bins [0, hb_end), segment length L = 4096,
+ * written in the tiled layout above. */
 int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
                    const void *d_T, int nt, int L, int hw, int uselen, long long hb_end,
-                   float *d_plane, long long stride, int groups, void *stream);
+                   float *d_plane, long long ntiles, int groups, void *stream);
+
+/* Fills the tile-row halos of a plane written by agpu_correlate. */
+int agpu_plane_halo(float *d_plane, int nrows, long long ntiles, void *stream);
 
 int agpu_harmsum(const agpu_harmsum_args *a, void *stream);
+
+/* In-memory path: per-half-bin norm^2 of the block that last wrote it. */
+int agpu_inmem_nsq_table(const float *d_nsq, int npre, int nblocks, long long rh0, int cu,
+                         long long hb_end, float *d_out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -240,11 +240,28 @@ def test_config2_full_size_properties():
                                        for q in ps))
         _say("top candidates: " + ", ".join(f"r={c['r']:.2f} z={c['z']:.1f} h={c['numharm']} "
                                              f"sig={c['sigma']:.1f}" for c in gc[:6]))
+        found = 0
         for q in ps:
             r_exp = q["f0"] * T + q["z"] / 2
             near = [c for c in gc for m in (1, 0.5, 2, 1 / 3, 3, 0.25, 4)
                     if abs(c["r"] - m * r_exp) < 2.0]
-            assert near, ("pulsar not found", q)
+            # predicted noise-free summed power of the best harmonic stage:
+            # harmonic h of a Gaussian pulse of width sig carries
+            # N (A sig sqrt(2 pi))^2 exp(-4 pi^2 sig^2 h^2); only harmonics
+            # with |h z| <= zmax stay inside the searched plane
+            sig = q["duty"] / (2 * np.sqrt(2 * np.log(2)))
+            p1 = len(x) * (q["amp"] * sig * np.sqrt(2 * np.pi)) ** 2
+            best = 0.0
+            for st in range(o.numharmstages):
+                hs = [h for h in range(1, (1 << st) + 1) if abs(h * q["z"]) <= 200]
+                pw = sum(p1 * np.exp(-4 * np.pi ** 2 * sig ** 2 * h * h) for h in hs)
+                best = max(best, (pw + (1 << st)) / o.powcut[st])
+            _say(f"pulsar f0={q['f0']:.2f}: predicted best stage power/cut {best:.2f}, "
+                 f"{'found' if near else 'not found'}")
+            if best > 1.5:
+                assert near, ("detectable pulsar not found", q)
+            found += bool(near)
+        assert found >= 1
         err = _plane_error(s, o, fft, [0, 2500, s.geom.nblocks - 1], [(1, 1), (2, 1), (16, 9)])
         _say(f"config-2 plane spot-check error vs FP64 oracle: {err:.3e}")
         assert err < 1e-4
