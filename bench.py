@@ -11,10 +11,20 @@ Prints one JSON line.  value = ffdot cells/s, where cells follow the
 reference geometry (blocks x corr_uselen x numz x max(numw,1)), so CPU and
 GPU figures are comparable; DM-trials/s and p50/p99 step latency are
 reported beside it.
+
+Roofline: per-kernel durations come from HIP events recorded on the
+search's own stream around each launch.  Algorithmic bytes per fundamental
+cell (the reference geometry's cells): harmonic sum 4 (1 + sum_f (k/H)^2)
+bytes (the fundamental plus each sub-harmonic fraction's share of the
+plane), correlation 4 bytes per raw-plane cell written plus the spectrum
+and template reads.  `traffic` is the measured HBM bytes per launch from
+the committed rocprofv3 counter summary profiles/plain_2e23_counters.md
+(FETCH_SIZE x 2 + WRITE_SIZE, the scale of a plain copy kernel).
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -25,6 +35,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DM-trials/s and ffdot cells/s at zmax=200,numharm=16; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 78.6   # FP32 vector FMA without packed math (256 CU x 64 lanes x 2 x 2.4 GHz)
+COUNTERS = os.path.join(ROOT, "profiles", "plain_2e23_counters.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary."""
+    try:
+        txt = open(COUNTERS).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic\s+%s\s+read_bytes=([0-9.e+]+)\s+write_bytes=([0-9.e+]+)"
+                  % re.escape(kernel), txt, re.M)
+    return float(m.group(1)) + float(m.group(2)) if m else None
 
 
 def cpu_baseline(fft, N, dt, max_blocks):
@@ -37,8 +60,8 @@ def cpu_baseline(fft, N, dt, max_blocks):
     t = time.perf_counter() - t0
     cells = nb * o.corr_uselen * o.numz
     return dict(value=cells / t, unit="ffdot cells/s", cores=1, kind="port",
-                sample=f"first {nb} of the workload's blocks (standard path, all 16 "
-                       f"harmonic stages), {t:.1f} s on 1 thread")
+                sample=f"first {nb} of the workload's {o.numz}-template blocks "
+                       f"(standard path, all 16 harmonic stages), {t:.1f} s on 1 thread")
 
 
 def main():
@@ -46,7 +69,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-blocks", type=int, default=80,
+    ap.add_argument("--cpu-blocks", type=int, default=250,
                     help="blocks of the oracle CPU baseline sample (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -66,28 +89,38 @@ def main():
     s.set_spectrum(fft)
     for _ in range(args.warmup):
         s.run()
-    lat, corr_ms, hs_ms, norm_ms = [], [], [], []
+    lat, tim = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        s.run()
+        s.run()          # returns after the stream has drained and hits are replayed
         lat.append(time.perf_counter() - ts)
-        t = s.timings()
-        corr_ms.append(t["corr"])
-        hs_ms.append(t["harmsum"])
-        norm_ms.append(t["norms"])
+        tim.append(s.timings())
     wall = time.perf_counter() - t0
     ncand = len(s.candidates())
-
-    cells = g.nblocks * g.corr_uselen * g.numz * max(g.numw, 1)
-    ms_step = 1e3 * wall / args.steps
     shape = s.corr_shape()
     plane_cells = s.plane_cells()
-    nt = g.numz * max(g.numw, 1)
-    corr_bytes = 4.0 * plane_cells + 8.0 * g.numbins + 8.0 * nt * shape["L"]
-    corr_avg = float(np.mean(corr_ms))
-    achieved = corr_bytes / (corr_avg * 1e-3) / 1e9
     s.close()
+
+    def avg(k):
+        return float(np.mean([t[k] for t in tim]))
+
+    numw = max(g.numw, 1)
+    cells = g.nblocks * g.corr_uselen * g.numz * numw
+    ms_step = 1e3 * wall / args.steps
+    nfrac = [(k, 1 << st) for st in range(1, g.numharmstages) for k in range(1, 1 << st, 2)]
+    hs_bytes_per_cell = 4.0 * (1.0 + sum((k / h) ** 2 for k, h in nfrac))
+    hs_bytes = hs_bytes_per_cell * cells
+    hs_ms = avg("harmsum")
+    corr_ms = avg("corr")
+    nt = g.numz * numw
+    corr_bytes = 4.0 * plane_cells + 8.0 * g.numbins + 8.0 * nt * shape["L"]
+    L, U = shape["L"], shape["uselen"]
+    fft_flops = 5.0 * L * np.log2(L)
+    # per (segment, template): inverse FFT, spectrum x template, |.|^2 of the
+    # kept outputs; per (segment, template group): the forward FFT
+    corr_flops = shape["nseg"] * (nt * (fft_flops + 6.0 * L + 3.0 * U)
+                                  + shape["groups"] * fft_flops)
 
     line = {
         "metric": METRIC,
@@ -107,16 +140,25 @@ def main():
                    "numbins": int(g.numbins), "blocks": int(g.nblocks),
                    "corr_uselen": int(g.corr_uselen), "numz": int(g.numz),
                    "path": "in-memory" if g.inmem else "standard",
-                   "segment_len": shape["L"], "segment_uselen": shape["uselen"]},
+                   "segment_len": L, "segment_uselen": U},
         "dm_trials_per_s": 1e3 / ms_step,
         "p50_ms": 1e3 * float(np.percentile(lat, 50)),
         "p99_ms": 1e3 * float(np.percentile(lat, 99)),
-        "kernel_ms": {"norms": float(np.mean(norm_ms)), "correlation": corr_avg,
-                      "harmonic_sum": float(np.mean(hs_ms))},
+        "kernel_ms": {"block_norms": avg("norms"), "correlation": corr_ms,
+                      "plane_halo": avg("halo"), "harmonic_sum": hs_ms},
         "candidates": ncand,
-        "roofline": {"bound": "hbm", "kernel": "k_corr4096", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "k_harmsum<5,0>",
+                     "achieved": hs_bytes / (hs_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s",
+                     "frac": hs_bytes / (hs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_harmsum<5, 0>"),
+                     "bytes_per_cell": hs_bytes_per_cell},
+        "roofline_correlation": {"bound": "valu", "kernel": "k_corr4096",
+                                 "achieved_gbs": corr_bytes / (corr_ms * 1e-3) / 1e9,
+                                 "hbm_frac": corr_bytes / (corr_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                 "achieved_tflops": corr_flops / (corr_ms * 1e-3) / 1e12,
+                                 "valu_peak_tflops": VALU_PEAK_TFLOPS,
+                                 "traffic": measured_traffic("k_corr4096")},
     }
     if args.cpu_blocks > 0:
         line["cpu_baseline"] = cpu_baseline(fft, N, dt, args.cpu_blocks)

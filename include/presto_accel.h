@@ -123,8 +123,9 @@ int accel_search_plane_row(const accel_search *s, int wi, int zi, long long hb0,
                            float *out);
 int accel_search_nwindows(const accel_search *s);
 int accel_search_norms(const accel_search *s, double *out, int n);
-/* Milliseconds of the last run: norms, correlation, harmonic sum, total. */
-int accel_search_timings(const accel_search *s, float *ms4);
+/* Milliseconds of the last run (HIP events on the search's stream): block
+ * norms, correlation, plane halo fill, harmonic sum, total. */
+int accel_search_timings(const accel_search *s, float *ms5);
 /* Launch shape of the correlation kernel: segment length, used half-bins
  * per segment, template groups, segments. */
 int accel_search_corr_shape(const accel_search *s, int *L, int *uselen, int *groups,

@@ -219,10 +219,10 @@ class Search:
         return lo[:n], nd[:n]
 
     def timings(self):
-        ms = np.zeros(4, dtype=np.float32)
+        ms = np.zeros(5, dtype=np.float32)
         lib.accel_search_timings(self.h, ms.ctypes.data)
-        return dict(norms=float(ms[0]), corr=float(ms[1]), harmsum=float(ms[2]),
-                    total=float(ms[3]))
+        return dict(norms=float(ms[0]), corr=float(ms[1]), halo=float(ms[2]),
+                    harmsum=float(ms[3]), total=float(ms[4]))
 
     def corr_shape(self):
         L, u, g, ns = _i(), _i(), _i(), _ll()

@@ -438,7 +438,7 @@ struct accel_search {
     int npre;             /* in-memory pre-population blocks */
     int nwin, maxn;
     void *stream;
-    void *ev[4];
+    void *ev[5];
     void *d_spec;
     int own_spec;
     void *d_T;
@@ -456,7 +456,7 @@ struct accel_search {
     long long nhits;
     cnode *cands;
     int ncands;
-    float ms[4];
+    float ms[5];
 };
 
 const accel_geom *accel_search_geom(const accel_search *s) { return &s->g; }
@@ -572,7 +572,7 @@ static int search_init(accel_search *s, int device)
     accel_geom *g = &s->g;
     GPU(agpu_init(device));
     GPU(agpu_stream_create(&s->stream));
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 5; i++)
         GPU(agpu_event_create(&s->ev[i]));
     s->numw1 = g->numw ? g->numw : 1;
     s->nt = s->numw1 * g->numz;
@@ -651,7 +651,7 @@ void accel_search_destroy(accel_search *s)
     agpu_free(s->d_wmap);
     agpu_free(s->d_hits);
     agpu_free(s->d_nhits);
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 5; i++)
         if (s->ev[i])
             agpu_event_destroy(s->ev[i]);
     if (s->stream)
@@ -720,11 +720,12 @@ int accel_search_launch(accel_search *s)
     GPU(agpu_event_record(s->ev[1], s->stream));
     GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw, s->uselen,
                        s->hb_end, s->d_plane, s->ntiles, s->groups, s->stream));
-    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, s->stream));
     GPU(agpu_event_record(s->ev[2], s->stream));
+    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, s->stream));
+    GPU(agpu_event_record(s->ev[3], s->stream));
     fill_harmsum_args(s, &a);
     GPU(agpu_harmsum(&a, s->stream));
-    GPU(agpu_event_record(s->ev[3], s->stream));
+    GPU(agpu_event_record(s->ev[4], s->stream));
     return 0;
 }
 
@@ -791,9 +792,9 @@ int accel_search_collect(accel_search *s)
             return -1;
         }
     }
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 4; i++)
         GPU(agpu_event_elapsed(&s->ms[i], s->ev[i], s->ev[i + 1]));
-    GPU(agpu_event_elapsed(&s->ms[3], s->ev[0], s->ev[3]));
+    GPU(agpu_event_elapsed(&s->ms[4], s->ev[0], s->ev[4]));
     free(s->hits);
     s->nhits = nh;
     s->hits = (accel_hit *) malloc(sizeof(accel_hit) * (nh ? nh : 1));
@@ -877,9 +878,9 @@ int accel_search_norms(const accel_search *s, double *out, int n)
     return n;
 }
 
-int accel_search_timings(const accel_search *s, float *ms4)
+int accel_search_timings(const accel_search *s, float *ms5)
 {
-    memcpy(ms4, s->ms, sizeof(s->ms));
+    memcpy(ms5, s->ms, sizeof(s->ms));
     return 0;
 }
 

@@ -1,18 +1,23 @@
 """Profiling helpers.
 
-`python -m presto_amd.profiling copy NBYTES` runs a plain device-to-device
-copy of NBYTES (hipMemcpyAsync, HIP's own copy kernel) a few times.  Under
-`rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` it gives the counter reading
-for a known number of bytes read and written, the scale factor used to
-turn those counters into HBM bytes for the search kernels."""
-import ctypes as C
-import sys
+python -m presto_amd.profiling copy NBYTES
+    A plain device-to-device copy of NBYTES (HIP's own copy kernel), five
+    times.  Under `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` it gives
+    the counter reading for a known number of bytes read and written: the
+    scale factor that turns those counters into HBM bytes.
 
-from presto_amd import accel as A
+python -m presto_amd.profiling summarize DB [DB ...]
+    Per-kernel averages from rocprofv3 result databases (*_results.db):
+    dispatch count, mean duration (us) and the mean of every collected
+    counter per dispatch.
+"""
+import collections
+import ctypes as C
+import sqlite3
+import sys
 
 
 def copy(nbytes, reps=5):
-    lib = A.lib
     hip = C.CDLL("libamdhip64.so")
     src, dst = C.c_void_p(), C.c_void_p()
     if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or \
@@ -24,12 +29,53 @@ def copy(nbytes, reps=5):
     hip.hipDeviceSynchronize()
     hip.hipFree(src)
     hip.hipFree(dst)
-    del lib
     print(f"copied {nbytes} bytes x {reps}", flush=True)
 
 
-if __name__ == "__main__":
-    if len(sys.argv) >= 3 and sys.argv[1] == "copy":
-        copy(int(sys.argv[2]))
+def _short(name):
+    return name.split("(")[0].replace("void ", "")
+
+
+def summarize(db):
+    """{kernel: {"calls": n, "mean_us": t, counter: mean per dispatch}}"""
+    con = sqlite3.connect(db)
+    cur = con.cursor()
+    out = collections.defaultdict(dict)
+    durs = collections.defaultdict(list)
+    for name, dur in cur.execute("select name, duration from kernels"):
+        durs[_short(name)].append(dur)
+    for k, d in durs.items():
+        out[k]["calls"] = len(d)
+        out[k]["mean_us"] = sum(d) / len(d) / 1e3
+    tabs = {r[0] for r in cur.execute("select name from sqlite_master")}
+    if "counters_collection" in tabs:
+        cols = [r[1] for r in cur.execute("pragma table_info(counters_collection)")]
+        ki, ci = cols.index("kernel_name"), cols.index("counter_name")
+        vi, di = cols.index("value"), cols.index("dispatch_id")
+        per = collections.defaultdict(float)
+        for r in cur.execute("select * from counters_collection"):
+            per[(_short(r[ki]), r[ci], r[di])] += r[vi]
+        acc = collections.defaultdict(list)
+        for (k, c, _), v in per.items():
+            acc[(k, c)].append(v)
+        for (k, c), v in acc.items():
+            out[k][c] = sum(v) / len(v)
+    return out
+
+
+def main(argv):
+    if len(argv) >= 2 and argv[0] == "copy":
+        copy(int(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "summarize":
+        for db in argv[1:]:
+            print(f"# {db}")
+            for k, d in sorted(summarize(db).items(), key=lambda kv: -kv[1].get("mean_us", 0)
+                               * kv[1].get("calls", 0)):
+                fields = "  ".join(f"{c}={v:.6g}" for c, v in d.items())
+                print(f"{k:40s} {fields}")
     else:
         print(__doc__)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
