@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include "agpu.h"
 
@@ -163,6 +164,54 @@ __global__ void k_resp_z(const int *zs, const int *nks, int nkmax, double2 *resp
         }
     }
     resp[(size_t) t * nkmax + k] = out;
+}
+
+/* Jerk template (|w| >= 1e-4), src/responses.c:333-418: the reference
+ * samples a chirp of npts points into FP32, zero-pads it to 2 npts and
+ * real-FFTs it, keeping numkern bins around 2 fbar.  Here: the same FP32
+ * samples (phase in FP64, rounded like the reference's data[] array), then
+ * the FP64 DFT at exactly the kept bins.  One workgroup per template. */
+__global__ __launch_bounds__(256) void k_resp_w(const int *zs, const int *ws, const int *nks,
+                                                const int *npts_arr, const int *tlist,
+                                                int nkmax, double2 *resp)
+{
+    extern __shared__ float xs[];
+    const double TWOPI = 6.28318530717958647692;
+    const int t = tlist[blockIdx.x];
+    const double z = (double) zs[t], w = (double) ws[t];
+    const int nk = nks[t], npts = npts_arr[t];
+    const double dt = 1.0 / (double) npts;
+    const double amp = 2.0 * dt;
+    const int fbar = npts / 4;
+    const double f = (double) fbar - 0.5 * z + w / 12.0;
+    const double fd = (z - 0.5 * w) / 2.0;
+    const double fdd = w / 6.0;
+    for (int ii = threadIdx.x; ii < npts; ii += blockDim.x) {
+        const double tt = ii * dt;
+        const double phase = TWOPI * (tt * (tt * (tt * fdd + fd) + f));
+        xs[ii] = (float) (amp * cos(phase));
+    }
+    __syncthreads();
+    const long long ntot = 2LL * npts;
+    const long long first = 2LL * fbar - nk / 2;
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+        const long long m = first + k;
+        double ws_, wc_;
+        sincospi(-2.0 * (double) (m % ntot) / (double) ntot, &ws_, &wc_);
+        double ar = 0.0, ai = 0.0, cr = 1.0, ci = 0.0;
+        for (int i = 0; i < npts; i++) {
+            if ((i & 511) == 0 && i) {   /* re-seed the rotation exactly */
+                sincospi(-2.0 * (double) ((m * i) % ntot) / (double) ntot, &ci, &cr);
+            }
+            const double x = (double) xs[i];
+            ar += x * cr;
+            ai += x * ci;
+            const double nr = cr * wc_ - ci * ws_;
+            ci = cr * ws_ + ci * wc_;
+            cr = nr;
+        }
+        resp[(size_t) t * nkmax + k] = make_double2(ar, ai);
+    }
 }
 
 /* T[t][f] = conj(sum_m resp[nk/2+m] e^{-2 pi i f m / L}) / L in FP64 */
@@ -883,37 +932,76 @@ int agpu_event_elapsed(float *ms, void *start, void *stop)
     return 0;
 }
 
-int agpu_gen_templates_z(const int *zs, const int *numkern, int nt, int L, void *d_T,
-                         void *stream)
+int agpu_gen_templates(const int *zs, const int *ws, const int *numkern, int nt, int L,
+                       void *d_T, void *stream)
 {
     hipStream_t st = (hipStream_t) stream;
     if (nt <= 0 || L <= 0 || nt > 65535)
-        return fail("agpu_gen_templates_z: bad template count or length");
-    int nkmax = 0;
+        return fail("agpu_gen_templates: bad template count or length");
+    int nkmax = 0, nw = 0, nptsmax = 0;
+    int *npts = (int *) malloc(sizeof(int) * nt);
+    int *tlist = (int *) malloc(sizeof(int) * nt);
     for (int t = 0; t < nt; t++) {
-        if (numkern[t] <= 0 || numkern[t] % 4 || numkern[t] > L)
-            return fail("agpu_gen_templates_z: kernel width must be a positive multiple of 4 <= L");
+        if (numkern[t] <= 0 || numkern[t] % 4 || numkern[t] > L) {
+            free(npts);
+            free(tlist);
+            return fail("agpu_gen_templates: kernel width must be a positive multiple of 4 <= L");
+        }
         if (numkern[t] > nkmax)
             nkmax = numkern[t];
+        npts[t] = 0;
+        if (abs(ws[t]) > 0) {
+            /* src/responses.c:381-382: next2(6 hw + 200 + numkern/2) */
+            long long want = 6LL * (numkern[t] / 4) + 200 + numkern[t] / 2, p2 = 1;
+            while (p2 < want)
+                p2 <<= 1;
+            npts[t] = (int) p2;
+            if (p2 > nptsmax)
+                nptsmax = (int) p2;
+            tlist[nw++] = t;
+        }
     }
-    int *d_z = NULL, *d_nk = NULL;
+    if (nptsmax * (long long) sizeof(float) > 60 * 1024) {
+        free(npts);
+        free(tlist);
+        return fail("agpu_gen_templates: jerk chirp longer than 15360 samples");
+    }
+    int *d_z = NULL, *d_w = NULL, *d_nk = NULL, *d_np = NULL, *d_tl = NULL;
     double2 *d_resp = NULL;
     AGPU_CHECK(hipMalloc(&d_z, sizeof(int) * nt));
+    AGPU_CHECK(hipMalloc(&d_w, sizeof(int) * nt));
     AGPU_CHECK(hipMalloc(&d_nk, sizeof(int) * nt));
+    AGPU_CHECK(hipMalloc(&d_np, sizeof(int) * nt));
+    AGPU_CHECK(hipMalloc(&d_tl, sizeof(int) * (nw > 0 ? nw : 1)));
     AGPU_CHECK(hipMalloc(&d_resp, sizeof(double2) * (size_t) nt * nkmax));
     AGPU_CHECK(hipMemcpyAsync(d_z, zs, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    AGPU_CHECK(hipMemcpyAsync(d_w, ws, sizeof(int) * nt, hipMemcpyHostToDevice, st));
     AGPU_CHECK(hipMemcpyAsync(d_nk, numkern, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    AGPU_CHECK(hipMemcpyAsync(d_np, npts, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    if (nw)
+        AGPU_CHECK(hipMemcpyAsync(d_tl, tlist, sizeof(int) * nw, hipMemcpyHostToDevice, st));
+    /* z responses for every template; jerk templates then overwrite theirs */
     dim3 g1((nkmax + 255) / 256, nt);
     hipLaunchKernelGGL(k_resp_z, g1, dim3(256), 0, st, d_z, d_nk, nkmax, d_resp);
     AGPU_CHECK(hipGetLastError());
+    if (nw) {
+        hipLaunchKernelGGL(k_resp_w, dim3(nw), dim3(256), sizeof(float) * nptsmax, st, d_z, d_w,
+                           d_nk, d_np, d_tl, nkmax, d_resp);
+        AGPU_CHECK(hipGetLastError());
+    }
     dim3 g2((L + 255) / 256, nt);
     hipLaunchKernelGGL(k_template_dft, g2, dim3(256), 0, st, d_resp, d_nk, nkmax, L,
                        (float2 *) d_T);
     AGPU_CHECK(hipGetLastError());
     AGPU_CHECK(hipStreamSynchronize(st));
     AGPU_CHECK(hipFree(d_z));
+    AGPU_CHECK(hipFree(d_w));
     AGPU_CHECK(hipFree(d_nk));
+    AGPU_CHECK(hipFree(d_np));
+    AGPU_CHECK(hipFree(d_tl));
     AGPU_CHECK(hipFree(d_resp));
+    free(npts);
+    free(tlist);
     return 0;
 }
 

@@ -543,22 +543,25 @@ static int build_tables(accel_search *s, agpu_window **win_out, int **zmap_out,
     return 0;
 }
 
+/* Template bank of the fundamental: (w, z) row-major, z from -zmax and w
+ * from -wmax (src/accel_utils.c:223-237). */
 static int gen_templates(accel_search *s)
 {
     const accel_geom *g = &s->g;
-    if (g->numw) {
-        set_err("jerk (w) templates are not built on this path yet");
-        return -1;
-    }
     int *zs = (int *) malloc(sizeof(int) * s->nt);
+    int *ws = (int *) malloc(sizeof(int) * s->nt);
     int *nk = (int *) malloc(sizeof(int) * s->nt);
-    for (int zi = 0; zi < g->numz; zi++) {
-        int z = (int) g->zlo + zi * ACCEL_DZ;
-        zs[zi] = z;
-        nk[zi] = 2 * ACCEL_NUMBETWEEN * z_resp_halfwidth(z);
+    for (int wi = 0; wi < s->numw1; wi++) {
+        for (int zi = 0; zi < g->numz; zi++) {
+            int t = wi * g->numz + zi;
+            zs[t] = (int) g->zlo + zi * ACCEL_DZ;
+            ws[t] = g->numw ? (int) g->wlo + wi * ACCEL_DW : 0;
+            nk[t] = 2 * ACCEL_NUMBETWEEN * w_resp_halfwidth(zs[t], ws[t]);
+        }
     }
-    int rc = agpu_gen_templates_z(zs, nk, s->nt, s->L, s->d_T, s->stream);
+    int rc = agpu_gen_templates(zs, ws, nk, s->nt, s->L, s->d_T, s->stream);
     free(zs);
+    free(ws);
     free(nk);
     if (rc) {
         set_err("template generation: %s", agpu_last_error());

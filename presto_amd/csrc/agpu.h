@@ -93,11 +93,13 @@ int agpu_event_destroy(void *e);
 int agpu_event_record(void *e, void *stream);
 int agpu_event_elapsed(float *ms, void *start, void *stop);
 
-/* FP64 z-response templates for one segment length L (power of two):
- * T[t][f] = conj(FFT_L(placed kernel of z[t]))[f] / L, rounded to FP32.
- * zs/numkern are host arrays of nt entries. */
-int agpu_gen_templates_z(const int *zs, const int *numkern, int nt, int L, void *d_T,
-                         void *stream);
+/* FP64 response templates for one segment length L (power of two):
+ * T[t][f] = conj(FFT_L(placed kernel of (z[t], w[t])))[f] / L, rounded to
+ * FP32.  |w| = 0: Fresnel z-response; otherwise the FP64 DFT of the
+ * reference's FP32-sampled chirp.  zs/ws/numkern are host arrays of nt
+ * entries. */
+int agpu_gen_templates(const int *zs, const int *ws, const int *numkern, int nt, int L,
+                       void *d_T, void *stream);
 
 /* Exact block medians of |x|^2 (FP32 powers) and the reference's
  * normalisation: norm = 1/sqrt(median/ln 2) (FP64), nsq = (float)(norm^2).

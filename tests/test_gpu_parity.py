@@ -90,13 +90,16 @@ def _plane_error(s, o, fft, blocks, fracs):
             nuse = o.corr_uselen if H == 1 else int(np.ceil(o.corr_uselen * k / H)) + 1
             nuse = min(nuse, meta["numrs"])
             hb0 = 2 * meta["rlo"]
-            for zi in range(meta["numzs"]):
-                z = meta["zlo"] + 2 * zi
-                row = (z - int(o.zlo)) // 2
-                raw = s.plane_row(0, row, hb0, nuse).astype(np.float64)
-                gpu = raw * np.float32(norms[widx] ** 2)
-                ref = plane[0, zi, :nuse]
-                worst = max(worst, float(np.max(np.abs(gpu - ref) / np.maximum(ref, 1.0))))
+            for wi in range(meta["numws"]):
+                wrow = (meta["wlo"] + 20 * wi - int(o.wlo)) // 20 if o.numw else 0
+                for zi in range(meta["numzs"]):
+                    z = meta["zlo"] + 2 * zi
+                    row = (z - int(o.zlo)) // 2
+                    raw = s.plane_row(wrow, row, hb0, nuse).astype(np.float64)
+                    gpu = raw * np.float32(norms[widx] ** 2)
+                    ref = plane[wi, zi, :nuse]
+                    worst = max(worst,
+                                float(np.max(np.abs(gpu - ref) / np.maximum(ref, 1.0))))
     banks.close()
     return worst
 
@@ -207,6 +210,42 @@ def test_edge_configurations(zmax, numharm, inmem):
     oh, oc, _ = O.accelsearch(o, fft)
     _compare_hits(gh, oh, list(o.powcut))
     _compare_cands(gc, oc)
+
+
+# ------------------------------------------------------------------ jerk (w) search
+@pytest.fixture(scope="module")
+def jerk_case():
+    _say("case: jerk search, N=2^16 dt=64us zmax=40 wmax=60 numharm=4, pulsars w=+-40")
+    fft, p, o = _case(1 << 16, 64e-6, 41,
+                      [dict(f0=211.9, z=8.0, w=40.0, duty=0.06, amp=0.5),
+                       dict(f0=97.3, z=0.0, w=-40.0, duty=0.04, amp=0.5)],
+                      zmax=40, wmax=60, numharm=4, sigma=2.0, force_standard=True)
+    assert o.numw == 7
+    s = _run_gpu(fft, p)
+    oh, oc, nb = O.accelsearch(o, fft)
+    yield fft, p, o, s, oh, oc
+    s.close()
+
+
+def test_jerk_plane_within_1e4_of_fp64(jerk_case):
+    fft, p, o, s, oh, oc = jerk_case
+    assert s.geom.numw == o.numw
+    err = _plane_error(s, o, fft, [0, s.geom.nblocks // 2, s.geom.nblocks - 1],
+                       [(1, 1), (2, 1), (4, 1), (4, 3)])
+    _say(f"jerk plane error vs FP64 oracle: {err:.3e}")
+    assert err < 1e-4
+
+
+def test_jerk_hits_and_candidates(jerk_case):
+    fft, p, o, s, oh, oc = jerk_case
+    gh = s.hits()
+    assert len(gh) > 0 and gh["wi"].max() > 0
+    worst = _compare_hits(gh, oh, list(o.powcut))
+    _say(f"jerk: {len(gh)} hits, worst power rel. diff {worst:.2e}, {len(oc)} candidates")
+    assert worst < 1e-4
+    gc = s.candidates()
+    _compare_cands(gc, oc)
+    assert any(abs(c["w"]) > 0 for c in gc)
 
 
 def test_repeat_runs_are_deterministic(std_case):

@@ -261,6 +261,72 @@ def test_subharmonic_plane_is_normalised_direct_correlation(small_case, numharm,
     banks.close()
 
 
+def test_jerk_volume_is_normalised_direct_correlation():
+    """Every w-plane of a block is norm^2 times the direct correlation of
+    the spectrum with that (z, w) response (src/accel_utils.c:1046-1121)."""
+    N, dt = 8192, 1e-3
+    x = synth.make_series(N, dt, 12, [dict(f0=40.3, z=4.0, w=20.0, duty=0.1, amp=0.4)])
+    fft = O.prepare_dat(x, dt)
+    obs = O.make_obs(len(fft), N, dt, zmax=10, wmax=40, numharm=2, sigma=2.0,
+                     force_standard=True)
+    banks = O.SubharmBanks(obs, fp64=True)
+    startr = obs.rlo + 2 * (obs.corr_uselen // 2)
+    lastr = startr + obs.corr_uselen // 2 - 0.5
+    plane, meta = O.subharm_fderivs_vol(1, 1, startr, lastr, banks, obs, fft, fp64=True)
+    assert meta["numws"] == obs.numw == 5
+    hw = O.lib.o_w_resp_halfwidth(obs.zhi, obs.whi, 0)
+    norm = O.block_norm(obs, fft, meta["rlo"] - hw, obs.fftlen // 2)
+    rng = np.random.default_rng(2)
+    for _ in range(25):
+        wi, zi = int(rng.integers(meta["numws"])), int(rng.integers(meta["numzs"]))
+        ri = int(rng.integers(meta["numrs"]))
+        z, w = meta["zlo"] + 2 * zi, meta["wlo"] + 20 * wi
+        nk = 4 * O.lib.o_w_resp_halfwidth(z, w, 0)
+        resp = O.gen_w_response(0.0, 2, z, w, nk, fp64=True)
+        r_half = 2 * meta["rlo"] + ri
+        acc = 0j
+        for m in range(-nk // 2, nk // 2):
+            j = r_half + m
+            if j % 2 == 0 and 0 <= j // 2 < obs.numbins:
+                acc += complex(fft[j // 2]) * np.conj(resp[nk // 2 + m])
+        expect = norm ** 2 * abs(acc) ** 2
+        assert abs(plane[wi, zi, ri] - expect) <= 1e-9 * max(expect, 1.0), (wi, zi, ri)
+    banks.close()
+
+
+def test_jerk_plane_peak_is_matched_filter_power():
+    """Independent of the response code: at the injected (r, z, w) the plane
+    holds the matched-filter power of a numpy-built cubic-phase chirp over
+    the same bins, normalised the same way."""
+    N, dt = 1 << 14, 1e-3
+    r0, z, w = 700.0, 10.0, 40.0
+    u = np.arange(N) / N
+    phase = (r0 - z / 2 + w / 12) * u + (z - w / 2) / 2 * u * u + w / 6 * u ** 3
+    rng = np.random.Generator(np.random.PCG64(13))
+    x = (rng.standard_normal(N) + 0.35 * np.cos(2 * np.pi * phase)).astype(np.float32)
+    fft = O.prepare_dat(x, dt)
+    obs = O.make_obs(len(fft), N, dt, zmax=20, wmax=60, numharm=1, sigma=2.0,
+                     force_standard=True)
+    banks = O.SubharmBanks(obs, fp64=True)
+    rstep = obs.corr_uselen // 2
+    startr = obs.rlo + int((r0 - obs.rlo) // rstep) * rstep
+    plane, meta = O.subharm_fderivs_vol(1, 1, startr, startr + rstep - 0.5, banks, obs, fft,
+                                        fp64=True)
+    wi, zi = int((w - meta["wlo"]) // 20), int((z - meta["zlo"]) // 2)
+    ri = int(2 * (r0 - meta["rlo"]))
+    hw = O.lib.o_w_resp_halfwidth(obs.zhi, obs.whi, 0)
+    norm = O.block_norm(obs, fft, meta["rlo"] - hw, obs.fftlen // 2)
+    chirp = np.fft.fft(np.exp(2j * np.pi * phase)) / N
+    k = np.arange(int(r0) - 40, int(r0) + 41)
+    mf = norm ** 2 * abs(np.sum(fft[k].astype(np.complex128) * np.conj(chirp[k]))) ** 2
+    got = plane[wi, zi, ri]
+    assert got > 20.0
+    assert abs(got - mf) < 0.01 * mf, (got, mf)
+    # and it is the largest cell of the neighbourhood
+    assert got >= plane[:, :, ri - 4:ri + 5].max() * 0.999
+    banks.close()
+
+
 def test_fp32_plane_close_to_fp64(small_case):
     fft, obs = small_case
     b32, b64 = O.SubharmBanks(obs), O.SubharmBanks(obs, fp64=True)
