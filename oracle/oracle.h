@@ -165,6 +165,44 @@ long long o_prepare_dat(float *series, long long n, double dt);
 int o_result_ncands(const o_result *res);
 void o_result_cands(const o_result *res, double *out7);  /* power,sigma,numharm,r,z,w,0 */
 
+/* ---- refinement: o_refine.c ---- */
+typedef struct {
+    double pow, phs, dpow, dphs, d2pow, d2phs, locpow;
+} o_rderivs;   /* include/presto.h:160-168 */
+typedef struct {
+    double r;
+    float rerr;
+    double z;
+    float zerr;
+    double w;
+    float werr;
+    float pow, powerr, sig, rawpow, phs, phserr, cen, cenerr, pur, purerr, locpow;
+} o_fourierprops;   /* include/presto.h:170-188 */
+typedef struct {
+    double p, pd, pdd, f, fd, fdd;
+    float perr, pderr, pdderr, ferr, fderr, fdderr;
+} o_rzwerrs;   /* include/presto.h:233-246 */
+
+void o_rz_interp(const o_fcomplex *data, long numdata, double r, double z, int khw,
+                 float *ar, float *ai);
+void o_rzw_interp(const o_fcomplex *data, long numdata, double r, double z, double w,
+                  int khw, float *ar, float *ai);
+double o_localpower3d(const o_fcomplex *data, long numdata, double r, double z, double w);
+void o_derivs3d(const o_fcomplex *data, long numdata, double r, double z, double w,
+                double locpow, o_rderivs *res);
+int o_calc_props(const o_rderivs *d, double r, double z, double w, o_fourierprops *p);
+int o_calc_rzwerrs(const o_fourierprops *p, double T, o_rzwerrs *e);
+double o_nm_minimize(double (*f)(const double *x, void *par), void *par, double *x,
+                     double step, double tol, int maxiter, int *iters);
+double o_max_rz_arr(const o_fcomplex *data, long numdata, double rin, double zin,
+                    double *rout, double *zout, o_rderivs *derivs);
+long o_max_rz_arr_harmonics(const o_fcomplex *data, long numdata, int numharm, double rin,
+                            double zin, double *rout, double *zout, o_rderivs *derivs,
+                            double *powers);
+void o_max_rzw_arr_harmonics(const o_fcomplex *data, long numdata, int numharm, double rin,
+                             double zin, double win, double *rout, double *zout, double *wout,
+                             o_rderivs *derivs, double *powers);
+
 #ifdef __cplusplus
 }
 #endif

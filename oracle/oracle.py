@@ -311,3 +311,107 @@ class CandList:
     def free(self):
         lib.o_free_cands(self.head)
         self.head = C.POINTER(OCand)()
+
+
+# ---------------------------------------------------------------- refinement
+class ORDerivs(C.Structure):
+    _fields_ = [(n, _d) for n in ("pow", "phs", "dpow", "dphs", "d2pow", "d2phs", "locpow")]
+
+
+class OFourierProps(C.Structure):
+    _fields_ = [("r", _d), ("rerr", _f), ("z", _d), ("zerr", _f), ("w", _d), ("werr", _f)] + \
+               [(n, _f) for n in ("pow", "powerr", "sig", "rawpow", "phs", "phserr", "cen",
+                                  "cenerr", "pur", "purerr", "locpow")]
+
+
+class ORzwErrs(C.Structure):
+    _fields_ = [(n, _d) for n in ("p", "pd", "pdd", "f", "fd", "fdd")] + \
+               [(n, _f) for n in ("perr", "pderr", "pdderr", "ferr", "fderr", "fdderr")]
+
+
+_sig("o_rz_interp", None, _cf, C.c_long, _d, _d, _i, C.POINTER(_f), C.POINTER(_f))
+_sig("o_rzw_interp", None, _cf, C.c_long, _d, _d, _d, _i, C.POINTER(_f), C.POINTER(_f))
+_sig("o_localpower3d", _d, _cf, C.c_long, _d, _d, _d)
+_sig("o_derivs3d", None, _cf, C.c_long, _d, _d, _d, _d, C.POINTER(ORDerivs))
+_sig("o_calc_props", _i, C.POINTER(ORDerivs), _d, _d, _d, C.POINTER(OFourierProps))
+_sig("o_calc_rzwerrs", _i, C.POINTER(OFourierProps), _d, C.POINTER(ORzwErrs))
+_sig("o_max_rz_arr", _d, _cf, C.c_long, _d, _d, C.POINTER(_d), C.POINTER(_d),
+     C.POINTER(ORDerivs))
+_sig("o_max_rz_arr_harmonics", C.c_long, _cf, C.c_long, _i, _d, _d, C.POINTER(_d),
+     C.POINTER(_d), C.POINTER(ORDerivs), _dp)
+_sig("o_max_rzw_arr_harmonics", None, _cf, C.c_long, _i, _d, _d, _d, C.POINTER(_d),
+     C.POINTER(_d), C.POINTER(_d), C.POINTER(ORDerivs), _dp)
+
+
+def _c64(data):
+    return np.ascontiguousarray(data, dtype=np.complex64)
+
+
+def rz_interp(data, r, z, khw):
+    d = _c64(data)
+    a, b = _f(), _f()
+    lib.o_rz_interp(d, len(d), r, z, khw, C.byref(a), C.byref(b))
+    return complex(a.value, b.value)
+
+
+def rzw_interp(data, r, z, w, khw):
+    d = _c64(data)
+    a, b = _f(), _f()
+    lib.o_rzw_interp(d, len(d), r, z, w, khw, C.byref(a), C.byref(b))
+    return complex(a.value, b.value)
+
+
+def localpower3d(data, r, z, w=0.0):
+    d = _c64(data)
+    return lib.o_localpower3d(d, len(d), r, z, w)
+
+
+def derivs3d(data, r, z, w, locpow):
+    d = _c64(data)
+    out = ORDerivs()
+    lib.o_derivs3d(d, len(d), r, z, w, locpow, C.byref(out))
+    return out
+
+
+def calc_props(rd, r, z, w):
+    out = OFourierProps()
+    if lib.o_calc_props(C.byref(rd), r, z, w, C.byref(out)):
+        raise ValueError("calc_props: power or local power out of range")
+    return out
+
+
+def calc_rzwerrs(props, T):
+    out = ORzwErrs()
+    if lib.o_calc_rzwerrs(C.byref(props), T, C.byref(out)):
+        raise ValueError("calc_rzwerrs: T out of range")
+    return out
+
+
+def max_rz_arr(data, r, z):
+    """[maxpow, rmax, zmax, rderivs] as src/maximize_rz.c:103-140."""
+    d = _c64(data)
+    ro, zo, rd = _d(), _d(), ORDerivs()
+    p = lib.o_max_rz_arr(d, len(d), r, z, C.byref(ro), C.byref(zo), C.byref(rd))
+    return p, ro.value, zo.value, rd
+
+
+def max_rz_arr_harmonics(data, r, z, numharm):
+    """[rmax, zmax, [rderivs], powers, nevals] as src/maximize_rz.c:171-228."""
+    d = _c64(data)
+    ro, zo = _d(), _d()
+    rds = (ORDerivs * numharm)()
+    pw = np.zeros(numharm)
+    nev = lib.o_max_rz_arr_harmonics(d, len(d), numharm, r, z, C.byref(ro), C.byref(zo),
+                                     rds, pw)
+    return ro.value, zo.value, list(rds), pw, nev
+
+
+def max_rzw_arr_harmonics(data, r, z, w, numharm):
+    """[rmax, zmax, wmax, [rderivs], powers] as src/maximize_rzw.c:84-191."""
+    d = _c64(data)
+    ro, zo, wo = _d(), _d(), _d()
+    rds = (ORDerivs * numharm)()
+    pw = np.zeros(numharm)
+    lib.o_max_rzw_arr_harmonics(d, len(d), numharm, r, z, w, C.byref(ro), C.byref(zo),
+                                C.byref(wo), rds, pw)
+    return ro.value, zo.value, wo.value, list(rds), pw
