@@ -148,6 +148,70 @@ long accel_prepare_dat_spectrum(accel_fcomplex *fft, long numbins, double N, dou
 double accel_candidate_sigma(double power, int numsum, double numtrials);
 double accel_power_for_sigma(double sigma, int numsum, double numtrials);
 
+/* ---- refinement and output ---- */
+
+/* include/presto.h:160-168 */
+typedef struct {
+    double pow, phs, dpow, dphs, d2pow, d2phs, locpow;
+} accel_rderivs;
+
+/* include/presto.h:170-188: the record of the .cand file (88 bytes) */
+typedef struct {
+    double r;
+    float rerr;
+    double z;
+    float zerr;
+    double w;
+    float werr;
+    float pow, powerr, sig, rawpow, phs, phserr, cen, cenerr, pur, purerr, locpow;
+} accel_fourierprops;
+
+/* include/presto.h:233-246 */
+typedef struct {
+    double p, pd, pdd, f, fd, fdd;
+    float perr, pderr, pdderr, ferr, fderr, fdderr;
+} accel_rzwerrs;
+
+/* Highest harmonic count a refined candidate may carry. */
+#define ACCEL_MAXHARM 32
+
+/* optimize_accelcand (src/accel_utils.c:552-626) for every candidate of
+ * the last run, as src/accelsearch.c:250-257 calls it, with harmonic
+ * polishing (the default): max_rz_arr_harmonics (src/maximize_rz.c:171)
+ * in FP64 on the GPU.  Jerk searches (numw > 0) return an error. */
+int accel_search_optimize(accel_search *s);
+/* Refined harmonics of candidate i (after accel_search_optimize): the
+ * accelcand fields hirs, hizs, hiws, pows and derivs
+ * (include/accel.h:83-95).  Returns the harmonic count. */
+int accel_search_cand_harmonics(const accel_search *s, int i, double *hirs, double *hizs,
+                                double *hiws, double *pows, accel_rderivs *derivs, int nmax);
+/* The fundamentals' fourierprops written to the .cand file
+ * (src/accelsearch.c:262-275).  Returns the count written. */
+int accel_search_fund_props(const accel_search *s, accel_fourierprops *props, int n);
+/* The ACCEL text file (output_fundamentals + output_harmonics,
+ * src/accel_utils.c:684-979) and, if candnm is not NULL, the .cand binary
+ * (src/accelsearch.c:281-284).  inf_path: the .inf to append to the text
+ * file as the reference does (src/accel_utils.c:975-977), or NULL. */
+int accel_search_write_outputs(const accel_search *s, const char *accelnm, const char *candnm,
+                               const char *inf_path);
+
+/* max_rz_arr_harmonics (src/maximize_rz.c:171-228) for n (r, z) starts on
+ * a host spectrum of numdata bins (r relative to its first bin), on the
+ * GPU.  pows and derivs hold n x ACCEL_MAXHARM entries. */
+int accel_max_rz_harmonics(const accel_fcomplex *spec, long long numdata, int n,
+                           const int *numharm, const double *rin, const double *zin,
+                           double *rout, double *zout, double *pows, accel_rderivs *derivs,
+                           int device);
+
+/* src/characteristics.c:195-262 and :330-396 */
+int accel_calc_props(const accel_rderivs *d, double r, double z, double w,
+                     accel_fourierprops *p);
+int accel_calc_rzwerrs(const accel_fourierprops *p, double T, accel_rzwerrs *e);
+
+/* The value(error) text of src/output.c:6-113 (numerr 1) and :114-216
+ * (numerr 2); returns the string length or -1. */
+int accel_nice_output(char *out, double val, double err, int len, int numerr);
+
 const char *accel_last_error(void);
 
 #ifdef __cplusplus

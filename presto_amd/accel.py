@@ -88,6 +88,32 @@ _sig("accel_candidate_sigma", _d, _d, _i, _d)
 _sig("accel_power_for_sigma", _d, _d, _i, _d)
 _sig("accel_last_error", C.c_char_p)
 
+
+class RDerivs(C.Structure):
+    _fields_ = [(n, _d) for n in ("pow", "phs", "dpow", "dphs", "d2pow", "d2phs", "locpow")]
+
+
+class FourierProps(C.Structure):
+    _fields_ = [("r", _d), ("rerr", _f), ("z", _d), ("zerr", _f), ("w", _d), ("werr", _f)] + \
+               [(n, _f) for n in ("pow", "powerr", "sig", "rawpow", "phs", "phserr", "cen",
+                                  "cenerr", "pur", "purerr", "locpow")]
+
+
+class RzwErrs(C.Structure):
+    _fields_ = [(n, _d) for n in ("p", "pd", "pdd", "f", "fd", "fdd")] + \
+               [(n, _f) for n in ("perr", "pderr", "pdderr", "ferr", "fderr", "fdderr")]
+
+
+MAXHARM = 32
+_sig("accel_search_optimize", _i, _p)
+_sig("accel_search_cand_harmonics", _i, _p, _i, _p, _p, _p, _p, _p, _i)
+_sig("accel_search_fund_props", _i, _p, _p, _i)
+_sig("accel_search_write_outputs", _i, _p, C.c_char_p, C.c_char_p, C.c_char_p)
+_sig("accel_max_rz_harmonics", _i, _p, _ll, _i, _p, _p, _p, _p, _p, _p, _p, _i)
+_sig("accel_calc_props", _i, C.POINTER(RDerivs), _d, _d, _d, C.POINTER(FourierProps))
+_sig("accel_calc_rzwerrs", _i, C.POINTER(FourierProps), _d, C.POINTER(RzwErrs))
+_sig("accel_nice_output", _i, C.c_char_p, _d, _d, _i, _i)
+
 EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_destroy", "accel_search_geom", "accel_search_set_spectrum",
             "accel_search_set_device_spectrum", "accel_search_run", "accel_search_launch",
@@ -96,7 +122,10 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_nwindows", "accel_search_norms", "accel_search_timings",
             "accel_search_corr_shape", "accel_search_plane_cells",
             "accel_search_windows", "accel_deredden", "accel_prepare_dat_spectrum",
-            "accel_candidate_sigma", "accel_power_for_sigma", "accel_last_error"]
+            "accel_candidate_sigma", "accel_power_for_sigma", "accel_search_optimize",
+            "accel_search_cand_harmonics", "accel_search_fund_props",
+            "accel_search_write_outputs", "accel_max_rz_harmonics", "accel_calc_props",
+            "accel_calc_rzwerrs", "accel_nice_output", "accel_last_error"]
 
 
 class AccelError(RuntimeError):
@@ -231,6 +260,73 @@ class Search:
 
     def plane_cells(self):
         return lib.accel_search_plane_cells(self.h)
+
+    def optimize(self):
+        """optimize_accelcand for every candidate (FP64, on the GPU)."""
+        if lib.accel_search_optimize(self.h):
+            raise _err()
+
+    def harmonics(self, i):
+        """(hirs, hizs, hiws, pows, [RDerivs]) of refined candidate i."""
+        hr, hz, hw, pw = (np.zeros(MAXHARM) for _ in range(4))
+        der = (RDerivs * MAXHARM)()
+        n = lib.accel_search_cand_harmonics(self.h, i, hr.ctypes.data, hz.ctypes.data,
+                                            hw.ctypes.data, pw.ctypes.data, der, MAXHARM)
+        if n < 0:
+            raise _err()
+        return hr[:n], hz[:n], hw[:n], pw[:n], list(der)[:n]
+
+    def fund_props(self):
+        n = lib.accel_search_ncands(self.h)
+        out = (FourierProps * max(n, 1))()
+        if n and lib.accel_search_fund_props(self.h, out, n) != n:
+            raise _err()
+        return list(out)[:n]
+
+    def write_outputs(self, accelnm, candnm=None, inf_path=None):
+        enc = (lambda x: x.encode() if x is not None else None)
+        if lib.accel_search_write_outputs(self.h, enc(accelnm), enc(candnm), enc(inf_path)):
+            raise _err()
+
+
+def max_rz_harmonics(spec, r, z, numharm, device=0):
+    """max_rz_arr_harmonics for arrays of starts on the GPU.  Returns
+    rout, zout, powers [n, MAXHARM], derivs (list of lists of RDerivs)."""
+    spec = np.ascontiguousarray(spec, dtype=np.complex64)
+    r = np.ascontiguousarray(np.atleast_1d(r), dtype=np.float64)
+    z = np.ascontiguousarray(np.atleast_1d(z), dtype=np.float64)
+    nh = np.ascontiguousarray(np.broadcast_to(numharm, r.shape), dtype=np.int32)
+    n = len(r)
+    ro, zo = np.zeros(n), np.zeros(n)
+    pw = np.zeros((n, MAXHARM))
+    der = (RDerivs * (n * MAXHARM))()
+    if lib.accel_max_rz_harmonics(spec.ctypes.data, len(spec), n, nh.ctypes.data,
+                                  r.ctypes.data, z.ctypes.data, ro.ctypes.data, zo.ctypes.data,
+                                  pw.ctypes.data, der, int(device)):
+        raise _err()
+    dl = [list(der)[i * MAXHARM:i * MAXHARM + nh[i]] for i in range(n)]
+    return ro, zo, pw, dl
+
+
+def calc_props(rd, r, z, w):
+    out = FourierProps()
+    if lib.accel_calc_props(C.byref(rd), r, z, w, C.byref(out)):
+        raise AccelError("calc_props: power or local power out of range")
+    return out
+
+
+def calc_rzwerrs(props, T):
+    out = RzwErrs()
+    if lib.accel_calc_rzwerrs(C.byref(props), T, C.byref(out)):
+        raise AccelError("calc_rzwerrs: T out of range")
+    return out
+
+
+def nice_output(val, err, numerr, length=0):
+    buf = C.create_string_buffer(96)
+    if lib.accel_nice_output(buf, val, err, length, numerr) < 0:
+        raise AccelError("nice_output: undefined case")
+    return buf.value.decode()
 
 
 def spectrum_from_series(x, dt):

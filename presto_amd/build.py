@@ -16,8 +16,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-I", CSRC, "-I", os.path.join(ROOT, "include")]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall",
            "-I", CSRC, "-I", os.path.join(ROOT, "include")]
-HIP_SRC = ["accel_gpu.hip"]
-C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c"]
+HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip"]
+C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c", "accel_output.c"]
 
 
 def run(cmd):

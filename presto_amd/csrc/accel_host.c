@@ -457,6 +457,12 @@ struct accel_search {
     cnode *cands;
     int ncands;
     float ms[5];
+    /* refinement of the current list (accel_search_optimize) */
+    int nopt;
+    int *opt_nh;
+    double *opt_r, *opt_z, *opt_w;    /* fundamental optimum; r includes lobin */
+    double *opt_pows;                 /* [nopt][ACCEL_MAXHARM] */
+    accel_rderivs *opt_derivs;        /* [nopt][ACCEL_MAXHARM] */
 };
 
 const accel_geom *accel_search_geom(const accel_search *s) { return &s->g; }
@@ -661,6 +667,12 @@ void accel_search_destroy(accel_search *s)
         agpu_stream_destroy(s->stream);
     free(s->hits);
     free_cands(s->cands);
+    free(s->opt_nh);
+    free(s->opt_r);
+    free(s->opt_z);
+    free(s->opt_w);
+    free(s->opt_pows);
+    free(s->opt_derivs);
     free(s);
 }
 
@@ -743,6 +755,8 @@ static int cmp_hit(const void *pa, const void *pb)
     return 0;
 }
 
+static void clear_opt(accel_search *s);
+
 /* Replays the hits in the reference's order: block, then stage, then
  * w, z, r (src/accelsearch.c:189-227, src/accel_utils.c:1419-1450). */
 static int replay(accel_search *s)
@@ -750,6 +764,7 @@ static int replay(accel_search *s)
     const accel_geom *g = &s->g;
     const int rstep = g->corr_uselen / 2;
     qsort(s->hits, (size_t) s->nhits, sizeof(accel_hit), cmp_hit);
+    clear_opt(s);
     free_cands(s->cands);
     s->cands = NULL;
     for (long long i = 0; i < s->nhits; i++) {
@@ -804,6 +819,20 @@ int accel_search_collect(accel_search *s)
     if (nh)
         GPU(agpu_d2h(s->hits, s->d_hits, sizeof(agpu_hit) * nh, s->stream));
     return replay(s);
+}
+
+static void clear_opt(accel_search *s)
+{
+    free(s->opt_nh);
+    free(s->opt_r);
+    free(s->opt_z);
+    free(s->opt_w);
+    free(s->opt_pows);
+    free(s->opt_derivs);
+    s->opt_nh = NULL;
+    s->opt_r = s->opt_z = s->opt_w = s->opt_pows = NULL;
+    s->opt_derivs = NULL;
+    s->nopt = 0;
 }
 
 int accel_search_run(accel_search *s)
@@ -900,4 +929,185 @@ int accel_search_corr_shape(const accel_search *s, int *L, int *uselen, int *gro
 long long accel_search_plane_cells(const accel_search *s)
 {
     return (long long) s->nt * s->hb_end;
+}
+
+/* ---- refinement: optimize_accelcand, src/accel_utils.c:552-626 ---- */
+
+/* Runs max_rz_arr_harmonics for n starts on the device spectrum d_spec. */
+static int refine_rz_device(const void *d_spec, long long numdata, int n, const int *nh,
+                            const double *rin, const double *zin, double *rout, double *zout,
+                            double *pows, accel_rderivs *derivs, void *stream)
+{
+    int rc = -1;
+    int *d_nh = NULL, *d_nev = NULL;
+    double *d_rin = NULL, *d_zin = NULL, *d_rout = NULL, *d_zout = NULL;
+    double *d_der = NULL, *d_pow = NULL;
+    const size_t nd = sizeof(double) * (size_t) n;
+    const size_t nhx = (size_t) n * ACCEL_MAXHARM;
+    for (int i = 0; i < n; i++)
+        if (nh[i] < 1 || nh[i] > ACCEL_MAXHARM) {
+            set_err("refinement: numharm %d out of range 1..%d", nh[i], ACCEL_MAXHARM);
+            return -1;
+        }
+    if (agpu_malloc((void **) &d_nh, sizeof(int) * n) ||
+        agpu_malloc((void **) &d_nev, sizeof(int) * n) ||
+        agpu_malloc((void **) &d_rin, nd) || agpu_malloc((void **) &d_zin, nd) ||
+        agpu_malloc((void **) &d_rout, nd) || agpu_malloc((void **) &d_zout, nd) ||
+        agpu_malloc((void **) &d_der, sizeof(double) * 7 * nhx) ||
+        agpu_malloc((void **) &d_pow, sizeof(double) * nhx)) {
+        set_err("refinement: %s", agpu_last_error());
+        goto done;
+    }
+    if (agpu_h2d(d_nh, nh, sizeof(int) * n, stream) || agpu_h2d(d_rin, rin, nd, stream) ||
+        agpu_h2d(d_zin, zin, nd, stream) ||
+        agpu_memset(d_der, 0, sizeof(double) * 7 * nhx, stream) ||
+        agpu_memset(d_pow, 0, sizeof(double) * nhx, stream)) {
+        set_err("refinement: %s", agpu_last_error());
+        goto done;
+    }
+    agpu_refine_args a;
+    a.spec = d_spec;
+    a.numdata = numdata;
+    a.ncand = n;
+    a.numharm = d_nh;
+    a.rin = d_rin;
+    a.zin = d_zin;
+    a.rout = d_rout;
+    a.zout = d_zout;
+    a.derivs = d_der;
+    a.powers = d_pow;
+    a.nevals = d_nev;
+    if (agpu_refine_rz(&a, stream) || agpu_d2h(rout, d_rout, nd, stream) ||
+        agpu_d2h(zout, d_zout, nd, stream) ||
+        agpu_d2h(pows, d_pow, sizeof(double) * nhx, stream) ||
+        agpu_d2h(derivs, d_der, sizeof(double) * 7 * nhx, stream)) {
+        set_err("refinement: %s", agpu_last_error());
+        goto done;
+    }
+    rc = 0;
+done:
+    agpu_free(d_nh);
+    agpu_free(d_nev);
+    agpu_free(d_rin);
+    agpu_free(d_zin);
+    agpu_free(d_rout);
+    agpu_free(d_zout);
+    agpu_free(d_der);
+    agpu_free(d_pow);
+    return rc;
+}
+
+int accel_search_optimize(accel_search *s)
+{
+    const accel_geom *g = &s->g;
+    clear_opt(s);
+    const int n = s->ncands;
+    if (n == 0)
+        return 0;
+    if (g->numw) {
+        set_err("accel_search_optimize: jerk (w) refinement is not built");
+        return -1;
+    }
+    int *nh = (int *) malloc(sizeof(int) * n);
+    double *rin = (double *) malloc(sizeof(double) * n);
+    double *zin = (double *) malloc(sizeof(double) * n);
+    s->opt_nh = (int *) malloc(sizeof(int) * n);
+    s->opt_r = (double *) malloc(sizeof(double) * n);
+    s->opt_z = (double *) malloc(sizeof(double) * n);
+    s->opt_w = (double *) calloc(n, sizeof(double));
+    s->opt_pows = (double *) calloc((size_t) n * ACCEL_MAXHARM, sizeof(double));
+    s->opt_derivs = (accel_rderivs *) calloc((size_t) n * ACCEL_MAXHARM, sizeof(accel_rderivs));
+    int i = 0;
+    for (cnode *c = s->cands; c; c = c->next, i++) {
+        nh[i] = c->c.numharm;
+        rin[i] = c->c.r - (double) g->lobin;
+        zin[i] = c->c.z;
+    }
+    int rc = refine_rz_device(s->d_spec, g->numbins, n, nh, rin, zin, s->opt_r, s->opt_z,
+                              s->opt_pows, s->opt_derivs, s->stream);
+    if (rc == 0) {
+        for (i = 0; i < n; i++) {
+            s->opt_nh[i] = nh[i];
+            s->opt_r[i] += (double) g->lobin;
+        }
+        s->nopt = n;
+    } else {
+        clear_opt(s);
+    }
+    free(nh);
+    free(rin);
+    free(zin);
+    return rc;
+}
+
+int accel_search_cand_harmonics(const accel_search *s, int i, double *hirs, double *hizs,
+                                double *hiws, double *pows, accel_rderivs *derivs, int nmax)
+{
+    if (i < 0 || i >= s->nopt) {
+        set_err("accel_search_cand_harmonics: candidate %d not refined", i);
+        return -1;
+    }
+    int nh = s->opt_nh[i];
+    int m = nh < nmax ? nh : nmax;
+    for (int k = 0; k < m; k++) {
+        /* src/accel_utils.c:577-581 */
+        if (hirs)
+            hirs[k] = s->opt_r[i] * (k + 1);
+        if (hizs)
+            hizs[k] = s->opt_z[i] * (k + 1);
+        if (hiws)
+            hiws[k] = s->g.numw ? s->opt_w[i] * (k + 1) : 0.0;
+        if (pows)
+            pows[k] = s->opt_pows[(size_t) i * ACCEL_MAXHARM + k];
+        if (derivs)
+            derivs[k] = s->opt_derivs[(size_t) i * ACCEL_MAXHARM + k];
+    }
+    return nh;
+}
+
+int accel_search_fund_props(const accel_search *s, accel_fourierprops *props, int n)
+{
+    if (s->nopt != s->ncands) {
+        set_err("accel_search_fund_props: run accel_search_optimize first");
+        return -1;
+    }
+    int i = 0;
+    for (cnode *c = s->cands; c && i < n; c = c->next, i++) {
+        /* src/accelsearch.c:262-275: props at the search's r, z, w with the
+         * fundamental's derivatives, errors from the search grid */
+        if (accel_calc_props(&s->opt_derivs[(size_t) i * ACCEL_MAXHARM], c->c.r, c->c.z,
+                             c->c.w, &props[i]))
+            return -1;
+        props[i].rerr = (float) (ACCEL_DR) / c->c.numharm;
+        props[i].zerr = (float) (ACCEL_DZ) / c->c.numharm;
+        props[i].werr = (float) (ACCEL_DW) / c->c.numharm;
+    }
+    return i;
+}
+
+int accel_max_rz_harmonics(const accel_fcomplex *spec, long long numdata, int n,
+                           const int *numharm, const double *rin, const double *zin,
+                           double *rout, double *zout, double *pows, accel_rderivs *derivs,
+                           int device)
+{
+    void *stream = NULL, *d_spec = NULL;
+    int rc = -1;
+    if (n <= 0 || numdata <= 0) {
+        set_err("accel_max_rz_harmonics: empty input");
+        return -1;
+    }
+    if (agpu_init(device) || agpu_stream_create(&stream) ||
+        agpu_malloc(&d_spec, sizeof(accel_fcomplex) * (size_t) numdata) ||
+        agpu_h2d(d_spec, spec, sizeof(accel_fcomplex) * (size_t) numdata, stream)) {
+        set_err("accel_max_rz_harmonics: %s", agpu_last_error());
+        goto done;
+    }
+    rc = refine_rz_device(d_spec, numdata, n, numharm, rin, zin, rout, zout, pows, derivs,
+                          stream);
+done:
+    if (d_spec)
+        agpu_free(d_spec);
+    if (stream)
+        agpu_stream_destroy(stream);
+    return rc;
 }

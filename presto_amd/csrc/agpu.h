@@ -78,6 +78,7 @@ typedef struct {
 
 int agpu_init(int device);
 const char *agpu_last_error(void);
+void agpu_set_error(const char *msg);
 int agpu_device_name(char *buf, int len);
 
 int agpu_malloc(void **p, size_t bytes);
@@ -123,6 +124,26 @@ int agpu_harmsum(const agpu_harmsum_args *a, void *stream);
 /* In-memory path: per-half-bin norm^2 of the block that last wrote it. */
 int agpu_inmem_nsq_table(const float *d_nsq, int npre, int nblocks, long long rh0, int cu,
                          long long hb_end, float *d_out, void *stream);
+
+/* Candidate refinement (harmonic polishing), one workgroup per candidate.
+ * spec: the resident spectrum (numdata complex FP32 bins, bin 0 = lobin);
+ * rin = candidate r - lobin; per candidate up to AGPU_REFINE_MAXH
+ * harmonics; derivs[c][h][7] = pow, phs, dpow, dphs, d2pow, d2phs, locpow;
+ * powers[c][h]; all output arrays are device memory. */
+#define AGPU_REFINE_MAXH 32
+typedef struct {
+    const void *spec;
+    long long numdata;
+    int ncand;
+    const int *numharm;
+    const double *rin, *zin;
+    double *rout, *zout;
+    double *derivs;
+    double *powers;
+    int *nevals;
+} agpu_refine_args;
+
+int agpu_refine_rz(const agpu_refine_args *args, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,393 @@
+/*
+ * refine_gpu.hip -- FP64 candidate refinement on the GPU: the work of
+ * optimize_accelcand (src/accel_utils.c:552-626) for the default
+ * harmonic-polishing path, one workgroup per candidate.
+ *
+ *   k_refine_rz : max_rz_arr_harmonics (src/maximize_rz.c:171-228): local
+ *       powers, a two-pass Nelder-Mead simplex over (r, z/4) maximising the
+ *       locally normalised summed power of all harmonics, then per
+ *       harmonic the power and the 5-point derivatives (get_localpower3d
+ *       and get_derivs3d, src/characteristics.c:79-192).
+ *
+ * Every function value is a set of Fourier interpolations (rz_interp,
+ * src/rzinterp.c:144-237): the response taps are generated on the fly in
+ * FP64 (Fresnel integrals), rounded to FP32 as the reference stores them,
+ * multiplied by the FP32 spectrum and summed in FP64 over the workgroup
+ * with a fixed-order tree, so every thread holds the same value and the
+ * simplex runs in lockstep, deterministically.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include "agpu.h"
+#include "dev_fresnel.h"
+
+#define RF_TPB 256
+#define RF_MAXH AGPU_REFINE_MAXH
+
+#define RF_CHECK(call)                                                            \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            char m_[512];                                                         \
+            snprintf(m_, sizeof(m_), "%s:%d %s: %s", __FILE__, __LINE__, #call,   \
+                     hipGetErrorString(e_));                                      \
+            agpu_set_error(m_);                                                   \
+            return -1;                                                            \
+        }                                                                         \
+    } while (0)
+
+/* z_resp_halfwidth, src/responses.c:27-46 */
+__device__ static int zr_halfwidth(double z, int highacc)
+{
+    return (int) (0.5 * 1.1 * fabs(z)) + (highacc ? 48 : 16);
+}
+
+/* Tap m of gen_z_response(roffset, 1, z, numkern) (src/responses.c:228-316;
+ * |z| < 1e-4: gen_r_response, :159-225), rounded to FP32. */
+__device__ static float2 resp_tap(double roffset, double z, int numkern, int m)
+{
+    const double PI = 3.14159265358979323846;
+    double re, im;
+    if (fabs(z) < 1e-4) {
+        if (roffset < 1e-3 && m == numkern / 2) {
+            double t = roffset * roffset;
+            re = 1 - 6.579736267392905746 * t;
+            im = roffset * (PI - 10.335425560099940058 * t);
+        } else {
+            double x = numkern / 2 + roffset - m;   /* response at pi x */
+            double s, c;
+            sincospi(x, &s, &c);
+            double sinc = (x == 0.0) ? 1.0 : s / (PI * x);
+            re = c * sinc;
+            im = s * sinc;
+        }
+    } else {
+        double ip;
+        double startr = roffset - 0.5 * z;
+        double sro = (startr < 0) ? 1.0 + modf(startr, &ip) : modf(startr, &ip);
+        double absz = fabs(z);
+        if (sro < 1e-3 && absz < 1e-3 && m == numkern / 2) {
+            double zz = z * z, xx = sro * sro;
+            re = 1.0 - 0.16449340668482264365 * zz;
+            im = -0.5235987755982988731 * z;
+            re += sro * 1.6449340668482264365 * z;
+            im += sro * (PI - 0.5167712780049970029 * zz);
+            re += xx * (-6.579736267392905746 + 0.9277056288952613070 * zz);
+            im += xx * (3.1006276680299820175 * z);
+        } else {
+            int signz = (z < 0.0) ? -1 : 1;
+            double zd = signz * 1.41421356237309504880 / sqrt(absz);
+            double cons = zd / 2.0;
+            double r = startr + numkern / 2 - m;
+            double yy = r * zd, zz = yy + z * zd;
+            double s, c;
+            sincospi(r * r / z, &s, &c);
+            double fsy, fcy, fsz, fcz;
+            d_fresnl(yy, &fsy, &fcy);
+            d_fresnl(zz, &fsz, &fcz);
+            double tre = signz * (fcz - fcy), tim = fsy - fsz;
+            re = (tre * c - tim * s) * cons;
+            im = -(tre * s + tim * c) * cons;
+        }
+    }
+    return make_float2((float) re, (float) im);
+}
+
+/* Workgroup sum in a fixed order; every thread gets the result. */
+__device__ static double2 wg_sum(double2 v, double2 *red)
+{
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = RF_TPB / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            red[t].x += red[t + s].x;
+            red[t].y += red[t + s].y;
+        }
+        __syncthreads();
+    }
+    double2 out = red[0];
+    __syncthreads();
+    return out;
+}
+
+/* rz_interp (src/rzinterp.c:144-237) by one workgroup */
+__device__ static double2 rz_interp_wg(const float2 *spec, long long numdata, double r, double z,
+                                       int khw, double2 *red)
+{
+    if (r > numdata - 1.0 || r < 0.0)
+        return make_double2(0.0, 0.0);
+    double dint;
+    const double frac = modf(r, &dint);
+    const long long intfreq = (long long) dint;
+    if (fabs(z) < 1e-4) {
+        if (frac < 1e-5) {
+            float2 d = spec[intfreq];
+            return make_double2(d.x, d.y);
+        }
+        if ((1.0 - frac) < 1e-5) {
+            float2 d = spec[intfreq + 1];
+            return make_double2(d.x, d.y);
+        }
+    }
+    const int numkern = 2 * khw;
+    long long lodata = intfreq - khw, loresp, hiresp;
+    if (lodata < 0) {
+        loresp = -lodata;
+        lodata = 0;
+    } else {
+        loresp = 0;
+    }
+    const long long hidata = intfreq + khw - 1;
+    hiresp = (hidata > numdata - 1) ? numkern - hidata + numdata - 1 : numkern;
+    const long long nsum = hiresp - loresp;
+    double2 acc = make_double2(0.0, 0.0);
+    for (long long ii = threadIdx.x; ii < nsum; ii += RF_TPB) {
+        const float2 q = resp_tap(frac, z, numkern, (int) (loresp + ii));
+        const float2 d = spec[lodata + ii];
+        acc.x += (double) d.x * q.x + (double) d.y * q.y;
+        acc.y += (double) d.y * q.x - (double) q.y * d.x;
+    }
+    return wg_sum(acc, red);
+}
+
+/* get_localpower3d with w = 0 (src/characteristics.c:79-136) */
+__device__ static double localpower_wg(const float2 *spec, long long numdata, double r, double z,
+                                       double2 *red)
+{
+    const int binsperside = 10;   /* NUMLOCPOWAVG / 2 */
+    const int khw = zr_halfwidth(z, 0);
+    double lo1 = r - 5 - binsperside, hi1 = lo1 + binsperside;   /* DELTAAVGBINS = 5 */
+    double lo2 = r + 5, hi2 = lo2 + binsperside;
+    const double nd = (double) numdata;
+    lo1 = lo1 < 0.0 ? 0.0 : (lo1 > nd ? nd : lo1);
+    lo2 = lo2 < 0.0 ? 0.0 : (lo2 > nd ? nd : lo2);
+    hi1 = hi1 < 0.0 ? 0.0 : (hi1 > nd ? nd : hi1);
+    hi2 = hi2 < 0.0 ? 0.0 : (hi2 > nd ? nd : hi2);
+    double sum = 0.0;
+    for (double f = lo1; f < hi1; f += 1.0) {
+        double2 a = rz_interp_wg(spec, numdata, f, z, khw, red);
+        sum += a.x * a.x + a.y * a.y;
+    }
+    for (double f = lo2; f < hi2; f += 1.0) {
+        double2 a = rz_interp_wg(spec, numdata, f, z, khw, red);
+        sum += a.x * a.x + a.y * a.y;
+    }
+    return sum / 20.0;
+}
+
+/* get_derivs3d with w = 0 (src/characteristics.c:139-192) */
+__device__ static void derivs_wg(const float2 *spec, long long numdata, double r, double z,
+                                 double locpow, double *out7, double2 *red)
+{
+    const double PI = 3.14159265358979323846, TWOPI = 6.28318530717958647692;
+    const double h = 0.003, twoh = 0.006, twohsqrd = 0.000036;
+    const int khw = zr_halfwidth(z, 1);
+    double pwr[5], phs[5];
+    double f = r - twoh;
+#pragma unroll
+    for (int k = 0; k < 5; k++, f += h) {
+        double2 a = rz_interp_wg(spec, numdata, f, z, khw, red);
+        pwr[k] = a.x * a.x + a.y * a.y;
+        double t = atan2(a.y, a.x);
+        phs[k] = t > 0.0 ? t : t + TWOPI;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (fabs(phs[k + 1] - phs[k]) > PI)
+            phs[k + 1] -= TWOPI;
+    out7[0] = pwr[2];
+    out7[1] = phs[2];
+    out7[2] = (pwr[3] - pwr[1]) / twoh;
+    out7[3] = (phs[3] - phs[1]) / twoh;
+    out7[4] = (pwr[4] - 2.0 * pwr[2] + pwr[0]) / twohsqrd;
+    out7[5] = (phs[4] - 2.0 * phs[2] + phs[0]) / twohsqrd;
+    out7[6] = locpow;
+}
+
+struct rf_ctx {
+    const float2 *spec;
+    long long numdata;
+    const float *locpow;   /* LDS, numharm entries */
+    int numharm, kernhw;
+    double2 *red;
+    int nevals;
+};
+
+/* power_call_rz_harmonics_gsl (src/maximize_rz.c:43-63): minus the summed
+ * locally normalised power at (r, z = zs * 4) over all harmonics */
+__device__ static double rf_fun(rf_ctx &c, double r, double zs)
+{
+    c.nevals++;
+    double tot = 0.0;
+    for (int k = 0; k < c.numharm; k++) {
+        const int n = k + 1;
+        double2 a = rz_interp_wg(c.spec, c.numdata, r * n, zs * 4.0 * n, c.kernhw, c.red);
+        tot += (a.x * a.x + a.y * a.y) / c.locpow[k];
+    }
+    if (!isfinite(tot))
+        return 0.0;
+    return -tot;
+}
+
+/* Nelder-Mead over 2 parameters, the nmsimplex2 scheme: reflect / expand /
+ * contract the worst corner, else shrink towards the best; incremental
+ * centre and squared rms size; stop when the size drops below tol or after
+ * maxiter iterations (src/maximize_rz.c:65-100). */
+__device__ static void rf_simplex(rf_ctx &c, double *x, double step, double tol, int maxiter)
+{
+    const double P = 3.0;
+    double X[3][2], Y[3], C[2], S2;
+    X[0][0] = x[0];
+    X[0][1] = x[1];
+    Y[0] = rf_fun(c, x[0], x[1]);
+    for (int i = 0; i < 2; i++) {
+        double xt0 = x[0] + (i == 0 ? step : 0.0), xt1 = x[1] + (i == 1 ? step : 0.0);
+        X[i + 1][0] = xt0;
+        X[i + 1][1] = xt1;
+        Y[i + 1] = rf_fun(c, xt0, xt1);
+    }
+    auto center_size = [&]() {
+        for (int j = 0; j < 2; j++)
+            C[j] = (X[0][j] + X[1][j] + X[2][j]) / 3.0;
+        double ss = 0.0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 2; j++)
+                ss += (X[i][j] - C[j]) * (X[i][j] - C[j]);
+        S2 = ss / 3.0;
+    };
+    auto update = [&](int i, double v0, double v1, double val) {
+        double d0 = v0 - X[i][0], d1 = v1 - X[i][1];
+        double xmcd = (X[i][0] - C[0]) * d0 + (X[i][1] - C[1]) * d1;
+        S2 += (2.0 / P) * xmcd + ((P - 1.0) / P) * ((d0 * d0 + d1 * d1) / P);
+        C[0] += -X[i][0] / P;
+        C[0] += v0 / P;
+        C[1] += -X[i][1] / P;
+        C[1] += v1 / P;
+        X[i][0] = v0;
+        X[i][1] = v1;
+        Y[i] = val;
+    };
+    center_size();
+    int it = 0, lo = 0;
+    double size;
+    do {
+        it++;
+        double dhi = Y[0], dlo = Y[0], ds_hi = Y[1];
+        int hi = 0;
+        lo = 0;
+        for (int i = 1; i < 3; i++) {
+            double v = Y[i];
+            if (v < dlo) {
+                dlo = v;
+                lo = i;
+            } else if (v > dhi) {
+                ds_hi = dhi;
+                dhi = v;
+                hi = i;
+            } else if (v > ds_hi) {
+                ds_hi = v;
+            }
+        }
+        /* corner moves: xc = a*centre + b*x_hi, a = (1-k) P/(P-1), b = (P k - 1)/(P-1) */
+        const double ar = 3.0, br = -2.0;      /* k = -1: reflection */
+        double r0 = ar * C[0] + br * X[hi][0], r1 = ar * C[1] + br * X[hi][1];
+        double val = rf_fun(c, r0, r1);
+        if (isfinite(val) && val < Y[lo]) {
+            const double ae = 4.5, be = -3.5;  /* k = -2: expansion */
+            double e0 = ae * C[0] + be * X[hi][0], e1 = ae * C[1] + be * X[hi][1];
+            double val2 = rf_fun(c, e0, e1);
+            if (isfinite(val2) && val2 < Y[lo])
+                update(hi, e0, e1, val2);
+            else
+                update(hi, r0, r1, val);
+        } else if (!isfinite(val) || val > ds_hi) {
+            if (isfinite(val) && val <= Y[hi])
+                update(hi, r0, r1, val);
+            const double ac = 0.75, bc = 0.25; /* k = 1/2: contraction */
+            double k0 = ac * C[0] + bc * X[hi][0], k1 = ac * C[1] + bc * X[hi][1];
+            double val2 = rf_fun(c, k0, k1);
+            if (isfinite(val2) && val2 <= Y[hi]) {
+                update(hi, k0, k1, val2);
+            } else {
+                for (int i = 0; i < 3; i++) {
+                    if (i == lo)
+                        continue;
+                    X[i][0] = 0.5 * (X[i][0] + X[lo][0]);
+                    X[i][1] = 0.5 * (X[i][1] + X[lo][1]);
+                    Y[i] = rf_fun(c, X[i][0], X[i][1]);
+                }
+                center_size();
+            }
+        } else {
+            update(hi, r0, r1, val);
+        }
+        lo = 0;
+        for (int i = 1; i < 3; i++)
+            if (Y[i] < Y[lo])
+                lo = i;
+        if (!(S2 > 0))
+            center_size();
+        size = sqrt(S2);
+    } while (!(size < tol) && it < maxiter);
+    x[0] = X[lo][0];
+    x[1] = X[lo][1];
+}
+
+__global__ __launch_bounds__(RF_TPB) void k_refine_rz(agpu_refine_args a)
+{
+    __shared__ double2 red[RF_TPB];
+    __shared__ float s_lp[RF_MAXH];
+    const int ci = blockIdx.x;
+    const int H = a.numharm[ci];
+    const float2 *spec = (const float2 *) a.spec;
+    const double rin = a.rin[ci], zin = a.zin[ci];
+    for (int k = 0; k < H; k++) {
+        const int n = k + 1;
+        double lp = localpower_wg(spec, a.numdata, rin * n, zin * n, red);
+        if (threadIdx.x == 0)
+            s_lp[k] = (float) lp;
+    }
+    __syncthreads();
+    rf_ctx c{spec, a.numdata, s_lp, H, 0, red, 0};
+    double x[2] = {rin, zin / 4.0};
+    c.kernhw = zr_halfwidth(fabs(zin * H) + 4.0, 0);
+    rf_simplex(c, x, 0.4, 1.0e-4, 5000);
+    c.kernhw = zr_halfwidth(fabs(x[1] * H) + 4.0, 1);
+    rf_simplex(c, x, 0.01, 1.0e-7, 5000);
+    const double rout = x[0], zout = x[1] * 4.0;
+    __syncthreads();
+    for (int k = 0; k < H; k++) {
+        const int n = k + 1;
+        const float lp = (float) localpower_wg(spec, a.numdata, rout * n, zout * n, red);
+        double2 amp = rz_interp_wg(spec, a.numdata, rout * n, zout / 4.0 * n * 4.0, c.kernhw,
+                                   red);
+        double d7[7];
+        derivs_wg(spec, a.numdata, rout * n, zout * n, lp, d7, red);
+        if (threadIdx.x == 0) {
+            a.powers[(size_t) ci * RF_MAXH + k] = amp.x * amp.x + amp.y * amp.y;
+            for (int j = 0; j < 7; j++)
+                a.derivs[((size_t) ci * RF_MAXH + k) * 7 + j] = d7[j];
+        }
+    }
+    if (threadIdx.x == 0) {
+        a.rout[ci] = rout;
+        a.zout[ci] = zout;
+        a.nevals[ci] = c.nevals;
+    }
+}
+
+int agpu_refine_rz(const agpu_refine_args *args, void *stream)
+{
+    if (args->ncand <= 0)
+        return 0;
+    if (args->ncand > 1 << 20) {
+        agpu_set_error("agpu_refine_rz: too many candidates");
+        return -2;
+    }
+    hipLaunchKernelGGL(k_refine_rz, dim3(args->ncand), dim3(RF_TPB), 0, (hipStream_t) stream,
+                       *args);
+    RF_CHECK(hipGetLastError());
+    return 0;
+}

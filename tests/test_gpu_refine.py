@@ -1,0 +1,181 @@
+"""GPU parity of the candidate refinement and the output files.
+
+* The reference's own optimiser battery (tests/test_maximize_rz.py of the
+  reference: injected (r, z), perturbed starts, its recovery tolerances) run
+  through the HIP kernel.
+* The HIP refinement against the oracle's restatement for the same starts:
+  both run the same Nelder-Mead scheme on the same function, so they must
+  end at the same optimum; bar 1e-4 bins in r, 4e-4 in z, powers within
+  1e-5 relative (the FP32 accumulation order inside each interpolation
+  differs: FP64 tree on the GPU, FP32 running sum in the reference).
+* A search -> optimize -> ACCEL / .cand round trip, read back with the
+  line rules of the reference's sifting.py (python/presto/sifting.py:40-41,
+  1075-1177)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from presto_amd import synth
+
+pytestmark = pytest.mark.gpu
+A = pytest.importorskip("presto_amd.accel")
+
+N = 2 ** 14
+R_RZ, R_HARM = N / 4.0, N / 16.0
+THEO_POW = N ** 2.0 / 4.0
+RZ_OFFSETS = [(0.0, 0.0), (0.12, -0.8), (-0.2, 1.3), (0.3, 2.0), (-0.35, -2.5)]
+HARM_OFFSETS = [(0.0, 0.0), (0.15, 1.0), (-0.25, -1.5)]
+Z_VALUES = [0.0, 1.0, 2.5, 5.0, -5.0, 10.0, -10.0, 30.0, -30.0, 55.0]
+HARM_Z_VALUES = [0.0, 5.0, -8.0, 20.0]
+TOL = {0.0: dict(dr=0.01, dz=0.03, plo=0.99, phi=1.01),
+       1.0: dict(dr=0.05, dz=0.20, plo=0.90, phi=1.12)}
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _signal(r, z, noise_amp, seed, numharm=1):
+    us = np.arange(N, dtype=np.float64) / N
+    sig = np.zeros(N)
+    for h in range(1, numharm + 1):
+        rh, zh = r * h, z * h
+        sig += np.cos(2.0 * np.pi * (us * (us * (zh / 2.0) + (rh - 0.5 * zh))))
+    if noise_amp:
+        sig = sig + noise_amp * np.random.default_rng(seed).standard_normal(N)
+    return O.realfft(sig.astype(np.float32)).view(np.complex64)
+
+
+@pytest.mark.parametrize("na,seed", [(0.0, 0), (1.0, 12345)])
+def test_gpu_refinement_passes_reference_battery(na, seed):
+    _say(f"case: reference optimiser battery on the GPU, noise {na}")
+    tol = TOL[na]
+    for z in Z_VALUES:
+        ft = _signal(R_RZ, z, na, seed)
+        rs = np.array([R_RZ + d for d, _ in RZ_OFFSETS])
+        zs = np.array([z + d for _, d in RZ_OFFSETS])
+        ro, zo, pw, _ = A.max_rz_harmonics(ft, rs, zs, 1)
+        assert np.all(np.abs(ro - R_RZ) <= tol["dr"]), (z, ro - R_RZ)
+        assert np.all(np.abs(zo - z) <= tol["dz"]), (z, zo - z)
+        pf = pw[:, 0] / THEO_POW
+        assert np.all((pf >= tol["plo"]) & (pf <= tol["phi"])), (z, pf)
+    for numharm in (2, 3, 4):
+        for z in HARM_Z_VALUES:
+            ft = _signal(R_HARM, z, na, seed, numharm)
+            rs = np.array([R_HARM + d for d, _ in HARM_OFFSETS])
+            zs = np.array([z + d for _, d in HARM_OFFSETS])
+            ro, zo, pw, der = A.max_rz_harmonics(ft, rs, zs, numharm)
+            total = np.array([sum(d.pow for d in dl) for dl in der])
+            assert np.all(np.abs(ro - R_HARM) <= tol["dr"]), (numharm, z, ro - R_HARM)
+            assert np.all(np.abs(zo - z) <= tol["dz"]), (numharm, z, zo - z)
+            pf = total / (numharm * THEO_POW)
+            assert np.all((pf >= tol["plo"]) & (pf <= tol["phi"])), (numharm, z, pf)
+
+
+def test_gpu_refinement_matches_oracle():
+    _say("case: GPU refinement vs oracle on noisy multi-harmonic starts")
+    worst_r = worst_z = worst_p = 0.0
+    for numharm, z, dr, dz in [(1, 7.0, 0.2, -1.0), (2, -12.0, -0.3, 2.0),
+                               (4, 20.0, 0.1, 1.5), (8, 3.0, 0.05, -0.5)]:
+        ft = _signal(R_HARM, z, 1.0, 99 + numharm, numharm)
+        ro, zo, pw, der = A.max_rz_harmonics(ft, [R_HARM + dr], [z + dz], numharm)
+        oro, ozo, ords, opw, _ = O.max_rz_arr_harmonics(ft, R_HARM + dr, z + dz, numharm)
+        worst_r = max(worst_r, abs(ro[0] - oro))
+        worst_z = max(worst_z, abs(zo[0] - ozo))
+        for k in range(numharm):
+            worst_p = max(worst_p, abs(pw[0, k] - opw[k]) / opw[k],
+                          abs(der[0][k].pow - ords[k].pow) / ords[k].pow,
+                          abs(der[0][k].locpow - ords[k].locpow) / ords[k].locpow)
+            assert abs(der[0][k].phs - ords[k].phs) < 1e-3 or \
+                abs(abs(der[0][k].phs - ords[k].phs) - 2 * np.pi) < 1e-3
+    _say(f"refinement vs oracle: |dr| {worst_r:.2e}  |dz| {worst_z:.2e}  "
+         f"power rel {worst_p:.2e}")
+    assert worst_r < 1e-4 and worst_z < 4e-4 and worst_p < 1e-5
+
+
+FUND_RE = re.compile(r"^\d")        # python/presto/sifting.py:40
+HARMS_RE = re.compile(r"^[ ]\d")    # python/presto/sifting.py:41
+
+
+def _parse_accel(path):
+    """The candidate and harmonic lines as sifting.py's
+    candlist_from_candfile reads them."""
+    cands, harms = [], {}
+    cur, nh = None, 0
+    for line in open(path):
+        if FUND_RE.match(line):
+            f = line.split()
+            cands.append(dict(num=int(f[0]), sigma=float(f[1]), ipow=float(f[2]),
+                              cpow=float(f[3]), numharm=int(f[4]),
+                              r=float(f[7].split("(")[0]), z=float(f[9].split("(")[0])))
+            continue
+        if HARMS_RE.match(line):
+            f = line.split()
+            cur = int(f[0])
+            harms[cur] = [(f[3], float(f[9].split("(")[0]))]
+            continue
+        if cur is not None and line.strip() and not line.startswith("-") and \
+                len(harms[cur]) < [c["numharm"] for c in cands if c["num"] == cur][0]:
+            f = line.split()
+            harms[cur].append((f[2], float(f[8].split("(")[0])))
+    return cands, harms
+
+
+def test_search_optimize_and_outputs(tmp_path):
+    _say("case: search -> optimize -> ACCEL/.cand, N=2^16 zmax 100 numharm 8")
+    dt = 64e-6
+    x = synth.make_series(1 << 16, dt, 57, [dict(f0=61.3, z=9.0, duty=0.05, amp=0.35),
+                                            dict(f0=377.7, z=-7.0, duty=0.08, amp=0.3)])
+    fft = A.spectrum_from_series(x, dt)
+    p = A.make_params(len(fft), len(x), dt, zmax=100, numharm=8, sigma=2.0)
+    with A.Search(p) as s:
+        s.set_spectrum(fft)
+        s.run()
+        cands = s.candidates()
+        assert len(cands) >= 2
+        s.optimize()
+        props = s.fund_props()
+        acc = str(tmp_path / "test_ACCEL_100")
+        cand = str(tmp_path / "test_ACCEL_100.cand")
+        inf = tmp_path / "test.inf"
+        inf.write_text(f" Number of bins in the time series   =  {len(x)}\n"
+                       f" Width of each time series bin (sec) =  {dt}\n")
+        s.write_outputs(acc, cand, str(inf))
+        # refined harmonics of the two strongest candidates against the oracle
+        for i in range(2):
+            c = cands[i]
+            hr, hz, hw, pw, der = s.harmonics(i)
+            assert len(hr) == c["numharm"]
+            oro, ozo, ords, opw, _ = O.max_rz_arr_harmonics(
+                fft, c["r"] - p.lobin, c["z"], int(c["numharm"]))
+            assert abs(hr[0] - (oro + p.lobin)) < 1e-4
+            assert abs(hz[0] - ozo) < 4e-4
+            assert np.allclose(pw, opw[:len(pw)], rtol=1e-5)
+            assert np.allclose(hr, hr[0] * np.arange(1, len(hr) + 1))
+    # the .cand file: one 88-byte fourierprops per candidate, as computed
+    raw = open(cand, "rb").read()
+    assert len(raw) == 88 * len(cands)
+    dt_fp = np.dtype([("r", "f8"), ("rerr", "f4"), ("pad0", "V4"), ("z", "f8"), ("zerr", "f4"),
+                      ("pad1", "V4"), ("w", "f8"), ("werr", "f4")] +
+                     [(n, "f4") for n in ("pow", "powerr", "sig", "rawpow", "phs", "phserr",
+                                          "cen", "cenerr", "pur", "purerr", "locpow")])
+    assert dt_fp.itemsize == 88
+    rec = np.frombuffer(raw, dtype=dt_fp)
+    for i, pr in enumerate(props):
+        assert rec["r"][i] == pr.r and rec["z"][i] == pr.z and rec["pow"][i] == np.float32(pr.pow)
+        assert rec["rerr"][i] == np.float32(0.5 / cands[i]["numharm"])
+    # the text file, read the way sifting.py reads it
+    fc, fh = _parse_accel(acc)
+    assert [c["num"] for c in fc] == list(range(1, len(cands) + 1))
+    for c, g in zip(fc, cands):
+        assert c["numharm"] == g["numharm"]
+        assert abs(c["sigma"] - g["sigma"]) <= 0.006
+        assert abs(c["r"] - g["r"]) <= 0.5 / g["numharm"] + 1e-6
+        assert abs(c["z"] - g["z"]) <= 2.0 / g["numharm"] + 1e-6
+        assert len(fh[c["num"]]) == g["numharm"]
+    txt = open(acc).read()
+    assert "Number of bins in the time series" in txt
+    _say(f"ACCEL file: {len(fc)} candidates, harmonics tables complete")
