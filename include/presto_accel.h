@@ -177,8 +177,9 @@ typedef struct {
 
 /* optimize_accelcand (src/accel_utils.c:552-626) for every candidate of
  * the last run, as src/accelsearch.c:250-257 calls it, with harmonic
- * polishing (the default): max_rz_arr_harmonics (src/maximize_rz.c:171)
- * in FP64 on the GPU.  Jerk searches (numw > 0) return an error. */
+ * polishing (the default), in FP64 on the GPU: max_rz_arr_harmonics
+ * (src/maximize_rz.c:171) or, for jerk searches, max_rzw_arr_harmonics
+ * (src/maximize_rzw.c:84). */
 int accel_search_optimize(accel_search *s);
 /* Refined harmonics of candidate i (after accel_search_optimize): the
  * accelcand fields hirs, hizs, hiws, pows and derivs
@@ -202,6 +203,13 @@ int accel_max_rz_harmonics(const accel_fcomplex *spec, long long numdata, int n,
                            const int *numharm, const double *rin, const double *zin,
                            double *rout, double *zout, double *pows, accel_rderivs *derivs,
                            int device);
+
+/* max_rzw_arr_harmonics (src/maximize_rzw.c:84-191) for n (r, z, w)
+ * starts on a host spectrum, on the GPU; outputs as above plus wout. */
+int accel_max_rzw_harmonics(const accel_fcomplex *spec, long long numdata, int n,
+                            const int *numharm, const double *rin, const double *zin,
+                            const double *win, double *rout, double *zout, double *wout,
+                            double *pows, accel_rderivs *derivs, int device);
 
 /* src/characteristics.c:195-262 and :330-396 */
 int accel_calc_props(const accel_rderivs *d, double r, double z, double w,

@@ -448,12 +448,13 @@ long o_max_rz_arr_harmonics(const o_fcomplex *data, long numdata, int numharm, d
 /* (src/rzwinterp.c:88-100, src/corr_routines.c:146-258); FP64 sums.    */
 /* ------------------------------------------------------------------ */
 static void rzw_vol_row(const o_fcomplex *data, long numdata, int nb, long lobin, double z,
-                        double w, int khw, int nout, double *pw)
+                        double w, int khw, int jlo, int nout, double *pw)
 {
     int numkern = 2 * nb * khw;
     o_fcomplex *R = (o_fcomplex *) malloc(sizeof(o_fcomplex) * numkern);
     o_gen_w_response(0.0, nb, z, w, numkern, R, NULL);
-    for (int j = 0; j < nout; j++) {
+    for (int jj = 0; jj < nout; jj++) {
+        const int j = jlo + jj;
         double sr = 0.0, si = 0.0;
         /* indices into R: numkern/2 + (k - lobin) nb - j in [0, numkern) */
         long kmin = lobin + (long) floor((j - numkern / 2) / (double) nb);
@@ -467,7 +468,7 @@ static void rzw_vol_row(const o_fcomplex *data, long numdata, int nb, long lobin
             si += (double) d.i * q.r - (double) d.r * q.i;
         }
         float ar = (float) sr, ai = (float) si;
-        pw[j] = (double) ar * ar + (double) ai * ai;
+        pw[jj] = (double) ar * ar + (double) ai * ai;
     }
     free(R);
 }
@@ -509,22 +510,27 @@ void o_max_rzw_arr_harmonics(const o_fcomplex *data, long numdata, int numharm, 
         double maxw = (fabs(wlo) < fabs(whi)) ? whi : wlo;
         int vkhw = o_w_resp_halfwidth(maxz, maxw, 0);
         (void) khw;   /* sizes the reference's FFT only */
-        /* the r positions of the harmonic grid this harmonic feeds */
-        int maxrind = 0;
+        /* the r positions of the harmonic grid this harmonic feeds; a lower
+         * harmonic's grid can start below lobin, where the reference indexes
+         * before the row it computed (src/maximize_rzw.c:148-150 with
+         * rind < 0): the correlation at that r is used instead */
+        int maxrind = -1000000, minrind = 1000000;
         int *rind = (int *) malloc(sizeof(int) * numr);
         for (int kk = 0; kk < numr; kk++) {
             rind[kk] = (int) (long) round(((hhlobin + kk * dr) * hfrac - lobin) * nb);
             if (rind[kk] > maxrind)
                 maxrind = rind[kk];
+            if (rind[kk] < minrind)
+                minrind = rind[kk];
         }
-        double *row = (double *) malloc(sizeof(double) * (maxrind + 1));
+        double *row = (double *) malloc(sizeof(double) * (maxrind - minrind + 1));
         for (int ii = 0; ii < numw; ii++) {
             for (int jj = 0; jj < numz; jj++) {
                 rzw_vol_row(data, numdata, nb, lobin, zlo + jj * dz, wlo + ii * dw, vkhw,
-                            maxrind + 1, row);
+                            minrind, maxrind - minrind + 1, row);
                 for (int kk = 0; kk < numr; kk++) {
                     float *ps = &powsum[((size_t) ii * numz + jj) * numr + kk];
-                    *ps += row[rind[kk]] / locpow;
+                    *ps += row[rind[kk] - minrind] / locpow;
                 }
             }
         }

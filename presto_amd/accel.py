@@ -110,6 +110,7 @@ _sig("accel_search_cand_harmonics", _i, _p, _i, _p, _p, _p, _p, _p, _i)
 _sig("accel_search_fund_props", _i, _p, _p, _i)
 _sig("accel_search_write_outputs", _i, _p, C.c_char_p, C.c_char_p, C.c_char_p)
 _sig("accel_max_rz_harmonics", _i, _p, _ll, _i, _p, _p, _p, _p, _p, _p, _p, _i)
+_sig("accel_max_rzw_harmonics", _i, _p, _ll, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i)
 _sig("accel_calc_props", _i, C.POINTER(RDerivs), _d, _d, _d, C.POINTER(FourierProps))
 _sig("accel_calc_rzwerrs", _i, C.POINTER(FourierProps), _d, C.POINTER(RzwErrs))
 _sig("accel_nice_output", _i, C.c_char_p, _d, _d, _i, _i)
@@ -124,7 +125,8 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_windows", "accel_deredden", "accel_prepare_dat_spectrum",
             "accel_candidate_sigma", "accel_power_for_sigma", "accel_search_optimize",
             "accel_search_cand_harmonics", "accel_search_fund_props",
-            "accel_search_write_outputs", "accel_max_rz_harmonics", "accel_calc_props",
+            "accel_search_write_outputs", "accel_max_rz_harmonics",
+            "accel_max_rzw_harmonics", "accel_calc_props",
             "accel_calc_rzwerrs", "accel_nice_output", "accel_last_error"]
 
 
@@ -306,6 +308,27 @@ def max_rz_harmonics(spec, r, z, numharm, device=0):
         raise _err()
     dl = [list(der)[i * MAXHARM:i * MAXHARM + nh[i]] for i in range(n)]
     return ro, zo, pw, dl
+
+
+def max_rzw_harmonics(spec, r, z, w, numharm, device=0):
+    """max_rzw_arr_harmonics for arrays of starts on the GPU.  Returns
+    rout, zout, wout, powers [n, MAXHARM], derivs (lists of RDerivs)."""
+    spec = np.ascontiguousarray(spec, dtype=np.complex64)
+    r = np.ascontiguousarray(np.atleast_1d(r), dtype=np.float64)
+    z = np.ascontiguousarray(np.atleast_1d(z), dtype=np.float64)
+    w = np.ascontiguousarray(np.atleast_1d(w), dtype=np.float64)
+    nh = np.ascontiguousarray(np.broadcast_to(numharm, r.shape), dtype=np.int32)
+    n = len(r)
+    ro, zo, wo = np.zeros(n), np.zeros(n), np.zeros(n)
+    pw = np.zeros((n, MAXHARM))
+    der = (RDerivs * (n * MAXHARM))()
+    if lib.accel_max_rzw_harmonics(spec.ctypes.data, len(spec), n, nh.ctypes.data,
+                                   r.ctypes.data, z.ctypes.data, w.ctypes.data, ro.ctypes.data,
+                                   zo.ctypes.data, wo.ctypes.data, pw.ctypes.data, der,
+                                   int(device)):
+        raise _err()
+    dl = [list(der)[i * MAXHARM:i * MAXHARM + nh[i]] for i in range(n)]
+    return ro, zo, wo, pw, dl
 
 
 def calc_props(rd, r, z, w):

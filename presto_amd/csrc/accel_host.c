@@ -933,67 +933,122 @@ long long accel_search_plane_cells(const accel_search *s)
 
 /* ---- refinement: optimize_accelcand, src/accel_utils.c:552-626 ---- */
 
-/* Runs max_rz_arr_harmonics for n starts on the device spectrum d_spec. */
-static int refine_rz_device(const void *d_spec, long long numdata, int n, const int *nh,
-                            const double *rin, const double *zin, double *rout, double *zout,
-                            double *pows, accel_rderivs *derivs, void *stream)
+/* Largest chirp length max_rzw_arr_harmonics can need for a start
+ * (zin, win) with numharm harmonics: every template of harmonic n has
+ * |z| <= |zin| n + 4 and |w| <= |win| n + 40, so its half width is at most
+ * 1.1 (13/12 |w| + 3/2 |z|) + 16 (w_resp_halfwidth, src/responses.c:48-84)
+ * and its chirp next2(6 hw + 200 + 2 hw_vol) points (src/responses.c:381). */
+static int rzw_npts_bound(int numharm, double zin, double win)
+{
+    double zb = fabs(zin) * numharm + 4.0, wb = fabs(win) * numharm + 40.0;
+    long long hw = (long long) (1.1 * (13.0 / 12.0 * wb + 1.5 * zb)) + 17;
+    return (int) next2(8 * hw + 200);
+}
+
+/* Runs max_rz_arr_harmonics (win == NULL) or max_rzw_arr_harmonics for n
+ * starts on the device spectrum d_spec, in chunks of at most 256
+ * candidates (one workgroup each). */
+static int refine_device(const void *d_spec, long long numdata, int n, const int *nh,
+                         const double *rin, const double *zin, const double *win, double *rout,
+                         double *zout, double *wout, double *pows, accel_rderivs *derivs,
+                         void *stream)
 {
     int rc = -1;
     int *d_nh = NULL, *d_nev = NULL;
-    double *d_rin = NULL, *d_zin = NULL, *d_rout = NULL, *d_zout = NULL;
-    double *d_der = NULL, *d_pow = NULL;
-    const size_t nd = sizeof(double) * (size_t) n;
-    const size_t nhx = (size_t) n * ACCEL_MAXHARM;
-    for (int i = 0; i < n; i++)
+    double *d_rin = NULL, *d_zin = NULL, *d_win = NULL, *d_rout = NULL, *d_zout = NULL;
+    double *d_wout = NULL, *d_der = NULL, *d_pow = NULL;
+    void *d_scr = NULL;
+    int *nev = NULL;
+    const int chunk = n < 256 ? n : 256;
+    const size_t nd = sizeof(double) * (size_t) chunk;
+    const size_t nhx = (size_t) chunk * ACCEL_MAXHARM;
+    int cap = 0;
+    for (int i = 0; i < n; i++) {
         if (nh[i] < 1 || nh[i] > ACCEL_MAXHARM) {
             set_err("refinement: numharm %d out of range 1..%d", nh[i], ACCEL_MAXHARM);
             return -1;
         }
-    if (agpu_malloc((void **) &d_nh, sizeof(int) * n) ||
-        agpu_malloc((void **) &d_nev, sizeof(int) * n) ||
+        if (win) {
+            int b = rzw_npts_bound(nh[i], zin[i], win[i]);
+            if (b > cap)
+                cap = b;
+        }
+    }
+    if (win && cap > (1 << 20)) {
+        set_err("refinement: jerk chirp of %d points is too long", cap);
+        return -1;
+    }
+    nev = (int *) malloc(sizeof(int) * chunk);
+    if (agpu_malloc((void **) &d_nh, sizeof(int) * chunk) ||
+        agpu_malloc((void **) &d_nev, sizeof(int) * chunk) ||
         agpu_malloc((void **) &d_rin, nd) || agpu_malloc((void **) &d_zin, nd) ||
-        agpu_malloc((void **) &d_rout, nd) || agpu_malloc((void **) &d_zout, nd) ||
+        agpu_malloc((void **) &d_win, nd) || agpu_malloc((void **) &d_rout, nd) ||
+        agpu_malloc((void **) &d_zout, nd) || agpu_malloc((void **) &d_wout, nd) ||
         agpu_malloc((void **) &d_der, sizeof(double) * 7 * nhx) ||
-        agpu_malloc((void **) &d_pow, sizeof(double) * nhx)) {
+        agpu_malloc((void **) &d_pow, sizeof(double) * nhx) ||
+        (win && agpu_malloc(&d_scr, (size_t) 16 * 24 * (size_t) cap * chunk))) {
         set_err("refinement: %s", agpu_last_error());
         goto done;
     }
-    if (agpu_h2d(d_nh, nh, sizeof(int) * n, stream) || agpu_h2d(d_rin, rin, nd, stream) ||
-        agpu_h2d(d_zin, zin, nd, stream) ||
-        agpu_memset(d_der, 0, sizeof(double) * 7 * nhx, stream) ||
-        agpu_memset(d_pow, 0, sizeof(double) * nhx, stream)) {
-        set_err("refinement: %s", agpu_last_error());
-        goto done;
-    }
-    agpu_refine_args a;
-    a.spec = d_spec;
-    a.numdata = numdata;
-    a.ncand = n;
-    a.numharm = d_nh;
-    a.rin = d_rin;
-    a.zin = d_zin;
-    a.rout = d_rout;
-    a.zout = d_zout;
-    a.derivs = d_der;
-    a.powers = d_pow;
-    a.nevals = d_nev;
-    if (agpu_refine_rz(&a, stream) || agpu_d2h(rout, d_rout, nd, stream) ||
-        agpu_d2h(zout, d_zout, nd, stream) ||
-        agpu_d2h(pows, d_pow, sizeof(double) * nhx, stream) ||
-        agpu_d2h(derivs, d_der, sizeof(double) * 7 * nhx, stream)) {
-        set_err("refinement: %s", agpu_last_error());
-        goto done;
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+        const int m = (n - c0) < chunk ? n - c0 : chunk;
+        const size_t md = sizeof(double) * (size_t) m, mhx = (size_t) m * ACCEL_MAXHARM;
+        if (agpu_h2d(d_nh, nh + c0, sizeof(int) * m, stream) ||
+            agpu_h2d(d_rin, rin + c0, md, stream) || agpu_h2d(d_zin, zin + c0, md, stream) ||
+            (win && agpu_h2d(d_win, win + c0, md, stream)) ||
+            agpu_memset(d_der, 0, sizeof(double) * 7 * mhx, stream) ||
+            agpu_memset(d_pow, 0, sizeof(double) * mhx, stream)) {
+            set_err("refinement: %s", agpu_last_error());
+            goto done;
+        }
+        agpu_refine_args a;
+        memset(&a, 0, sizeof(a));
+        a.spec = d_spec;
+        a.numdata = numdata;
+        a.ncand = m;
+        a.numharm = d_nh;
+        a.rin = d_rin;
+        a.zin = d_zin;
+        a.win = d_win;
+        a.rout = d_rout;
+        a.zout = d_zout;
+        a.wout = d_wout;
+        a.derivs = d_der;
+        a.powers = d_pow;
+        a.nevals = d_nev;
+        a.scratch = d_scr;
+        a.npts_cap = cap;
+        if ((win ? agpu_refine_rzw(&a, stream) : agpu_refine_rz(&a, stream)) ||
+            agpu_d2h(rout + c0, d_rout, md, stream) || agpu_d2h(zout + c0, d_zout, md, stream) ||
+            (wout && agpu_d2h(wout + c0, d_wout, md, stream)) ||
+            agpu_d2h(nev, d_nev, sizeof(int) * m, stream) ||
+            agpu_d2h(pows + (size_t) c0 * ACCEL_MAXHARM, d_pow, sizeof(double) * mhx, stream) ||
+            agpu_d2h(derivs + (size_t) c0 * ACCEL_MAXHARM, d_der, sizeof(double) * 7 * mhx,
+                     stream)) {
+            set_err("refinement: %s", agpu_last_error());
+            goto done;
+        }
+        for (int i = 0; i < m; i++)
+            if (nev[i] < 0) {
+                set_err("refinement: candidate %d exceeded the chirp bound", c0 + i);
+                goto done;
+            }
     }
     rc = 0;
 done:
+    free(nev);
     agpu_free(d_nh);
     agpu_free(d_nev);
     agpu_free(d_rin);
     agpu_free(d_zin);
+    agpu_free(d_win);
     agpu_free(d_rout);
     agpu_free(d_zout);
+    agpu_free(d_wout);
     agpu_free(d_der);
     agpu_free(d_pow);
+    if (d_scr)
+        agpu_free(d_scr);
     return rc;
 }
 
@@ -1004,13 +1059,10 @@ int accel_search_optimize(accel_search *s)
     const int n = s->ncands;
     if (n == 0)
         return 0;
-    if (g->numw) {
-        set_err("accel_search_optimize: jerk (w) refinement is not built");
-        return -1;
-    }
     int *nh = (int *) malloc(sizeof(int) * n);
     double *rin = (double *) malloc(sizeof(double) * n);
     double *zin = (double *) malloc(sizeof(double) * n);
+    double *win = (double *) malloc(sizeof(double) * n);
     s->opt_nh = (int *) malloc(sizeof(int) * n);
     s->opt_r = (double *) malloc(sizeof(double) * n);
     s->opt_z = (double *) malloc(sizeof(double) * n);
@@ -1022,9 +1074,12 @@ int accel_search_optimize(accel_search *s)
         nh[i] = c->c.numharm;
         rin[i] = c->c.r - (double) g->lobin;
         zin[i] = c->c.z;
+        win[i] = c->c.w;
     }
-    int rc = refine_rz_device(s->d_spec, g->numbins, n, nh, rin, zin, s->opt_r, s->opt_z,
-                              s->opt_pows, s->opt_derivs, s->stream);
+    /* src/accel_utils.c:563-575: max_rzw_arr_harmonics when numw, else
+     * max_rz_arr_harmonics */
+    int rc = refine_device(s->d_spec, g->numbins, n, nh, rin, zin, g->numw ? win : NULL,
+                           s->opt_r, s->opt_z, s->opt_w, s->opt_pows, s->opt_derivs, s->stream);
     if (rc == 0) {
         for (i = 0; i < n; i++) {
             s->opt_nh[i] = nh[i];
@@ -1037,6 +1092,7 @@ int accel_search_optimize(accel_search *s)
     free(nh);
     free(rin);
     free(zin);
+    free(win);
     return rc;
 }
 
@@ -1102,8 +1158,35 @@ int accel_max_rz_harmonics(const accel_fcomplex *spec, long long numdata, int n,
         set_err("accel_max_rz_harmonics: %s", agpu_last_error());
         goto done;
     }
-    rc = refine_rz_device(d_spec, numdata, n, numharm, rin, zin, rout, zout, pows, derivs,
-                          stream);
+    rc = refine_device(d_spec, numdata, n, numharm, rin, zin, NULL, rout, zout, NULL, pows,
+                       derivs, stream);
+done:
+    if (d_spec)
+        agpu_free(d_spec);
+    if (stream)
+        agpu_stream_destroy(stream);
+    return rc;
+}
+
+int accel_max_rzw_harmonics(const accel_fcomplex *spec, long long numdata, int n,
+                            const int *numharm, const double *rin, const double *zin,
+                            const double *win, double *rout, double *zout, double *wout,
+                            double *pows, accel_rderivs *derivs, int device)
+{
+    void *stream = NULL, *d_spec = NULL;
+    int rc = -1;
+    if (n <= 0 || numdata <= 0) {
+        set_err("accel_max_rzw_harmonics: empty input");
+        return -1;
+    }
+    if (agpu_init(device) || agpu_stream_create(&stream) ||
+        agpu_malloc(&d_spec, sizeof(accel_fcomplex) * (size_t) numdata) ||
+        agpu_h2d(d_spec, spec, sizeof(accel_fcomplex) * (size_t) numdata, stream)) {
+        set_err("accel_max_rzw_harmonics: %s", agpu_last_error());
+        goto done;
+    }
+    rc = refine_device(d_spec, numdata, n, numharm, rin, zin, win, rout, zout, wout, pows,
+                       derivs, stream);
 done:
     if (d_spec)
         agpu_free(d_spec);

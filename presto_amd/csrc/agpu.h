@@ -136,14 +136,21 @@ typedef struct {
     long long numdata;
     int ncand;
     const int *numharm;
-    const double *rin, *zin;
-    double *rout, *zout;
+    const double *rin, *zin, *win;
+    double *rout, *zout, *wout;
     double *derivs;
     double *powers;
     int *nevals;
+    /* jerk refinement only: ncand x 24 x npts_cap complex FP64 scratch */
+    void *scratch;
+    int npts_cap;
 } agpu_refine_args;
 
+/* max_rz_arr_harmonics per candidate */
 int agpu_refine_rz(const agpu_refine_args *args, void *stream);
+/* max_rzw_arr_harmonics per candidate; a candidate whose chirp length
+ * exceeds npts_cap gets rout = NaN, nevals = -1 */
+int agpu_refine_rzw(const agpu_refine_args *args, void *stream);
 
 #ifdef __cplusplus
 }

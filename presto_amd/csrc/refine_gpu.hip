@@ -43,9 +43,9 @@ __device__ static int zr_halfwidth(double z, int highacc)
     return (int) (0.5 * 1.1 * fabs(z)) + (highacc ? 48 : 16);
 }
 
-/* Tap m of gen_z_response(roffset, 1, z, numkern) (src/responses.c:228-316;
+/* Tap m of gen_z_response(roffset, nb, z, numkern) (src/responses.c:228-316;
  * |z| < 1e-4: gen_r_response, :159-225), rounded to FP32. */
-__device__ static float2 resp_tap(double roffset, double z, int numkern, int m)
+__device__ static float2 resp_tap_nb(double roffset, int nb, double z, int numkern, int m)
 {
     const double PI = 3.14159265358979323846;
     double re, im;
@@ -55,7 +55,7 @@ __device__ static float2 resp_tap(double roffset, double z, int numkern, int m)
             re = 1 - 6.579736267392905746 * t;
             im = roffset * (PI - 10.335425560099940058 * t);
         } else {
-            double x = numkern / 2 + roffset - m;   /* response at pi x */
+            double x = (numkern / 2 - m) / (double) nb + roffset;   /* response at pi x */
             double s, c;
             sincospi(x, &s, &c);
             double sinc = (x == 0.0) ? 1.0 : s / (PI * x);
@@ -79,7 +79,7 @@ __device__ static float2 resp_tap(double roffset, double z, int numkern, int m)
             int signz = (z < 0.0) ? -1 : 1;
             double zd = signz * 1.41421356237309504880 / sqrt(absz);
             double cons = zd / 2.0;
-            double r = startr + numkern / 2 - m;
+            double r = startr + (numkern / 2 - m) / (double) nb;
             double yy = r * zd, zz = yy + z * zd;
             double s, c;
             sincospi(r * r / z, &s, &c);
@@ -92,6 +92,11 @@ __device__ static float2 resp_tap(double roffset, double z, int numkern, int m)
         }
     }
     return make_float2((float) re, (float) im);
+}
+
+__device__ static float2 resp_tap(double roffset, double z, int numkern, int m)
+{
+    return resp_tap_nb(roffset, 1, z, numkern, m);
 }
 
 /* Workgroup sum in a fixed order; every thread gets the result. */
@@ -387,6 +392,395 @@ int agpu_refine_rz(const agpu_refine_args *args, void *stream)
         return -2;
     }
     hipLaunchKernelGGL(k_refine_rz, dim3(args->ncand), dim3(RF_TPB), 0, (hipStream_t) stream,
+                       *args);
+    RF_CHECK(hipGetLastError());
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Jerk refinement: max_rzw_arr_harmonics (src/maximize_rzw.c:84-191)   */
+/* ------------------------------------------------------------------ */
+
+/* w_resp_halfwidth, src/responses.c:48-84 */
+__device__ static int wr_halfwidth(double z, double w, int highacc)
+{
+    if (fabs(w) < 1.0e-7)
+        return zr_halfwidth(z, highacc);
+    double r0 = 0.5 * (w / 6.0 - z), r1 = 0.5 * (w / 6.0 + z);
+    double maxdev = fabs(r0) > fabs(r1) ? fabs(r0) : fabs(r1);
+    double u_ext = 0.5 - z / w;
+    if (u_ext > 0.0 && u_ext < 1.0) {
+        double z0 = z - w / 2.0;
+        double r_ext = 0.5 * w * u_ext * u_ext + z0 * u_ext + r0;
+        maxdev = fabs(r_ext) > maxdev ? fabs(r_ext) : maxdev;
+    }
+    return (int) (1.1 * maxdev) + (highacc ? 48 : 16);
+}
+
+__device__ static long long next2_dev(long long x)
+{
+    long long t = 1;
+    while (t < x)
+        t <<= 1;
+    return t;
+}
+
+/* Sample n of the chirp gen_w_response builds (src/responses.c:384-395):
+ * FP64 phase, stored as FP32. */
+__device__ static float chirp_x(int n, int npts, double roffset, double z, double w)
+{
+    const double TWOPI = 6.28318530717958647692;
+    const double dt = 1.0 / (double) npts;
+    const double f = (double) (npts / 4) + roffset - 0.5 * z + w / 12.0;
+    const double fd = (z - 0.5 * w) / 2.0, fdd = w / 6.0;
+    const double t = n * dt;
+    return (float) (2.0 * dt * cos(TWOPI * (t * (t * (t * fdd + fd) + f))));
+}
+
+/* i^n */
+__device__ static double2 ipow(int n)
+{
+    switch (n & 3) {
+    case 0: return make_double2(1.0, 0.0);
+    case 1: return make_double2(0.0, 1.0);
+    case 2: return make_double2(-1.0, 0.0);
+    default: return make_double2(0.0, -1.0);
+    }
+}
+
+/* S(n) = sum_{m=mlo}^{mhi-1} d[k0+m] e^{2 pi i m n / npts}, bins outside
+ * [0, numdata) taken as zero (the zero padding of src/corr_routines.c:
+ * 159-170).  The phasor is re-seeded exactly every 128 steps. */
+__device__ static double2 window_dft(const float2 *spec, long long numdata, long long k0, int mlo,
+                                     int mhi, int n, int npts)
+{
+    long long a = k0 + mlo < 0 ? -k0 : mlo;
+    long long b = k0 + mhi > numdata ? numdata - k0 : mhi;
+    double2 acc = make_double2(0.0, 0.0);
+    double ss, cs;
+    sincospi(2.0 * (double) n / (double) npts, &ss, &cs);
+    double pr = 0.0, pi = 0.0;
+    for (long long m = a; m < b; m++) {
+        if (((m - a) & 127) == 0) {
+            long long q = (m * (long long) n) % npts;
+            if (q < 0)
+                q += npts;
+            sincospi(2.0 * (double) q / (double) npts, &pi, &pr);
+        }
+        const float2 d = spec[k0 + m];
+        acc.x += d.x * pr - d.y * pi;
+        acc.y += d.x * pi + d.y * pr;
+        const double t = pr * cs - pi * ss;
+        pi = pr * ss + pi * cs;
+        pr = t;
+    }
+    return acc;
+}
+
+/* rzw_interp (src/rzwinterp.c:145-230) by one workgroup.  With R the
+ * response of gen_w_response(frac, 1, z, w, 2 khw) -- the DFT of the
+ * chirp x at bins npts/4 - khw + kk -- the sum over data bins
+ * sum_k d_k conj(R) regroups into sum_n x_n i^n S(n), S the DFT of the
+ * data window around intfreq. */
+__device__ static double2 rzw_interp_wg(const float2 *spec, long long numdata, double r,
+                                        double z, double w, int khw, double2 *red)
+{
+    if (r > numdata - 1.0 || r < 0.0)
+        return make_double2(0.0, 0.0);
+    if (fabs(w) < 1e-4)
+        return rz_interp_wg(spec, numdata, r, z, khw, red);
+    double dint;
+    const double frac = modf(r, &dint);
+    const long long intfreq = (long long) dint;
+    const int numkern = 2 * khw;
+    const int npts = (int) next2_dev(6 * wr_halfwidth(z, w, 0) + 200 + numkern);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int n = threadIdx.x; n < npts; n += RF_TPB) {
+        const double x = chirp_x(n, npts, frac, z, w);
+        const double2 S = window_dft(spec, numdata, intfreq, -khw, khw, n, npts);
+        const double2 ph = ipow(n);
+        acc.x += x * (ph.x * S.x - ph.y * S.y);
+        acc.y += x * (ph.x * S.y + ph.y * S.x);
+    }
+    return wg_sum(acc, red);
+}
+
+__device__ static double localpower3d_wg(const float2 *spec, long long numdata, double r,
+                                         double z, double w, double2 *red)
+{
+    const int khw = wr_halfwidth(z, w, 0);
+    double lo1 = r - 5 - 10, hi1 = lo1 + 10;
+    double lo2 = r + 5, hi2 = lo2 + 10;
+    const double nd = (double) numdata;
+    lo1 = lo1 < 0.0 ? 0.0 : (lo1 > nd ? nd : lo1);
+    lo2 = lo2 < 0.0 ? 0.0 : (lo2 > nd ? nd : lo2);
+    hi1 = hi1 < 0.0 ? 0.0 : (hi1 > nd ? nd : hi1);
+    hi2 = hi2 < 0.0 ? 0.0 : (hi2 > nd ? nd : hi2);
+    double sum = 0.0;
+    for (double f = lo1; f < hi1; f += 1.0) {
+        double2 a = rzw_interp_wg(spec, numdata, f, z, w, khw, red);
+        sum += a.x * a.x + a.y * a.y;
+    }
+    for (double f = lo2; f < hi2; f += 1.0) {
+        double2 a = rzw_interp_wg(spec, numdata, f, z, w, khw, red);
+        sum += a.x * a.x + a.y * a.y;
+    }
+    return sum / 20.0;
+}
+
+__device__ static void derivs3d_wg(const float2 *spec, long long numdata, double r, double z,
+                                   double w, double locpow, double *out7, double2 *red)
+{
+    const double PI = 3.14159265358979323846, TWOPI = 6.28318530717958647692;
+    const double h = 0.003, twoh = 0.006, twohsqrd = 0.000036;
+    const int khw = wr_halfwidth(z, w, 1);
+    double pwr[5], phs[5];
+    double f = r - twoh;
+    for (int k = 0; k < 5; k++, f += h) {
+        double2 a = rzw_interp_wg(spec, numdata, f, z, w, khw, red);
+        pwr[k] = a.x * a.x + a.y * a.y;
+        double t = atan2(a.y, a.x);
+        phs[k] = t > 0.0 ? t : t + TWOPI;
+    }
+    for (int k = 0; k < 4; k++)
+        if (fabs(phs[k + 1] - phs[k]) > PI)
+            phs[k + 1] -= TWOPI;
+    out7[0] = pwr[2];
+    out7[1] = phs[2];
+    out7[2] = (pwr[3] - pwr[1]) / twoh;
+    out7[3] = (phs[3] - phs[1]) / twoh;
+    out7[4] = (pwr[4] - 2.0 * pwr[2] + pwr[0]) / twohsqrd;
+    out7[5] = (phs[4] - 2.0 * phs[2] + phs[0]) / twohsqrd;
+    out7[6] = locpow;
+}
+
+#define RZW_NR 24    /* 3 * interpfac * ACCEL_RDR */
+#define RZW_NZ 17    /* 2 * numwid * interpfac + 1 */
+#define RZW_NB 8     /* interpfac * ACCEL_RDR */
+#define RZW_NC 7     /* data windows c = -1 .. 5 */
+
+/* Output j (in 1/8 bins from lobin) correlates the data bins
+ * [c - khw, c + khw) around lobin, c = ceil(j / 8).  j in [-8, 47]: a
+ * lower harmonic's grid can start up to one bin below its lobin; the
+ * reference then indexes before the row it computed (src/maximize_rzw.c:
+ * 148-150 with rind < 0) -- here the correlation at that r is used. */
+__device__ static int rzw_window(int j)
+{
+    return j >= 0 ? (j + RZW_NB - 1) / RZW_NB : -((-j) / RZW_NB);
+}
+
+/* Workgroup sum of RZW_NR complex values per thread (wave shuffles, then
+ * one LDS pass over the 4 waves); lane 0 of wave 0 ... every thread gets
+ * the result in out. */
+__device__ static void wg_sum_vec(double2 *v, double2 *out, double2 *red4)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int k = 0; k < RZW_NR; k++) {
+        double x = v[k].x, y = v[k].y;
+        for (int off = 32; off > 0; off >>= 1) {
+            x += __shfl_xor(x, off);
+            y += __shfl_xor(y, off);
+        }
+        if (lane == 0)
+            red4[wv * RZW_NR + k] = make_double2(x, y);
+    }
+    __syncthreads();
+    if (threadIdx.x < RZW_NR) {
+        const int k = threadIdx.x;
+        double2 s = red4[k];
+        for (int q = 1; q < RF_TPB / 64; q++) {
+            s.x += red4[q * RZW_NR + k].x;
+            s.y += red4[q * RZW_NR + k].y;
+        }
+        out[k] = s;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
+{
+    __shared__ double2 red[RF_TPB];
+    __shared__ double2 red4[(RF_TPB / 64) * RZW_NR];
+    __shared__ double2 vol[RZW_NR];
+    __shared__ float powsum[RZW_NZ * RZW_NZ * RZW_NR];
+    __shared__ int rind[RZW_NR];
+    const int ci = blockIdx.x;
+    const int H = a.numharm[ci];
+    const float2 *spec = (const float2 *) a.spec;
+    const long long numdata = a.numdata;
+    const double rin = a.rin[ci], zin = a.zin[ci], win = a.win[ci];
+    double2 *Q = (double2 *) a.scratch + (size_t) ci * RZW_NR * a.npts_cap;
+    const double dr = 1.0 / RZW_NB;
+    for (int i = threadIdx.x; i < RZW_NZ * RZW_NZ * RZW_NR; i += RF_TPB)
+        powsum[i] = 0.0f;
+    __syncthreads();
+    long long hhlobin = 0;
+    int bad = 0;
+    for (int hind = 0; hind < H && !bad; hind++) {
+        const int n = H - hind;
+        const double rh = rin * n, zh = zin * n, wh = win * n;
+        const double hfrac = n / (double) H;
+        const float locpow = (float) localpower3d_wg(spec, numdata, rh, zh, wh, red);
+        double rh_int;
+        modf(rh, &rh_int);
+        const long long lobin = (long long) rh_int - 1;
+        if (hind == 0)
+            hhlobin = lobin;
+        double zlo = zh - 2 * hfrac * 2.0, zhi = zh + 2 * hfrac * 2.0;
+        double wlo = wh - 2 * hfrac * 20.0, whi = wh + 2 * hfrac * 20.0;
+        const double dz = (zhi - zlo) / (RZW_NZ - 1), dw = (whi - wlo) / (RZW_NZ - 1);
+        const double maxz = (fabs(zlo) < fabs(zhi)) ? zhi : zlo;
+        const double maxw = (fabs(wlo) < fabs(whi)) ? whi : wlo;
+        const int khw = wr_halfwidth(maxz, maxw, 0);
+        const int numkern = 2 * RZW_NB * khw;
+        if (threadIdx.x < RZW_NR)
+            rind[threadIdx.x] =
+                (int) llround(((hhlobin + threadIdx.x * dr) * hfrac - lobin) * RZW_NB);
+        __syncthreads();
+        for (int kk = 0; kk < RZW_NR; kk++)
+            if (rind[kk] < -RZW_NB || rzw_window(rind[kk]) > RZW_NC - 2)
+                bad = 1;
+        if (bad)
+            break;
+        int npts_cur = -1;   /* npts of the Q tables in scratch */
+        for (int wi = 0; wi < RZW_NZ && !bad; wi++) {
+            for (int zi = 0; zi < RZW_NZ; zi++) {
+                const double z = zlo + zi * dz, w = wlo + wi * dw;
+                if (fabs(w) < 1e-4) {
+                    /* z-response template: direct sums with Fresnel taps,
+                     * window [ceil(j/8) - khw, ceil(j/8) + khw) around lobin */
+                    double2 part[RZW_NR];
+                    for (int kk = 0; kk < RZW_NR; kk++) {
+                        const int j = rind[kk];
+                        const int c = rzw_window(j);
+                        double2 acc = make_double2(0.0, 0.0);
+                        for (int m = c - khw + (int) threadIdx.x; m < c + khw; m += RF_TPB) {
+                            const long long k = lobin + m;
+                            if (k < 0 || k >= numdata)
+                                continue;
+                            const int idx = numkern / 2 + m * RZW_NB - j;
+                            const float2 q = resp_tap_nb(0.0, RZW_NB, z, numkern, idx);
+                            const float2 d = spec[k];
+                            acc.x += (double) d.x * q.x + (double) d.y * q.y;
+                            acc.y += (double) d.y * q.x - (double) q.y * d.x;
+                        }
+                        part[kk] = acc;
+                    }
+                    wg_sum_vec(part, vol, red4);
+                } else {
+                    const int npts =
+                        (int) next2_dev(6 * wr_halfwidth(z, w, 0) + 200 + numkern / RZW_NB);
+                    if (npts > a.npts_cap) {
+                        bad = 1;
+                        break;
+                    }
+                    if (npts != npts_cur) {
+                        /* Q_kk(n) = i^n e^{-2 pi i j n / (8 npts)} S_c(n), S_c the
+                         * DFT of the data window [c - khw, c + khw) around lobin,
+                         * c = ceil(j / 8) in [-1, 5] */
+                        for (int nn = threadIdx.x; nn < npts; nn += RF_TPB) {
+                            double2 S[RZW_NC];
+                            S[0] = window_dft(spec, numdata, lobin, -1 - khw, khw - 1, nn, npts);
+                            for (int c = 0; c < RZW_NC - 1; c++) {
+                                double2 e0 = window_dft(spec, numdata, lobin, c - 1 - khw,
+                                                        c - khw, nn, npts);
+                                double2 e1 = window_dft(spec, numdata, lobin, khw + c - 1,
+                                                        khw + c, nn, npts);
+                                S[c + 1] = make_double2(S[c].x - e0.x + e1.x,
+                                                        S[c].y - e0.y + e1.y);
+                            }
+                            const double2 ph = ipow(nn);
+                            for (int kk = 0; kk < RZW_NR; kk++) {
+                                const int j = rind[kk];
+                                const int c = rzw_window(j);
+                                const long long q = ((long long) j * nn) % (RZW_NB * (long long) npts);
+                                double sv, cv;
+                                sincospi(-2.0 * (double) q / (RZW_NB * (double) npts), &sv, &cv);
+                                const double er = ph.x * cv - ph.y * sv, ei = ph.x * sv + ph.y * cv;
+                                const double2 Sc = S[c + 1];
+                                Q[(size_t) kk * a.npts_cap + nn] =
+                                    make_double2(er * Sc.x - ei * Sc.y, er * Sc.y + ei * Sc.x);
+                            }
+                        }
+                        __syncthreads();
+                        npts_cur = npts;
+                    }
+                    double2 part[RZW_NR];
+                    for (int kk = 0; kk < RZW_NR; kk++)
+                        part[kk] = make_double2(0.0, 0.0);
+                    for (int nn = threadIdx.x; nn < npts; nn += RF_TPB) {
+                        const double x = chirp_x(nn, npts, 0.0, z, w);
+                        for (int kk = 0; kk < RZW_NR; kk++) {
+                            const double2 q = Q[(size_t) kk * a.npts_cap + nn];
+                            part[kk].x += x * q.x;
+                            part[kk].y += x * q.y;
+                        }
+                    }
+                    wg_sum_vec(part, vol, red4);
+                }
+                /* powsum += |vol|^2 / locpow (src/maximize_rzw.c:145-153);
+                 * the reference's volume is FP32 */
+                if (threadIdx.x < RZW_NR) {
+                    const int kk = threadIdx.x;
+                    const float vr = (float) vol[kk].x, vi = (float) vol[kk].y;
+                    float *ps = &powsum[(wi * RZW_NZ + zi) * RZW_NR + kk];
+                    *ps += ((double) vr * vr + (double) vi * vi) / locpow;
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (threadIdx.x == 0) {
+            a.rout[ci] = NAN;
+            a.nevals[ci] = -1;
+        }
+        return;
+    }
+    /* argmax in (w, z, r) order, first maximum wins (src/maximize_rzw.c:160-173) */
+    float maxpow = 0.0f;
+    int best = 0;
+    for (int i = 0; i < RZW_NZ * RZW_NZ * RZW_NR; i++)
+        if (powsum[i] > maxpow) {
+            maxpow = powsum[i];
+            best = i;
+        }
+    const int wind = best / (RZW_NZ * RZW_NR), zind = (best / RZW_NR) % RZW_NZ,
+              rix = best % RZW_NR;
+    const double nh = H;
+    const double rout = (hhlobin + rix * dr) / nh;
+    const double zout = ((zin * nh) - 2 * 2.0 + zind * 2.0 / 4.0) / nh;
+    const double wout = ((win * nh) - 2 * 20.0 + wind * 20.0 / 4.0) / nh;
+    for (int k = 0; k < H; k++) {
+        const int hh = k + 1;
+        const double rr = rout * hh, zz = zout * hh, ww = wout * hh;
+        const float lp = (float) localpower3d_wg(spec, numdata, rr, zz, ww, red);
+        double d7[7];
+        derivs3d_wg(spec, numdata, rr, zz, ww, lp, d7, red);
+        if (threadIdx.x == 0) {
+            a.powers[(size_t) ci * RF_MAXH + k] = d7[0];
+            for (int j = 0; j < 7; j++)
+                a.derivs[((size_t) ci * RF_MAXH + k) * 7 + j] = d7[j];
+        }
+    }
+    if (threadIdx.x == 0) {
+        a.rout[ci] = rout;
+        a.zout[ci] = zout;
+        a.wout[ci] = wout;
+        a.nevals[ci] = 0;
+    }
+}
+
+int agpu_refine_rzw(const agpu_refine_args *args, void *stream)
+{
+    if (args->ncand <= 0)
+        return 0;
+    if (args->ncand > 1 << 20 || args->npts_cap <= 0) {
+        agpu_set_error("agpu_refine_rzw: bad candidate count or scratch size");
+        return -2;
+    }
+    hipLaunchKernelGGL(k_refine_rzw, dim3(args->ncand), dim3(RF_TPB), 0, (hipStream_t) stream,
                        *args);
     RF_CHECK(hipGetLastError());
     return 0;

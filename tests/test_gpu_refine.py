@@ -179,3 +179,74 @@ def test_search_optimize_and_outputs(tmp_path):
     txt = open(acc).read()
     assert "Number of bins in the time series" in txt
     _say(f"ACCEL file: {len(fc)} candidates, harmonics tables complete")
+
+
+def _jerk_signal(r, z, w, noise_amp, seed, numharm=1):
+    us = np.arange(N, dtype=np.float64) / N
+    sig = np.zeros(N)
+    for h in range(1, numharm + 1):
+        rh, zh, wh = r * h, z * h, w * h
+        r0, z0 = rh - 0.5 * zh + wh / 12.0, zh - 0.5 * wh
+        sig += np.cos(2.0 * np.pi * (us * (us * (us * wh / 6.0 + z0 / 2.0) + r0)))
+    if noise_amp:
+        sig = sig + noise_amp * np.random.default_rng(seed).standard_normal(N)
+    return O.realfft(sig.astype(np.float32)).view(np.complex64)
+
+
+def test_gpu_jerk_refinement_matches_oracle():
+    """max_rzw_arr_harmonics is a grid search over an oversampled volume:
+    the GPU and the oracle must pick the same grid point, and agree on the
+    powers at it (derivative fields d2pow/d2phs are finite differences of
+    FP32-level values and are compared loosely)."""
+    _say("case: GPU jerk refinement vs oracle")
+    cases = [(1, R_HARM + 0.37, 10.0, 60.0, 0.1, 0.6, 7.0),
+             (2, R_HARM + 0.11, -6.0, -40.0, -0.05, -0.4, 5.0),
+             (4, R_HARM + 0.62, 4.0, 20.0, 0.08, 0.3, -6.0),
+             (2, R_HARM + 0.9, 0.0, 0.0, 0.1, 0.5, 0.0)]
+    for nh, r, z, w, dr, dz, dw in cases:
+        ft = _jerk_signal(r, z, w, 0.5, 7 + nh, nh)
+        ro, zo, wo, pw, der = A.max_rzw_harmonics(ft, [r + dr], [z + dz], [w + dw], nh)
+        oro, ozo, owo, ords, opw = O.max_rzw_arr_harmonics(ft, r + dr, z + dz, w + dw, nh)
+        _say(f"  numharm {nh}: gpu ({ro[0]:.4f}, {zo[0]:.3f}, {wo[0]:.2f}) "
+             f"oracle ({oro:.4f}, {ozo:.3f}, {owo:.2f})")
+        assert abs(ro[0] - oro) < 1e-9 and abs(zo[0] - ozo) < 1e-9 and abs(wo[0] - owo) < 1e-9
+        assert abs(ro[0] - r) <= 0.5 / (8 * nh) + 0.1 / nh
+        for k in range(nh):
+            assert abs(pw[0, k] - opw[k]) <= 2e-5 * opw[k]
+            assert abs(der[0][k].locpow - ords[k].locpow) <= 2e-5 * ords[k].locpow
+            dphs = abs(der[0][k].phs - ords[k].phs)
+            assert min(dphs, abs(dphs - 2 * np.pi)) < 1e-3
+
+
+def test_jerk_search_optimize_and_outputs(tmp_path):
+    _say("case: jerk search -> optimize -> ACCEL, N=2^16 zmax 40 wmax 60 numharm 4")
+    dt = 64e-6
+    x = synth.make_series(1 << 16, dt, 41, [dict(f0=211.9, z=8.0, w=40.0, duty=0.06, amp=0.5),
+                                            dict(f0=97.3, z=0.0, w=-40.0, duty=0.04, amp=0.5)])
+    fft = A.spectrum_from_series(x, dt)
+    p = A.make_params(len(fft), len(x), dt, zmax=40, wmax=60, numharm=4, sigma=2.0,
+                      force_standard=True)
+    with A.Search(p) as s:
+        s.set_spectrum(fft)
+        s.run()
+        cands = s.candidates()
+        assert len(cands) >= 1
+        s.optimize()
+        for i in range(min(2, len(cands))):
+            c = cands[i]
+            hr, hz, hw, pw, der = s.harmonics(i)
+            oro, ozo, owo, ords, opw = O.max_rzw_arr_harmonics(
+                fft, c["r"] - p.lobin, c["z"], c["w"], int(c["numharm"]))
+            assert abs(hr[0] - (oro + p.lobin)) < 1e-9
+            assert abs(hz[0] - ozo) < 1e-9 and abs(hw[0] - owo) < 1e-9
+            assert np.allclose(pw, opw[:len(pw)], rtol=2e-5)
+        acc = str(tmp_path / "jerk_ACCEL_40_JERK_60")
+        s.write_outputs(acc, str(tmp_path / "jerk.cand"))
+    lines = [l for l in open(acc) if FUND_RE.match(l)]
+    assert len(lines) == len(cands)
+    for l, c in zip(lines, cands):
+        f = l.split()
+        # jerk files carry FFT 'w' after FFT 'z' (src/accel_utils.c:825-831)
+        assert abs(float(f[10].split("(")[0]) - c["w"]) <= 20.0 / c["numharm"] + 1e-6
+    head = open(acc).readline()
+    assert "FFT 'w'" in head
