@@ -144,6 +144,23 @@ long accel_deredden(accel_fcomplex *fft, long numbins, int startwidth, int endwi
  * 200, 0.5/dt, N*dt) then bin 0 := 1+1i (src/accel_utils.c:1561,1621-1624). */
 long accel_prepare_dat_spectrum(accel_fcomplex *fft, long numbins, double N, double dt);
 
+/* get_birdies (src/birdzap.c:6-90): reads a zaplist (frequency and width
+ * in Hz per line, '#' comments, 'B' prefix = already barycentric) into
+ * sorted bin ranges for an observation of T seconds at radial velocity
+ * baryv (v/c).  Fills at most maxbirds entries; returns the number of
+ * birdies in the file or -1. */
+int accel_read_birdies(const char *zaplist, double T, double baryv, double *lobins,
+                       double *hibins, int maxbirds);
+/* The -zaplist step of src/accelsearch.c:87-110 on an in-memory spectrum
+ * of an N-sample series: zapbirds (src/zapping.c:58-160) over the sorted
+ * birdies, replacing each with chi^2(2) noise at the local power level
+ * (MT19937 seeded with 1, as the reference's ZAPSEED). */
+int accel_zap_birdies(accel_fcomplex *fft, long long numbins, double N, const double *lobins,
+                      const double *hibins, int nbirds);
+/* n raw 32-bit outputs of MT19937 seeded with init_genrand(seed), the
+ * generator behind the zapping noise (for tests). */
+int accel_mt19937_raw(unsigned int seed, unsigned int *out, int n);
+
 /* Significance (src/characteristics.c:526-559), for callers and tests. */
 double accel_candidate_sigma(double power, int numsum, double numtrials);
 double accel_power_for_sigma(double sigma, int numsum, double numtrials);

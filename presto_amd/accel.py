@@ -114,6 +114,9 @@ _sig("accel_max_rzw_harmonics", _i, _p, _ll, _i, _p, _p, _p, _p, _p, _p, _p, _p,
 _sig("accel_calc_props", _i, C.POINTER(RDerivs), _d, _d, _d, C.POINTER(FourierProps))
 _sig("accel_calc_rzwerrs", _i, C.POINTER(FourierProps), _d, C.POINTER(RzwErrs))
 _sig("accel_nice_output", _i, C.c_char_p, _d, _d, _i, _i)
+_sig("accel_read_birdies", _i, C.c_char_p, _d, _d, _p, _p, _i)
+_sig("accel_zap_birdies", _i, _p, _ll, _d, _p, _p, _i)
+_sig("accel_mt19937_raw", _i, C.c_uint, _p, _i)
 
 EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_destroy", "accel_search_geom", "accel_search_set_spectrum",
@@ -127,7 +130,8 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_cand_harmonics", "accel_search_fund_props",
             "accel_search_write_outputs", "accel_max_rz_harmonics",
             "accel_max_rzw_harmonics", "accel_calc_props",
-            "accel_calc_rzwerrs", "accel_nice_output", "accel_last_error"]
+            "accel_calc_rzwerrs", "accel_nice_output", "accel_read_birdies",
+            "accel_zap_birdies", "accel_mt19937_raw", "accel_last_error"]
 
 
 class AccelError(RuntimeError):
@@ -342,6 +346,34 @@ def calc_rzwerrs(props, T):
     out = RzwErrs()
     if lib.accel_calc_rzwerrs(C.byref(props), T, C.byref(out)):
         raise AccelError("calc_rzwerrs: T out of range")
+    return out
+
+
+def read_birdies(zaplist, T, baryv=0.0):
+    """get_birdies: sorted (lobins, hibins) of a zaplist file."""
+    n = lib.accel_read_birdies(zaplist.encode(), T, baryv, None, None, 0)
+    if n < 0:
+        raise AccelError(f"cannot read zaplist {zaplist}")
+    lo, hi = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    lib.accel_read_birdies(zaplist.encode(), T, baryv, lo.ctypes.data, hi.ctypes.data, n)
+    return lo[:n], hi[:n]
+
+
+def zap_birdies(fft, N, lobins, hibins):
+    """The -zaplist step on a spectrum (in place on a complex64 copy,
+    returned)."""
+    out = np.ascontiguousarray(fft, dtype=np.complex64).copy()
+    lo = np.ascontiguousarray(lobins, dtype=np.float64)
+    hi = np.ascontiguousarray(hibins, dtype=np.float64)
+    if lib.accel_zap_birdies(out.ctypes.data, len(out), float(N), lo.ctypes.data,
+                             hi.ctypes.data, len(lo)):
+        raise AccelError("zapping failed (birdie outside the spectrum)")
+    return out
+
+
+def mt19937_raw(seed, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib.accel_mt19937_raw(seed, out.ctypes.data, n)
     return out
 
 
