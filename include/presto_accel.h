@@ -44,6 +44,7 @@ typedef struct {
     int inmem;           /* -inmem */
     int force_standard;  /* never pick the in-memory path (testing) */
     int dat_input;       /* the spectrum came from a .dat time series */
+    int noharmremove;    /* -noharmremove: keep harmonically related candidates */
 } accel_params;
 
 /* Fills the command-line defaults: numharm 8, zmax 200, sigma 2.0,
@@ -135,6 +136,12 @@ long long accel_search_plane_cells(const accel_search *s);
 
 /* Normalisation windows of the last plan: bins [lobin, lobin+numdata). */
 int accel_search_windows(const accel_search *s, long long *lobin, int *numdata, int n);
+
+/* realfft(data, n, -1) (src/fastffts.c:163-243) of a host series on the
+ * GPU: n a power of two; out receives n/2 complex bins, out[0] =
+ * (DC, Nyquist).  Replaces the FFT of create_accelobs's .dat input
+ * (src/accel_utils.c:1547). */
+int accel_realfft_forward(const float *series, long long n, accel_fcomplex *out, int device);
 
 /* Running-median de-reddening of a spectrum in place; replaces deredden
  * (src/misc_utils.c:1054).  Returns the bins written or -1. */

@@ -24,7 +24,7 @@ class Params(C.Structure):
                 ("numharm", _i), ("zmax", _i), ("wmax", _i), ("wmaxP", _i),
                 ("sigma", _f), ("flo", _d), ("fhi", _d), ("rlo", _d), ("rhi", _d),
                 ("floP", _i), ("fhiP", _i), ("rloP", _i), ("rhiP", _i),
-                ("inmem", _i), ("force_standard", _i), ("dat_input", _i)]
+                ("inmem", _i), ("force_standard", _i), ("dat_input", _i), ("noharmremove", _i)]
 
 
 class Geom(C.Structure):
@@ -117,6 +117,7 @@ _sig("accel_nice_output", _i, C.c_char_p, _d, _d, _i, _i)
 _sig("accel_read_birdies", _i, C.c_char_p, _d, _d, _p, _p, _i)
 _sig("accel_zap_birdies", _i, _p, _ll, _d, _p, _p, _i)
 _sig("accel_mt19937_raw", _i, C.c_uint, _p, _i)
+_sig("accel_realfft_forward", _i, _p, _ll, _p, _i)
 
 EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_destroy", "accel_search_geom", "accel_search_set_spectrum",
@@ -131,7 +132,8 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_write_outputs", "accel_max_rz_harmonics",
             "accel_max_rzw_harmonics", "accel_calc_props",
             "accel_calc_rzwerrs", "accel_nice_output", "accel_read_birdies",
-            "accel_zap_birdies", "accel_mt19937_raw", "accel_last_error"]
+            "accel_zap_birdies", "accel_mt19937_raw", "accel_realfft_forward",
+            "accel_last_error"]
 
 
 class AccelError(RuntimeError):
@@ -368,6 +370,15 @@ def zap_birdies(fft, N, lobins, hibins):
     if lib.accel_zap_birdies(out.ctypes.data, len(out), float(N), lo.ctypes.data,
                              hi.ctypes.data, len(lo)):
         raise AccelError("zapping failed (birdie outside the spectrum)")
+    return out
+
+
+def realfft_gpu(x, device=0):
+    """PRESTO-packed forward real FFT of a float32 series on the GPU."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(len(x) // 2, dtype=np.complex64)
+    if lib.accel_realfft_forward(x.ctypes.data, len(x), out.ctypes.data, int(device)):
+        raise _err()
     return out
 
 

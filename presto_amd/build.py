@@ -9,6 +9,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpresto_accel.so")
+CLI = os.path.join(HERE, "accelsearch")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -16,7 +17,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-I", CSRC, "-I", os.path.join(ROOT, "include")]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall",
            "-I", CSRC, "-I", os.path.join(ROOT, "include")]
-HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip"]
+HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip", "fft_gpu.hip"]
 C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c", "accel_output.c"]
 
 
@@ -42,6 +43,12 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
     os.replace(tmp, LIB)
+    # the accelsearch command, linked against the library next to it
+    cli_obj = os.path.join(BUILD, "accelsearch_cli.o")
+    run(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, "accelsearch_cli.c"), "-o", cli_obj])
+    tmp = CLI + ".tmp"
+    run(["gcc", "-o", tmp, cli_obj, "-L", HERE, "-lpresto_accel", "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(tmp, CLI)
     if verbose:
         print("built", LIB)
     return LIB

@@ -784,7 +784,8 @@ static int replay(accel_search *s)
         n++;
     if (n) {
         s->cands = sort_cands(s->cands);
-        if (s->p.numharm > 1)
+        /* src/accelsearch.c:239-241 */
+        if (s->p.numharm > 1 && !s->p.noharmremove)
             s->cands = eliminate_harmonics(s->cands, &n);
     }
     s->ncands = n;
@@ -1190,6 +1191,37 @@ int accel_max_rzw_harmonics(const accel_fcomplex *spec, long long numdata, int n
 done:
     if (d_spec)
         agpu_free(d_spec);
+    if (stream)
+        agpu_stream_destroy(stream);
+    return rc;
+}
+
+int accel_realfft_forward(const float *series, long long n, accel_fcomplex *out, int device)
+{
+    void *stream = NULL, *d_x = NULL, *d_out = NULL, *d_work = NULL;
+    int rc = -1;
+    if (n < 4 || (n & (n - 1))) {
+        set_err("accel_realfft_forward: length %lld is not a power of two >= 4", n);
+        return -1;
+    }
+    if (agpu_init(device) || agpu_stream_create(&stream) ||
+        agpu_malloc(&d_x, sizeof(float) * (size_t) n) ||
+        agpu_malloc(&d_out, sizeof(accel_fcomplex) * (size_t) (n / 2)) ||
+        agpu_malloc(&d_work, sizeof(accel_fcomplex) * (size_t) (3 * n / 4)) ||
+        agpu_h2d(d_x, series, sizeof(float) * (size_t) n, stream) ||
+        agpu_realfft((const float *) d_x, n, d_out, d_work, stream) ||
+        agpu_d2h(out, d_out, sizeof(accel_fcomplex) * (size_t) (n / 2), stream)) {
+        set_err("accel_realfft_forward: %s", agpu_last_error());
+        goto done;
+    }
+    rc = 0;
+done:
+    if (d_x)
+        agpu_free(d_x);
+    if (d_out)
+        agpu_free(d_out);
+    if (d_work)
+        agpu_free(d_work);
     if (stream)
         agpu_stream_destroy(stream);
     return rc;

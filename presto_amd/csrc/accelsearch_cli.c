@@ -1,0 +1,290 @@
+/*
+ * accelsearch_cli.c -- the accelsearch command (src/accelsearch.c:82-313,
+ * options of clig/accelsearch_cmd.cli:13-48) over libpresto_accel.
+ *
+ *   accelsearch [-numharm n] [-zmax z] [-wmax w] [-sigma s] [-flo f] [-fhi f]
+ *               [-rlo r] [-rhi r] [-lobin b] [-inmem] [-median]
+ *               [-zaplist file] [-baryv v] [-noharmremove] [-ncpus n]
+ *               [-device d] file.{dat,sdat,fft}
+ *
+ * Reads <root>.inf (N, dt, telescope) and the data file; for time series:
+ * GPU real FFT, de-reddening and bin 0 := 1+1i, optional birdie zapping;
+ * then the search, the FP64 refinement and the outputs
+ * <root>_ACCEL_<zmax>[_JERK_<wmax>] (text, .inf appended) and .cand.
+ * Normalisations other than the block median (-photon, -locpow), the
+ * unpolished optimiser (-noharmpolish) and -otheropt are not built and
+ * are refused.  -ncpus is accepted and ignored (the work runs on the GPU).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "presto_accel.h"
+
+static void usage(void)
+{
+    fprintf(stderr,
+            "usage: accelsearch [-numharm n] [-zmax z] [-wmax w] [-sigma s] [-flo Hz] [-fhi Hz]\n"
+            "                   [-rlo bin] [-rhi bin] [-lobin bin] [-inmem] [-median]\n"
+            "                   [-zaplist file] [-baryv v] [-noharmremove] [-ncpus n]\n"
+            "                   [-device d] infile.{dat,sdat,fft}\n");
+}
+
+typedef struct {
+    double N, dt;
+    char telescope[100];
+} infinfo;
+
+/* The two numbers accelsearch needs from the .inf (src/ioinf.c:294-297),
+ * found by their keys. */
+static int read_inf(const char *path, infinfo *inf)
+{
+    FILE *f = fopen(path, "r");
+    if (!f)
+        return -1;
+    char line[512];
+    int got = 0;
+    inf->telescope[0] = '\0';
+    while (fgets(line, sizeof(line), f)) {
+        char *eq = strchr(line, '=');
+        if (!eq)
+            continue;
+        const char *val = eq + 1;
+        while (*val == ' ')
+            val++;
+        if (strstr(line, "Number of bins in the time series")) {
+            inf->N = atof(val);
+            got |= 1;
+        } else if (strstr(line, "Width of each time series bin")) {
+            inf->dt = atof(val);
+            got |= 2;
+        } else if (strstr(line, "Telescope used")) {
+            snprintf(inf->telescope, sizeof(inf->telescope), "%s", val);
+            inf->telescope[strcspn(inf->telescope, "\r\n")] = '\0';
+        }
+    }
+    fclose(f);
+    return got == 3 ? 0 : -1;
+}
+
+static void *read_file(const char *path, size_t elem, long long *count)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f)
+        return NULL;
+    fseek(f, 0, SEEK_END);
+    long long bytes = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    *count = bytes / (long long) elem;
+    void *buf = malloc((size_t) (*count > 0 ? *count : 1) * elem);
+    if (*count && fread(buf, elem, (size_t) *count, f) != (size_t) *count) {
+        free(buf);
+        fclose(f);
+        return NULL;
+    }
+    fclose(f);
+    return buf;
+}
+
+int main(int argc, char **argv)
+{
+    accel_params p;
+    accel_params_default(&p);
+    const char *infile = NULL, *zaplist = NULL;
+    double baryv = 0.0;
+    int device = 0;
+    p.floP = 1;
+    p.fhiP = 1;
+    for (int i = 1; i < argc; i++) {
+        const char *a = argv[i];
+#define NEXT() (i + 1 < argc ? argv[++i] : (usage(), exit(1), ""))
+        if (!strcmp(a, "-numharm"))
+            p.numharm = atoi(NEXT());
+        else if (!strcmp(a, "-zmax"))
+            p.zmax = atoi(NEXT());
+        else if (!strcmp(a, "-wmax")) {
+            p.wmax = atoi(NEXT());
+            p.wmaxP = 1;
+        } else if (!strcmp(a, "-sigma"))
+            p.sigma = (float) atof(NEXT());
+        else if (!strcmp(a, "-flo")) {
+            p.flo = atof(NEXT());
+            p.floP = 1;
+        } else if (!strcmp(a, "-fhi")) {
+            p.fhi = atof(NEXT());
+            p.fhiP = 1;
+        } else if (!strcmp(a, "-rlo")) {
+            p.rlo = atof(NEXT());
+            p.rloP = 1;
+            p.floP = 0;
+        } else if (!strcmp(a, "-rhi")) {
+            p.rhi = atof(NEXT());
+            p.rhiP = 1;
+            p.fhiP = 0;
+        } else if (!strcmp(a, "-lobin"))
+            p.lobin = atoll(NEXT());
+        else if (!strcmp(a, "-inmem"))
+            p.inmem = 1;
+        else if (!strcmp(a, "-median"))
+            ;
+        else if (!strcmp(a, "-zaplist"))
+            zaplist = NEXT();
+        else if (!strcmp(a, "-baryv"))
+            baryv = atof(NEXT());
+        else if (!strcmp(a, "-noharmremove"))
+            p.noharmremove = 1;
+        else if (!strcmp(a, "-ncpus"))
+            (void) NEXT();
+        else if (!strcmp(a, "-device"))
+            device = atoi(NEXT());
+        else if (!strcmp(a, "-photon") || !strcmp(a, "-locpow") || !strcmp(a, "-otheropt") ||
+                 !strcmp(a, "-noharmpolish")) {
+            fprintf(stderr, "accelsearch: option %s is not built in this implementation\n", a);
+            return 1;
+        } else if (a[0] == '-') {
+            fprintf(stderr, "accelsearch: unknown option %s\n", a);
+            usage();
+            return 1;
+        } else
+            infile = a;
+#undef NEXT
+    }
+    if (!infile) {
+        usage();
+        return 1;
+    }
+    /* root name and input type (src/accel_utils.c:1459-1488) */
+    const char *dot = strrchr(infile, '.');
+    if (!dot || (strcmp(dot, ".fft") && strcmp(dot, ".dat") && strcmp(dot, ".sdat"))) {
+        fprintf(stderr, "\nInput file ('%s') must be an '.fft' or '.[s]dat' file!\n\n", infile);
+        return 1;
+    }
+    char root[4096];
+    snprintf(root, sizeof(root), "%.*s", (int) (dot - infile), infile);
+    char infpath[4200];
+    snprintf(infpath, sizeof(infpath), "%s.inf", root);
+    infinfo inf;
+    if (read_inf(infpath, &inf)) {
+        fprintf(stderr, "accelsearch: cannot read N and dt from '%s'\n", infpath);
+        return 1;
+    }
+    const int dat = strcmp(dot, ".fft") != 0;
+    accel_fcomplex *fft = NULL;
+    long long numbins = 0;
+    clock_t t0 = clock();
+    if (dat) {
+        long long n = 0;
+        float *x = NULL;
+        if (!strcmp(dot, ".sdat")) {
+            short *sx = (short *) read_file(infile, sizeof(short), &n);
+            if (sx) {
+                x = (float *) malloc(sizeof(float) * (size_t) (n ? n : 1));
+                for (long long k = 0; k < n; k++)
+                    x[k] = (float) sx[k];
+                free(sx);
+            }
+        } else {
+            x = (float *) read_file(infile, sizeof(float), &n);
+        }
+        if (!x) {
+            fprintf(stderr, "accelsearch: cannot read '%s'\n", infile);
+            return 1;
+        }
+        if (n > 67108864) {   /* src/accel_utils.c:1520 */
+            fprintf(stderr, "\nThe input time series is too large.  Use 'realfft' first.\n\n");
+            return 1;
+        }
+        numbins = n / 2;
+        fft = (accel_fcomplex *) malloc(sizeof(accel_fcomplex) * (size_t) numbins);
+        printf("Reading and FFTing the time series...");
+        fflush(stdout);
+        if (accel_realfft_forward(x, n, fft, device)) {
+            fprintf(stderr, "\naccelsearch: %s\n", accel_last_error());
+            return 1;
+        }
+        free(x);
+        printf("done.\nRemoving red-noise...");
+        if (accel_prepare_dat_spectrum(fft, numbins, (double) n, inf.dt) < 0) {
+            fprintf(stderr, "\naccelsearch: de-reddening failed\n");
+            return 1;
+        }
+        printf("done.\n\n");
+        p.N = (double) n;
+        p.dat_input = 1;
+    } else {
+        fft = (accel_fcomplex *) read_file(infile, sizeof(accel_fcomplex), &numbins);
+        if (!fft) {
+            fprintf(stderr, "accelsearch: cannot read '%s'\n", infile);
+            return 1;
+        }
+        p.N = inf.N;
+    }
+    p.dt = inf.dt;
+    p.numbins = numbins;
+    if (zaplist && dat) {
+        /* src/accelsearch.c:87-110 */
+        const double T = p.N * p.dt;
+        int nb = accel_read_birdies(zaplist, T, baryv, NULL, NULL, 0);
+        if (nb < 0) {
+            fprintf(stderr, "accelsearch: cannot read zaplist '%s'\n", zaplist);
+            return 1;
+        }
+        double *lo = (double *) malloc(sizeof(double) * (nb ? nb : 1));
+        double *hi = (double *) malloc(sizeof(double) * (nb ? nb : 1));
+        accel_read_birdies(zaplist, T, baryv, lo, hi, nb);
+        printf("Read %d birdies from '%s'.\n", nb, zaplist);
+        printf("Zapping them using a barycentric velocity of %.5gc.\n\n", baryv);
+        if (accel_zap_birdies(fft, numbins, p.N, lo, hi, nb)) {
+            fprintf(stderr, "accelsearch: zapping failed\n");
+            return 1;
+        }
+        free(lo);
+        free(hi);
+    }
+    accel_search *s = accel_search_create(&p, device);
+    if (!s) {
+        fprintf(stderr, "accelsearch: %s\n", accel_last_error());
+        return 1;
+    }
+    const accel_geom *g = accel_search_geom(s);
+    printf("Searching with up to %d harmonics summed:\n", 1 << (g->numharmstages - 1));
+    printf("  f = %.1f to %.1f Hz\n", g->rlo / g->T, g->rhi / g->T);
+    printf("  r = %.1f to %.1f Fourier bins\n", g->rlo, g->rhi);
+    printf("  z = %.1f to %.1f Fourier bins drifted\n", g->zlo, g->zhi);
+    if (g->numw)
+        printf("  w = %.1f to %.1f Fourier-derivative bins drifted\n", g->wlo, g->whi);
+    if (accel_search_set_spectrum(s, fft) || accel_search_run(s)) {
+        fprintf(stderr, "accelsearch: %s\n", accel_last_error());
+        return 1;
+    }
+    const int ncand = accel_search_ncands(s);
+    printf("\nDone searching.  Now optimizing each candidate.\n\n");
+    char accelnm[4300], candnm[4400];
+    if (g->numw)
+        snprintf(accelnm, sizeof(accelnm), "%s_ACCEL_%d_JERK_%d", root, (int) g->zhi,
+                 (int) g->whi);
+    else
+        snprintf(accelnm, sizeof(accelnm), "%s_ACCEL_%d", root, (int) g->zhi);
+    snprintf(candnm, sizeof(candnm), "%s.cand", accelnm);
+    if (ncand) {
+        if (accel_search_optimize(s) ||
+            accel_search_write_outputs(s, accelnm, candnm, infpath)) {
+            fprintf(stderr, "accelsearch: %s\n", accel_last_error());
+            return 1;
+        }
+        printf("Wrote %d candidates to '%s' and '%s'.\n\n", ncand, accelnm, candnm);
+    } else {
+        printf("No candidates above sigma = %.2f were found.\n\n", g->sigma);
+    }
+    printf("Searched the following approx numbers of independent points:\n");
+    printf("  %d harmonic:   %9lld\n", 1, g->numindep[0]);
+    for (int i = 1; i < g->numharmstages; i++)
+        printf("  %d harmonics:  %9lld\n", 1 << i, g->numindep[i]);
+    float ms[5];
+    accel_search_timings(s, ms);
+    printf("\nTiming summary:\n    GPU search %.3f ms, total %.2f s (CPU clock)\n", ms[4],
+           (double) (clock() - t0) / CLOCKS_PER_SEC);
+    accel_search_destroy(s);
+    free(fft);
+    return 0;
+}

@@ -125,6 +125,10 @@ int agpu_harmsum(const agpu_harmsum_args *a, void *stream);
 int agpu_inmem_nsq_table(const float *d_nsq, int npre, int nblocks, long long rh0, int cu,
                          long long hb_end, float *d_out, void *stream);
 
+/* Forward real FFT of n (a power of two) FP32 samples in PRESTO's packing
+ * into d_out (n/2 complex); d_work holds 3n/4 complex values. */
+int agpu_realfft(const float *d_x, long long n, void *d_out, void *d_work, void *stream);
+
 /* Candidate refinement (harmonic polishing), one workgroup per candidate.
  * spec: the resident spectrum (numdata complex FP32 bins, bin 0 = lobin);
  * rin = candidate r - lobin; per candidate up to AGPU_REFINE_MAXH
