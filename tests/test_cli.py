@@ -49,8 +49,11 @@ def test_gpu_realfft_matches_fp64():
         x = synth.make_series(n, 1e-3, n, [dict(f0=37.3, duty=0.1, amp=0.5)])
         got = A.realfft_gpu(x)
         ref = np.fft.rfft(x.astype(np.float64))
-        err = np.max(np.abs(got[1:] - ref[1:n // 2])) / np.sqrt(n)
-        assert err < 2e-6, (n, err)
+        # FP32 radix-2 passes: worst bin within 1e-5 of the rms bin amplitude
+        rms = np.sqrt(np.mean(np.abs(ref[1:n // 2]) ** 2))
+        err = np.max(np.abs(got[1:] - ref[1:n // 2])) / rms
+        print(f"realfft n={n}: worst error / rms amplitude {err:.2e}", flush=True)
+        assert err < 1e-5, (n, err)
         assert abs(got[0].real - ref[0].real) < 1e-3 * np.sqrt(n)
         assert abs(got[0].imag - ref[n // 2].real) < 1e-3 * np.sqrt(n)
 
