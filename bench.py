@@ -50,18 +50,28 @@ def measured_traffic(kernel):
     return float(m.group(1)) + float(m.group(2)) if m else None
 
 
-def cpu_baseline(fft, N, dt, max_blocks):
-    """The oracle (C restatement of the reference's search, 1 thread) on the
+def _stage(msg):
+    """Progress line on stderr with the process's peak resident memory."""
+    import resource
+    rss = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1024
+    print(f"bench: {msg} (peak rss {rss:.0f} MiB)", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(fft, N, dt, max_blocks, threads):
+    """The oracle (C restatement of the reference's search, templates of a
+    block in parallel over OpenMP threads as the reference's loop) on the
     first max_blocks blocks of the same spectrum."""
     from oracle import oracle as O
+    O.lib.o_set_threads(threads)
     o = O.make_obs(len(fft), N, dt, zmax=200, numharm=16, sigma=2.0)
     t0 = time.perf_counter()
     _, _, nb = O.accelsearch(o, fft, max_blocks=max_blocks)
     t = time.perf_counter() - t0
     cells = nb * o.corr_uselen * o.numz
-    return dict(value=cells / t, unit="ffdot cells/s", cores=1, kind="port",
+    return dict(value=cells / t, unit="ffdot cells/s", cores=threads, kind="port",
                 sample=f"first {nb} of the workload's {o.numz}-template blocks "
-                       f"(standard path, all 16 harmonic stages), {t:.1f} s on 1 thread")
+                       f"(standard path, all 16 harmonic stages), {t:.1f} s on "
+                       f"{threads} OpenMP threads")
 
 
 def main():
@@ -69,8 +79,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-blocks", type=int, default=250,
+    ap.add_argument("--cpu-blocks", type=int, default=600,
                     help="blocks of the oracle CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                    help="OpenMP threads of the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
@@ -83,8 +96,10 @@ def main():
     x, pulsars = synth.config2_series()
     N = len(x)
     fft = A.spectrum_from_series(x, dt)
+    _stage("spectrum ready")
     p = A.make_params(len(fft), N, dt, zmax=200, numharm=16, sigma=2.0)
     s = A.Search(p, device=0)
+    _stage("search handle created")
     g = s.geom
     s.set_spectrum(fft)
     for _ in range(args.warmup):
@@ -97,10 +112,27 @@ def main():
         lat.append(time.perf_counter() - ts)
         tim.append(s.timings())
     wall = time.perf_counter() - t0
+    _stage(f"{args.steps} timed steps done")
     ncand = len(s.candidates())
     shape = s.corr_shape()
     plane_cells = s.plane_cells()
+    # the rest of a DM trial: FP64 refinement of every candidate on the GPU
+    # and the ACCEL / .cand files (src/accelsearch.c:243-284)
+    import tempfile
+    t_opt, t_out = [], []
+    with tempfile.TemporaryDirectory() as td:
+        for _ in range(3):
+            ts = time.perf_counter()
+            s.optimize()
+            t_opt.append(time.perf_counter() - ts)
+            ts = time.perf_counter()
+            s.write_outputs(os.path.join(td, "bench_ACCEL_200"),
+                            os.path.join(td, "bench_ACCEL_200.cand"))
+            t_out.append(time.perf_counter() - ts)
     s.close()
+    _stage("refinement and outputs timed")
+    opt_ms = 1e3 * float(np.median(t_opt))
+    out_ms = 1e3 * float(np.median(t_out))
 
     def avg(k):
         return float(np.mean([t[k] for t in tim]))
@@ -142,6 +174,9 @@ def main():
                    "path": "in-memory" if g.inmem else "standard",
                    "segment_len": L, "segment_uselen": U},
         "dm_trials_per_s": 1e3 / ms_step,
+        "optimize_ms": opt_ms,
+        "accel_output_ms": out_ms,
+        "dm_trials_per_s_through_accel": 1e3 / (ms_step + opt_ms + out_ms),
         "p50_ms": 1e3 * float(np.percentile(lat, 50)),
         "p99_ms": 1e3 * float(np.percentile(lat, 99)),
         "kernel_ms": {"block_norms": avg("norms"), "correlation": corr_ms,
@@ -161,7 +196,9 @@ def main():
                                  "traffic": measured_traffic("k_corr4096")},
     }
     if args.cpu_blocks > 0:
-        line["cpu_baseline"] = cpu_baseline(fft, N, dt, args.cpu_blocks)
+        _stage(f"CPU baseline on {args.cpu_threads} threads")
+        line["cpu_baseline"] = cpu_baseline(fft, N, dt, args.cpu_blocks, args.cpu_threads)
+        _stage("CPU baseline done")
     print(json.dumps(line), flush=True)
 
 

@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "oracle.h"
+#include <omp.h>
 
 #define NEAREST_INT(x) ((int) ((x) < 0 ? (x) - 0.5 : (x) + 0.5))
 
@@ -358,9 +359,12 @@ o_ffdot *o_subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
         o_cfft_f(pdata, fftlen, -1);
         ffdot->powers = (float *) malloc(sizeof(float) * nplane);
         const float pnorm = 1.0 / ((double) fftlen * fftlen);
-        o_fcomplex *tmp = (o_fcomplex *) malloc(sizeof(o_fcomplex) * fftlen);
+        /* templates in parallel, as the reference's OpenMP loop
+         * (src/accel_utils.c:1141-1193) */
+#pragma omp parallel for collapse(2) schedule(static)
         for (int ii = 0; ii < ffdot->numws; ii++) {
             for (int jj = 0; jj < ffdot->numzs; jj++) {
+                o_fcomplex *tmp = (o_fcomplex *) malloc(sizeof(o_fcomplex) * fftlen);
                 const o_fcomplex *k = shi->kern[ii * shi->numkern_zdim + jj].data;
                 float *outp = ffdot->powers + ((size_t) ii * ffdot->numzs + jj) * ffdot->numrs;
                 for (int kk = 0; kk < fftlen; kk++) {
@@ -375,9 +379,9 @@ o_ffdot *o_subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
                     outp[kk] = fmaf(tmp[ind].r, tmp[ind].r,
                                     fmaf(tmp[ind].i, tmp[ind].i, 0.0f)) * pnorm;
                 }
+                free(tmp);
             }
         }
-        free(tmp);
         free(pdata);
     } else {
         o_dcomplex *pdata = (o_dcomplex *) calloc(fftlen, sizeof(o_dcomplex));
@@ -388,9 +392,10 @@ o_ffdot *o_subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
         o_cfft_d(pdata, fftlen, -1);
         ffdot->dpowers = (double *) malloc(sizeof(double) * nplane);
         const double pnorm = 1.0 / ((double) fftlen * fftlen);
-        o_dcomplex *tmp = (o_dcomplex *) malloc(sizeof(o_dcomplex) * fftlen);
+#pragma omp parallel for collapse(2) schedule(static)
         for (int ii = 0; ii < ffdot->numws; ii++) {
             for (int jj = 0; jj < ffdot->numzs; jj++) {
+                o_dcomplex *tmp = (o_dcomplex *) malloc(sizeof(o_dcomplex) * fftlen);
                 const o_dcomplex *k = shi->kern[ii * shi->numkern_zdim + jj].ddata;
                 double *outp = ffdot->dpowers + ((size_t) ii * ffdot->numzs + jj) * ffdot->numrs;
                 for (int kk = 0; kk < fftlen; kk++) {
@@ -402,9 +407,9 @@ o_ffdot *o_subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
                     const int ind = kk + offset;
                     outp[kk] = (tmp[ind].r * tmp[ind].r + tmp[ind].i * tmp[ind].i) * pnorm;
                 }
+                free(tmp);
             }
         }
-        free(tmp);
         free(pdata);
     }
     free(data);
@@ -840,4 +845,10 @@ long long o_prepare_dat(float *series, long long n, double dt)
     fft[0].r = 1.0f;
     fft[0].i = 1.0f;
     return numbins;
+}
+
+/* threads of the template loop (the accelsearch -ncpus option) */
+void o_set_threads(int n)
+{
+    omp_set_num_threads(n > 0 ? n : 1);
 }

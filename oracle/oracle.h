@@ -160,6 +160,7 @@ typedef struct {
 int o_accelsearch(const o_obs *obs, const o_fcomplex *fft, int fp64, int max_blocks,
                   o_result *res);
 void o_free_result(o_result *res);
+void o_set_threads(int n);
 long long o_prepare_dat(float *series, long long n, double dt);
 /* flat accessors for ctypes */
 int o_result_ncands(const o_result *res);

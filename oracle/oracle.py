@@ -97,6 +97,7 @@ _sig("o_deredden", C.c_long, _cf, C.c_long, _i, _i, _d, _d)
 _sig("o_create_accelobs", _i, C.POINTER(OObs))
 _sig("o_accelsearch", _i, C.POINTER(OObs), _cf, _i, _i, C.POINTER(OResult))
 _sig("o_free_result", None, C.POINTER(OResult))
+_sig("o_set_threads", None, _i)
 _sig("o_result_ncands", _i, C.POINTER(OResult))
 _sig("o_result_cands", None, C.POINTER(OResult), _dp)
 _sig("o_prepare_dat", _ll, _fp, _ll, _d)

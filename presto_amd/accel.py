@@ -182,7 +182,8 @@ class Search:
         self.h = lib.accel_search_create(C.byref(params), int(device))
         if not self.h:
             raise _err()
-        self.geom = lib.accel_search_geom(self.h).contents
+        # a copy: the handle's own struct is freed with the handle
+        self.geom = Geom.from_buffer_copy(lib.accel_search_geom(self.h).contents)
 
     def close(self):
         if self.h:

@@ -88,3 +88,17 @@ def test_spectrum_from_series_close_to_oracle_realfft():
     a = A.spectrum_from_series(x, 1e-3)
     b = O.prepare_dat(x, 1e-3)
     assert np.max(np.abs(a[1:] - b[1:])) < 1e-3
+
+
+@pytest.mark.gpu
+def test_search_geometry_outlives_the_handle():
+    """Search.geom is an owned copy, still valid after close() and after
+    other allocations reuse the handle's memory."""
+    import presto_amd.accel as A
+    p = A.make_params(1 << 15, 1 << 16, 1e-3, zmax=20, numharm=8)
+    s = A.Search(p)
+    before = bytes(s.geom)
+    s.close()
+    junk = [bytearray(4096) for _ in range(256)]
+    assert bytes(s.geom) == before and s.geom.numharmstages == 4
+    del junk
