@@ -244,6 +244,37 @@ int accel_calc_rzwerrs(const accel_fourierprops *p, double T, accel_rzwerrs *e);
  * (numerr 2); returns the string length or -1. */
 int accel_nice_output(char *out, double val, double err, int len, int numerr);
 
+/* ---- batches of DM-trial spectra ---- */
+
+/* One spectrum of a batch: inputs, what to do with it, and its results. */
+typedef struct {
+    accel_params params;          /* search parameters of this spectrum */
+    const accel_fcomplex *fft;    /* params.numbins host bins */
+    int optimize;                 /* refine the candidates (and write files) */
+    const char *accelnm, *candnm, *inf_path;   /* outputs, or accelnm NULL */
+    accel_cand *cands;            /* caller's array for the candidates, or NULL */
+    int maxcands;
+    int ncands;                   /* out: candidates found */
+    int status;                   /* out: 0 or -1 */
+    float gpu_ms;                 /* out: device time of its search */
+} accel_batch_item;
+
+typedef struct accel_batch accel_batch;
+
+/* A pool of search handles on one device, reused by every spectrum of the
+ * same parameters.  No counterpart in the reference, which runs one
+ * accelsearch process per spectrum (src/accelsearch.c:82-313). */
+accel_batch *accel_batch_create(int device);
+void accel_batch_destroy(accel_batch *b);
+/* Search handles the pool holds (each with its resident templates). */
+int accel_batch_handles(const accel_batch *b);
+/* Runs every item as accelsearch would (search, candidates, and when
+ * optimize is set the refinement and output files), pipelining the items:
+ * the upload and search of item i+1 overlap the hit replay, refinement and
+ * output of item i.  Results equal separate runs.  Returns 0, or -1 when
+ * any item failed (its status says which). */
+int accel_batch_run(accel_batch *b, accel_batch_item *items, int n);
+
 const char *accel_last_error(void);
 
 #ifdef __cplusplus

@@ -133,7 +133,8 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_max_rzw_harmonics", "accel_calc_props",
             "accel_calc_rzwerrs", "accel_nice_output", "accel_read_birdies",
             "accel_zap_birdies", "accel_mt19937_raw", "accel_realfft_forward",
-            "accel_last_error"]
+            "accel_batch_create", "accel_batch_destroy", "accel_batch_handles",
+            "accel_batch_run", "accel_last_error"]
 
 
 class AccelError(RuntimeError):
@@ -298,6 +299,73 @@ class Search:
             raise _err()
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("params", Params), ("fft", _p), ("optimize", _i), ("accelnm", C.c_char_p),
+                ("candnm", C.c_char_p), ("inf_path", C.c_char_p), ("cands", _p),
+                ("maxcands", _i), ("ncands", _i), ("status", _i), ("gpu_ms", _f)]
+
+
+_sig("accel_batch_create", _p, _i)
+_sig("accel_batch_destroy", None, _p)
+_sig("accel_batch_handles", _i, _p)
+_sig("accel_batch_run", _i, _p, _p, _i)
+
+
+class Batch:
+    """A pool of search handles on one device; run() pipelines many
+    spectra through search, refinement and output."""
+
+    def __init__(self, device=0):
+        self.h = lib.accel_batch_create(int(device))
+
+    def close(self):
+        if self.h:
+            lib.accel_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def handles(self):
+        return lib.accel_batch_handles(self.h)
+
+    def run(self, jobs, optimize=True, maxcands=4096):
+        """jobs: list of (params, spectrum, accelnm or None, candnm or None).
+        Returns a list of candidate arrays (CAND_DTYPE), one per job, and
+        the per-job device times (ms)."""
+        n = len(jobs)
+        items = (BatchItem * n)()
+        keep = []
+        outs = []
+        for it, (p, fft, acc, cand) in zip(items, jobs):
+            fft = np.ascontiguousarray(fft, dtype=np.complex64)
+            if len(fft) != p.numbins:
+                raise ValueError("spectrum length differs from params.numbins")
+            out = np.zeros(maxcands, dtype=CAND_DTYPE)
+            keep.append(fft)
+            outs.append(out)
+            it.params = p
+            it.fft = fft.ctypes.data
+            it.optimize = int(optimize)
+            it.accelnm = acc.encode() if acc else None
+            it.candnm = cand.encode() if cand else None
+            it.inf_path = None
+            it.cands = out.ctypes.data
+            it.maxcands = maxcands
+        rc = lib.accel_batch_run(self.h, items, n)
+        if rc:
+            bad = [i for i in range(n) if items[i].status]
+            raise AccelError(f"batch items {bad[:8]} failed: {lib.accel_last_error().decode()}")
+        return ([o[:min(it.ncands, maxcands)].copy() for o, it in zip(outs, items)],
+                [it.gpu_ms for it in items])
+
+
 def max_rz_harmonics(spec, r, z, numharm, device=0):
     """max_rz_arr_harmonics for arrays of starts on the GPU.  Returns
     rout, zout, powers [n, MAXHARM], derivs (list of lists of RDerivs)."""
@@ -380,6 +448,15 @@ def realfft_gpu(x, device=0):
     out = np.zeros(len(x) // 2, dtype=np.complex64)
     if lib.accel_realfft_forward(x.ctypes.data, len(x), out.ctypes.data, int(device)):
         raise _err()
+    return out
+
+
+def spectrum_from_series_gpu(x, dt, device=0):
+    """spectrum_from_series with the real FFT on the GPU (FP32, as the
+    accelsearch command does it)."""
+    out = realfft_gpu(x, device)
+    if lib.accel_prepare_dat_spectrum(out.ctypes.data, len(out), float(len(x)), float(dt)) < 0:
+        raise AccelError("deredden failed (spectrum too short)")
     return out
 
 

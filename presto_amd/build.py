@@ -18,7 +18,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall",
            "-I", CSRC, "-I", os.path.join(ROOT, "include")]
 HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip", "fft_gpu.hip"]
-C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c", "accel_output.c"]
+C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c", "accel_output.c", "accel_batch.c"]
 
 
 def run(cmd):

@@ -425,6 +425,8 @@ static void free_cands(cnode *l)
 }
 
 /* ---- the search ---- */
+typedef struct refine_ws refine_ws;
+static void ws_free(refine_ws *w);
 struct accel_search {
     accel_params p;
     accel_geom g;
@@ -463,6 +465,7 @@ struct accel_search {
     double *opt_r, *opt_z, *opt_w;    /* fundamental optimum; r includes lobin */
     double *opt_pows;                 /* [nopt][ACCEL_MAXHARM] */
     accel_rderivs *opt_derivs;        /* [nopt][ACCEL_MAXHARM] */
+    refine_ws *ws;                    /* refinement device buffers */
 };
 
 const accel_geom *accel_search_geom(const accel_search *s) { return &s->g; }
@@ -673,6 +676,10 @@ void accel_search_destroy(accel_search *s)
     free(s->opt_w);
     free(s->opt_pows);
     free(s->opt_derivs);
+    if (s->ws) {
+        ws_free(s->ws);
+        free(s->ws);
+    }
     free(s);
 }
 
@@ -946,23 +953,78 @@ static int rzw_npts_bound(int numharm, double zin, double win)
     return (int) next2(8 * hw + 200);
 }
 
-/* Runs max_rz_arr_harmonics (win == NULL) or max_rzw_arr_harmonics for n
- * starts on the device spectrum d_spec, in chunks of at most 256
- * candidates (one workgroup each). */
-static int refine_device(const void *d_spec, long long numdata, int n, const int *nh,
-                         const double *rin, const double *zin, const double *win, double *rout,
-                         double *zout, double *wout, double *pows, accel_rderivs *derivs,
-                         void *stream)
+/* Device buffers of the refinement, kept by their owner between calls:
+ * freeing device memory waits for the whole device, which would stop a
+ * refinement from overlapping the next search. */
+#define REFINE_CHUNK 256
+struct refine_ws {
+    int ready;
+    size_t scr_bytes;
+    int *d_nh, *d_nev;
+    double *d_rin, *d_zin, *d_win, *d_rout, *d_zout, *d_wout, *d_der, *d_pow;
+    void *d_scr;
+};
+
+static void ws_free(refine_ws *w)
 {
-    int rc = -1;
-    int *d_nh = NULL, *d_nev = NULL;
-    double *d_rin = NULL, *d_zin = NULL, *d_win = NULL, *d_rout = NULL, *d_zout = NULL;
-    double *d_wout = NULL, *d_der = NULL, *d_pow = NULL;
-    void *d_scr = NULL;
-    int *nev = NULL;
-    const int chunk = n < 256 ? n : 256;
-    const size_t nd = sizeof(double) * (size_t) chunk;
-    const size_t nhx = (size_t) chunk * ACCEL_MAXHARM;
+    if (!w->ready)
+        return;
+    agpu_free(w->d_nh);
+    agpu_free(w->d_nev);
+    agpu_free(w->d_rin);
+    agpu_free(w->d_zin);
+    agpu_free(w->d_win);
+    agpu_free(w->d_rout);
+    agpu_free(w->d_zout);
+    agpu_free(w->d_wout);
+    agpu_free(w->d_der);
+    agpu_free(w->d_pow);
+    if (w->d_scr)
+        agpu_free(w->d_scr);
+    memset(w, 0, sizeof(*w));
+}
+
+static int ws_reserve(refine_ws *w, size_t scr_bytes)
+{
+    const size_t nd = sizeof(double) * REFINE_CHUNK;
+    const size_t nhx = (size_t) REFINE_CHUNK * ACCEL_MAXHARM;
+    if (!w->ready) {
+        if (agpu_malloc((void **) &w->d_nh, sizeof(int) * REFINE_CHUNK) ||
+            agpu_malloc((void **) &w->d_nev, sizeof(int) * REFINE_CHUNK) ||
+            agpu_malloc((void **) &w->d_rin, nd) || agpu_malloc((void **) &w->d_zin, nd) ||
+            agpu_malloc((void **) &w->d_win, nd) || agpu_malloc((void **) &w->d_rout, nd) ||
+            agpu_malloc((void **) &w->d_zout, nd) || agpu_malloc((void **) &w->d_wout, nd) ||
+            agpu_malloc((void **) &w->d_der, sizeof(double) * 7 * nhx) ||
+            agpu_malloc((void **) &w->d_pow, sizeof(double) * nhx)) {
+            set_err("refinement: %s", agpu_last_error());
+            w->ready = 1;
+            ws_free(w);
+            return -1;
+        }
+        w->ready = 1;
+    }
+    if (scr_bytes > w->scr_bytes) {
+        if (w->d_scr)
+            agpu_free(w->d_scr);
+        w->d_scr = NULL;
+        w->scr_bytes = 0;
+        if (agpu_malloc(&w->d_scr, scr_bytes)) {
+            set_err("refinement scratch: %s", agpu_last_error());
+            return -1;
+        }
+        w->scr_bytes = scr_bytes;
+    }
+    return 0;
+}
+
+/* Runs max_rz_arr_harmonics (win == NULL) or max_rzw_arr_harmonics for n
+ * starts on the device spectrum d_spec, in chunks of at most REFINE_CHUNK
+ * candidates (one workgroup each), with the buffers of w. */
+static int refine_device(refine_ws *w, const void *d_spec, long long numdata, int n,
+                         const int *nh, const double *rin, const double *zin, const double *win,
+                         double *rout, double *zout, double *wout, double *pows,
+                         accel_rderivs *derivs, void *stream)
+{
     int cap = 0;
     for (int i = 0; i < n; i++) {
         if (nh[i] < 1 || nh[i] > ACCEL_MAXHARM) {
@@ -979,78 +1041,58 @@ static int refine_device(const void *d_spec, long long numdata, int n, const int
         set_err("refinement: jerk chirp of %d points is too long", cap);
         return -1;
     }
-    nev = (int *) malloc(sizeof(int) * chunk);
-    if (agpu_malloc((void **) &d_nh, sizeof(int) * chunk) ||
-        agpu_malloc((void **) &d_nev, sizeof(int) * chunk) ||
-        agpu_malloc((void **) &d_rin, nd) || agpu_malloc((void **) &d_zin, nd) ||
-        agpu_malloc((void **) &d_win, nd) || agpu_malloc((void **) &d_rout, nd) ||
-        agpu_malloc((void **) &d_zout, nd) || agpu_malloc((void **) &d_wout, nd) ||
-        agpu_malloc((void **) &d_der, sizeof(double) * 7 * nhx) ||
-        agpu_malloc((void **) &d_pow, sizeof(double) * nhx) ||
-        (win && agpu_malloc(&d_scr, (size_t) 16 * 24 * (size_t) cap * chunk))) {
-        set_err("refinement: %s", agpu_last_error());
-        goto done;
-    }
+    const int chunk = n < REFINE_CHUNK ? n : REFINE_CHUNK;
+    if (ws_reserve(w, win ? (size_t) 16 * 24 * (size_t) cap * chunk : 0))
+        return -1;
+    int nev[REFINE_CHUNK];
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int m = (n - c0) < chunk ? n - c0 : chunk;
         const size_t md = sizeof(double) * (size_t) m, mhx = (size_t) m * ACCEL_MAXHARM;
-        if (agpu_h2d(d_nh, nh + c0, sizeof(int) * m, stream) ||
-            agpu_h2d(d_rin, rin + c0, md, stream) || agpu_h2d(d_zin, zin + c0, md, stream) ||
-            (win && agpu_h2d(d_win, win + c0, md, stream)) ||
-            agpu_memset(d_der, 0, sizeof(double) * 7 * mhx, stream) ||
-            agpu_memset(d_pow, 0, sizeof(double) * mhx, stream)) {
+        if (agpu_h2d(w->d_nh, nh + c0, sizeof(int) * m, stream) ||
+            agpu_h2d(w->d_rin, rin + c0, md, stream) ||
+            agpu_h2d(w->d_zin, zin + c0, md, stream) ||
+            (win && agpu_h2d(w->d_win, win + c0, md, stream))) {
             set_err("refinement: %s", agpu_last_error());
-            goto done;
+            return -1;
         }
         agpu_refine_args a;
         memset(&a, 0, sizeof(a));
         a.spec = d_spec;
         a.numdata = numdata;
         a.ncand = m;
-        a.numharm = d_nh;
-        a.rin = d_rin;
-        a.zin = d_zin;
-        a.win = d_win;
-        a.rout = d_rout;
-        a.zout = d_zout;
-        a.wout = d_wout;
-        a.derivs = d_der;
-        a.powers = d_pow;
-        a.nevals = d_nev;
-        a.scratch = d_scr;
+        a.numharm = w->d_nh;
+        a.rin = w->d_rin;
+        a.zin = w->d_zin;
+        a.win = w->d_win;
+        a.rout = w->d_rout;
+        a.zout = w->d_zout;
+        a.wout = w->d_wout;
+        a.derivs = w->d_der;
+        a.powers = w->d_pow;
+        a.nevals = w->d_nev;
+        a.scratch = w->d_scr;
         a.npts_cap = cap;
-        if ((win ? agpu_refine_rzw(&a, stream) : agpu_refine_rz(&a, stream)) ||
-            agpu_d2h(rout + c0, d_rout, md, stream) || agpu_d2h(zout + c0, d_zout, md, stream) ||
-            (wout && agpu_d2h(wout + c0, d_wout, md, stream)) ||
-            agpu_d2h(nev, d_nev, sizeof(int) * m, stream) ||
-            agpu_d2h(pows + (size_t) c0 * ACCEL_MAXHARM, d_pow, sizeof(double) * mhx, stream) ||
-            agpu_d2h(derivs + (size_t) c0 * ACCEL_MAXHARM, d_der, sizeof(double) * 7 * mhx,
+        if (agpu_memset(w->d_der, 0, sizeof(double) * 7 * mhx, stream) ||
+            agpu_memset(w->d_pow, 0, sizeof(double) * mhx, stream) ||
+            (win ? agpu_refine_rzw(&a, stream) : agpu_refine_rz(&a, stream)) ||
+            agpu_d2h(rout + c0, w->d_rout, md, stream) ||
+            agpu_d2h(zout + c0, w->d_zout, md, stream) ||
+            (wout && agpu_d2h(wout + c0, w->d_wout, md, stream)) ||
+            agpu_d2h(nev, w->d_nev, sizeof(int) * m, stream) ||
+            agpu_d2h(pows + (size_t) c0 * ACCEL_MAXHARM, w->d_pow, sizeof(double) * mhx,
+                     stream) ||
+            agpu_d2h(derivs + (size_t) c0 * ACCEL_MAXHARM, w->d_der, sizeof(double) * 7 * mhx,
                      stream)) {
             set_err("refinement: %s", agpu_last_error());
-            goto done;
+            return -1;
         }
         for (int i = 0; i < m; i++)
             if (nev[i] < 0) {
                 set_err("refinement: candidate %d exceeded the chirp bound", c0 + i);
-                goto done;
+                return -1;
             }
     }
-    rc = 0;
-done:
-    free(nev);
-    agpu_free(d_nh);
-    agpu_free(d_nev);
-    agpu_free(d_rin);
-    agpu_free(d_zin);
-    agpu_free(d_win);
-    agpu_free(d_rout);
-    agpu_free(d_zout);
-    agpu_free(d_wout);
-    agpu_free(d_der);
-    agpu_free(d_pow);
-    if (d_scr)
-        agpu_free(d_scr);
-    return rc;
+    return 0;
 }
 
 int accel_search_optimize(accel_search *s)
@@ -1079,7 +1121,9 @@ int accel_search_optimize(accel_search *s)
     }
     /* src/accel_utils.c:563-575: max_rzw_arr_harmonics when numw, else
      * max_rz_arr_harmonics */
-    int rc = refine_device(s->d_spec, g->numbins, n, nh, rin, zin, g->numw ? win : NULL,
+    if (!s->ws)
+        s->ws = (refine_ws *) calloc(1, sizeof(refine_ws));
+    int rc = refine_device(s->ws, s->d_spec, g->numbins, n, nh, rin, zin, g->numw ? win : NULL,
                            s->opt_r, s->opt_z, s->opt_w, s->opt_pows, s->opt_derivs, s->stream);
     if (rc == 0) {
         for (i = 0; i < n; i++) {
@@ -1159,8 +1203,11 @@ int accel_max_rz_harmonics(const accel_fcomplex *spec, long long numdata, int n,
         set_err("accel_max_rz_harmonics: %s", agpu_last_error());
         goto done;
     }
-    rc = refine_device(d_spec, numdata, n, numharm, rin, zin, NULL, rout, zout, NULL, pows,
+    refine_ws w;
+    memset(&w, 0, sizeof(w));
+    rc = refine_device(&w, d_spec, numdata, n, numharm, rin, zin, NULL, rout, zout, NULL, pows,
                        derivs, stream);
+    ws_free(&w);
 done:
     if (d_spec)
         agpu_free(d_spec);
@@ -1186,8 +1233,11 @@ int accel_max_rzw_harmonics(const accel_fcomplex *spec, long long numdata, int n
         set_err("accel_max_rzw_harmonics: %s", agpu_last_error());
         goto done;
     }
-    rc = refine_device(d_spec, numdata, n, numharm, rin, zin, win, rout, zout, wout, pows,
+    refine_ws w;
+    memset(&w, 0, sizeof(w));
+    rc = refine_device(&w, d_spec, numdata, n, numharm, rin, zin, win, rout, zout, wout, pows,
                        derivs, stream);
+    ws_free(&w);
 done:
     if (d_spec)
         agpu_free(d_spec);

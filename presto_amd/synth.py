@@ -46,3 +46,35 @@ def config2_series(N=1 << 23, dt=64e-6, seed=2):
                        duty=float(rng.uniform(0.02, 0.10)),
                        amp=float(rng.uniform(0.01, 0.05))))
     return make_series(N, dt, seed, ps), ps
+
+
+def make_series_f32(N, dt, seed, pulsars=()):
+    """As make_series, with the noise drawn directly in float32 (the batched
+    workload's generator: PCG64(seed) float32 normals)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = rng.standard_normal(N, dtype=np.float32)
+    for p in pulsars:
+        x += pulsar_series(N, dt, **p).astype(np.float32)
+    return x
+
+
+def config3_trials(n=256, seed0=3000):
+    """The batched DM-trial workload: n trials, lengths 2^23 / 2^22 / 2^21
+    samples (dt 64 / 128 / 256 us, so every trial spans 536.87 s) in a
+    2:1:1 ratio as the DM (trial index) grows and the data are downsampled;
+    seeded noise (seed0 + i); about 10 % of the trials carry one pulsar
+    whose amplitude peaks at trial i0 and falls off as a Gaussian of width
+    3 trials.  Yields (i, N, dt, seed, pulsars)."""
+    i0 = n // 6
+    for i in range(n):
+        if i < n // 2:
+            N, dt = 1 << 23, 64e-6
+        elif i < 3 * n // 4:
+            N, dt = 1 << 22, 128e-6
+        else:
+            N, dt = 1 << 21, 256e-6
+        pulsars = []
+        if abs(i - i0) <= max(1, int(0.05 * n)):
+            amp = 0.04 * float(np.exp(-0.5 * ((i - i0) / 3.0) ** 2))
+            pulsars = [dict(f0=150.7, z=30.0, duty=0.05, amp=amp)]
+        yield i, N, dt, seed0 + i, pulsars
