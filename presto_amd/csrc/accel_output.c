@@ -32,6 +32,8 @@ int accel_calc_props(const accel_rderivs *d, double r, double z, double w,
 {
     if (fabs(d->pow) < DBLCORRECT || d->locpow < DBLCORRECT)
         return -1;
+    /* the record is written whole to .cand files: clear its padding too */
+    memset(p, 0, sizeof(*p));
     const double pw = d->pow / d->locpow, d2 = d->d2pow / d->locpow;
     p->pur = sqrt(1.5 * fabs(d2) / pw) / PI;
     if (fabs(pw) < DBLCORRECT)
@@ -321,7 +323,7 @@ int accel_search_write_outputs(const accel_search *s, const char *accelnm, const
     const int jerk = g->numw != 0;
     accel_cand *cands = (accel_cand *) malloc(sizeof(accel_cand) * (n ? n : 1));
     accel_fourierprops *props =
-        (accel_fourierprops *) malloc(sizeof(accel_fourierprops) * (n ? n : 1));
+        (accel_fourierprops *) calloc(n ? n : 1, sizeof(accel_fourierprops));
     accel_rderivs der[ACCEL_MAXHARM];
     double hirs[ACCEL_MAXHARM], hizs[ACCEL_MAXHARM], hiws[ACCEL_MAXHARM];
     char *notes = (char *) malloc((size_t) 20 * (n ? n : 1) + 1);
