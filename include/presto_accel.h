@@ -97,6 +97,18 @@ const accel_geom *accel_search_geom(const accel_search *s);
 /* Copies the spectrum (numbins complex values, first one at bin lobin) to
  * HBM.  Replaces the mmap / .dat read of create_accelobs. */
 int accel_search_set_spectrum(accel_search *s, const accel_fcomplex *fft);
+/* Takes a time series of n samples (n a power of two, n/2 = numbins,
+ * lobin 0) and prepares its spectrum on the GPU as accelsearch prepares a
+ * .dat: realfft, deredden and bin 0 := 1+1i (src/accel_utils.c:1507-1562,
+ * 1621-1624). */
+int accel_search_set_series(accel_search *s, const float *series, long long n);
+/* Copies the first n bins of the search's current spectrum to the host. */
+int accel_search_spectrum(const accel_search *s, accel_fcomplex *out, long long n);
+/* The median blocks of deredden (src/misc_utils.c:940-1046): starts and
+ * lengths tiling [1, numbins); fills at most max entries, returns the
+ * block count or -1. */
+int accel_dered_blocks(long numbins, int startwidth, int endwidth, double endfreq, double T,
+                       long long *start, int *len, int max);
 /* Uses a spectrum already in device memory (not copied, not owned). */
 int accel_search_set_device_spectrum(accel_search *s, const void *d_fft);
 

@@ -67,6 +67,9 @@ _sig("accel_search_destroy", None, _p)
 _sig("accel_search_geom", _GP, _p)
 _sig("accel_search_set_spectrum", _i, _p, _p)
 _sig("accel_search_set_device_spectrum", _i, _p, _p)
+_sig("accel_search_set_series", _i, _p, _p, _ll)
+_sig("accel_search_spectrum", _i, _p, _p, _ll)
+_sig("accel_dered_blocks", _i, C.c_long, _i, _i, _d, _d, _p, _p, _i)
 _sig("accel_search_run", _i, _p)
 _sig("accel_search_launch", _i, _p)
 _sig("accel_search_collect", _i, _p)
@@ -120,6 +123,7 @@ _sig("accel_mt19937_raw", _i, C.c_uint, _p, _i)
 _sig("accel_realfft_forward", _i, _p, _ll, _p, _i)
 
 EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
+            "accel_search_set_series", "accel_search_spectrum", "accel_dered_blocks",
             "accel_search_destroy", "accel_search_geom", "accel_search_set_spectrum",
             "accel_search_set_device_spectrum", "accel_search_run", "accel_search_launch",
             "accel_search_collect", "accel_search_ncands", "accel_search_cands",
@@ -206,6 +210,18 @@ class Search:
             raise ValueError("spectrum length differs from params.numbins")
         if lib.accel_search_set_spectrum(self.h, fft.ctypes.data):
             raise _err()
+
+    def set_series(self, x):
+        """Time series in; realfft + deredden on the GPU."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if lib.accel_search_set_series(self.h, x.ctypes.data, len(x)):
+            raise _err()
+
+    def spectrum(self):
+        out = np.zeros(self.geom.numbins, dtype=np.complex64)
+        if lib.accel_search_spectrum(self.h, out.ctypes.data, len(out)):
+            raise _err()
+        return out
 
     def set_device_spectrum(self, dptr):
         if lib.accel_search_set_device_spectrum(self.h, C.c_void_p(int(dptr))):

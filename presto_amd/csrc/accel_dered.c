@@ -152,6 +152,39 @@ long accel_deredden(accel_fcomplex *fft, long numbins, int startwidth, int endwi
 /* After realfft of a .dat of N samples (dt each): de-redden with the
  * window growing to 200 bins at Nyquist, then bin 0 := 1 + 1i
  * (src/accel_utils.c:1561, 1621-1624). */
+int accel_dered_blocks(long numbins, int startwidth, int endwidth, double endfreq, double T,
+                       long long *start, int *len, int max)
+{
+    if (numbins < 1 + startwidth || startwidth < 1 || endwidth < startwidth)
+        return -1;
+    const float Tf = (float) T;
+    long rpos = 1, binnum = 1;
+    int nb = 0;
+    int bufflen = startwidth;
+    for (;;) {
+        long avail = numbins - rpos;
+        int n = (int) (avail < bufflen ? avail : bufflen);
+        if (n <= 0)
+            break;
+        if (nb < max) {
+            start[nb] = rpos;
+            len[nb] = n;
+        }
+        nb++;
+        rpos += n;
+        binnum += n;
+        /* the window growth of src/misc_utils.c:980-984 and :1036-1042 */
+        if (nb == 1 || (float) binnum / Tf < endfreq) {
+            bufflen = startwidth * log(binnum);
+            if (bufflen > endwidth)
+                bufflen = endwidth;
+        } else {
+            bufflen = endwidth;
+        }
+    }
+    return nb;
+}
+
 long accel_prepare_dat_spectrum(accel_fcomplex *fft, long numbins, double N, double dt)
 {
     long n = accel_deredden(fft, numbins, 6, 200, 0.5 / dt, dt * N);

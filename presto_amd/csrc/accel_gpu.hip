@@ -219,20 +219,18 @@ __device__ static unsigned radix_select(const unsigned *keys, int n, int k, unsi
     return prefix;
 }
 
-__global__ __launch_bounds__(256) void k_block_norms(const float2 *spec, long long numbins,
-                                                     long long lobin_file,
-                                                     const agpu_window *win, double *norm,
-                                                     float *nsq)
+/* Exact median of the FP32 powers |x|^2 of spectrum bins [lo, lo+n)
+ * (lo relative to the array; bins outside [0, numbins) count as 0), as
+ * PRESTO's median(): the middle order statistic, or for even n the mean of
+ * the two middle ones formed in FP64 and rounded to FP32.  Called by the
+ * whole workgroup; keys holds n entries of dynamic LDS. */
+__device__ static float block_median(const float2 *spec, long long numbins, long long lo, int n,
+                                     unsigned *keys, unsigned *hist, unsigned *sh)
 {
     /* the reference forms r*r + i*i with two roundings (no FMA) */
 #pragma clang fp contract(off)
-    extern __shared__ unsigned keys[];
-    __shared__ unsigned hist[256];
-    __shared__ unsigned sh[8];
-    const agpu_window w = win[blockIdx.x];
-    const int n = w.numdata;
     for (int i = threadIdx.x; i < n; i += 256) {
-        long long b = w.lobin + i - lobin_file;
+        long long b = lo + i;
         float p = 0.0f;
         if (b >= 0 && b < numbins) {
             float2 v = spec[b];
@@ -265,11 +263,110 @@ __global__ __launch_bounds__(256) void k_block_norms(const float2 *spec, long lo
         const unsigned kb = (sh[4] >= (unsigned) rhs + 1) ? ka : sh[5];
         med = (float) (0.5 * ((double) __uint_as_float(ka) + (double) __uint_as_float(kb)));
     }
+    __syncthreads();
+    return med;
+}
+
+__global__ __launch_bounds__(256) void k_block_norms(const float2 *spec, long long numbins,
+                                                     long long lobin_file,
+                                                     const agpu_window *win, double *norm,
+                                                     float *nsq)
+{
+    extern __shared__ unsigned keys[];
+    __shared__ unsigned hist[256];
+    __shared__ unsigned sh[8];
+    const agpu_window w = win[blockIdx.x];
+    const float med = block_median(spec, numbins, w.lobin - lobin_file, w.numdata, keys, hist, sh);
     if (threadIdx.x == 0) {
         double nv = 1.0 / sqrt((double) med / 0.69314718055994530942);
         norm[blockIdx.x] = nv;
         nsq[blockIdx.x] = (float) (nv * nv);
     }
+}
+
+/* De-reddening (src/misc_utils.c:912-1067), block medians: mean_b =
+ * median(|x|^2 of block b) / ln 2, rounded to FP32 as the reference. */
+__global__ __launch_bounds__(256) void k_dered_means(const float2 *spec, long long numbins,
+                                                     const agpu_window *blk, float *mean)
+{
+    extern __shared__ unsigned keys[];
+    __shared__ unsigned hist[256];
+    __shared__ unsigned sh[8];
+    const agpu_window w = blk[blockIdx.x];
+    const float med = block_median(spec, numbins, w.lobin, w.numdata, keys, hist, sh);
+    if (threadIdx.x == 0)
+        mean[blockIdx.x] = (float) ((double) med / 0.69314718055994530942);
+}
+
+/* invsqrtf of src/misc_utils.c:291-302: one Newton step from the
+ * 0x5f3759df estimate, in FP32 without fused operations. */
+__device__ static float dered_invsqrt(float x)
+{
+#pragma clang fp contract(off)
+    float y = __int_as_float(0x5f3759df - (__float_as_int(x) >> 1));
+    return y * (1.5f - 0.5f * x * y * y);
+}
+
+/* De-reddening, scaling: bin 0 -> (1, 0); the first half of block 0 by
+ * invsqrt(mean_0); then each span [middle of block b, middle of block
+ * b+1) by invsqrt(mean_b + dslope_b ii), ii counted from the span start,
+ * dslope_b = (mean_{b+1} - mean_b) / ((n_b + n_{b+1}) / 2); the last
+ * half-block continues with the last slope (1 if there is one block). */
+__global__ void k_dered_apply(float2 *spec, long long numbins, const agpu_window *blk,
+                              const float *mean, int nblk)
+{
+#pragma clang fp contract(off)
+    const long long k = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= numbins)
+        return;
+    if (k == 0) {
+        spec[0] = make_float2(1.0f, 0.0f);
+        return;
+    }
+    const long long mid0 = blk[0].lobin + blk[0].numdata / 2;
+    float norm;
+    if (k < mid0) {
+        norm = dered_invsqrt(mean[0]);
+    } else {
+        /* the span holding k: largest b with lobin_b + n_b/2 <= k */
+        int lo = 0, hi = nblk - 1;
+        while (lo < hi) {
+            int m = (lo + hi + 1) >> 1;
+            if (blk[m].lobin + blk[m].numdata / 2 <= k)
+                lo = m;
+            else
+                hi = m - 1;
+        }
+        const int b = lo;
+        const int ii = (int) (k - (blk[b].lobin + blk[b].numdata / 2));
+        float dslope;
+        if (b + 1 < nblk)
+            dslope = (float) ((mean[b + 1] - mean[b]) /
+                              (0.5 * (blk[b].numdata + blk[b + 1].numdata)));
+        else if (nblk > 1)
+            dslope = (float) ((mean[b] - mean[b - 1]) /
+                              (0.5 * (blk[b - 1].numdata + blk[b].numdata)));
+        else
+            dslope = 1.0f;
+        norm = dered_invsqrt(mean[b] + dslope * (float) ii);
+    }
+    float2 v = spec[k];
+    spec[k] = make_float2(v.x * norm, v.y * norm);
+}
+
+int agpu_deredden(void *d_spec, long long numbins, const agpu_window *d_blk, int nblk,
+                  int maxn, float *d_mean, void *stream)
+{
+    hipStream_t st = (hipStream_t) stream;
+    if (nblk <= 0 || maxn <= 0 || maxn > NORM_MAXN)
+        return fail("agpu_deredden: bad block table");
+    hipLaunchKernelGGL(k_dered_means, dim3(nblk), dim3(256), sizeof(unsigned) * maxn, st,
+                       (const float2 *) d_spec, numbins, d_blk, d_mean);
+    AGPU_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_dered_apply, dim3((unsigned) ((numbins + 255) / 256)), dim3(256), 0, st,
+                       (float2 *) d_spec, numbins, d_blk, (const float *) d_mean, nblk);
+    AGPU_CHECK(hipGetLastError());
+    return 0;
 }
 
 /* In-memory path: norm^2 of the fundamental block that last wrote each

@@ -466,6 +466,11 @@ struct accel_search {
     double *opt_pows;                 /* [nopt][ACCEL_MAXHARM] */
     accel_rderivs *opt_derivs;        /* [nopt][ACCEL_MAXHARM] */
     refine_ws *ws;                    /* refinement device buffers */
+    /* time-series input on the device (accel_search_set_series) */
+    void *d_fftwork;
+    agpu_window *d_dblk;
+    float *d_dmean;
+    int ndblk, dmaxn;
 };
 
 const accel_geom *accel_search_geom(const accel_search *s) { return &s->g; }
@@ -680,6 +685,9 @@ void accel_search_destroy(accel_search *s)
         ws_free(s->ws);
         free(s->ws);
     }
+    agpu_free(s->d_fftwork);
+    agpu_free(s->d_dblk);
+    agpu_free(s->d_dmean);
     free(s);
 }
 
@@ -1275,4 +1283,68 @@ done:
     if (stream)
         agpu_stream_destroy(stream);
     return rc;
+}
+
+int accel_search_set_series(accel_search *s, const float *series, long long n)
+{
+    const accel_geom *g = &s->g;
+    if (n < 4 || (n & (n - 1)) || n / 2 != g->numbins || g->lobin != 0) {
+        set_err("accel_search_set_series: %lld samples do not give the %lld-bin spectrum "
+                "of this search (power-of-two length, lobin 0)", n, g->numbins);
+        return -1;
+    }
+    if (!s->own_spec) {
+        GPU(agpu_malloc(&s->d_spec, sizeof(accel_fcomplex) * (size_t) g->numbins));
+        s->own_spec = 1;
+    }
+    if (!s->d_fftwork)
+        GPU(agpu_malloc(&s->d_fftwork, sizeof(accel_fcomplex) * (size_t) (3 * n / 4)));
+    if (!s->d_dblk) {
+        /* src/accel_utils.c:1561: deredden(fft, numbins, 6, 200, 0.5/dt, dt N) */
+        const double T = g->dt * (double) n;
+        int nb = accel_dered_blocks((long) g->numbins, 6, 200, 0.5 / g->dt, T, NULL, NULL, 0);
+        if (nb <= 0) {
+            set_err("accel_search_set_series: spectrum too short to de-redden");
+            return -1;
+        }
+        long long *st = (long long *) malloc(sizeof(long long) * nb);
+        int *ln = (int *) malloc(sizeof(int) * nb);
+        accel_dered_blocks((long) g->numbins, 6, 200, 0.5 / g->dt, T, st, ln, nb);
+        agpu_window *w = (agpu_window *) calloc(nb, sizeof(agpu_window));
+        int maxn = 0;
+        for (int i = 0; i < nb; i++) {
+            w[i].lobin = st[i];
+            w[i].numdata = ln[i];
+            if (ln[i] > maxn)
+                maxn = ln[i];
+        }
+        int rc = agpu_malloc((void **) &s->d_dblk, sizeof(agpu_window) * nb) ||
+                 agpu_malloc((void **) &s->d_dmean, sizeof(float) * nb) ||
+                 agpu_h2d(s->d_dblk, w, sizeof(agpu_window) * nb, s->stream);
+        free(st);
+        free(ln);
+        free(w);
+        if (rc) {
+            set_err("accel_search_set_series: %s", agpu_last_error());
+            return -1;
+        }
+        s->ndblk = nb;
+        s->dmaxn = maxn;
+    }
+    /* realfft (src/accel_utils.c:1547), deredden (:1561), bin 0 := 1+1i
+     * (:1621-1624), all on the device */
+    static const accel_fcomplex one = {1.0f, 1.0f};
+    GPU(agpu_h2d(s->d_spec, series, sizeof(float) * (size_t) n, s->stream));
+    GPU(agpu_realfft((const float *) s->d_spec, n, s->d_spec, s->d_fftwork, s->stream));
+    GPU(agpu_deredden(s->d_spec, g->numbins, s->d_dblk, s->ndblk, s->dmaxn, s->d_dmean,
+                      s->stream));
+    GPU(agpu_h2d(s->d_spec, &one, sizeof(one), s->stream));
+    return 0;
+}
+
+int accel_search_spectrum(const accel_search *s, accel_fcomplex *out, long long n)
+{
+    long long m = n < s->g.numbins ? n : s->g.numbins;
+    GPU(agpu_d2h(out, s->d_spec, sizeof(accel_fcomplex) * (size_t) m, s->stream));
+    return 0;
 }

@@ -125,6 +125,12 @@ int agpu_harmsum(const agpu_harmsum_args *a, void *stream);
 int agpu_inmem_nsq_table(const float *d_nsq, int npre, int nblocks, long long rh0, int cu,
                          long long hb_end, float *d_out, void *stream);
 
+/* De-reddening of a device spectrum in place (src/misc_utils.c:912-1067):
+ * d_blk = the reference's median blocks {first bin, length} tiling
+ * [1, numbins), maxn = the longest; d_mean receives nblk floats. */
+int agpu_deredden(void *d_spec, long long numbins, const agpu_window *d_blk, int nblk,
+                  int maxn, float *d_mean, void *stream);
+
 /* Forward real FFT of n (a power of two) FP32 samples in PRESTO's packing
  * into d_out (n/2 complex); d_work holds 3n/4 complex values. */
 int agpu_realfft(const float *d_x, long long n, void *d_out, void *d_work, void *stream);

@@ -88,7 +88,8 @@ int agpu_realfft(const float *d_x, long long n, void *d_out, void *d_work, void 
     hipLaunchKernelGGL(k_fft_twiddles, dim3((half + FFT_TPB - 1) / FFT_TPB), blk, 0, st, tw,
                        half, M);
     FFT_CHECK(hipGetLastError());
-    FFT_CHECK(hipMemcpyAsync(a, d_x, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    if ((const void *) d_x != (const void *) a)
+        FFT_CHECK(hipMemcpyAsync(a, d_x, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
     float2 *src = a, *dst = b;
     for (long long Ns = 1; Ns < M; Ns <<= 1) {
         hipLaunchKernelGGL(k_fft_pass, dim3((half + FFT_TPB - 1) / FFT_TPB), blk, 0, st, src,
