@@ -256,6 +256,10 @@ int accel_calc_rzwerrs(const accel_fourierprops *p, double T, accel_rzwerrs *e);
  * (numerr 2); returns the string length or -1. */
 int accel_nice_output(char *out, double val, double err, int len, int numerr);
 
+/* Device memory (bytes) a search handle for p needs, including series
+ * input and refinement buffers (for admission by memory footprint). */
+long long accel_search_footprint(const accel_params *p);
+
 /* ---- batches of DM-trial spectra ---- */
 
 /* One spectrum of a batch: inputs, what to do with it, and its results. */
@@ -286,6 +290,32 @@ int accel_batch_handles(const accel_batch *b);
  * output of item i.  Results equal separate runs.  Returns 0, or -1 when
  * any item failed (its status says which). */
 int accel_batch_run(accel_batch *b, accel_batch_item *items, int n);
+
+/* ---- a request loop (serving) ---- */
+
+typedef struct accel_server accel_server;
+
+/* Starts `workers` threads serving requests on one device within
+ * hbm_budget bytes of device memory.  No counterpart in the reference
+ * (one accelsearch process per input, src/accelsearch.c:82-313). */
+accel_server *accel_server_create(int device, long long hbm_budget, int workers);
+/* Queues a time series of n samples (copied) for an accelsearch with
+ * parameters p (numbins = n/2); optimize: refine the candidates.  Requests
+ * are admitted in arrival order when their memory fits beside the running
+ * ones, and run concurrently on their own streams.  Returns a ticket. */
+int accel_server_submit(accel_server *sv, const accel_params *p, const float *series,
+                        long long n, int optimize);
+/* Waits for a request; latency = submit to done, service = start to done
+ * (milliseconds).  Returns its status (0 or -1). */
+int accel_server_wait(accel_server *sv, int ticket, double *latency_ms, double *service_ms);
+/* Results of a finished request: candidates (returns the count) and, when
+ * refined, the fundamentals' fourierprops. */
+int accel_server_cands(accel_server *sv, int ticket, accel_cand *out, int n);
+int accel_server_props(accel_server *sv, int ticket, accel_fourierprops *out, int n);
+/* Device memory held by live handles (running and cached). */
+long long accel_server_memory(accel_server *sv);
+/* Finishes every queued request, then stops the workers. */
+void accel_server_destroy(accel_server *sv);
 
 const char *accel_last_error(void);
 

@@ -138,7 +138,10 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_calc_rzwerrs", "accel_nice_output", "accel_read_birdies",
             "accel_zap_birdies", "accel_mt19937_raw", "accel_realfft_forward",
             "accel_batch_create", "accel_batch_destroy", "accel_batch_handles",
-            "accel_batch_run", "accel_last_error"]
+            "accel_batch_run", "accel_search_footprint", "accel_server_create",
+            "accel_server_submit", "accel_server_wait", "accel_server_cands",
+            "accel_server_props", "accel_server_memory", "accel_server_destroy",
+            "accel_last_error"]
 
 
 class AccelError(RuntimeError):
@@ -380,6 +383,76 @@ class Batch:
             raise AccelError(f"batch items {bad[:8]} failed: {lib.accel_last_error().decode()}")
         return ([o[:min(it.ncands, maxcands)].copy() for o, it in zip(outs, items)],
                 [it.gpu_ms for it in items])
+
+
+_sig("accel_search_footprint", _ll, _PP)
+_sig("accel_server_create", _p, _i, _ll, _i)
+_sig("accel_server_submit", _i, _p, _PP, _p, _ll, _i)
+_sig("accel_server_wait", _i, _p, _i, C.POINTER(_d), C.POINTER(_d))
+_sig("accel_server_cands", _i, _p, _i, _p, _i)
+_sig("accel_server_props", _i, _p, _i, _p, _i)
+_sig("accel_server_memory", _ll, _p)
+_sig("accel_server_destroy", None, _p)
+
+
+def footprint(params):
+    """Device bytes a search with these parameters needs."""
+    return lib.accel_search_footprint(C.byref(params))
+
+
+class Server:
+    """The request loop: submit time series as they arrive, wait for each;
+    requests run concurrently when their memory fits the budget."""
+
+    def __init__(self, device=0, budget_bytes=200 << 30, workers=4):
+        self.h = lib.accel_server_create(int(device), int(budget_bytes), int(workers))
+        if not self.h:
+            raise AccelError("cannot start the server")
+
+    def submit(self, params, series, optimize=True):
+        x = np.ascontiguousarray(series, dtype=np.float32)
+        t = lib.accel_server_submit(self.h, C.byref(params), x.ctypes.data, len(x), int(optimize))
+        if t < 0:
+            raise AccelError("submit failed (bad parameters)")
+        return t
+
+    def wait(self, ticket):
+        """(status, latency_ms, service_ms)"""
+        lat, svc = _d(), _d()
+        st = lib.accel_server_wait(self.h, ticket, C.byref(lat), C.byref(svc))
+        return st, lat.value, svc.value
+
+    def candidates(self, ticket):
+        n = lib.accel_server_cands(self.h, ticket, None, 0)
+        out = np.zeros(max(n, 0), dtype=CAND_DTYPE)
+        if n > 0:
+            lib.accel_server_cands(self.h, ticket, out.ctypes.data, n)
+        return out
+
+    def props(self, ticket):
+        n = lib.accel_server_props(self.h, ticket, None, 0)
+        if n <= 0:
+            return []
+        out = (FourierProps * n)()
+        lib.accel_server_props(self.h, ticket, out, n)
+        return list(out)
+
+    def memory(self):
+        return lib.accel_server_memory(self.h)
+
+    def close(self):
+        if self.h:
+            lib.accel_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
 
 def max_rz_harmonics(spec, r, z, numharm, device=0):

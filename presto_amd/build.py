@@ -18,7 +18,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall",
            "-I", CSRC, "-I", os.path.join(ROOT, "include")]
 HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip", "fft_gpu.hip"]
-C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c", "accel_output.c", "accel_batch.c"]
+C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c", "accel_output.c", "accel_batch.c",
+         "accel_server.c"]
 
 
 def run(cmd):
@@ -41,7 +42,8 @@ def build(verbose=False):
         run(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+        ["-lm", "-lpthread"])
     os.replace(tmp, LIB)
     # the accelsearch command, linked against the library next to it
     cli_obj = os.path.join(BUILD, "accelsearch_cli.o")

@@ -22,7 +22,7 @@
 #include <string.h>
 #include "agpu.h"
 
-static char g_err[512];
+static thread_local char g_err[512];   /* per host thread */
 
 #define AGPU_CHECK(call)                                                          \
     do {                                                                          \

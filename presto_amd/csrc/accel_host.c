@@ -40,7 +40,7 @@
 #define CORR_L 4096
 #define HIT_CAP0 (1u << 20)
 
-static char g_err[1024];
+static __thread char g_err[1024];   /* per host thread */
 
 static void set_err(const char *fmt, ...)
 {
@@ -1347,4 +1347,27 @@ int accel_search_spectrum(const accel_search *s, accel_fcomplex *out, long long 
     long long m = n < s->g.numbins ? n : s->g.numbins;
     GPU(agpu_d2h(out, s->d_spec, sizeof(accel_fcomplex) * (size_t) m, s->stream));
     return 0;
+}
+
+long long accel_search_footprint(const accel_params *p)
+{
+    accel_geom g;
+    if (accel_geom_create(p, &g))
+        return -1;
+    const long long numw1 = g.numw ? g.numw : 1, nt = numw1 * g.numz;
+    const long long hb_end = (long long) (2.0 * g.rlo) + (long long) g.nblocks * g.corr_uselen;
+    const long long ntiles = (hb_end + AGPU_TILE - 1) / AGPU_TILE + 1;
+    const int nfrac = (1 << (g.numharmstages - 1)) - 1;
+    long long bytes = 4LL * nt * ntiles * AGPU_ROWLEN;        /* raw plane */
+    bytes += 8LL * g.numbins;                                 /* spectrum */
+    bytes += 8LL * nt * CORR_L;                               /* templates */
+    bytes += 24LL * (long long) (g.nblocks + 64) * (nfrac + 1); /* windows, norms */
+    if (g.inmem)
+        bytes += 4LL * ntiles * AGPU_TILE;                    /* per-half-bin norm^2 */
+    bytes += 24LL * HIT_CAP0;                                 /* hit buffer */
+    bytes += 12LL * g.numbins;                                /* FFT work of series input */
+    bytes += 64LL << 20;                                      /* refinement, tables, slack */
+    if (g.numw)   /* jerk refinement scratch for a full chunk of candidates */
+        bytes += 16LL * 24 * REFINE_CHUNK * rzw_npts_bound(1, g.zhi, g.whi);
+    return bytes;
 }

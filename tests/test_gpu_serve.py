@@ -1,0 +1,66 @@
+"""The request loop (accel_server_*): requests of mixed sizes and search
+parameters give the same candidates and refined properties whether each is
+served alone or together with the others, and admission by memory keeps
+the device within the budget."""
+import numpy as np
+import pytest
+
+from presto_amd import synth
+
+pytestmark = pytest.mark.gpu
+A = pytest.importorskip("presto_amd.accel")
+
+
+def _requests():
+    spec = [(1 << 15, 1e-3, 0, None, 4), (1 << 16, 5e-4, 20, None, 8), (1 << 17, 2.5e-4, 50, None, 8),
+            (1 << 16, 5e-4, 20, 40, 4), (1 << 15, 1e-3, 50, None, 2), (1 << 17, 2.5e-4, 0, None, 8),
+            (1 << 16, 5e-4, 20, None, 8), (1 << 16, 5e-4, 10, 20, 2)]
+    out = []
+    for i, (N, dt, zmax, wmax, nh) in enumerate(spec):
+        ps = [dict(f0=41.3 + 3 * i, z=2.0 * i - 6, duty=0.06, amp=0.4)] if i % 3 != 2 else []
+        x = synth.make_series(N, dt, 900 + i, ps)
+        p = A.make_params(N // 2, N, dt, zmax=zmax, numharm=nh, sigma=2.0, wmax=wmax)
+        out.append((p, x))
+    return out
+
+
+def _serve(reqs, workers, budget, together):
+    res = []
+    with A.Server(budget_bytes=budget, workers=workers) as sv:
+        if together:
+            tickets = [sv.submit(p, x) for p, x in reqs]
+            for t in tickets:
+                st, lat, svc = sv.wait(t)
+                assert st == 0
+                res.append((sv.candidates(t), sv.props(t)))
+        else:
+            for p, x in reqs:
+                t = sv.submit(p, x)
+                st, lat, svc = sv.wait(t)
+                assert st == 0
+                res.append((sv.candidates(t), sv.props(t)))
+    return res
+
+
+def test_results_equal_alone_and_in_the_mix():
+    print("case: 8 mixed requests, alone vs 4 workers together", flush=True)
+    reqs = _requests()
+    big = max(A.footprint(p) for p, _ in reqs)
+    alone = _serve(reqs, 1, 64 << 30, together=False)
+    mixed = _serve(reqs, 4, 64 << 30, together=True)
+    tight = _serve(reqs, 4, big + (64 << 20), together=True)   # one big request at a time
+    assert sum(len(c) for c, _ in alone) >= 4
+    for (ca, pa), (cm, pm), (ct, pt) in zip(alone, mixed, tight):
+        assert np.array_equal(ca, cm) and np.array_equal(ca, ct)
+        assert len(pa) == len(pm) == len(pt)
+        for a, b, c in zip(pa, pm, pt):
+            assert bytes(a) == bytes(b) == bytes(c)
+
+
+def test_request_larger_than_the_budget_fails_cleanly():
+    p, x = _requests()[2]
+    with A.Server(budget_bytes=A.footprint(p) // 2, workers=2) as sv:
+        t = sv.submit(p, x)
+        st, _, _ = sv.wait(t)
+        assert st == -1
+        assert sv.memory() == 0
