@@ -1050,7 +1050,7 @@ static int refine_device(refine_ws *w, const void *d_spec, long long numdata, in
         return -1;
     }
     const int chunk = n < REFINE_CHUNK ? n : REFINE_CHUNK;
-    if (ws_reserve(w, win ? (size_t) 16 * 24 * (size_t) cap * chunk : 0))
+    if (ws_reserve(w, win ? (size_t) 16 * 26 * (size_t) cap * chunk : 0))
         return -1;
     int nev[REFINE_CHUNK];
     for (int c0 = 0; c0 < n; c0 += chunk) {
@@ -1368,6 +1368,6 @@ long long accel_search_footprint(const accel_params *p)
     bytes += 12LL * g.numbins;                                /* FFT work of series input */
     bytes += 64LL << 20;                                      /* refinement, tables, slack */
     if (g.numw)   /* jerk refinement scratch for a full chunk of candidates */
-        bytes += 16LL * 24 * REFINE_CHUNK * rzw_npts_bound(1, g.zhi, g.whi);
+        bytes += 16LL * 26 * REFINE_CHUNK * rzw_npts_bound(1, g.zhi, g.whi);
     return bytes;
 }

@@ -151,7 +151,7 @@ typedef struct {
     double *derivs;
     double *powers;
     int *nevals;
-    /* jerk refinement only: ncand x 24 x npts_cap complex FP64 scratch */
+    /* jerk refinement only: ncand x 26 x npts_cap complex FP64 scratch */
     void *scratch;
     int npts_cap;
 } agpu_refine_args;

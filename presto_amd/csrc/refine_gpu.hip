@@ -477,13 +477,63 @@ __device__ static double2 window_dft(const float2 *spec, long long numdata, long
     return acc;
 }
 
+/* Forward DFT (e^{-2 pi i}) of a[0..npts), npts a power of two, by the
+ * whole workgroup: radix-2 Stockham passes between a and b in global
+ * memory (the candidate's scratch), FP64 twiddles.  Returns the buffer
+ * holding the result. */
+__device__ static double2 *wg_fft64(double2 *a, double2 *b, int npts)
+{
+    const int half = npts >> 1;
+    double2 *src = a, *dst = b;
+    for (int Ns = 1; Ns < npts; Ns <<= 1) {
+        for (int j = threadIdx.x; j < half; j += RF_TPB) {
+            const int k = j & (Ns - 1);
+            double s, c;
+            sincospi(-(double) k / (double) Ns, &s, &c);
+            const double2 x0 = src[j], x1 = src[j + half];
+            const double2 y = make_double2(x1.x * c - x1.y * s, x1.x * s + x1.y * c);
+            const int d = ((j - k) << 1) + k;
+            dst[d] = make_double2(x0.x + y.x, x0.y + y.y);
+            dst[d + Ns] = make_double2(x0.x - y.x, x0.y - y.y);
+        }
+        __syncthreads();
+        double2 *t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+/* S(n) = sum_{m=mlo}^{mhi-1} d[k0+m] e^{+2 pi i m n / npts} for all n <
+ * npts (bins outside [0, numdata) as zero; mhi - mlo <= npts), through
+ * wg_fft64 of the conjugated window: S = conj(FFT(conj(window))).  The
+ * result is left conjugated in the returned buffer (read it with conj). */
+__device__ static double2 *window_spectrum(const float2 *spec, long long numdata, long long k0,
+                                           int mlo, int mhi, int npts, double2 *a, double2 *b)
+{
+    for (int j = threadIdx.x; j < npts; j += RF_TPB)
+        a[j] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int m = mlo + (int) threadIdx.x; m < mhi; m += RF_TPB) {
+        const long long k = k0 + m;
+        if (k < 0 || k >= numdata)
+            continue;
+        const float2 d = spec[k];
+        const int j = m >= 0 ? m : m + npts;
+        a[j] = make_double2(d.x, -d.y);
+    }
+    __syncthreads();
+    return wg_fft64(a, b, npts);
+}
+
 /* rzw_interp (src/rzwinterp.c:145-230) by one workgroup.  With R the
  * response of gen_w_response(frac, 1, z, w, 2 khw) -- the DFT of the
  * chirp x at bins npts/4 - khw + kk -- the sum over data bins
  * sum_k d_k conj(R) regroups into sum_n x_n i^n S(n), S the DFT of the
  * data window around intfreq. */
 __device__ static double2 rzw_interp_wg(const float2 *spec, long long numdata, double r,
-                                        double z, double w, int khw, double2 *red)
+                                        double z, double w, int khw, double2 *red,
+                                        double2 *fa, double2 *fb, int npts_cap, int *bad)
 {
     if (r > numdata - 1.0 || r < 0.0)
         return make_double2(0.0, 0.0);
@@ -494,19 +544,27 @@ __device__ static double2 rzw_interp_wg(const float2 *spec, long long numdata, d
     const long long intfreq = (long long) dint;
     const int numkern = 2 * khw;
     const int npts = (int) next2_dev(6 * wr_halfwidth(z, w, 0) + 200 + numkern);
+    if (npts > npts_cap) {
+        *bad = 1;
+        return make_double2(0.0, 0.0);
+    }
+    const double2 *S = window_spectrum(spec, numdata, intfreq, -khw, khw, npts, fa, fb);
     double2 acc = make_double2(0.0, 0.0);
     for (int n = threadIdx.x; n < npts; n += RF_TPB) {
         const double x = chirp_x(n, npts, frac, z, w);
-        const double2 S = window_dft(spec, numdata, intfreq, -khw, khw, n, npts);
+        const double2 Sc = S[n];   /* conj of S(n) */
         const double2 ph = ipow(n);
-        acc.x += x * (ph.x * S.x - ph.y * S.y);
-        acc.y += x * (ph.x * S.y + ph.y * S.x);
+        /* x i^n S(n), S(n) = (Sc.x, -Sc.y) */
+        acc.x += x * (ph.x * Sc.x + ph.y * Sc.y);
+        acc.y += x * (ph.y * Sc.x - ph.x * Sc.y);
     }
+    __syncthreads();   /* fa/fb are reused by the next call */
     return wg_sum(acc, red);
 }
 
 __device__ static double localpower3d_wg(const float2 *spec, long long numdata, double r,
-                                         double z, double w, double2 *red)
+                                         double z, double w, double2 *red, double2 *fa,
+                                         double2 *fb, int npts_cap, int *bad)
 {
     const int khw = wr_halfwidth(z, w, 0);
     double lo1 = r - 5 - 10, hi1 = lo1 + 10;
@@ -518,18 +576,19 @@ __device__ static double localpower3d_wg(const float2 *spec, long long numdata, 
     hi2 = hi2 < 0.0 ? 0.0 : (hi2 > nd ? nd : hi2);
     double sum = 0.0;
     for (double f = lo1; f < hi1; f += 1.0) {
-        double2 a = rzw_interp_wg(spec, numdata, f, z, w, khw, red);
+        double2 a = rzw_interp_wg(spec, numdata, f, z, w, khw, red, fa, fb, npts_cap, bad);
         sum += a.x * a.x + a.y * a.y;
     }
     for (double f = lo2; f < hi2; f += 1.0) {
-        double2 a = rzw_interp_wg(spec, numdata, f, z, w, khw, red);
+        double2 a = rzw_interp_wg(spec, numdata, f, z, w, khw, red, fa, fb, npts_cap, bad);
         sum += a.x * a.x + a.y * a.y;
     }
     return sum / 20.0;
 }
 
 __device__ static void derivs3d_wg(const float2 *spec, long long numdata, double r, double z,
-                                   double w, double locpow, double *out7, double2 *red)
+                                   double w, double locpow, double *out7, double2 *red,
+                                   double2 *fa, double2 *fb, int npts_cap, int *bad)
 {
     const double PI = 3.14159265358979323846, TWOPI = 6.28318530717958647692;
     const double h = 0.003, twoh = 0.006, twohsqrd = 0.000036;
@@ -537,7 +596,7 @@ __device__ static void derivs3d_wg(const float2 *spec, long long numdata, double
     double pwr[5], phs[5];
     double f = r - twoh;
     for (int k = 0; k < 5; k++, f += h) {
-        double2 a = rzw_interp_wg(spec, numdata, f, z, w, khw, red);
+        double2 a = rzw_interp_wg(spec, numdata, f, z, w, khw, red, fa, fb, npts_cap, bad);
         pwr[k] = a.x * a.x + a.y * a.y;
         double t = atan2(a.y, a.x);
         phs[k] = t > 0.0 ? t : t + TWOPI;
@@ -558,6 +617,7 @@ __device__ static void derivs3d_wg(const float2 *spec, long long numdata, double
 #define RZW_NZ 17    /* 2 * numwid * interpfac + 1 */
 #define RZW_NB 8     /* interpfac * ACCEL_RDR */
 #define RZW_NC 7     /* data windows c = -1 .. 5 */
+#define RZW_SCR (RZW_NR + 2)   /* scratch rows per candidate: Q tables + 2 FFT buffers */
 
 /* Output j (in 1/8 bins from lobin) correlates the data bins
  * [c - khw, c + khw) around lobin, c = ceil(j / 8).  j in [-8, 47]: a
@@ -609,7 +669,9 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
     const float2 *spec = (const float2 *) a.spec;
     const long long numdata = a.numdata;
     const double rin = a.rin[ci], zin = a.zin[ci], win = a.win[ci];
-    double2 *Q = (double2 *) a.scratch + (size_t) ci * RZW_NR * a.npts_cap;
+    /* scratch of this candidate: Q tables [RZW_NR][cap], two FFT buffers */
+    double2 *Q = (double2 *) a.scratch + (size_t) ci * RZW_SCR * a.npts_cap;
+    double2 *fa = Q + (size_t) RZW_NR * a.npts_cap, *fb = fa + a.npts_cap;
     const double dr = 1.0 / RZW_NB;
     for (int i = threadIdx.x; i < RZW_NZ * RZW_NZ * RZW_NR; i += RF_TPB)
         powsum[i] = 0.0f;
@@ -620,7 +682,8 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
         const int n = H - hind;
         const double rh = rin * n, zh = zin * n, wh = win * n;
         const double hfrac = n / (double) H;
-        const float locpow = (float) localpower3d_wg(spec, numdata, rh, zh, wh, red);
+        const float locpow =
+            (float) localpower3d_wg(spec, numdata, rh, zh, wh, red, fa, fb, a.npts_cap, &bad);
         double rh_int;
         modf(rh, &rh_int);
         const long long lobin = (long long) rh_int - 1;
@@ -678,9 +741,11 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
                         /* Q_kk(n) = i^n e^{-2 pi i j n / (8 npts)} S_c(n), S_c the
                          * DFT of the data window [c - khw, c + khw) around lobin,
                          * c = ceil(j / 8) in [-1, 5] */
+                        const double2 *D = window_spectrum(spec, numdata, lobin, -1 - khw,
+                                                           khw - 1, npts, fa, fb);
                         for (int nn = threadIdx.x; nn < npts; nn += RF_TPB) {
                             double2 S[RZW_NC];
-                            S[0] = window_dft(spec, numdata, lobin, -1 - khw, khw - 1, nn, npts);
+                            S[0] = make_double2(D[nn].x, -D[nn].y);
                             for (int c = 0; c < RZW_NC - 1; c++) {
                                 double2 e0 = window_dft(spec, numdata, lobin, c - 1 - khw,
                                                         c - khw, nn, npts);
@@ -755,9 +820,12 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
     for (int k = 0; k < H; k++) {
         const int hh = k + 1;
         const double rr = rout * hh, zz = zout * hh, ww = wout * hh;
-        const float lp = (float) localpower3d_wg(spec, numdata, rr, zz, ww, red);
+        const float lp =
+            (float) localpower3d_wg(spec, numdata, rr, zz, ww, red, fa, fb, a.npts_cap, &bad);
         double d7[7];
-        derivs3d_wg(spec, numdata, rr, zz, ww, lp, d7, red);
+        derivs3d_wg(spec, numdata, rr, zz, ww, lp, d7, red, fa, fb, a.npts_cap, &bad);
+        if (bad)
+            break;
         if (threadIdx.x == 0) {
             a.powers[(size_t) ci * RF_MAXH + k] = d7[0];
             for (int j = 0; j < 7; j++)
@@ -765,10 +833,10 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
         }
     }
     if (threadIdx.x == 0) {
-        a.rout[ci] = rout;
+        a.rout[ci] = bad ? NAN : rout;
         a.zout[ci] = zout;
         a.wout[ci] = wout;
-        a.nevals[ci] = 0;
+        a.nevals[ci] = bad ? -1 : 0;
     }
 }
 
