@@ -154,6 +154,106 @@ def batched_section(ntrials, threads):
             "candidates": int(sum(len(c) for c in bc))}
 
 
+def serving_section(nreq, rate, threads, budget_gib, workers):
+    """The serving workload: requests (time series + search parameters)
+    arrive as a Poisson stream and go to the request loop (accel_server),
+    which admits them by device memory and runs them concurrently; each is
+    taken through spectrum preparation on the GPU, search and refinement.
+    Latency = arrival to result."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    from presto_amd import accel as A, synth
+
+    reqs = list(synth.config5_requests(nreq, rate=rate))
+
+    def gen(q):
+        i, t, N, dt, zmax, wmax, nh, seed, ps = q
+        return synth.make_series_f32(N, dt, seed, ps)
+
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        series = list(ex.map(gen, reqs))
+    params = [A.make_params(q[2] // 2, q[2], q[3], zmax=q[4], numharm=q[6], sigma=2.0,
+                            wmax=(q[5] if q[5] else None)) for q in reqs]
+    cells = 0
+    for p in params:
+        g = A.geometry(p)
+        cells += g.nblocks * g.corr_uselen * g.numz * max(g.numw, 1)
+    _stage(f"serving workload: {nreq} requests prepared")
+    lat, svc, status = [None] * nreq, [None] * nreq, [None] * nreq
+    with A.Server(budget_bytes=int(budget_gib) << 30, workers=workers) as sv:
+        tickets = [None] * nreq
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for i in range(nreq):
+                ready.acquire()
+                status[i], lat[i], svc[i] = sv.wait(tickets[i])
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for i, q in enumerate(reqs):
+            dtw = t0 + q[1] - time.perf_counter()
+            if dtw > 0:
+                time.sleep(dtw)
+            tickets[i] = sv.submit(params[i], series[i])
+            ready.release()
+        th.join()
+        wall = time.perf_counter() - t0
+    ok = [i for i in range(nreq) if status[i] == 0]
+    lat_ok = [lat[i] for i in ok]
+    _stage(f"serving workload: {len(ok)}/{nreq} served in {wall:.2f} s")
+    return {"workload": f"{nreq} requests, Poisson arrivals at {rate:g}/s, lengths 2^20..2^24 "
+                        "(log-uniform, powers of two), zmax {0,50,200}, wmax {0,100}, "
+                        "numharm {8,16}, 20% with a pulsar; GPU spectrum preparation, search "
+                        "and refinement per request",
+            "requests": nreq, "served": len(ok), "workers": workers,
+            "hbm_budget_gib": budget_gib,
+            "dm_trials_per_s": len(ok) / wall, "cells_per_s": cells / wall,
+            "offered_rate_per_s": rate,
+            "p50_ms": float(np.percentile(lat_ok, 50)) if ok else None,
+            "p99_ms": float(np.percentile(lat_ok, 99)) if ok else None,
+            "service_p50_ms": float(np.percentile([svc[i] for i in ok], 50)) if ok else None}
+
+
+def jerk_section():
+    """One jerk-search spectrum at full size: 2^22 samples (dt 64 us),
+    zmax 200, wmax 300, numharm 8, a pulsar with z = 60 and w = 150;
+    search and refinement timed."""
+    from presto_amd import accel as A, synth
+    N, dt = 1 << 22, 64e-6
+    x = synth.make_series_f32(N, dt, 4000, [dict(f0=180.3, z=60.0, w=150.0, duty=0.05,
+                                                 amp=0.05)])
+    p = A.make_params(N // 2, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=300)
+    g = A.geometry(p)
+    cells = g.nblocks * g.corr_uselen * g.numz * g.numw
+    with A.Search(p) as s:
+        s.set_series(x)
+        s.run()                       # warm
+        t0 = time.perf_counter()
+        s.run()
+        t_search = time.perf_counter() - t0
+        tim = s.timings()
+        nc = len(s.candidates())
+        t0 = time.perf_counter()
+        if nc:
+            s.optimize()
+        t_opt = time.perf_counter() - t0
+        plane_gb = s.plane_cells() * 4 / 1e9
+        best = s.candidates()[:1]
+    _stage(f"jerk workload: search {1e3 * t_search:.1f} ms, refinement {1e3 * t_opt:.1f} ms")
+    return {"workload": "one spectrum from 2^22 float32 samples, dt 64 us, zmax 200, wmax 300 "
+                        "(201 x 31 templates), numharm 8, one pulsar z 60 w 150",
+            "cells": cells, "search_ms": 1e3 * t_search, "cells_per_s": cells / t_search,
+            "kernel_ms": {"correlation": tim["corr"], "harmonic_sum": tim["harmsum"],
+                          "block_norms": tim["norms"], "plane_halo": tim["halo"]},
+            "raw_plane_gb": plane_gb, "candidates": nc, "optimize_ms": 1e3 * t_opt,
+            "dm_trials_per_s_through_refinement": 1.0 / (t_search + t_opt),
+            "best": ({"r": float(best[0]["r"]), "z": float(best[0]["z"]), "w": float(best[0]["w"]),
+                      "sigma": float(best[0]["sigma"]), "numharm": int(best[0]["numharm"])}
+                     if len(best) else None)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +263,12 @@ def main():
                     help="blocks of the oracle CPU baseline sample (0: skip)")
     ap.add_argument("--batch-trials", type=int, default=256,
                     help="spectra of the batched DM-trial section (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=120,
+                    help="requests of the serving section (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-budget-gib", type=int, default=240)
+    ap.add_argument("--jerk", type=int, default=1, help="run the full-size jerk section")
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
                     help="OpenMP threads of the CPU baseline")
@@ -279,6 +385,12 @@ def main():
     }
     if args.batch_trials > 0:
         line["batched"] = batched_section(args.batch_trials, args.cpu_threads)
+    if args.jerk:
+        line["jerk"] = jerk_section()
+    if args.serve_requests > 0:
+        line["serving"] = serving_section(args.serve_requests, args.serve_rate,
+                                          args.cpu_threads, args.serve_budget_gib,
+                                          args.serve_workers)
     if args.cpu_blocks > 0:
         _stage(f"CPU baseline on {args.cpu_threads} threads")
         line["cpu_baseline"] = cpu_baseline(fft, N, dt, args.cpu_blocks, args.cpu_threads)

@@ -78,3 +78,26 @@ def config3_trials(n=256, seed0=3000):
             amp = 0.04 * float(np.exp(-0.5 * ((i - i0) / 3.0) ** 2))
             pulsars = [dict(f0=150.7, z=30.0, duty=0.05, amp=amp)]
         yield i, N, dt, seed0 + i, pulsars
+
+
+def config5_requests(n, seed_arrivals=5, seed0=5_000_000, rate=40.0):
+    """The serving workload: n requests with Poisson arrival times (rate per
+    second, PCG64(seed_arrivals)); per request (PCG64(seed0 + i)): a length
+    log-uniform in 2^20..2^24 samples, rounded to a power of two (the GPU
+    real FFT takes powers of two), dt 64 us, zmax from {0, 50, 200}, wmax
+    from {0, 100} (0: no jerk search), numharm from {8, 16}, and a pulsar
+    in 20 % of them.  Yields (i, t_arrival, N, dt, zmax, wmax, numharm,
+    seed, pulsars)."""
+    ra = np.random.Generator(np.random.PCG64(seed_arrivals))
+    t = np.cumsum(ra.exponential(1.0 / rate, n))
+    for i in range(n):
+        r = np.random.Generator(np.random.PCG64(seed0 + i))
+        N = 1 << int(round(r.uniform(20.0, 24.0)))
+        zmax = int(r.choice([0, 50, 200]))
+        wmax = int(r.choice([0, 100]))
+        nh = int(r.choice([8, 16]))
+        ps = []
+        if r.uniform() < 0.2:
+            ps = [dict(f0=float(r.uniform(20, 500)), z=float(r.uniform(-0.5, 0.5) * zmax),
+                       w=float(r.uniform(-0.5, 0.5) * wmax), duty=0.05, amp=0.05)]
+        yield i, float(t[i]), N, 64e-6, zmax, wmax, nh, seed0 + i, ps
