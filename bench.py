@@ -218,12 +218,12 @@ def serving_section(nreq, rate, threads, budget_gib, workers):
 
 def jerk_section():
     """One jerk-search spectrum at full size: 2^22 samples (dt 64 us),
-    zmax 200, wmax 300, numharm 8, a pulsar with z = 60 and w = 150;
-    search and refinement timed."""
+    zmax 200, wmax 300, numharm 8, a pulsar with average z = 40 (start
+    f-dot -20) and w = 120; search and refinement timed."""
     from presto_amd import accel as A, synth
     N, dt = 1 << 22, 64e-6
-    x = synth.make_series_f32(N, dt, 4000, [dict(f0=180.3, z=60.0, w=150.0, duty=0.05,
-                                                 amp=0.05)])
+    x = synth.make_series_f32(N, dt, 4000, [dict(f0=180.3, z=-20.0, w=120.0, duty=0.05,
+                                                 amp=0.12)])
     p = A.make_params(N // 2, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=300)
     g = A.geometry(p)
     cells = g.nblocks * g.corr_uselen * g.numz * g.numw
@@ -243,7 +243,7 @@ def jerk_section():
         best = s.candidates()[:1]
     _stage(f"jerk workload: search {1e3 * t_search:.1f} ms, refinement {1e3 * t_opt:.1f} ms")
     return {"workload": "one spectrum from 2^22 float32 samples, dt 64 us, zmax 200, wmax 300 "
-                        "(201 x 31 templates), numharm 8, one pulsar z 60 w 150",
+                        "(201 x 31 templates), numharm 8, one pulsar with average z 40, w 120",
             "cells": cells, "search_ms": 1e3 * t_search, "cells_per_s": cells / t_search,
             "kernel_ms": {"correlation": tim["corr"], "harmonic_sum": tim["harmsum"],
                           "block_norms": tim["norms"], "plane_halo": tim["halo"]},
@@ -267,7 +267,9 @@ def main():
                     help="requests of the serving section (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-workers", type=int, default=4)
-    ap.add_argument("--serve-budget-gib", type=int, default=240)
+    ap.add_argument("--serve-budget-gib", type=int, default=190,
+                    help="device memory for requests; allocations slow down sharply "
+                         "above ~200 GB in use on a 288 GB MI355X")
     ap.add_argument("--jerk", type=int, default=1, help="run the full-size jerk section")
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),

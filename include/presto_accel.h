@@ -312,6 +312,10 @@ int accel_server_wait(accel_server *sv, int ticket, double *latency_ms, double *
  * refined, the fundamentals' fourierprops. */
 int accel_server_cands(accel_server *sv, int ticket, accel_cand *out, int n);
 int accel_server_props(accel_server *sv, int ticket, accel_fourierprops *out, int n);
+/* Where a finished request's time went (ms): queued, evicting cached
+ * handles, creating its handle, preparation + search + refinement, the
+ * whole service; then its device-memory footprint (bytes). */
+int accel_server_timing(accel_server *sv, int ticket, double *ms6);
 /* Device memory held by live handles (running and cached). */
 long long accel_server_memory(accel_server *sv);
 /* Finishes every queued request, then stops the workers. */

@@ -141,6 +141,7 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_batch_run", "accel_search_footprint", "accel_server_create",
             "accel_server_submit", "accel_server_wait", "accel_server_cands",
             "accel_server_props", "accel_server_memory", "accel_server_destroy",
+            "accel_server_timing",
             "accel_last_error"]
 
 
@@ -392,6 +393,7 @@ _sig("accel_server_wait", _i, _p, _i, C.POINTER(_d), C.POINTER(_d))
 _sig("accel_server_cands", _i, _p, _i, _p, _i)
 _sig("accel_server_props", _i, _p, _i, _p, _i)
 _sig("accel_server_memory", _ll, _p)
+_sig("accel_server_timing", _i, _p, _i, _p)
 _sig("accel_server_destroy", None, _p)
 
 
@@ -439,6 +441,13 @@ class Server:
 
     def memory(self):
         return lib.accel_server_memory(self.h)
+
+    def timing(self, ticket):
+        """dict of queued, evict, create, work, service (ms) and footprint bytes"""
+        v = np.zeros(6)
+        if lib.accel_server_timing(self.h, ticket, v.ctypes.data):
+            raise AccelError("request not finished")
+        return dict(zip(("queued", "evict", "create", "work", "service", "bytes"), v))
 
     def close(self):
         if self.h:

@@ -34,6 +34,7 @@ typedef struct {
     int state;                  /* 0 queued, 1 running, 2 done */
     int status;
     double t_submit, t_start, t_done;
+    double ms_evict, ms_create, ms_work;   /* inside the service time */
     int ncands;
     accel_cand *cands;
     accel_fourierprops *props;  /* when optimized */
@@ -73,7 +74,8 @@ static double now_s(void)
 /* With the lock held: a handle for params p, reusing an idle cached one or
  * creating one when memory allows (evicting idle handles LRU).  Returns the
  * handle index, -1 if the request must wait, -2 if it can never fit. */
-static int take_handle(accel_server *sv, const accel_params *p, long long bytes)
+static int take_handle(accel_server *sv, const accel_params *p, long long bytes,
+                       double *ms_evict)
 {
     for (int i = 0; i < sv->nhd; i++)
         if (!sv->hd[i].busy && !memcmp(&sv->hd[i].p, p, sizeof(*p))) {
@@ -89,7 +91,9 @@ static int take_handle(accel_server *sv, const accel_params *p, long long bytes)
                 lru = i;
         if (lru < 0)
             return -1;   /* running requests hold the memory: wait */
+        double t0 = now_s();
         accel_search_destroy(sv->hd[lru].s);
+        *ms_evict += 1e3 * (now_s() - t0);
         sv->in_use -= sv->hd[lru].bytes;
         sv->hd[lru] = sv->hd[--sv->nhd];
     }
@@ -144,7 +148,9 @@ static void *worker(void *arg)
             continue;
         }
         const int ti = sv->head;
-        int hi = take_handle(sv, &sv->req[ti].p, sv->req[ti].bytes);
+        double ms_evict = 0.0;
+        int hi = take_handle(sv, &sv->req[ti].p, sv->req[ti].bytes, &ms_evict);
+        sv->req[ti].ms_evict += ms_evict;
         if (hi == -1) {
             pthread_cond_wait(&sv->cv, &sv->mu);
             continue;
@@ -168,13 +174,16 @@ static void *worker(void *arg)
         pthread_mutex_unlock(&sv->mu);
         /* the current device is per host thread */
         agpu_init(sv->device);
+        double t0 = now_s();
         if (!s)
             s = accel_search_create(&p, sv->device);
+        double t1 = now_s();
         srv_req local = *r;   /* the request array may move while unlocked */
         if (s)
             run_request(sv, &local, s);
         else
             snprintf(local.err, sizeof(local.err), "%s", accel_last_error());
+        double t2 = now_s();
         pthread_mutex_lock(&sv->mu);
         /* the handle array may have moved too: find this handle again */
         for (int i = 0; i < sv->nhd; i++)
@@ -190,6 +199,8 @@ static void *worker(void *arg)
                 break;
             }
         r = &sv->req[ti];
+        r->ms_create = 1e3 * (t1 - t0);
+        r->ms_work = 1e3 * (t2 - t1);
         r->status = local.status;
         r->ncands = local.ncands;
         r->cands = local.cands;
@@ -264,6 +275,24 @@ int accel_server_wait(accel_server *sv, int ticket, double *latency_ms, double *
     int st = r->status;
     pthread_mutex_unlock(&sv->mu);
     return st;
+}
+
+int accel_server_timing(accel_server *sv, int ticket, double *ms6)
+{
+    pthread_mutex_lock(&sv->mu);
+    int rc = -1;
+    if (ticket >= 0 && ticket < sv->nreq && sv->req[ticket].state == 2) {
+        srv_req *r = &sv->req[ticket];
+        ms6[0] = 1e3 * (r->t_start - r->t_submit);   /* queued */
+        ms6[1] = r->ms_evict;                         /* evicting cached handles */
+        ms6[2] = r->ms_create;                        /* creating the handle */
+        ms6[3] = r->ms_work;                          /* preparation, search, refinement */
+        ms6[4] = 1e3 * (r->t_done - r->t_start);      /* service */
+        ms6[5] = (double) r->bytes;
+        rc = 0;
+    }
+    pthread_mutex_unlock(&sv->mu);
+    return rc;
 }
 
 int accel_server_cands(accel_server *sv, int ticket, accel_cand *out, int n)
