@@ -16,6 +16,7 @@
  *       (src/accel_utils.c:1261-1452).
  */
 #include <hip/hip_runtime.h>
+#include <pthread.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -871,16 +872,157 @@ int agpu_device_name(char *buf, int len)
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* Device memory arena.  Freed device memory is not immediately cheap to  */
+/* hand out again (the next large allocation can stall for seconds), and  */
+/* every hipFree waits for the whole device.  A long-running caller (the  */
+/* request loop) therefore reserves one block once and every allocation   */
+/* is carved from it: best fit, free neighbours coalesced, 64 KiB units.  */
+/* Without an arena, allocations go straight to hipMalloc / hipFree.     */
+/* ------------------------------------------------------------------ */
+#define ARENA_UNIT ((size_t) 64 << 10)
+
+struct arena_seg {
+    size_t off, len;
+};
+
+static pthread_mutex_t g_arena_mu = PTHREAD_MUTEX_INITIALIZER;
+static char *g_arena = NULL;
+static size_t g_arena_len = 0, g_arena_used = 0;
+static arena_seg *g_free = NULL, *g_used = NULL;
+static int g_nfree = 0, g_nused = 0, g_capfree = 0, g_capused = 0;
+
+static void seg_push(arena_seg **v, int *n, int *cap, arena_seg s)
+{
+    if (*n == *cap) {
+        *cap = *cap ? 2 * *cap : 64;
+        *v = (arena_seg *) realloc(*v, sizeof(arena_seg) * *cap);
+    }
+    (*v)[(*n)++] = s;
+}
+
+int agpu_arena_init(size_t bytes)
+{
+    pthread_mutex_lock(&g_arena_mu);
+    int rc = 0;
+    if (g_arena) {
+        rc = (g_arena_len >= bytes) ? 0 : -2;
+        if (rc)
+            snprintf(g_err, sizeof(g_err), "agpu_arena_init: an arena of %zu bytes exists",
+                     g_arena_len);
+    } else {
+        bytes = (bytes + ARENA_UNIT - 1) / ARENA_UNIT * ARENA_UNIT;
+        hipError_t e = hipMalloc((void **) &g_arena, bytes);
+        if (e != hipSuccess) {
+            snprintf(g_err, sizeof(g_err), "agpu_arena_init: hipMalloc(%zu): %s", bytes,
+                     hipGetErrorString(e));
+            g_arena = NULL;
+            rc = -1;
+        } else {
+            g_arena_len = bytes;
+            g_arena_used = 0;
+            g_nfree = g_nused = 0;
+            seg_push(&g_free, &g_nfree, &g_capfree, arena_seg{0, bytes});
+        }
+    }
+    pthread_mutex_unlock(&g_arena_mu);
+    return rc;
+}
+
+int agpu_arena_release(void)
+{
+    pthread_mutex_lock(&g_arena_mu);
+    int rc = 0;
+    if (g_arena && g_nused) {
+        snprintf(g_err, sizeof(g_err), "agpu_arena_release: %d blocks still allocated", g_nused);
+        rc = -2;
+    } else if (g_arena) {
+        hipFree(g_arena);
+        g_arena = NULL;
+        g_arena_len = g_arena_used = 0;
+        g_nfree = 0;
+    }
+    pthread_mutex_unlock(&g_arena_mu);
+    return rc;
+}
+
+size_t agpu_arena_used(void)
+{
+    pthread_mutex_lock(&g_arena_mu);
+    size_t u = g_arena_used;
+    pthread_mutex_unlock(&g_arena_mu);
+    return u;
+}
+
 int agpu_malloc(void **p, size_t bytes)
 {
-    AGPU_CHECK(hipMalloc(p, bytes));
+    pthread_mutex_lock(&g_arena_mu);
+    if (!g_arena) {
+        pthread_mutex_unlock(&g_arena_mu);
+        AGPU_CHECK(hipMalloc(p, bytes));
+        return 0;
+    }
+    const size_t len = (bytes + ARENA_UNIT - 1) / ARENA_UNIT * ARENA_UNIT;
+    int best = -1;
+    for (int i = 0; i < g_nfree; i++)
+        if (g_free[i].len >= len && (best < 0 || g_free[i].len < g_free[best].len))
+            best = i;
+    if (best < 0) {
+        pthread_mutex_unlock(&g_arena_mu);
+        snprintf(g_err, sizeof(g_err), "device arena: no free block of %zu bytes", len);
+        return -1;
+    }
+    arena_seg blk = {g_free[best].off, len};
+    g_free[best].off += len;
+    g_free[best].len -= len;
+    if (g_free[best].len == 0)
+        g_free[best] = g_free[--g_nfree];
+    seg_push(&g_used, &g_nused, &g_capused, blk);
+    g_arena_used += len;
+    *p = g_arena + blk.off;
+    pthread_mutex_unlock(&g_arena_mu);
     return 0;
 }
 
 int agpu_free(void *p)
 {
-    if (p)
-        AGPU_CHECK(hipFree(p));
+    if (!p)
+        return 0;
+    pthread_mutex_lock(&g_arena_mu);
+    if (g_arena && (char *) p >= g_arena && (char *) p < g_arena + g_arena_len) {
+        const size_t off = (size_t) ((char *) p - g_arena);
+        int u = -1;
+        for (int i = 0; i < g_nused; i++)
+            if (g_used[i].off == off) {
+                u = i;
+                break;
+            }
+        if (u < 0) {
+            pthread_mutex_unlock(&g_arena_mu);
+            return fail("device arena: free of an unknown block");
+        }
+        arena_seg blk = g_used[u];
+        g_used[u] = g_used[--g_nused];
+        g_arena_used -= blk.len;
+        /* coalesce with the free blocks just before and after */
+        for (int i = 0; i < g_nfree;) {
+            if (g_free[i].off + g_free[i].len == blk.off) {
+                blk.off = g_free[i].off;
+                blk.len += g_free[i].len;
+                g_free[i] = g_free[--g_nfree];
+            } else if (blk.off + blk.len == g_free[i].off) {
+                blk.len += g_free[i].len;
+                g_free[i] = g_free[--g_nfree];
+            } else {
+                i++;
+            }
+        }
+        seg_push(&g_free, &g_nfree, &g_capfree, blk);
+        pthread_mutex_unlock(&g_arena_mu);
+        return 0;
+    }
+    pthread_mutex_unlock(&g_arena_mu);
+    AGPU_CHECK(hipFree(p));
     return 0;
 }
 

@@ -220,6 +220,11 @@ accel_server *accel_server_create(int device, long long hbm_budget, int workers)
 {
     if (workers < 1 || workers > SRV_MAXW || hbm_budget <= 0)
         return NULL;
+    /* every handle of the loop lives in one arena: no device memory is
+     * returned to the driver between requests */
+    if (agpu_init(device) ||
+        agpu_arena_init((size_t) hbm_budget + (size_t) hbm_budget / 20 + ((size_t) 1 << 30)))
+        return NULL;
     accel_server *sv = (accel_server *) calloc(1, sizeof(accel_server));
     sv->device = device;
     sv->budget = hbm_budget;
@@ -351,6 +356,7 @@ void accel_server_destroy(accel_server *sv)
     }
     free(sv->req);
     free(sv->hd);
+    agpu_arena_release();
     pthread_mutex_destroy(&sv->mu);
     pthread_cond_destroy(&sv->cv);
     free(sv);

@@ -82,6 +82,11 @@ void agpu_set_error(const char *msg);
 int agpu_device_name(char *buf, int len);
 
 int agpu_malloc(void **p, size_t bytes);
+/* One device block reserved up front; while it exists every agpu_malloc /
+ * agpu_free is carved from / returned to it (no hipMalloc / hipFree). */
+int agpu_arena_init(size_t bytes);
+int agpu_arena_release(void);
+size_t agpu_arena_used(void);
 int agpu_free(void *p);
 int agpu_h2d(void *dst, const void *src, size_t bytes, void *stream);
 int agpu_d2h(void *dst, const void *src, size_t bytes, void *stream);
