@@ -256,6 +256,10 @@ int accel_calc_rzwerrs(const accel_fourierprops *p, double T, accel_rzwerrs *e);
  * (numerr 2); returns the string length or -1. */
 int accel_nice_output(char *out, double val, double err, int len, int numerr);
 
+/* Template banks are generated once per (z, w) grid and shared by all
+ * handles; this frees the banks no handle uses. */
+void accel_template_cache_clear(void);
+
 /* Device memory (bytes) a search handle for p needs, including series
  * input and refinement buffers (for admission by memory footprint). */
 long long accel_search_footprint(const accel_params *p);

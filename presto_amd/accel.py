@@ -138,7 +138,8 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_calc_rzwerrs", "accel_nice_output", "accel_read_birdies",
             "accel_zap_birdies", "accel_mt19937_raw", "accel_realfft_forward",
             "accel_batch_create", "accel_batch_destroy", "accel_batch_handles",
-            "accel_batch_run", "accel_search_footprint", "accel_server_create",
+            "accel_batch_run", "accel_search_footprint", "accel_template_cache_clear",
+            "accel_server_create",
             "accel_server_submit", "accel_server_wait", "accel_server_cands",
             "accel_server_props", "accel_server_memory", "accel_server_destroy",
             "accel_server_timing",
@@ -387,6 +388,7 @@ class Batch:
 
 
 _sig("accel_search_footprint", _ll, _PP)
+_sig("accel_template_cache_clear", None)
 _sig("accel_server_create", _p, _i, _ll, _i)
 _sig("accel_server_submit", _i, _p, _PP, _p, _ll, _i)
 _sig("accel_server_wait", _i, _p, _i, C.POINTER(_d), C.POINTER(_d))

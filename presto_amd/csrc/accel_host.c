@@ -20,6 +20,7 @@
  * the reference would have used for that block and fraction.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -557,6 +558,85 @@ static int build_tables(accel_search *s, agpu_window **win_out, int **zmap_out,
     return 0;
 }
 
+/* Template banks depend only on the (w, z) grid: they are generated once
+ * per grid and shared by every handle (refcounted, kept while unused until
+ * accel_template_cache_clear). */
+typedef struct tbank {
+    int numz, numw, zlo, wlo, L, refs;
+    void *d_T;
+    struct tbank *next;
+} tbank;
+static tbank *g_banks = NULL;
+static pthread_mutex_t g_bank_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static int gen_templates(accel_search *s);
+
+static int bank_get(accel_search *s)
+{
+    const accel_geom *g = &s->g;
+    pthread_mutex_lock(&g_bank_mu);
+    for (tbank *b = g_banks; b; b = b->next)
+        if (b->numz == g->numz && b->numw == g->numw && b->zlo == (int) g->zlo &&
+            b->wlo == (int) g->wlo && b->L == s->L) {
+            b->refs++;
+            s->d_T = b->d_T;
+            pthread_mutex_unlock(&g_bank_mu);
+            return 0;
+        }
+    int rc = -1;
+    if (agpu_malloc(&s->d_T, sizeof(accel_fcomplex) * (size_t) s->nt * s->L)) {
+        set_err("template bank: %s", agpu_last_error());
+        s->d_T = NULL;
+    } else if (gen_templates(s)) {
+        agpu_free(s->d_T);
+        s->d_T = NULL;
+    } else {
+        tbank *b = (tbank *) calloc(1, sizeof(tbank));
+        b->numz = g->numz;
+        b->numw = g->numw;
+        b->zlo = (int) g->zlo;
+        b->wlo = (int) g->wlo;
+        b->L = s->L;
+        b->refs = 1;
+        b->d_T = s->d_T;
+        b->next = g_banks;
+        g_banks = b;
+        rc = 0;
+    }
+    pthread_mutex_unlock(&g_bank_mu);
+    return rc;
+}
+
+static void bank_put(void *d_T)
+{
+    if (!d_T)
+        return;
+    pthread_mutex_lock(&g_bank_mu);
+    for (tbank *b = g_banks; b; b = b->next)
+        if (b->d_T == d_T) {
+            b->refs--;
+            break;
+        }
+    pthread_mutex_unlock(&g_bank_mu);
+}
+
+void accel_template_cache_clear(void)
+{
+    pthread_mutex_lock(&g_bank_mu);
+    tbank **pp = &g_banks;
+    while (*pp) {
+        tbank *b = *pp;
+        if (b->refs <= 0) {
+            agpu_free(b->d_T);
+            *pp = b->next;
+            free(b);
+        } else {
+            pp = &b->next;
+        }
+    }
+    pthread_mutex_unlock(&g_bank_mu);
+}
+
 /* Template bank of the fundamental: (w, z) row-major, z from -zmax and w
  * from -wmax (src/accel_utils.c:223-237). */
 static int gen_templates(accel_search *s)
@@ -612,7 +692,6 @@ static int search_init(accel_search *s, int device)
     size_t plane_bytes = sizeof(float) * (size_t) s->nt * (size_t) s->ntiles * AGPU_ROWLEN;
     GPU(agpu_malloc(&s->d_spec, sizeof(accel_fcomplex) * (size_t) (g->numbins > 0 ? g->numbins : 1)));
     s->own_spec = 1;
-    GPU(agpu_malloc(&s->d_T, sizeof(accel_fcomplex) * (size_t) s->nt * s->L));
     GPU(agpu_malloc((void **) &s->d_plane, plane_bytes > 0 ? plane_bytes : 4));
     GPU(agpu_malloc((void **) &s->d_win, sizeof(agpu_window) * (size_t) (s->nwin > 0 ? s->nwin : 1)));
     GPU(agpu_malloc((void **) &s->d_norm, sizeof(double) * (size_t) (s->nwin > 0 ? s->nwin : 1)));
@@ -631,7 +710,7 @@ static int search_init(accel_search *s, int device)
     free(win);
     free(zmap);
     free(wmap);
-    if (gen_templates(s))
+    if (bank_get(s))
         return -1;
     return 0;
 }
@@ -658,7 +737,7 @@ void accel_search_destroy(accel_search *s)
         agpu_stream_sync(s->stream);
     if (s->own_spec)
         agpu_free(s->d_spec);
-    agpu_free(s->d_T);
+    bank_put(s->d_T);
     agpu_free(s->d_plane);
     agpu_free(s->d_win);
     agpu_free(s->d_norm);

@@ -356,6 +356,7 @@ void accel_server_destroy(accel_server *sv)
     }
     free(sv->req);
     free(sv->hd);
+    accel_template_cache_clear();
     agpu_arena_release();
     pthread_mutex_destroy(&sv->mu);
     pthread_cond_destroy(&sv->cv);
