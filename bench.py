@@ -234,6 +234,8 @@ def jerk_section():
         s.run()
         t_search = time.perf_counter() - t0
         tim = s.timings()
+        host = s.host_timings()
+        nhits = len(s.hits())
         nc = len(s.candidates())
         t0 = time.perf_counter()
         if nc:
@@ -246,7 +248,9 @@ def jerk_section():
                         "(201 x 31 templates), numharm 8, one pulsar with average z 40, w 120",
             "cells": cells, "search_ms": 1e3 * t_search, "cells_per_s": cells / t_search,
             "kernel_ms": {"correlation": tim["corr"], "harmonic_sum": tim["harmsum"],
-                          "block_norms": tim["norms"], "plane_halo": tim["halo"]},
+                          "block_norms": tim["norms"], "plane_halo": tim["halo"],
+                          "device_total": tim["total"]},
+            "host_ms": host, "hits": nhits,
             "raw_plane_gb": plane_gb, "candidates": nc, "optimize_ms": 1e3 * t_opt,
             "dm_trials_per_s_through_refinement": 1.0 / (t_search + t_opt),
             "best": ({"r": float(best[0]["r"]), "z": float(best[0]["z"]), "w": float(best[0]["w"]),
@@ -294,13 +298,14 @@ def main():
     s.set_spectrum(fft)
     for _ in range(args.warmup):
         s.run()
-    lat, tim = [], []
+    lat, tim, host = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
         s.run()          # returns after the stream has drained and hits are replayed
         lat.append(time.perf_counter() - ts)
         tim.append(s.timings())
+        host.append(s.host_timings())
     wall = time.perf_counter() - t0
     _stage(f"{args.steps} timed steps done")
     ncand = len(s.candidates())
@@ -370,7 +375,9 @@ def main():
         "p50_ms": 1e3 * float(np.percentile(lat, 50)),
         "p99_ms": 1e3 * float(np.percentile(lat, 99)),
         "kernel_ms": {"block_norms": avg("norms"), "correlation": corr_ms,
-                      "plane_halo": avg("halo"), "harmonic_sum": hs_ms},
+                      "plane_halo": avg("halo"), "harmonic_sum": hs_ms,
+                      "device_total": avg("total")},
+        "host_ms": {k: float(np.mean([h[k] for h in host])) for k in host[0]},
         "candidates": ncand,
         "roofline": {"bound": "hbm", "kernel": "k_harmsum<5,0>",
                      "achieved": hs_bytes / (hs_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,

@@ -139,6 +139,10 @@ int accel_search_norms(const accel_search *s, double *out, int n);
 /* Milliseconds of the last run (HIP events on the search's stream): block
  * norms, correlation, plane halo fill, harmonic sum, total. */
 int accel_search_timings(const accel_search *s, float *ms5);
+/* Host wall-clock milliseconds of the last run: enqueueing the kernels,
+ * waiting for them, copying the hits back, replaying the hits into the
+ * candidate list (merge, sort, harmonic removal). */
+int accel_search_host_timings(const accel_search *s, double *ms4);
 /* Launch shape of the correlation kernel: segment length, used half-bins
  * per segment, template groups, segments. */
 int accel_search_corr_shape(const accel_search *s, int *L, int *uselen, int *groups,

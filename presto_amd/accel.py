@@ -76,6 +76,7 @@ _sig("accel_search_collect", _i, _p)
 _sig("accel_search_ncands", _i, _p)
 _sig("accel_search_cands", _i, _p, _p, _i)
 _sig("accel_search_nhits", _ll, _p)
+_sig("accel_search_host_timings", _i, _p, _p)
 _sig("accel_search_hits", _i, _p, _p, _ll)
 _sig("accel_search_plane_row", _i, _p, _i, _i, _ll, _i, _p)
 _sig("accel_search_nwindows", _i, _p)
@@ -129,6 +130,7 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_collect", "accel_search_ncands", "accel_search_cands",
             "accel_search_nhits", "accel_search_hits", "accel_search_plane_row",
             "accel_search_nwindows", "accel_search_norms", "accel_search_timings",
+            "accel_search_host_timings",
             "accel_search_corr_shape", "accel_search_plane_cells",
             "accel_search_windows", "accel_deredden", "accel_prepare_dat_spectrum",
             "accel_candidate_sigma", "accel_power_for_sigma", "accel_search_optimize",
@@ -283,6 +285,12 @@ class Search:
         lib.accel_search_timings(self.h, ms.ctypes.data)
         return dict(norms=float(ms[0]), corr=float(ms[1]), halo=float(ms[2]),
                     harmsum=float(ms[3]), total=float(ms[4]))
+
+    def host_timings(self):
+        ms = np.zeros(4, dtype=np.float64)
+        lib.accel_search_host_timings(self.h, ms.ctypes.data)
+        return dict(launch=float(ms[0]), wait=float(ms[1]), copy=float(ms[2]),
+                    replay=float(ms[3]))
 
     def corr_shape(self):
         L, u, g, ns = _i(), _i(), _i(), _ll()

@@ -285,6 +285,23 @@ __global__ __launch_bounds__(256) void k_block_norms(const float2 *spec, long lo
     }
 }
 
+/* One window given by value (the accel.h surface normalises one block
+ * per subharm_fderivs_vol call). */
+__global__ __launch_bounds__(256) void k_block_norm1(const float2 *spec, long long numbins,
+                                                     long long lobin_file, long long lobin,
+                                                     int numdata, double *norm, float *nsq)
+{
+    extern __shared__ unsigned keys[];
+    __shared__ unsigned hist[256];
+    __shared__ unsigned sh[8];
+    const float med = block_median(spec, numbins, lobin - lobin_file, numdata, keys, hist, sh);
+    if (threadIdx.x == 0) {
+        double nv = 1.0 / sqrt((double) med / 0.69314718055994530942);
+        norm[0] = nv;
+        nsq[0] = (float) (nv * nv);
+    }
+}
+
 /* De-reddening (src/misc_utils.c:912-1067), block medians: mean_b =
  * median(|x|^2 of block b) / ln 2, rounded to FP32 as the reference. */
 __global__ __launch_bounds__(256) void k_dered_means(const float2 *spec, long long numbins,
@@ -711,6 +728,10 @@ __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const fl
                                             const float *nr2, const float *nr3, int rh_c0,
                                             int rh_first, float *acc)
 {
+    /* each normalised sub-harmonic power is rounded to FP32 before it is
+     * added, as the reference adds stored FP32 planes
+     * (src/accel_utils.c:1261-1282, 1312-1362): no fused multiply-add */
+#pragma clang fp contract(off)
     static_assert(HS_CPT == 4, "four cells per thread");
     constexpr int H = 1 << S;
     constexpr int F0 = (H >> 1) - 1;
@@ -736,7 +757,7 @@ __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const fl
         }
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            acc[j] = __builtin_fmaf(lp[j * step], nn[j], acc[j]);
+            acc[j] = acc[j] + lp[j * step] * nn[j];
         wl += hs_frac_wl(k, H);
     }
 }
@@ -1177,6 +1198,18 @@ int agpu_block_norms(const void *d_spec, long long numbins, long long lobin_file
     hipLaunchKernelGGL(k_block_norms, dim3(nwin), dim3(256), sizeof(unsigned) * maxn,
                        (hipStream_t) stream, (const float2 *) d_spec, numbins, lobin_file,
                        d_win, d_norm, d_nsq);
+    AGPU_CHECK(hipGetLastError());
+    return 0;
+}
+
+int agpu_block_norm1(const void *d_spec, long long numbins, long long lobin_file,
+                     long long lobin, int numdata, double *d_norm, float *d_nsq, void *stream)
+{
+    if (numdata <= 0 || numdata > NORM_MAXN)
+        return fail("agpu_block_norm1: window longer than the LDS selection buffer");
+    hipLaunchKernelGGL(k_block_norm1, dim3(1), dim3(256), sizeof(unsigned) * numdata,
+                       (hipStream_t) stream, (const float2 *) d_spec, numbins, lobin_file,
+                       lobin, numdata, d_norm, d_nsq);
     AGPU_CHECK(hipGetLastError());
     return 0;
 }

@@ -25,7 +25,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+#include <unistd.h>
 #include "presto_accel.h"
+#include "accel_internal.h"
 #include "agpu.h"
 
 #define ACCEL_NUMBETWEEN 2
@@ -53,6 +56,14 @@ static void set_err(const char *fmt, ...)
 
 const char *accel_last_error(void) { return g_err; }
 
+void accel_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
 #define GPU(call)                                                                \
     do {                                                                         \
         if ((call) != 0) {                                                       \
@@ -60,6 +71,13 @@ const char *accel_last_error(void) { return g_err; }
             return -1;                                                           \
         }                                                                        \
     } while (0)
+
+static double now_ms(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return 1e3 * (double) t.tv_sec + 1e-6 * (double) t.tv_nsec;
+}
 
 /* ---- index rules: src/accel_utils.c:36, 54-102 ---- */
 static int nearest_int(double x) { return (int) (x < 0 ? x - 0.5 : x + 0.5); }
@@ -460,6 +478,7 @@ struct accel_search {
     cnode *cands;
     int ncands;
     float ms[5];
+    double hms[4];                    /* host: launch, wait, copy, replay */
     /* refinement of the current list (accel_search_optimize) */
     int nopt;
     int *opt_nh;
@@ -819,6 +838,7 @@ int accel_search_launch(accel_search *s)
 {
     const accel_geom *g = &s->g;
     agpu_harmsum_args a;
+    const double t0 = now_ms();
     GPU(agpu_memset(s->d_nhits, 0, sizeof(unsigned), s->stream));
     GPU(agpu_event_record(s->ev[0], s->stream));
     GPU(agpu_block_norms(s->d_spec, g->numbins, g->lobin, s->d_win, s->nwin, s->maxn,
@@ -835,6 +855,7 @@ int accel_search_launch(accel_search *s)
     fill_harmsum_args(s, &a);
     GPU(agpu_harmsum(&a, s->stream));
     GPU(agpu_event_record(s->ev[4], s->stream));
+    s->hms[0] = now_ms() - t0;
     return 0;
 }
 
@@ -851,13 +872,212 @@ static int cmp_hit(const void *pa, const void *pb)
 
 static void clear_opt(accel_search *s);
 
+static int bits_for(unsigned v)
+{
+    int b = 0;
+    while (v) {
+        b++;
+        v >>= 1;
+    }
+    return b;
+}
+
+/* Puts the hits into the reference's search order.  Every (block, stage, w,
+ * z, r) appears at most once, so the order is total: the fields are packed
+ * into one 64-bit key (widths from the largest value of each field) and
+ * sorted by an LSD radix sort of 11-bit digits.  Falls back to qsort if a
+ * field is negative or the key would not fit. */
+static int sort_hits(accel_hit *h, long long n)
+{
+    if (n < 2)
+        return 0;
+    unsigned mx[5] = {0, 0, 0, 0, 0};
+    for (long long i = 0; i < n; i++) {
+        const int f[5] = {h[i].block, h[i].stage, h[i].wi, h[i].zi, h[i].ii};
+        for (int k = 0; k < 5; k++) {
+            if (f[k] < 0) {
+                qsort(h, (size_t) n, sizeof(accel_hit), cmp_hit);
+                return 0;
+            }
+            if ((unsigned) f[k] > mx[k])
+                mx[k] = (unsigned) f[k];
+        }
+    }
+    int nb[5], tot = 0;
+    for (int k = 0; k < 5; k++)
+        tot += nb[k] = bits_for(mx[k]);
+    if (tot > 64 || n > 0xffffffffLL) {
+        qsort(h, (size_t) n, sizeof(accel_hit), cmp_hit);
+        return 0;
+    }
+    unsigned long long *key = (unsigned long long *) malloc(sizeof(unsigned long long) * 2 * n);
+    unsigned *idx = (unsigned *) malloc(sizeof(unsigned) * 2 * n);
+    accel_hit *tmp = (accel_hit *) malloc(sizeof(accel_hit) * n);
+    if (!key || !idx || !tmp) {
+        free(key);
+        free(idx);
+        free(tmp);
+        set_err("out of host memory sorting %lld hits", n);
+        return -1;
+    }
+    for (long long i = 0; i < n; i++) {
+        const unsigned f[5] = {(unsigned) h[i].block, (unsigned) h[i].stage, (unsigned) h[i].wi,
+                               (unsigned) h[i].zi, (unsigned) h[i].ii};
+        unsigned long long kv = 0;
+        for (int k = 0; k < 5; k++)
+            kv = (nb[k] ? (kv << nb[k]) : kv) | f[k];
+        key[i] = kv;
+        idx[i] = (unsigned) i;
+    }
+    unsigned long long *ka = key, *kb = key + n;
+    unsigned *ia = idx, *ib = idx + n;
+    enum { DIG = 11, NBK = 1 << DIG };
+    static __thread long long cnt[NBK];
+    for (int sh = 0; sh < tot; sh += DIG) {
+        memset(cnt, 0, sizeof(cnt));
+        for (long long i = 0; i < n; i++)
+            cnt[(ka[i] >> sh) & (NBK - 1)]++;
+        long long run = 0;
+        for (int d = 0; d < NBK; d++) {
+            long long c = cnt[d];
+            cnt[d] = run;
+            run += c;
+        }
+        for (long long i = 0; i < n; i++) {
+            long long o = cnt[(ka[i] >> sh) & (NBK - 1)]++;
+            kb[o] = ka[i];
+            ib[o] = ia[i];
+        }
+        unsigned long long *tk = ka;
+        ka = kb;
+        kb = tk;
+        unsigned *ti = ia;
+        ia = ib;
+        ib = ti;
+    }
+    for (long long i = 0; i < n; i++)
+        tmp[i] = h[ia[i]];
+    memcpy(h, tmp, sizeof(accel_hit) * n);
+    free(key);
+    free(idx);
+    free(tmp);
+    return 0;
+}
+
+/* Sigma of every hit (src/accel_utils.c:1434-1437), spread over host
+ * threads when there are many: each value is a pure function of its hit,
+ * so the result does not depend on the split. */
+typedef struct {
+    const accel_hit *h;
+    const accel_geom *g;
+    float *sig;
+    long long lo, hi;
+} sig_job;
+
+static void *sig_worker(void *arg)
+{
+    sig_job *j = (sig_job *) arg;
+    for (long long i = j->lo; i < j->hi; i++) {
+        const accel_hit *h = &j->h[i];
+        j->sig[i] = (float) accel_candidate_sigma(h->pow, 1 << h->stage,
+                                                  (double) j->g->numindep[h->stage]);
+    }
+    return NULL;
+}
+
+#define SIG_MAX_THREADS 16
+#define SIG_PER_THREAD 2048
+
+static void hit_sigmas(const accel_hit *h, long long n, const accel_geom *g, float *sig)
+{
+    long np = sysconf(_SC_NPROCESSORS_ONLN);
+    long long nt = n / SIG_PER_THREAD;
+    if (nt > np)
+        nt = np;
+    if (nt > SIG_MAX_THREADS)
+        nt = SIG_MAX_THREADS;
+    sig_job jb[SIG_MAX_THREADS];
+    pthread_t th[SIG_MAX_THREADS];
+    int started[SIG_MAX_THREADS] = {0};
+    if (nt < 2) {
+        sig_job one = {h, g, sig, 0, n};
+        sig_worker(&one);
+        return;
+    }
+    for (int t = 0; t < nt; t++) {
+        jb[t] = (sig_job){h, g, sig, n * t / nt, n * (t + 1) / nt};
+        if (t > 0)
+            started[t] = pthread_create(&th[t], NULL, sig_worker, &jb[t]) == 0;
+    }
+    sig_worker(&jb[0]);
+    for (int t = 1; t < nt; t++) {
+        if (started[t])
+            pthread_join(th[t], NULL);
+        else
+            sig_worker(&jb[t]);
+    }
+}
+
+typedef struct {
+    const float *pow;
+    float *sig;
+    int numharm;
+    double numindep;
+    long long lo, hi;
+} psig_job;
+
+static void *psig_worker(void *arg)
+{
+    psig_job *j = (psig_job *) arg;
+    for (long long i = j->lo; i < j->hi; i++)
+        j->sig[i] = (float) accel_candidate_sigma(j->pow[i], j->numharm, j->numindep);
+    return NULL;
+}
+
+void accel_sigmas(const float *pow, long long n, int numharm, double numindep, float *sig)
+{
+    long np = sysconf(_SC_NPROCESSORS_ONLN);
+    long long nt = n / SIG_PER_THREAD;
+    if (nt > np)
+        nt = np;
+    if (nt > SIG_MAX_THREADS)
+        nt = SIG_MAX_THREADS;
+    psig_job jb[SIG_MAX_THREADS];
+    pthread_t th[SIG_MAX_THREADS];
+    int started[SIG_MAX_THREADS] = {0};
+    if (nt < 2) {
+        psig_job one = {pow, sig, numharm, numindep, 0, n};
+        psig_worker(&one);
+        return;
+    }
+    for (int t = 0; t < nt; t++) {
+        jb[t] = (psig_job){pow, sig, numharm, numindep, n * t / nt, n * (t + 1) / nt};
+        if (t > 0)
+            started[t] = pthread_create(&th[t], NULL, psig_worker, &jb[t]) == 0;
+    }
+    psig_worker(&jb[0]);
+    for (int t = 1; t < nt; t++) {
+        if (started[t])
+            pthread_join(th[t], NULL);
+        else
+            psig_worker(&jb[t]);
+    }
+}
+
 /* Replays the hits in the reference's order: block, then stage, then
  * w, z, r (src/accelsearch.c:189-227, src/accel_utils.c:1419-1450). */
 static int replay(accel_search *s)
 {
     const accel_geom *g = &s->g;
     const int rstep = g->corr_uselen / 2;
-    qsort(s->hits, (size_t) s->nhits, sizeof(accel_hit), cmp_hit);
+    if (sort_hits(s->hits, s->nhits))
+        return -1;
+    float *sigs = (float *) malloc(sizeof(float) * (s->nhits ? s->nhits : 1));
+    if (!sigs) {
+        set_err("out of host memory for %lld hit sigmas", s->nhits);
+        return -1;
+    }
+    hit_sigmas(s->hits, s->nhits, g, sigs);
     clear_opt(s);
     free_cands(s->cands);
     s->cands = NULL;
@@ -865,14 +1085,14 @@ static int replay(accel_search *s)
         const accel_hit *h = &s->hits[i];
         int numharm = 1 << h->stage;
         double startr = g->rlo + (double) h->block * rstep;
-        float sig = (float) accel_candidate_sigma(h->pow, numharm,
-                                                  (double) g->numindep[h->stage]);
+        float sig = sigs[i];
         double rr = ((long long) startr + h->ii * (double) ACCEL_DR) / (double) numharm;
         double zz = (g->zlo + h->zi * (double) ACCEL_DZ) / (double) numharm;
         double ww = (g->wlo + h->wi * (double) ACCEL_DW) / (double) numharm;
         int added;
         s->cands = insert_cand(s->cands, h->pow, sig, numharm, rr, zz, ww, &added);
     }
+    free(sigs);
     int n = 0;
     for (cnode *c = s->cands; c; c = c->next)
         n++;
@@ -889,6 +1109,7 @@ static int replay(accel_search *s)
 int accel_search_collect(accel_search *s)
 {
     unsigned nh = 0;
+    double t0 = now_ms();
     GPU(agpu_d2h(&nh, s->d_nhits, sizeof(unsigned), s->stream));
     if (nh > s->cap) {
         /* too many crossings for the buffer: grow it and sum again */
@@ -908,12 +1129,18 @@ int accel_search_collect(accel_search *s)
     for (int i = 0; i < 4; i++)
         GPU(agpu_event_elapsed(&s->ms[i], s->ev[i], s->ev[i + 1]));
     GPU(agpu_event_elapsed(&s->ms[4], s->ev[0], s->ev[4]));
+    double t1 = now_ms();
+    s->hms[1] = t1 - t0;
     free(s->hits);
     s->nhits = nh;
     s->hits = (accel_hit *) malloc(sizeof(accel_hit) * (nh ? nh : 1));
     if (nh)
         GPU(agpu_d2h(s->hits, s->d_hits, sizeof(agpu_hit) * nh, s->stream));
-    return replay(s);
+    t0 = now_ms();
+    s->hms[2] = t0 - t1;
+    int rc = replay(s);
+    s->hms[3] = now_ms() - t0;
+    return rc;
 }
 
 static void clear_opt(accel_search *s)
@@ -1008,6 +1235,12 @@ int accel_search_norms(const accel_search *s, double *out, int n)
 int accel_search_timings(const accel_search *s, float *ms5)
 {
     memcpy(ms5, s->ms, sizeof(s->ms));
+    return 0;
+}
+
+int accel_search_host_timings(const accel_search *s, double *ms4)
+{
+    memcpy(ms4, s->hms, sizeof(s->hms));
     return 0;
 }
 
@@ -1226,6 +1459,44 @@ int accel_search_optimize(accel_search *s)
     free(zin);
     free(win);
     return rc;
+}
+
+int accel_search_refine_dev(accel_search *s, int n, const int *nh, const double *rin,
+                            const double *zin, const double *win, double *rout, double *zout,
+                            double *wout, double *pows, accel_rderivs *derivs)
+{
+    if (!s->ws)
+        s->ws = (refine_ws *) calloc(1, sizeof(refine_ws));
+    if (!s->ws) {
+        set_err("refinement: out of host memory");
+        return -1;
+    }
+    return refine_device(s->ws, s->d_spec, s->g.numbins, n, nh, rin, zin, win, rout, zout, wout,
+                         pows, derivs, s->stream);
+}
+
+void accel_search_dev_view(const accel_search *s, accel_dev_view *v)
+{
+    v->d_spec = s->d_spec;
+    v->numbins = s->g.numbins;
+    v->lobin = s->g.lobin;
+    v->d_plane = s->d_plane;
+    v->ntiles = s->ntiles;
+    v->hb_end = s->hb_end;
+    v->nrows = s->nt;
+    v->numz = s->g.numz;
+    v->numw1 = s->numw1;
+    v->stream = s->stream;
+}
+
+int accel_search_plane_only(accel_search *s)
+{
+    const accel_geom *g = &s->g;
+    GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw, s->uselen,
+                       s->hb_end, s->d_plane, s->ntiles, s->groups, s->stream));
+    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, s->stream));
+    GPU(agpu_stream_sync(s->stream));
+    return 0;
 }
 
 int accel_search_cand_harmonics(const accel_search *s, int i, double *hirs, double *hizs,

@@ -21,6 +21,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "presto_accel.h"
+#include "accel_internal.h"
 
 #define PI 3.14159265358979323846
 #define TWOPI 6.28318530717958647692
@@ -315,43 +316,27 @@ static void header(FILE *f, const col *c, int n, int jerk)
     }
 }
 
-int accel_search_write_outputs(const accel_search *s, const char *accelnm, const char *candnm,
-                               const char *inf_path)
+int accel_out_fundamentals(FILE *f, const accel_out_cand *cands,
+                           const accel_fourierprops *props, int n, double T, int jerk)
 {
-    const accel_geom *g = accel_search_geom(s);
-    const int n = accel_search_ncands(s);
-    const int jerk = g->numw != 0;
-    accel_cand *cands = (accel_cand *) malloc(sizeof(accel_cand) * (n ? n : 1));
-    accel_fourierprops *props =
-        (accel_fourierprops *) calloc(n ? n : 1, sizeof(accel_fourierprops));
-    accel_rderivs der[ACCEL_MAXHARM];
-    double hirs[ACCEL_MAXHARM], hizs[ACCEL_MAXHARM], hiws[ACCEL_MAXHARM];
     char *notes = (char *) malloc((size_t) 20 * (n ? n : 1) + 1);
-    FILE *f = NULL;
-    int rc = -1;
-    accel_search_cands(s, cands, n);
-    if (n && accel_search_fund_props(s, props, n) != n)
-        goto done;
-    f = fopen(accelnm, "w");
-    if (!f)
-        goto done;
+    if (!notes)
+        return -1;
     memset(notes, ' ', (size_t) 20 * n);
     compare_cands(props, n, notes);
-
-    /* fundamentals, src/accel_utils.c:684-848 */
     header(f, FUND, 13, jerk);
     for (int i = 0; i < n; i++) {
-        const accel_cand *c = &cands[i];
+        const accel_out_cand *c = &cands[i];
         accel_rzwerrs e;
-        accel_calc_rzwerrs(&props[i], g->T, &e);
-        accel_search_cand_harmonics(s, i, NULL, NULL, NULL, NULL, der, ACCEL_MAXHARM);
+        accel_calc_rzwerrs(&props[i], T, &e);
         /* coherently summed power, src/accel_utils.c:780-797 */
-        double cr = 0.0, ci = 0.0, phs0 = der[0].phs;
+        double cr = 0.0, ci = 0.0, phs0 = c->derivs[0].phs;
         for (int j = 0; j < c->numharm; j++) {
-            double amp = sqrt(der[j].pow / der[j].locpow);
+            const accel_rderivs *d = &c->derivs[j];
+            double amp = sqrt(d->pow / d->locpow);
             double corr = phs0 - fmod((j + 1.0) * phs0, TWOPI);
-            cr += amp * cos(der[j].phs + corr);
-            ci += amp * sin(der[j].phs + corr);
+            cr += amp * cos(d->phs + corr);
+            ci += amp * sin(d->phs + corr);
         }
         char t[64];
         snprintf(t, sizeof(t), "%-4d", i + 1);
@@ -371,24 +356,27 @@ int accel_search_write_outputs(const accel_search *s, const char *accelnm, const
         put_valerr(f, props[i].z, props[i].zerr, FUND[9].numerr, FUND[9].width);
         if (jerk)
             put_valerr(f, props[i].w, props[i].werr, FUND[10].numerr, FUND[10].width);
-        double accel = props[i].z * SOL / (g->T * g->T * e.f);
-        double accelerr = props[i].zerr * SOL / (g->T * g->T * e.f);
+        double accel = props[i].z * SOL / (T * T * e.f);
+        double accelerr = props[i].zerr * SOL / (T * T * e.f);
         put_valerr(f, accel, accelerr, FUND[11].numerr, FUND[11].width);
         fprintf(f, "  %.20s\n", notes + i * 20);
     }
     fprintf(f, "\n\n");
+    free(notes);
+    return 0;
+}
 
-    /* harmonics, src/accel_utils.c:851-979 */
+int accel_out_harmonics(FILE *f, const accel_out_cand *cands, int n, double T, int jerk)
+{
     header(f, HARM, 15, jerk);
     for (int i = 0; i < n; i++) {
-        const accel_cand *c = &cands[i];
-        int nh = accel_search_cand_harmonics(s, i, hirs, hizs, hiws, NULL, der, ACCEL_MAXHARM);
-        for (int j = 0; j < nh; j++) {
+        const accel_out_cand *c = &cands[i];
+        for (int j = 0; j < c->numharm; j++) {
             accel_fourierprops hp;
             accel_rzwerrs e;
-            if (accel_calc_props(&der[j], hirs[j], hizs[j], hiws[j], &hp))
+            if (accel_calc_props(&c->derivs[j], c->hirs[j], c->hizs[j], c->hiws[j], &hp))
                 memset(&hp, 0, sizeof(hp));
-            accel_calc_rzwerrs(&hp, g->T, &e);
+            accel_calc_rzwerrs(&hp, T, &e);
             char t[64];
             if (j == 0)
                 snprintf(t, sizeof(t), " %-4d", i + 1);
@@ -420,16 +408,64 @@ int accel_search_write_outputs(const accel_search *s, const char *accelnm, const
         }
     }
     fprintf(f, "\n\n");
-    if (inf_path) {
-        FILE *inf = fopen(inf_path, "r");
-        if (inf) {
-            char buf[4096];
-            size_t k;
-            while ((k = fread(buf, 1, sizeof(buf), inf)) > 0)
-                fwrite(buf, 1, k, f);
-            fclose(inf);
-        }
+    return 0;
+}
+
+void accel_out_append_file(FILE *f, const char *path)
+{
+    FILE *in = path ? fopen(path, "r") : NULL;
+    if (!in)
+        return;
+    char buf[4096];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof(buf), in)) > 0)
+        fwrite(buf, 1, k, f);
+    fclose(in);
+}
+
+int accel_search_write_outputs(const accel_search *s, const char *accelnm, const char *candnm,
+                               const char *inf_path)
+{
+    const accel_geom *g = accel_search_geom(s);
+    const int n = accel_search_ncands(s);
+    const int jerk = g->numw != 0;
+    const int m = n ? n : 1;
+    accel_cand *cands = (accel_cand *) malloc(sizeof(accel_cand) * m);
+    accel_fourierprops *props = (accel_fourierprops *) calloc(m, sizeof(accel_fourierprops));
+    accel_out_cand *oc = (accel_out_cand *) calloc(m, sizeof(accel_out_cand));
+    accel_rderivs *der = (accel_rderivs *) calloc((size_t) m * ACCEL_MAXHARM,
+                                                  sizeof(accel_rderivs));
+    double *hi = (double *) calloc((size_t) 3 * m * ACCEL_MAXHARM, sizeof(double));
+    FILE *f = NULL;
+    int rc = -1;
+    if (!cands || !props || !oc || !der || !hi)
+        goto done;
+    accel_search_cands(s, cands, n);
+    if (n && accel_search_fund_props(s, props, n) != n)
+        goto done;
+    for (int i = 0; i < n; i++) {
+        double *hr = hi + (size_t) 3 * i * ACCEL_MAXHARM;
+        accel_rderivs *d = der + (size_t) i * ACCEL_MAXHARM;
+        accel_search_cand_harmonics(s, i, hr, hr + ACCEL_MAXHARM, hr + 2 * ACCEL_MAXHARM, NULL,
+                                    d, ACCEL_MAXHARM);
+        oc[i].power = cands[i].power;
+        oc[i].sigma = cands[i].sigma;
+        oc[i].numharm = cands[i].numharm;
+        oc[i].r = cands[i].r;
+        oc[i].z = cands[i].z;
+        oc[i].w = cands[i].w;
+        oc[i].derivs = d;
+        oc[i].hirs = hr;
+        oc[i].hizs = hr + ACCEL_MAXHARM;
+        oc[i].hiws = hr + 2 * ACCEL_MAXHARM;
     }
+    f = fopen(accelnm, "w");
+    if (!f)
+        goto done;
+    /* src/accel_utils.c:666-979 */
+    accel_out_fundamentals(f, oc, props, n, g->T, jerk);
+    accel_out_harmonics(f, oc, n, g->T, jerk);
+    accel_out_append_file(f, inf_path);
     if (fclose(f)) {
         f = NULL;
         goto done;
@@ -452,6 +488,8 @@ done:
         fclose(f);
     free(cands);
     free(props);
-    free(notes);
+    free(oc);
+    free(der);
+    free(hi);
     return rc;
 }

@@ -114,6 +114,50 @@ int agpu_block_norms(const void *d_spec, long long numbins, long long lobin_file
                      const agpu_window *d_win, int nwin, int maxn, double *d_norm,
                      float *d_nsq, void *stream);
 
+/* The same for one window given by value: d_norm[0], d_nsq[0]. */
+int agpu_block_norm1(const void *d_spec, long long numbins, long long lobin_file,
+                     long long lobin, int numdata, double *d_norm, float *d_nsq, void *stream);
+
+/* ---- accel.h surface (surface_gpu.hip); planes are [numws][numzs][numrs] ---- */
+
+/* Block plane from the raw plane: out = nsq[0] * raw(row (w+woff)*numz_plane
+ * + z+zoff, half-bin hb0 + r). */
+int agpu_sf_extract(const float *d_plane, int nrows, long long ntiles, int numz_plane, int woff,
+                    int zoff, int numws, int numzs, int numrs, long long hb0, const float *d_nsq,
+                    float *d_out, void *stream);
+
+/* add_ffdotpows: the sub-harmonic's index tables are rebuilt on the device
+ * from the values the reference builds them from. */
+typedef struct {
+    double hf;              /* harmnum / numharm */
+    double sub_fullrlo;     /* fullrlo the sub-harmonic plane was made for */
+    long long sub_rlo;
+    int sub_zlo, sub_wlo;
+    double obs_zlo;
+    int fund_wlo;
+    int fnumws, fnumzs, fnumrs;
+    int snumws, snumzs, snumrs;
+} agpu_sf_add_args;
+int agpu_sf_add(float *d_fund, const float *d_sub, const agpu_sf_add_args *a, void *stream);
+
+/* fund_to_ffdotplane / inmem_add_ffdotpows on a device full plane of
+ * numz rows of rlen half-bins */
+int agpu_sf_inmem_store(const float *d_fund, int numzs, int numrs, float *d_full,
+                        long long rlen, long long col0, void *stream);
+int agpu_sf_inmem_add(float *d_fund, int numzs, int numrs, const float *d_full, long long rlen,
+                      int rlo, double hf, int zlo, void *stream);
+
+/* Every element of a plane above powcut (unordered; the count may exceed
+ * cap, then only cap entries are written). */
+typedef struct {
+    long long idx;
+    float pow;
+    int pad;
+} agpu_sf_hit;
+int agpu_sf_threshold(const float *d_pow, long long n, float powcut, agpu_sf_hit *d_hits,
+                      unsigned *d_count, unsigned cap, void *stream);
+int agpu_copy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+
 /* Fused overlap-save correlation: raw |corr|^2 plane rows for templates
  * [0, nt) (row = template), half-bins [0, hb_end), segment length L = 4096,
  * written in the tiled layout above. */

@@ -1,0 +1,212 @@
+"""The accel.h surface on the GPU, driven block by block exactly as the
+reference's main loop (src/accelsearch.c:82-290) drives src/accel_utils.c:
+
+* block planes (subharm_fderivs_vol) against the FP64 oracle's planes of the
+  same block and harmonic, within 1e-4 of the power scale;
+* the index tables it fills on the host equal the oracle's, and the
+  device-side add_ffdotpows gathers with exactly those tables (bit-equal
+  FP32 sums);
+* whole runs -- in-memory, standard and jerk -- give the same candidate
+  list as the handle path (and, for the in-memory case, as the oracle's
+  block loop), and the same ACCEL and .cand bytes."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from presto_amd import synth
+
+pytestmark = pytest.mark.gpu
+A = pytest.importorskip("presto_amd.accel")
+S = pytest.importorskip("presto_amd.surface")
+
+
+def _write_dat(tmp_path, name, x, dt):
+    root = str(tmp_path / name)
+    x.astype(np.float32).tofile(root + ".dat")
+    S.write_inf(root, len(x), dt)
+    return root
+
+
+def _handle_run(x, dt, tmp_path, tag, inf, **kw):
+    p = A.make_params(len(x) // 2, len(x), dt, **kw)
+    with A.Search(p) as s:
+        s.set_series(x)
+        s.run()
+        c = s.candidates()
+        acc = str(tmp_path / f"{tag}_handle_ACCEL")
+        if len(c):
+            s.optimize()
+            s.write_outputs(acc, acc + ".cand", inf)
+        return c, acc, s.geom
+
+
+def _compare_run(tmp_path, name, x, dt, zmax, numharm, wmax=None, flo=1.0, fhi=10000.0,
+                 expect_inmem=None):
+    root = _write_dat(tmp_path, name, x, dt)
+    cmd = S.make_cmdline(root + ".dat", zmax=zmax, numharm=numharm, wmax=wmax, flo=flo,
+                         fhi=fhi)
+    info, cands = S.drive(root + ".dat", cmd)
+    if expect_inmem is not None:
+        assert info["inmem"] == expect_inmem
+    hc, hacc, g = _handle_run(x, dt, tmp_path, name, root + ".inf", zmax=zmax,
+                              numharm=numharm, wmax=wmax, flo=flo, fhi=fhi)
+    assert info["inmem"] == g.inmem and info["corr_uselen"] == g.corr_uselen
+    assert len(cands) == len(hc) > 0
+    for a, b in zip(cands, hc):
+        assert (a["numharm"], a["r"], a["z"], a["w"]) == \
+            (int(b["numharm"]), float(b["r"]), float(b["z"]), float(b["w"]))
+        assert np.float32(a["power"]) == b["power"] and np.float32(a["sigma"]) == b["sigma"]
+    assert open(info["accelnm"]).read() == open(hacc).read()
+    assert open(info["candnm"], "rb").read() == open(hacc + ".cand", "rb").read()
+    return info, cands
+
+
+def test_inmem_run_equals_handle_and_oracle(tmp_path):
+    print("case: surface in-memory run, 2^17 samples, zmax 50, numharm 8", flush=True)
+    N, dt = 1 << 17, 64e-6
+    x = synth.make_series_f32(N, dt, 71, [dict(f0=123.456, z=12.0, duty=0.05, amp=0.3),
+                                          dict(f0=377.7, z=-20.0, duty=0.08, amp=0.25)])
+    info, cands = _compare_run(tmp_path, "inmem", x, dt, 50, 8, expect_inmem=1)
+    # the oracle's block loop on the same spectrum (FP32-faithful mode)
+    with A.Search(A.make_params(N // 2, N, dt, zmax=50, numharm=8)) as s:
+        s.set_series(x)
+        spec = s.spectrum()
+    obs = O.make_obs(len(spec), N, dt, zmax=50, numharm=8, sigma=2.0)
+    _, oc, _ = O.accelsearch(obs, spec)
+    ours = sorted((c["numharm"], round(c["r"], 6), round(c["z"], 6)) for c in cands)
+    ref = sorted((int(c[2]), round(c[3], 6), round(c[4], 6)) for c in oc)
+    assert ours == ref
+
+
+def test_standard_run_equals_handle(tmp_path):
+    print("case: surface standard run, 2^23 samples, 4990-5000 Hz, zmax 200, numharm 8",
+          flush=True)
+    N, dt = 1 << 23, 64e-6
+    x = synth.make_series_f32(N, dt, 72, [dict(f0=4995.37, z=30.0, duty=0.2, amp=0.08)])
+    info, cands = _compare_run(tmp_path, "std", x, dt, 200, 8, flo=4990.0, fhi=5000.0,
+                               expect_inmem=0)
+    assert any(abs(c["r"] / (N * dt) - 4995.37) < 0.05 for c in cands)
+
+
+def test_jerk_run_equals_handle(tmp_path):
+    print("case: surface jerk run, 2^16 samples, zmax 20, wmax 60, numharm 4", flush=True)
+    N, dt = 1 << 16, 64e-6
+    x = synth.make_series_f32(N, dt, 73, [dict(f0=211.3, z=6.0, w=40.0, duty=0.05, amp=0.5)])
+    _compare_run(tmp_path, "jerk", x, dt, 20, 4, wmax=60, expect_inmem=0)
+
+
+class _OShi(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("numharm", "harmnum", "zmax", "wmax", "numkern_zdim",
+                                       "numkern_wdim", "numkern")] + \
+               [("kern", C.c_void_p), ("rinds", C.POINTER(C.c_ushort)),
+                ("zinds", C.POINTER(C.c_ushort))]
+
+
+def _synced(f):
+    assert S.lib.accel_ffdotpows_sync(f) == 0
+    d = f.contents
+    n = d.numws * d.numzs * d.numrs
+    flat = np.ctypeslib.as_array(d.powers[0][0], shape=(n,)).copy()
+    return flat.reshape(d.numws, d.numzs, d.numrs)
+
+
+@pytest.mark.parametrize("wmax", [None, 60])
+def test_block_planes_tables_and_adds(tmp_path, wmax):
+    print(f"case: surface block planes vs FP64 oracle, wmax {wmax}", flush=True)
+    N, dt = 1 << 16, 64e-6
+    x = synth.make_series_f32(N, dt, 74, [dict(f0=301.1, z=8.0, duty=0.05, amp=0.3)])
+    root = _write_dat(tmp_path, "blk", x, dt)
+    zmax, numharm = 30, 8
+    cmd = S.make_cmdline(root + ".dat", zmax=zmax, numharm=numharm, wmax=wmax)
+    obs, idata = S.AccelObs(), S.InfoData()
+    S.lib.create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), 1)
+    S._check()
+    # a plain 2^16 search is in-memory by the reference's memory rule (only
+    # its fundamental planes are checked here); a jerk search is standard
+    shis = S.lib.create_subharminfos(C.byref(obs))
+    S._check()
+    try:
+        spec = np.ctypeslib.as_array(C.cast(obs.fft, C.POINTER(C.c_float)),
+                                     shape=(2 * obs.numbins,)).copy().view(np.complex64)
+        oobs = O.make_obs(obs.numbins, N, dt, zmax=zmax, numharm=numharm, sigma=2.0, wmax=wmax,
+                          force_standard=True)
+        banks = O.SubharmBanks(oobs, fp64=True)
+        banks32 = O.SubharmBanks(oobs, fp64=False)
+        rstep = obs.corr_uselen * 0.5
+        for blk in (0, 3):
+            startr = obs.rlo + blk * rstep
+            lastr = startr + rstep - 0.5
+            fund = S.lib.subharm_fderivs_vol(1, 1, startr, lastr, shis[0], C.byref(obs))
+            S._check()
+            fa = _synced(fund)
+            ref, meta = O.subharm_fderivs_vol(1, 1, startr, lastr, banks, oobs, spec, fp64=True)
+            d = fund.contents
+            assert (d.rlo, d.zlo, d.wlo, d.numrs, d.numzs, d.numws) == \
+                (meta["rlo"], meta["zlo"], meta["wlo"], meta["numrs"], meta["numzs"],
+                 meta["numws"])
+            assert np.max(np.abs(fa - ref) / np.maximum(1.0, np.abs(ref))) < 1e-4
+            if obs.inmem:
+                S.lib.free_ffdotpows(fund)
+                continue
+            acc = fa.copy()
+            for stage, harm in ((1, 1), (2, 1), (2, 3), (3, 5)):
+                h = 1 << stage
+                sub = S.lib.subharm_fderivs_vol(h, harm, startr, lastr,
+                                                C.byref(shis[stage][harm - 1]), C.byref(obs))
+                S._check()
+                sa = _synced(sub)
+                sref, smeta = O.subharm_fderivs_vol(h, harm, startr, lastr, banks, oobs, spec,
+                                                    fp64=True)
+                sd = sub.contents
+                assert (sd.rlo, sd.zlo, sd.wlo, sd.numrs, sd.numzs, sd.numws) == \
+                    (smeta["rlo"], smeta["zlo"], smeta["wlo"], smeta["numrs"], smeta["numzs"],
+                     smeta["numws"])
+                assert np.max(np.abs(sa - sref) / np.maximum(1.0, np.abs(sref))) < 1e-4
+                # host tables == the oracle's (its FP32 banks fill them too)
+                O.subharm_fderivs_vol(h, harm, startr, lastr, banks32, oobs, spec, fp64=False)
+                oshi = C.cast(banks32.shi(stage, harm), C.POINTER(_OShi)).contents
+                ours = shis[stage][harm - 1]
+                nr, nz = obs.corr_uselen, obs.numz
+                rt = np.ctypeslib.as_array(ours.rinds, shape=(nr,)).copy()
+                zt = np.ctypeslib.as_array(ours.zinds, shape=(nz,)).copy()
+                assert np.array_equal(rt, np.ctypeslib.as_array(oshi.rinds, shape=(nr,)))
+                assert np.array_equal(zt, np.ctypeslib.as_array(oshi.zinds, shape=(nz,)))
+                # the device add gathers with exactly those tables
+                wlo = d.wlo
+                for i in range(d.numws):
+                    ww = wlo + i * 20
+                    subw = int(np.floor(0.05 * ww * (harm / h) + 0.5)) * 20 if ww >= 0 else \
+                        int(np.ceil(0.05 * ww * (harm / h) - 0.5)) * 20
+                    wind = int((subw - sd.wlo) * 0.05 + 1e-14)
+                    acc[i] += sa[wind][zt[:d.numzs]][:, rt[:d.numrs]]
+                S.lib.add_ffdotpows(fund, sub, h, harm)
+                S._check()
+                S.lib.free_ffdotpows(sub)
+            got = _synced(fund)
+            assert np.array_equal(got, acc)
+            S.lib.free_ffdotpows(fund)
+        banks.close()
+        banks32.close()
+    finally:
+        S.lib.free_subharminfos(C.byref(obs), shis)
+        S.lib.free_accelobs(C.byref(obs))
+
+
+def test_refusals_return_errors(tmp_path):
+    print("case: surface refusals (photon, bad suffix)", flush=True)
+    N, dt = 1 << 14, 64e-6
+    x = synth.make_series_f32(N, dt, 75)
+    root = _write_dat(tmp_path, "ref", x, dt)
+    obs, idata = S.AccelObs(), S.InfoData()
+    cmd = S.make_cmdline(root + ".dat")
+    cmd.photonP = 1
+    S.lib.create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), 1)
+    with pytest.raises(A.AccelError, match="photon"):
+        S._check()
+    cmd = S.make_cmdline(root + ".txt")
+    S.lib.create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), 1)
+    with pytest.raises(A.AccelError, match="must be an"):
+        S._check()
