@@ -622,8 +622,9 @@ __host__ __device__ constexpr int hs_stage_wl(int S)
     return n;
 }
 
-/* first LDS wave-load slot (64 floats) of stage S; stages 1-4 are resident
- * together, stage 5 (32 harmonics) reuses the buffer from slot 0 */
+/* first LDS wave-load slot (64 floats) of stage S; stages 1-3 are resident
+ * together, stage 4 and stage 5 (32 harmonics) each reuse the buffer from
+ * slot 0 */
 __host__ __device__ constexpr int hs_stage_base(int S)
 {
     int n = 0;
@@ -632,9 +633,15 @@ __host__ __device__ constexpr int hs_stage_base(int S)
     return n;
 }
 
-constexpr int HS_WL_1TO4 = hs_stage_base(5);
-constexpr int HS_WL_5 = hs_stage_wl(5);
-constexpr int HS_MAX_WL = HS_WL_1TO4 > HS_WL_5 ? HS_WL_1TO4 : HS_WL_5;
+__host__ __device__ constexpr int hs_max(int a, int b) { return a > b ? a : b; }
+
+/* LDS wave-load slots of a search with NSTAGES stages */
+__host__ __device__ constexpr int hs_lds_wl(int nstages)
+{
+    return nstages <= 4 ? hs_stage_base(nstages)
+                        : hs_max(hs_max(hs_stage_base(4), hs_stage_wl(4)),
+                                 nstages > 5 ? hs_stage_wl(5) : 0);
+}
 static_assert(hs_frac_wl(15, 16) * 64 <= AGPU_HALO && hs_frac_wl(31, 32) * 64 <= AGPU_HALO &&
                   hs_frac_wl(1, 2) * 64 <= AGPU_HALO,
               "every sub-harmonic span fits in a tile-row halo");
@@ -768,15 +775,17 @@ __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const fl
  * c % 8, so the sub-harmonic rows that the rows of a chunk share are read
  * once into that XCD's L2.  The fundamental values and every sub-harmonic
  * span of stages 1-4 are loaded with plain global loads, all in flight at
- * once, then the spans go to LDS; a 32-harmonic search loads stage 5 in a
- * second round.  Adding 256 to a fundamental half-bin adds an even
+ * once.  The spans of stages 1-3 go to LDS; stage 4's wait in registers
+ * and take the same LDS once stages 1-3 are summed, which halves the LDS
+ * of a 16-harmonic workgroup (18.8 KB: 5-6 workgroups per CU instead of
+ * 4).  A 32-harmonic search loads stage 5 in a third round.  Adding 256 to a fundamental half-bin adds an even
  * 256 k / 2^s to the rounded quotient, so one rounding per thread and
  * fraction serves all 4 cells. */
 template <int NSTAGES, int INMEM>
 __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int nchunks)
 {
     __shared__ float s_norm[3 * 32];   /* norm^2 of blocks b0..b0+2 x fraction */
-    constexpr int NWL = NSTAGES > 5 ? HS_MAX_WL : hs_stage_base(NSTAGES);
+    constexpr int NWL = hs_lds_wl(NSTAGES);
     __shared__ float lds[(NWL > 0 ? NWL : 1) * 64];
     const int nrows = a.numz * a.numw;
     const int bid = blockIdx.x;
@@ -814,7 +823,6 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int nch
     if (NSTAGES > 1) hs_store<1>(lds, hs_stage_base(1), v + hs_regs_upto(1), wv, lane);
     if (NSTAGES > 2) hs_store<2>(lds, hs_stage_base(2), v + hs_regs_upto(2), wv, lane);
     if (NSTAGES > 3) hs_store<3>(lds, hs_stage_base(3), v + hs_regs_upto(3), wv, lane);
-    if (NSTAGES > 4) hs_store<4>(lds, hs_stage_base(4), v + hs_regs_upto(4), wv, lane);
     /* cells outside [0, ncell) read valid plane memory (the chunk before the
      * first cell or the padding tile) and are never emitted */
     const int rh_first = rh_c0 + t;
@@ -852,7 +860,15 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int nch
     if (NSTAGES > 1) { HS_STAGE(1) }
     if (NSTAGES > 2) { HS_STAGE(2) }
     if (NSTAGES > 3) { HS_STAGE(3) }
-    if (NSTAGES > 4) { HS_STAGE(4) }
+    if (NSTAGES > 4) {
+        /* stage 4's spans waited in registers; they take the LDS of
+         * stages 1-3 once those are consumed */
+        __syncthreads();
+        hs_store<4>(lds, 0, v + hs_regs_upto(4), wv, lane);
+        __syncthreads();
+        hs_consume4<4, INMEM>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, rh_c0, rh_first, acc);
+        HS_EMIT(4)
+    }
     if (NSTAGES > 5) {
         float v5[hs_stage_regs(5)];
         hs_load<5, INMEM>(a, v5, zi, wi, nrows, rh_c0, wv, lane);
