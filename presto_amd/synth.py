@@ -80,6 +80,26 @@ def config3_trials(n=256, seed0=3000):
         yield i, N, dt, seed0 + i, pulsars
 
 
+def config4_trials(n=64, seed0=4000):
+    """The jerk workload: n spectra of 2^22 samples (dt 64 us, T = 268.4 s),
+    seeded noise (seed0 + i) plus one binary-like signal each, drawn from
+    PCG64(seed0 + 100000 + i): f0 in [50, 700] Hz, average z in
+    [-100, 100], w in [-300, 300], duty 5 %, amplitude 0.1.  The series
+    generator takes the starting f-dot, z0 = z_avg - w / 2.  Yields
+    (i, N, dt, seed, pulsars, (r_avg, z_avg, w))."""
+    N, dt = 1 << 22, 64e-6
+    T = N * dt
+    for i in range(n):
+        rng = np.random.Generator(np.random.PCG64(seed0 + 100000 + i))
+        f0 = float(rng.uniform(50.0, 700.0))
+        z_avg = float(rng.uniform(-100.0, 100.0))
+        w = float(rng.uniform(-300.0, 300.0))
+        z0 = z_avg - 0.5 * w
+        r_avg = f0 * T + 0.5 * z0 + w / 6.0
+        yield i, N, dt, seed0 + i, [dict(f0=f0, z=z0, w=w, duty=0.05, amp=0.1)], \
+            (r_avg, z_avg, w)
+
+
 def config5_requests(n, seed_arrivals=5, seed0=5_000_000, rate=40.0):
     """The serving workload: n requests with Poisson arrival times (rate per
     second, PCG64(seed_arrivals)); per request (PCG64(seed0 + i)): a length
