@@ -209,6 +209,11 @@ int accel_surface_set_device(int device);
  * (powers[w][z][r], one contiguous block, as gen_f3Darr makes it); the
  * host copy is freed by free_ffdotpows().  0 or a negative code. */
 int accel_ffdotpows_sync(ffdotpows *ffd);
+/* readinf / writeinf (src/ioinf.c:121-350): the <root>.inf of a data set
+ * into / from an infodata, in the reference's line layout.  0 or -1 with
+ * accel_last_error() set. */
+int accel_readinf(infodata *d, const char *root);
+int accel_writeinf(const infodata *d);
 /* Frees the nodes (not the data) of a list built by this library. */
 void accel_gslist_free(GSList *list);
 /* Layout report for the structs above: for struct `name`, writes its size

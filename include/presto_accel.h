@@ -159,7 +159,22 @@ int accel_search_windows(const accel_search *s, long long *lobin, int *numdata, 
  * (src/accel_utils.c:1547). */
 int accel_realfft_forward(const float *series, long long n, accel_fcomplex *out, int device);
 
-/* Running-median de-reddening of a spectrum in place; replaces deredden
+/* realfft(data, n, +1) (src/fastffts.c:163-243): the inverse of the above,
+ * scaled by 2/n so that it returns the original series; on the GPU, n a
+ * power of two.  Replaces the in-core inverse of realfft -inv
+ * (src/realfft.c:331). */
+int accel_realfft_inverse(const accel_fcomplex *spec, long long n, float *out, int device);
+
+/* deredden (src/misc_utils.c:1054-1067) of a host spectrum, computed on
+ * the GPU (exact block medians, the reference's FP32 interpolation and
+ * invsqrtf); bin 0 becomes (1, 0).  Returns the bins written or -1.  The
+ * de-reddening of realfft -rednoise and rednoise (src/realfft.c:342,
+ * src/rednoise.c:86). */
+long accel_deredden_gpu(accel_fcomplex *fft, long long numbins, int startwidth, int endwidth,
+                        double endfreq, double T, int device);
+
+/* Running-median de-reddening of a spectrum in place, on the host (the
+ * restatement the GPU version is checked against); replaces deredden
  * (src/misc_utils.c:1054).  Returns the bins written or -1. */
 long accel_deredden(accel_fcomplex *fft, long numbins, int startwidth, int endwidth,
                     double endfreq, double T);
@@ -180,6 +195,13 @@ int accel_read_birdies(const char *zaplist, double T, double baryv, double *lobi
  * (MT19937 seeded with 1, as the reference's ZAPSEED). */
 int accel_zap_birdies(accel_fcomplex *fft, long long numbins, double N, const double *lobins,
                       const double *hibins, int nbirds);
+/* The automatic mode of the zapbirds command (src/zapbirds.c:273-303) on a
+ * whole .fft held in memory: birdies are zapped while lobin < N/2 - 1,
+ * hibin clipped to N/2 - 1, over numbins = min(N/2, bins held);
+ * constamp: the constant-amplitude replacement of -constamp
+ * (src/zapping.c:124-128) instead of noise. */
+int accel_zapbirds_fft(accel_fcomplex *fft, long long numbins, double N, const double *lobins,
+                       const double *hibins, int nbirds, int constamp);
 /* n raw 32-bit outputs of MT19937 seeded with init_genrand(seed), the
  * generator behind the zapping noise (for tests). */
 int accel_mt19937_raw(unsigned int seed, unsigned int *out, int n);

@@ -82,3 +82,46 @@ def zap_birdies(fft, N, lobins, hibins):
             out[ilo + i] = np.complex64(complex(np.float32(amp * np.cos(ph)),
                                                 np.float32(amp * np.sin(ph))))
     return out
+
+
+def zapbirds_tool(fft, N, lobins, hibins, constamp=False):
+    """The automatic mode of the zapbirds command (src/zapbirds.c:285-303):
+    birdies zapped while lobin < N/2 - 1, over numbins = N/2 bins, and with
+    -constamp every zapped bin set to sqrt(local mean power) with zero phase
+    (src/zapping.c:124-128) instead of noise."""
+    out = np.array(fft, dtype=np.complex64, copy=True)
+    hibin = N / 2
+    numbins = min(int(hibin), len(out))
+    g = _Stream()
+    for lo, hi in zip(lobins, hibins):
+        if lo >= hibin - 1:
+            break
+        if hi >= hibin:
+            hi = hibin - 1
+        ilo, ihi = int(np.floor(lo)), int(np.ceil(hi))
+        nz = ihi - ilo
+        if nz < 1:
+            continue
+        lostart, histart = ilo - GUARDBINS - MEDIANBINS, ihi + GUARDBINS
+        uselo, usehi = lostart >= 1, histart + MEDIANBINS <= numbins
+        if not (uselo or usehi):
+            continue
+        med, sides = np.float32(0.0), 0
+        if uselo:
+            med = np.float32(med + _median_power(out[lostart:lostart + MEDIANBINS]))
+            sides += 1
+        if usehi:
+            med = np.float32(med + _median_power(out[histart:histart + MEDIANBINS]))
+            sides += 1
+        med = np.float32(med / np.float32(sides))
+        meanpow = np.float32(float(med) / -np.log(0.5))
+        if constamp:
+            out[ilo:ilo + nz] = np.complex64(complex(np.float32(np.sqrt(float(meanpow))), 0.0))
+            continue
+        for i in range(nz):
+            u = g.next() / 4294967296.0
+            amp = np.sqrt(-float(meanpow) * np.log1p(-u))
+            ph = 2 * np.pi * (g.next() / 4294967296.0)
+            out[ilo + i] = np.complex64(complex(np.float32(amp * np.cos(ph)),
+                                                np.float32(amp * np.sin(ph))))
+    return out

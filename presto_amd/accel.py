@@ -122,6 +122,9 @@ _sig("accel_read_birdies", _i, C.c_char_p, _d, _d, _p, _p, _i)
 _sig("accel_zap_birdies", _i, _p, _ll, _d, _p, _p, _i)
 _sig("accel_mt19937_raw", _i, C.c_uint, _p, _i)
 _sig("accel_realfft_forward", _i, _p, _ll, _p, _i)
+_sig("accel_realfft_inverse", _i, _p, _ll, _p, _i)
+_sig("accel_deredden_gpu", C.c_long, _p, _ll, _i, _i, _d, _d, _i)
+_sig("accel_zapbirds_fft", _i, _p, _ll, _d, _p, _p, _i, _i)
 
 EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_set_series", "accel_search_spectrum", "accel_dered_blocks",
@@ -139,6 +142,7 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_max_rzw_harmonics", "accel_calc_props",
             "accel_calc_rzwerrs", "accel_nice_output", "accel_read_birdies",
             "accel_zap_birdies", "accel_mt19937_raw", "accel_realfft_forward",
+            "accel_realfft_inverse", "accel_deredden_gpu", "accel_zapbirds_fft",
             "accel_batch_create", "accel_batch_destroy", "accel_batch_handles",
             "accel_batch_run", "accel_search_footprint", "accel_template_cache_clear",
             "accel_server_create",
@@ -557,6 +561,26 @@ def realfft_gpu(x, device=0):
     if lib.accel_realfft_forward(x.ctypes.data, len(x), out.ctypes.data, int(device)):
         raise _err()
     return out
+
+
+def realfft_inverse_gpu(spec, device=0):
+    """realfft(data, n, +1) on the GPU: the series whose PRESTO-packed
+    spectrum is spec (n/2 complex bins)."""
+    spec = np.ascontiguousarray(spec, dtype=np.complex64)
+    out = np.zeros(2 * len(spec), dtype=np.float32)
+    if lib.accel_realfft_inverse(spec.ctypes.data, len(out), out.ctypes.data, int(device)):
+        raise _err()
+    return out
+
+
+def deredden_gpu(fft, startwidth, endwidth, endfreq, T, device=0):
+    """deredden of a spectrum computed on the GPU (a de-reddened copy)."""
+    out = np.ascontiguousarray(fft, dtype=np.complex64).copy()
+    n = lib.accel_deredden_gpu(out.ctypes.data, len(out), int(startwidth), int(endwidth),
+                               float(endfreq), float(T), int(device))
+    if n < 0:
+        raise _err()
+    return out, n
 
 
 def spectrum_from_series_gpu(x, dt, device=0):

@@ -51,6 +51,15 @@ def build(verbose=False):
     tmp = CLI + ".tmp"
     run(["gcc", "-o", tmp, cli_obj, "-L", HERE, "-lpresto_accel", "-Wl,-rpath,$ORIGIN", "-lm"])
     os.replace(tmp, CLI)
+    # realfft, rednoise, zapbirds: one source, three commands
+    for tool in ("realfft", "rednoise", "zapbirds"):
+        obj = os.path.join(BUILD, tool + ".o")
+        run(["gcc"] + C_FLAGS + ["-DTOOL_" + tool.upper(), "-c",
+                                 os.path.join(CSRC, "fft_tools.c"), "-o", obj])
+        out = os.path.join(HERE, tool)
+        run(["gcc", "-o", out + ".tmp", obj, "-L", HERE, "-lpresto_accel",
+             "-Wl,-rpath,$ORIGIN", "-lm"])
+        os.replace(out + ".tmp", out)
     if verbose:
         print("built", LIB)
     return LIB

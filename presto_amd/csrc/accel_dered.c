@@ -313,7 +313,7 @@ static float median_power(const accel_fcomplex *d, int n)
  * [floor(lo), ceil(hi)) become chi^2(2)-distributed noise at the local
  * mean power measured MEDIANBINS wide, GUARDBINS away on each side. */
 static int zap_one(accel_fcomplex *fft, long long numbins, double lobin, double hibin,
-                   mt19937 *g)
+                   mt19937 *g, int constamp)
 {
     const double TWOPI = 6.28318530717958647692;
     long long ilo = (long long) floor(lobin), ihi = (long long) ceil(hibin);
@@ -338,6 +338,15 @@ static int zap_one(accel_fcomplex *fft, long long numbins, double lobin, double 
     }
     medpow /= sides;
     const float meanpow = medpow / -log(0.5);
+    if (constamp) {
+        /* the PRESTO 6.0.1 behaviour: sqrt(local mean power), zero phase */
+        const float avgamp = sqrt(meanpow);
+        for (long long i = 0; i < nz; i++) {
+            fft[ilo + i].r = avgamp;
+            fft[ilo + i].i = 0.0f;
+        }
+        return 0;
+    }
     for (long long i = 0; i < nz; i++) {
         /* gsl_ran_exponential(rng, mu) = -mu log1p(-u), u = next / 2^32 */
         double u = mt_next(g) / 4294967296.0;
@@ -361,7 +370,26 @@ int accel_zap_birdies(accel_fcomplex *fft, long long numbins, double N, const do
             break;
         if (hi >= hibin)
             hi = hibin - 1;
-        if (zap_one(fft, numbins, lo, hi, &g))
+        if (zap_one(fft, numbins, lo, hi, &g, 0))
+            return -1;
+    }
+    return 0;
+}
+
+int accel_zapbirds_fft(accel_fcomplex *fft, long long numbins, double N, const double *lobins,
+                       const double *hibins, int nbirds, int constamp)
+{
+    mt19937 g;
+    mt_seed(&g, 1);
+    const double hibin = N / 2;
+    const long long nb = (long long) hibin < numbins ? (long long) hibin : numbins;
+    for (int i = 0; i < nbirds; i++) {
+        double lo = lobins[i], hi = hibins[i];
+        if (lo >= hibin - 1)
+            break;
+        if (hi >= hibin)
+            hi = hibin - 1;
+        if (zap_one(fft, nb, lo, hi, &g, constamp))
             return -1;
     }
     return 0;

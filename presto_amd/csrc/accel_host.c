@@ -1635,6 +1635,89 @@ done:
     return rc;
 }
 
+int accel_realfft_inverse(const accel_fcomplex *spec, long long n, float *out, int device)
+{
+    void *stream = NULL, *d_x = NULL, *d_work = NULL;
+    int rc = -1;
+    if (n < 4 || (n & (n - 1))) {
+        set_err("accel_realfft_inverse: length %lld is not a power of two >= 4", n);
+        return -1;
+    }
+    if (agpu_init(device) || agpu_stream_create(&stream) ||
+        agpu_malloc(&d_x, sizeof(float) * (size_t) n) ||
+        agpu_malloc(&d_work, sizeof(accel_fcomplex) * (size_t) (3 * n / 4)) ||
+        agpu_h2d(d_x, spec, sizeof(float) * (size_t) n, stream) ||
+        agpu_realfft_inverse(d_x, n, (float *) d_x, d_work, stream) ||
+        agpu_d2h(out, d_x, sizeof(float) * (size_t) n, stream)) {
+        set_err("accel_realfft_inverse: %s", agpu_last_error());
+        goto done;
+    }
+    rc = 0;
+done:
+    if (d_x)
+        agpu_free(d_x);
+    if (d_work)
+        agpu_free(d_work);
+    if (stream)
+        agpu_stream_destroy(stream);
+    return rc;
+}
+
+long accel_deredden_gpu(accel_fcomplex *fft, long long numbins, int startwidth, int endwidth,
+                        double endfreq, double T, int device)
+{
+    void *stream = NULL, *d_spec = NULL, *d_blk = NULL, *d_mean = NULL;
+    long rc = -1;
+    const int nb = accel_dered_blocks((long) numbins, startwidth, endwidth, endfreq, T, NULL,
+                                      NULL, 0);
+    if (nb <= 0) {
+        set_err("accel_deredden_gpu: spectrum of %lld bins too short for a %d-bin window",
+                numbins, startwidth);
+        return -1;
+    }
+    long long *st = (long long *) malloc(sizeof(long long) * nb);
+    int *ln = (int *) malloc(sizeof(int) * nb);
+    agpu_window *w = (agpu_window *) calloc(nb, sizeof(agpu_window));
+    if (!st || !ln || !w) {
+        set_err("accel_deredden_gpu: out of host memory");
+        goto done;
+    }
+    accel_dered_blocks((long) numbins, startwidth, endwidth, endfreq, T, st, ln, nb);
+    int maxn = 0;
+    for (int i = 0; i < nb; i++) {
+        w[i].lobin = st[i];
+        w[i].numdata = ln[i];
+        if (ln[i] > maxn)
+            maxn = ln[i];
+    }
+    if (agpu_init(device) || agpu_stream_create(&stream) ||
+        agpu_malloc(&d_spec, sizeof(accel_fcomplex) * (size_t) numbins) ||
+        agpu_malloc(&d_blk, sizeof(agpu_window) * nb) ||
+        agpu_malloc(&d_mean, sizeof(float) * nb) ||
+        agpu_h2d(d_blk, w, sizeof(agpu_window) * nb, stream) ||
+        agpu_h2d(d_spec, fft, sizeof(accel_fcomplex) * (size_t) numbins, stream) ||
+        agpu_deredden(d_spec, numbins, (const agpu_window *) d_blk, nb, maxn, (float *) d_mean,
+                      stream) ||
+        agpu_d2h(fft, d_spec, sizeof(accel_fcomplex) * (size_t) numbins, stream)) {
+        set_err("accel_deredden_gpu: %s", agpu_last_error());
+        goto done;
+    }
+    rc = (long) numbins;
+done:
+    free(st);
+    free(ln);
+    free(w);
+    if (d_spec)
+        agpu_free(d_spec);
+    if (d_blk)
+        agpu_free(d_blk);
+    if (d_mean)
+        agpu_free(d_mean);
+    if (stream)
+        agpu_stream_destroy(stream);
+    return rc;
+}
+
 int accel_search_set_series(accel_search *s, const float *series, long long n)
 {
     const accel_geom *g = &s->g;

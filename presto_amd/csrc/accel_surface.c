@@ -404,47 +404,40 @@ static int inf_value(FILE *f, char *out, size_t outlen)
     return 0;
 }
 
+/* ra_dec_from_string (src/misc_utils.c:715-737): hh:mm:ss.ssss; a '-'
+ * before a zero hour/degree field is carried by minutes and seconds */
 static void ra_dec(const char *s, int *h, int *m, double *sec)
 {
-    int sign = 1;
-    while (*s == ' ')
+    while (*s == ' ' || *s == '\t')
         s++;
-    if (*s == '-') {
-        sign = -1;
-        s++;
-    }
-    int a = 0, b = 0;
-    double c = 0.0;
-    sscanf(s, "%d:%d:%lf", &a, &b, &c);
-    *h = sign * a;
-    *m = sign * b;
-    *sec = sign * c;
-    /* the reference keeps the sign only on the leading non-zero field */
-    if (sign < 0 && a != 0) {
-        *m = b;
-        *sec = c;
-    } else if (sign < 0 && b != 0) {
-        *sec = c;
+    *h = 0;
+    *m = 0;
+    *sec = 0.0;
+    sscanf(s, "%d:%d:%lf", h, m, sec);
+    if (s[0] == '-' && *h == 0) {
+        *m = -*m;
+        *sec = -*sec;
     }
 }
 
-static int sf_readinf(infodata *d, const char *root)
+int accel_readinf(infodata *d, const char *root)
 {
     char path[4200], t[256];
     snprintf(path, sizeof(path), "%s.inf", root);
     FILE *f = fopen(path, "r");
     if (!f) {
-        sf_fail(-1, "Error opening file '%s'", path);
+        accel_set_error("Error opening file '%s'", path);
         return -1;
     }
     memset(d, 0, sizeof(*d));
-#define RD(dst)                                                  \
-    do {                                                         \
-        if (inf_value(f, dst, sizeof(dst))) {                    \
-            fclose(f);                                           \
-            sf_fail(EXIT_FAILURE, "Error reading '%s'", path);   \
-            return -1;                                           \
-        }                                                        \
+#define RD(dst)                                                                 \
+    do {                                                                        \
+        if (inf_value(f, dst, sizeof(dst))) {                                   \
+            fclose(f);                                                          \
+            accel_set_error("Error:  end-of-file or blank line reading '%s'",   \
+                            path);                                              \
+            return -1;                                                          \
+        }                                                                       \
     } while (0)
     RD(d->name);
     RD(d->telescope);
@@ -483,8 +476,7 @@ static int sf_readinf(infodata *d, const char *root)
             RD(t);
             if (sscanf(t, "%lf %*[ ,] %lf", &d->onoff[ii], &d->onoff[ii + 1]) != 2) {
                 fclose(f);
-                sf_fail(EXIT_FAILURE, "Error:  can not parse on-off pair (%d) in readinf()",
-                        ii / 2);
+                accel_set_error("Error:  can not parse on-off pair (%d) in readinf()", ii / 2);
                 return -1;
             }
             ii += 2;
@@ -550,6 +542,92 @@ static int sf_readinf(infodata *d, const char *root)
         }
     }
     fclose(f);
+    return 0;
+}
+
+/* ra_dec_to_string (src/misc_utils.c:700-712) */
+static void ra_dec_str(char *out, size_t n, int h, int m, double s)
+{
+    int off = 0;
+    if (h == 0 && (m < 0 || s < 0.0)) {
+        out[0] = '-';
+        off = 1;
+    }
+    snprintf(out + off, n - off, "%.2d:%.2d:%07.4f", h, abs(m), fabs(s));
+}
+
+static const char *or_unknown(const char *s, char *buf, size_t n)
+{
+    snprintf(buf, n, "%s", s);
+    return *trim(buf) ? s : "Unknown";
+}
+
+int accel_writeinf(const infodata *d)
+{
+    char path[4200], t[128], u[512];
+    snprintf(path, sizeof(path), "%s.inf", d->name);
+    FILE *f = fopen(path, "w");
+    if (!f) {
+        accel_set_error("Error opening file '%s' for writing", path);
+        return -1;
+    }
+    const int fake = strcmp(d->telescope, "None (Artificial Data Set)") == 0;
+    fprintf(f, " Data file name without suffix          =  %s\n", d->name);
+    fprintf(f, " Telescope used                         =  %s\n",
+            or_unknown(d->telescope, u, sizeof(u)));
+    if (!fake) {
+        fprintf(f, " Instrument used                        =  %s\n",
+                or_unknown(d->instrument, u, sizeof(u)));
+        fprintf(f, " Object being observed                  =  %s\n",
+                or_unknown(d->object, u, sizeof(u)));
+        ra_dec_str(t, sizeof(t), d->ra_h, d->ra_m, d->ra_s);
+        fprintf(f, " J2000 Right Ascension (hh:mm:ss.ssss)  =  %s\n", t);
+        ra_dec_str(t, sizeof(t), d->dec_d, d->dec_m, d->dec_s);
+        fprintf(f, " J2000 Declination     (dd:mm:ss.ssss)  =  %s\n", t);
+        fprintf(f, " Data observed by                       =  %s\n",
+                or_unknown(d->observer, u, sizeof(u)));
+        char fr[64];
+        snprintf(fr, sizeof(fr), "%.15f", d->mjd_f);
+        const char *dot = strchr(fr, '.');
+        fprintf(f, " Epoch of observation (MJD)             =  %d.%s\n", d->mjd_i,
+                dot ? dot + 1 : "0");
+        fprintf(f, " Barycentered?           (1 yes, 0 no)  =  %d\n", d->bary);
+    }
+    fprintf(f, " Number of bins in the time series      =  %-11.0f\n", d->N);
+    fprintf(f, " Width of each time series bin (sec)    =  %.15g\n", d->dt);
+    fprintf(f, " Any breaks in the data? (1 yes, 0 no)  =  %d\n", d->numonoff > 1 ? 1 : 0);
+    if (d->numonoff > 1)
+        for (int i = 0; i < d->numonoff; i++)
+            fprintf(f, " On/Off bin pair #%3d                   =  %-11.0f, %-11.0f\n", i + 1,
+                    d->onoff[2 * i], d->onoff[2 * i + 1]);
+    if (!fake) {
+        fprintf(f, " Type of observation (EM band)          =  %s\n", d->band);
+        if (strcmp(d->band, "Radio") == 0) {
+            fprintf(f, " Beam diameter (arcsec)                 =  %.0f\n", d->fov);
+            fprintf(f, " Dispersion measure (cm-3 pc)           =  %.12g\n", d->dm);
+            fprintf(f, " Central freq of low channel (MHz)      =  %.12g\n", d->freq);
+            fprintf(f, " Total bandwidth (MHz)                  =  %.12g\n", d->freqband);
+            fprintf(f, " Number of channels                     =  %d\n", d->num_chan);
+            fprintf(f, " Channel bandwidth (MHz)                =  %.12g\n", d->chan_wid);
+        } else if (strcmp(d->band, "X-ray") == 0 || strcmp(d->band, "Gamma") == 0) {
+            fprintf(f, " Field-of-view diameter (arcsec)        =  %.2f\n", d->fov);
+            fprintf(f, " Central energy (kev)                   =  %.1f\n", d->energy);
+            fprintf(f, " Energy bandpass (kev)                  =  %.1f\n", d->energyband);
+        } else {
+            fprintf(f, " Photometric filter used                =  %s\n",
+                    or_unknown(d->filt, u, sizeof(u)));
+            fprintf(f, " Field-of-view diameter (arcsec)        =  %.2f\n", d->fov);
+            fprintf(f, " Central wavelength (nm)                =  %.1f\n", d->wavelen);
+            fprintf(f, " Bandpass (nm)                          =  %.1f\n", d->waveband);
+        }
+    }
+    fprintf(f, " Data analyzed by                       =  %s\n",
+            or_unknown(d->analyzer, u, sizeof(u)));
+    fprintf(f, " Any additional notes:\n    %s\n\n", d->notes);
+    if (fclose(f)) {
+        accel_set_error("Error writing '%s'", path);
+        return -1;
+    }
     return 0;
 }
 
@@ -626,8 +704,10 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
         return;
     }
     obs->use_harmonic_polishing = 1;
-    if (sf_readinf(idata, obs->rootfilenm))
+    if (accel_readinf(idata, obs->rootfilenm)) {
+        sf_fail(-1, "%s", accel_last_error());
         return;
+    }
     if (strlen(trim(idata->object)) > 0)
         printf("Analyzing %s data from '%s'.\n\n", idata->object, cmd->argv[0]);
     else

@@ -204,10 +204,14 @@ int main(int argc, char **argv)
         }
         free(x);
         printf("done.\nRemoving red-noise...");
-        if (accel_prepare_dat_spectrum(fft, numbins, (double) n, inf.dt) < 0) {
-            fprintf(stderr, "\naccelsearch: de-reddening failed\n");
+        /* src/accel_utils.c:1561 and 1621-1624, the de-reddening on the GPU */
+        if (accel_deredden_gpu(fft, numbins, 6, 200, 0.5 / inf.dt, inf.dt * (double) n,
+                               device) < 0) {
+            fprintf(stderr, "\naccelsearch: de-reddening failed: %s\n", accel_last_error());
             return 1;
         }
+        fft[0].r = 1.0f;
+        fft[0].i = 1.0f;
         printf("done.\n\n");
         p.N = (double) n;
         p.dat_input = 1;

@@ -183,6 +183,9 @@ int agpu_deredden(void *d_spec, long long numbins, const agpu_window *d_blk, int
 /* Forward real FFT of n (a power of two) FP32 samples in PRESTO's packing
  * into d_out (n/2 complex); d_work holds 3n/4 complex values. */
 int agpu_realfft(const float *d_x, long long n, void *d_out, void *d_work, void *stream);
+/* Its inverse (realfft with isign = +1, scaled by 2/n): n/2 complex bins in
+ * PRESTO's packing to n samples; d_X may be d_out; the same d_work size. */
+int agpu_realfft_inverse(const void *d_X, long long n, float *d_out, void *d_work, void *stream);
 
 /* Candidate refinement (harmonic polishing), one workgroup per candidate.
  * spec: the resident spectrum (numdata complex FP32 bins, bin 0 = lobin);

@@ -74,6 +74,79 @@ __global__ void k_realfft_post(const float2 *Z, float2 *X, long long M)
     X[k] = make_float2((float) (er + c * or_ - s * oi), (float) (ei + c * oi + s * or_));
 }
 
+/* Inverse split (realfft with isign = +1, src/fastffts.c:186-242): from
+ * X (PRESTO packing) form Z[k] = E[k] + i O[k], E = (X[k] + conj X[M-k]) / 2,
+ * O = (X[k] - conj X[M-k]) / (2 e^{-2 pi i k / N}), Z[0] = ((DC + Nyq) / 2,
+ * (DC - Nyq) / 2); written conjugated, so that the forward passes give
+ * conj of the inverse transform.  Arithmetic in FP64. */
+__global__ void k_realfft_pre_inv(const float2 *X, float2 *Zc, long long M)
+{
+    long long k = (long long) blockIdx.x * FFT_TPB + threadIdx.x;
+    if (k >= M)
+        return;
+    if (k == 0) {
+        const double dc = X[0].x, ny = X[0].y;
+        Zc[0] = make_float2((float) (0.5 * (dc + ny)), (float) (-0.5 * (dc - ny)));
+        return;
+    }
+    const float2 xk = X[k], xm = X[M - k];
+    const double er = 0.5 * ((double) xk.x + xm.x), ei = 0.5 * ((double) xk.y - xm.y);
+    const double dr = 0.5 * ((double) xk.x - xm.x), di = 0.5 * ((double) xk.y + xm.y);
+    double s, c;
+    sincospi((double) k / (double) M, &s, &c);   /* 1 / e^{-2 pi i k / N} */
+    const double or_ = dr * c - di * s, oi = dr * s + di * c;
+    /* Z = E + i O */
+    Zc[k] = make_float2((float) (er - oi), (float) (-(ei + or_)));
+}
+
+/* x[2n] + i x[2n+1] = conj(F[n]) / M, F the forward transform of conj Z */
+__global__ void k_realfft_post_inv(const float2 *F, float2 *x, long long M)
+{
+    long long n = (long long) blockIdx.x * FFT_TPB + threadIdx.x;
+    if (n >= M)
+        return;
+    const double sc = 1.0 / (double) M;
+    const float2 f = F[n];
+    x[n] = make_float2((float) (f.x * sc), (float) (-f.y * sc));
+}
+
+int agpu_realfft_inverse(const void *d_X, long long n, float *d_out, void *d_work, void *stream)
+{
+    hipStream_t st = (hipStream_t) stream;
+    if (n < 4 || (n & (n - 1))) {
+        agpu_set_error("agpu_realfft_inverse: the length must be a power of two >= 4");
+        return -2;
+    }
+    const long long M = n / 2, half = M / 2;
+    float2 *a = (float2 *) d_out, *b = (float2 *) d_work;
+    float2 *tw = b + M;
+    const dim3 blk(FFT_TPB);
+    hipLaunchKernelGGL(k_fft_twiddles, dim3((half + FFT_TPB - 1) / FFT_TPB), blk, 0, st, tw,
+                       half, M);
+    FFT_CHECK(hipGetLastError());
+    /* conj Z into b (the input may be d_out itself) */
+    hipLaunchKernelGGL(k_realfft_pre_inv, dim3((M + FFT_TPB - 1) / FFT_TPB), blk, 0, st,
+                       (const float2 *) d_X, b, M);
+    FFT_CHECK(hipGetLastError());
+    float2 *src = b, *dst = a;
+    for (long long Ns = 1; Ns < M; Ns <<= 1) {
+        hipLaunchKernelGGL(k_fft_pass, dim3((half + FFT_TPB - 1) / FFT_TPB), blk, 0, st, src,
+                           dst, tw, M, Ns);
+        FFT_CHECK(hipGetLastError());
+        float2 *t = src;
+        src = dst;
+        dst = t;
+    }
+    /* src holds F; the post pass writes x into the other buffer */
+    float2 *xo = (src == a) ? b : a;
+    hipLaunchKernelGGL(k_realfft_post_inv, dim3((M + FFT_TPB - 1) / FFT_TPB), blk, 0, st, src,
+                       xo, M);
+    FFT_CHECK(hipGetLastError());
+    if (xo != a)
+        FFT_CHECK(hipMemcpyAsync(a, xo, sizeof(float2) * M, hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
 int agpu_realfft(const float *d_x, long long n, void *d_out, void *d_work, void *stream)
 {
     hipStream_t st = (hipStream_t) stream;

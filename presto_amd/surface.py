@@ -132,6 +132,8 @@ _sig("accel_surface_set_device", _i, _i)
 _sig("accel_ffdotpows_sync", _i, _FP)
 _sig("accel_gslist_free", None, _LP)
 _sig("accel_surface_layout", _i, _cp, C.POINTER(_ll), _i)
+_sig("accel_readinf", _i, C.POINTER(InfoData), _cp)
+_sig("accel_writeinf", _i, C.POINTER(InfoData))
 _sig("accel_calc_props", _i, C.POINTER(RDerivs), _d, _d, _d, C.POINTER(FourierProps))
 _sig("accel_last_error", _cp)
 
@@ -143,7 +145,7 @@ EXPORTED = ["create_accelobs", "free_accelobs", "create_subharminfos", "free_sub
             "output_fundamentals", "output_harmonics", "free_accelcand", "print_accelcand",
             "get_fourier_amplitudes", "accel_surface_set_exit", "accel_surface_failed",
             "accel_surface_clear", "accel_surface_set_device", "accel_ffdotpows_sync",
-            "accel_gslist_free", "accel_surface_layout"]
+            "accel_gslist_free", "accel_surface_layout", "accel_readinf", "accel_writeinf"]
 
 # failures raise in Python instead of calling exit()
 lib.accel_surface_set_exit(0)
@@ -161,6 +163,18 @@ def layout(name):
     if n < 0:
         raise KeyError(name)
     return list(out[:n])
+
+
+def readinf(root):
+    d = InfoData()
+    if lib.accel_readinf(C.byref(d), root.encode()):
+        raise AccelError(lib.accel_last_error().decode())
+    return d
+
+
+def writeinf(d):
+    if lib.accel_writeinf(C.byref(d)):
+        raise AccelError(lib.accel_last_error().decode())
 
 
 def write_inf(root, N, dt, telescope="None (Artificial Data Set)"):
