@@ -513,12 +513,13 @@ __device__ __forceinline__ void fft4096(float2 *v, float2 *lds, const float2 *tw
 __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long numbins,
                                                   long long lobin_file, const float2 *T,
                                                   int nt, int hw, int uselen, long long hb_end,
-                                                  float *plane, int groups, int tpg)
+                                                  float *plane, int groups, int tpg,
+                                                  long long seg0)
 {
     __shared__ float2 lds[FFT_LDS];
     const int j = threadIdx.x;
     const int g = blockIdx.x % groups;
-    const long long seg = blockIdx.x / groups;
+    const long long seg = seg0 + blockIdx.x / groups;
     const long long seg_hb = seg * (long long) uselen;
     const long long lobin = seg_hb / 2 - hw;
     const int t0 = g * tpg;
@@ -571,10 +572,11 @@ __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long 
 
 /* Tile-row halos: row r of tile t repeats the first AGPU_HALO half-bins of
  * row r of tile t+1.  One workgroup per (tile, row), float4 copies. */
-__global__ __launch_bounds__(256) void k_plane_halo(float *plane, int nrows, int ntiles)
+__global__ __launch_bounds__(256) void k_plane_halo(float *plane, int nrows, int ntiles,
+                                                    int tile0)
 {
     const int row = blockIdx.x % nrows;
-    const int tile = blockIdx.x / nrows;
+    const int tile = tile0 + blockIdx.x / nrows;
     if (tile + 1 >= ntiles)
         return;
     const float4 *src = (const float4 *) (plane + AGPU_PLANE_IDX(row, (long long) (tile + 1) * AGPU_TILE, nrows));
@@ -782,7 +784,7 @@ __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const fl
  * 256 k / 2^s to the rounded quotient, so one rounding per thread and
  * fraction serves all 4 cells. */
 template <int NSTAGES, int INMEM>
-__global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int nchunks)
+__global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_lo, int c_hi)
 {
     __shared__ float s_norm[3 * 32];   /* norm^2 of blocks b0..b0+2 x fraction */
     constexpr int NWL = hs_lds_wl(NSTAGES);
@@ -791,8 +793,8 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int nch
     const int bid = blockIdx.x;
     const int xcd = bid & 7, local = bid >> 3;
     const int m = local / nrows, row = local - m * nrows;
-    const int chunk = xcd + 8 * m;
-    if (chunk >= nchunks)
+    const int chunk = c_lo + xcd + 8 * m;
+    if (chunk >= c_hi)
         return;
     const int wi = row / a.numz, zi = row - wi * a.numz;
     const int ncell = a.nblocks * a.corr_uselen;
@@ -1232,7 +1234,8 @@ int agpu_block_norm1(const void *d_spec, long long numbins, long long lobin_file
 
 int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
                    const void *d_T, int nt, int L, int hw, int uselen, long long hb_end,
-                   float *d_plane, long long ntiles, int groups, void *stream)
+                   float *d_plane, long long ntiles, int groups, long long seg_lo,
+                   long long seg_hi, void *stream)
 {
     if (L != 4096)
         return fail("agpu_correlate: only L = 4096 is built");
@@ -1243,31 +1246,55 @@ int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
     if (hb_end > ntiles * AGPU_TILE)
         return fail("agpu_correlate: plane tiles do not cover the half-bin range");
     long long nseg = (hb_end + uselen - 1) / uselen;
+    if (seg_hi < 0 || seg_hi > nseg)
+        seg_hi = nseg;
+    if (seg_lo < 0 || seg_lo > seg_hi)
+        return fail("agpu_correlate: bad segment range");
+    if (seg_lo == seg_hi)
+        return 0;
     int tpg = (nt + groups - 1) / groups;
-    long long nblk = nseg * groups;
+    long long nblk = (seg_hi - seg_lo) * groups;
     if (nblk > 0x7fffffffLL)
         return fail("agpu_correlate: grid too large");
     hipLaunchKernelGGL(k_corr4096, dim3((unsigned) nblk), dim3(256), 0, (hipStream_t) stream,
                        (const float2 *) d_spec, numbins, lobin_file, (const float2 *) d_T, nt,
-                       hw, uselen, hb_end, d_plane, groups, tpg);
+                       hw, uselen, hb_end, d_plane, groups, tpg, seg_lo);
     AGPU_CHECK(hipGetLastError());
     return 0;
 }
 
-int agpu_plane_halo(float *d_plane, int nrows, long long ntiles, void *stream)
+int agpu_plane_halo(float *d_plane, int nrows, long long ntiles, long long tile_lo,
+                    long long tile_hi, void *stream)
 {
-    long long nblk = (long long) nrows * ntiles;
+    if (tile_hi < 0 || tile_hi > ntiles)
+        tile_hi = ntiles;
+    if (tile_lo < 0 || tile_lo > tile_hi)
+        return fail("agpu_plane_halo: bad tile range");
+    long long nblk = (long long) nrows * (tile_hi - tile_lo);
     if (nblk > 0x7fffffffLL)
         return fail("agpu_plane_halo: grid too large");
     if (nblk == 0)
         return 0;
     hipLaunchKernelGGL(k_plane_halo, dim3((unsigned) nblk), dim3(256), 0, (hipStream_t) stream,
-                       d_plane, nrows, (int) ntiles);
+                       d_plane, nrows, (int) ntiles, (int) tile_lo);
     AGPU_CHECK(hipGetLastError());
     return 0;
 }
 
-int agpu_harmsum(const agpu_harmsum_args *a, void *stream)
+long long agpu_harmsum_nchunks(const agpu_harmsum_args *a)
+{
+    const long long ncell = (long long) a->nblocks * a->corr_uselen;
+    if (ncell <= 0)
+        return 0;
+    return (a->rh0 + ncell - 1) / HS_CHUNK - a->rh0 / HS_CHUNK + 1;
+}
+
+long long agpu_harmsum_chunk_hb(const agpu_harmsum_args *a, long long c)
+{
+    return (a->rh0 / HS_CHUNK + c) * HS_CHUNK;
+}
+
+int agpu_harmsum(const agpu_harmsum_args *a, long long c_lo, long long c_hi, void *stream)
 {
     long long ncell = (long long) a->nblocks * a->corr_uselen;
     if (ncell <= 0)
@@ -1288,19 +1315,25 @@ int agpu_harmsum(const agpu_harmsum_args *a, void *stream)
     if ((rh_end + HS_CHUNK) * (long long) ((1 << (a->nstages - 1)) - 1) > 0x7fffffffLL)
         return fail("agpu_harmsum: spectrum too long for 32-bit sub-harmonic indices "
                     "(half-bins x highest harmonic must stay below 2^31)");
+    if (c_hi < 0 || c_hi > nchunks)
+        c_hi = nchunks;
+    if (c_lo < 0 || c_lo > c_hi)
+        return fail("agpu_harmsum: bad chunk range");
+    if (c_lo == c_hi)
+        return 0;
     long long rows = (long long) a->numz * a->numw;
-    long long nblk = 8 * ((nchunks + 7) / 8) * rows;
+    long long nblk = 8 * ((c_hi - c_lo + 7) / 8) * rows;
     if (nblk > 0x7fffffffLL)
         return fail("agpu_harmsum: grid too large");
     dim3 grid((unsigned) nblk);
     hipStream_t st = (hipStream_t) stream;
-    int nc = (int) nchunks;
-#define HS_CASE(NS)                                                                     \
-    case NS:                                                                            \
-        if (a->inmem)                                                                   \
-            hipLaunchKernelGGL((k_harmsum<NS, 1>), grid, dim3(HS_TPB), 0, st, *a, nc);  \
-        else                                                                            \
-            hipLaunchKernelGGL((k_harmsum<NS, 0>), grid, dim3(HS_TPB), 0, st, *a, nc);  \
+    const int cl = (int) c_lo, ch = (int) c_hi;
+#define HS_CASE(NS)                                                                         \
+    case NS:                                                                                \
+        if (a->inmem)                                                                       \
+            hipLaunchKernelGGL((k_harmsum<NS, 1>), grid, dim3(HS_TPB), 0, st, *a, cl, ch);  \
+        else                                                                                \
+            hipLaunchKernelGGL((k_harmsum<NS, 0>), grid, dim3(HS_TPB), 0, st, *a, cl, ch);  \
         break;
     switch (a->nstages) {
         HS_CASE(1)

@@ -848,12 +848,12 @@ int accel_search_launch(accel_search *s)
                                  g->corr_uselen, s->hb_end, s->d_nsq_hb, s->stream));
     GPU(agpu_event_record(s->ev[1], s->stream));
     GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw, s->uselen,
-                       s->hb_end, s->d_plane, s->ntiles, s->groups, s->stream));
+                       s->hb_end, s->d_plane, s->ntiles, s->groups, 0, -1, s->stream));
     GPU(agpu_event_record(s->ev[2], s->stream));
-    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, s->stream));
+    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, 0, -1, s->stream));
     GPU(agpu_event_record(s->ev[3], s->stream));
     fill_harmsum_args(s, &a);
-    GPU(agpu_harmsum(&a, s->stream));
+    GPU(agpu_harmsum(&a, 0, -1, s->stream));
     GPU(agpu_event_record(s->ev[4], s->stream));
     s->hms[0] = now_ms() - t0;
     return 0;
@@ -1119,7 +1119,7 @@ int accel_search_collect(accel_search *s)
         agpu_harmsum_args a;
         GPU(agpu_memset(s->d_nhits, 0, sizeof(unsigned), s->stream));
         fill_harmsum_args(s, &a);
-        GPU(agpu_harmsum(&a, s->stream));
+        GPU(agpu_harmsum(&a, 0, -1, s->stream));
         GPU(agpu_d2h(&nh, s->d_nhits, sizeof(unsigned), s->stream));
         if (nh > s->cap) {
             set_err("hit buffer overflow after regrowth (%u > %u)", nh, s->cap);
@@ -1493,8 +1493,8 @@ int accel_search_plane_only(accel_search *s)
 {
     const accel_geom *g = &s->g;
     GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw, s->uselen,
-                       s->hb_end, s->d_plane, s->ntiles, s->groups, s->stream));
-    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, s->stream));
+                       s->hb_end, s->d_plane, s->ntiles, s->groups, 0, -1, s->stream));
+    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, 0, -1, s->stream));
     GPU(agpu_stream_sync(s->stream));
     return 0;
 }

@@ -160,15 +160,24 @@ int agpu_copy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 
 /* Fused overlap-save correlation: raw |corr|^2 plane rows for templates
  * [0, nt) (row = template), half-bins [0, hb_end), segment length L = 4096,
- * written in the tiled layout above. */
+ * written in the tiled layout above.  Only segments [seg_lo, seg_hi)
+ * (segment s: half-bins [s uselen, (s+1) uselen); seg_hi -1: to the end). */
 int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
                    const void *d_T, int nt, int L, int hw, int uselen, long long hb_end,
-                   float *d_plane, long long ntiles, int groups, void *stream);
+                   float *d_plane, long long ntiles, int groups, long long seg_lo,
+                   long long seg_hi, void *stream);
 
-/* Fills the tile-row halos of a plane written by agpu_correlate. */
-int agpu_plane_halo(float *d_plane, int nrows, long long ntiles, void *stream);
+/* Fills the tile-row halos of tiles [tile_lo, tile_hi) (-1: to the end) of
+ * a plane written by agpu_correlate; tile t needs half-bins up to
+ * (t+1) AGPU_TILE + AGPU_HALO computed. */
+int agpu_plane_halo(float *d_plane, int nrows, long long ntiles, long long tile_lo,
+                    long long tile_hi, void *stream);
 
-int agpu_harmsum(const agpu_harmsum_args *a, void *stream);
+/* Harmonic sums of fundamental chunks [c_lo, c_hi) (-1: to the end); chunk
+ * c covers the 1024 half-bins from agpu_harmsum_chunk_hb(a, c). */
+int agpu_harmsum(const agpu_harmsum_args *a, long long c_lo, long long c_hi, void *stream);
+long long agpu_harmsum_nchunks(const agpu_harmsum_args *a);
+long long agpu_harmsum_chunk_hb(const agpu_harmsum_args *a, long long c);
 
 /* In-memory path: per-half-bin norm^2 of the block that last wrote it. */
 int agpu_inmem_nsq_table(const float *d_nsq, int npre, int nblocks, long long rh0, int cu,
