@@ -648,17 +648,44 @@ static_assert(hs_frac_wl(15, 16) * 64 <= AGPU_HALO && hs_frac_wl(31, 32) * 64 <=
                   hs_frac_wl(1, 2) * 64 <= AGPU_HALO,
               "every sub-harmonic span fits in a tile-row halo");
 
-/* rint (half-even, standard path, src/accel_utils.c:59) or (int)(x+0.5)
- * (in-memory path, :1326) of rh * k / 2^s.  32-bit: the launcher checks
- * that every rh * k fits. */
-template <int INMEM>
+/* Index-rule variants of the harmonic sum:
+ *   HS_STD   standard path, sub-harmonic half-bin rint((hb + rh_off) k / H)
+ *            (src/accel_utils.c:59, 1042-1047 with the block start r on the
+ *            half-bin grid);
+ *   HS_INMEM in-memory path, (int)(hb k / H + 0.5) (src/accel_utils.c:1326);
+ *   HS_FRAC  standard path with a block start r off the half-bin grid: the
+ *            reference's FP64 expression rint(2 (startr + ii/2) k / H),
+ *            evaluated per cell. */
+#define HS_STD 0
+#define HS_INMEM 1
+#define HS_FRAC 2
+
+/* rint (half-even) or, for HS_INMEM, (int)(x+0.5) of rh * k / 2^s.
+ * 32-bit: the launcher checks that every rh * k fits. */
+template <int MODE>
 __device__ __forceinline__ int hs_round(int rh, int k, int s)
 {
     const unsigned q = (unsigned) rh * (unsigned) k;
     const unsigned H = 1u << s;
-    if (INMEM)
+    if (MODE == HS_INMEM)
         return (int) ((q + (H >> 1)) >> s);
     return (int) ((q + (H >> 1) - 1 + ((q >> s) & 1u)) >> s);
+}
+
+/* First half-bin of a chunk's span for fraction k / 2^s.  HS_FRAC starts
+ * one half-bin below the grid rule's: its lookups lie within
+ * (hb - 1/2, hb + 2) k / H, so the span [lo, lo + CHUNK k / H + 4) holds
+ * them, inside the hs_frac_wl slots (whose 64-float rounding leaves at
+ * least 31 spare half-bins). */
+template <int MODE>
+__device__ __forceinline__ int hs_span_lo(const agpu_harmsum_args &a, int rh_c0, int k, int s)
+{
+    if (MODE == HS_STD)
+        return hs_round<HS_STD>(rh_c0 + a.rh_off, k, s);
+    if (MODE == HS_INMEM)
+        return hs_round<HS_INMEM>(rh_c0, k, s);
+    const int lo = hs_round<HS_STD>(rh_c0, k, s) - 1;
+    return lo > 0 ? lo : 0;
 }
 
 /* per-thread staging registers of one fraction: its wave loads are dealt
@@ -688,7 +715,7 @@ __host__ __device__ constexpr int hs_regs_upto(int S)   /* stages 1..S-1 */
  * wave load i (64 consecutive half-bins) goes to wave i % HS_WAVES.  The row
  * base is wave-uniform and the span never leaves the tile-row plus its halo,
  * so every load is base + lane + immediate. */
-template <int S, int INMEM>
+template <int S, int MODE>
 __device__ __forceinline__ void hs_load(const agpu_harmsum_args &a, float *v, int zi, int wi,
                                         int nrows, int rh_c0, int wv, int lane)
 {
@@ -700,7 +727,7 @@ __device__ __forceinline__ void hs_load(const agpu_harmsum_args &a, float *v, in
         const int f = F0 + (k - 1) / 2;
         const int zs = a.zmap[f * a.numz + zi];
         const int ws = a.wmap[f * a.numw + wi];
-        const int lo = hs_round<INMEM>(rh_c0, k, S);
+        const int lo = hs_span_lo<MODE>(a, rh_c0, k, S);
         const int srow = ws * a.numz + zs;
         /* the span starts in lo's tile-row and ends within its halo */
         const float *b1 = a.plane + AGPU_PLANE_IDX(srow, lo, nrows) + wv * 64;
@@ -731,11 +758,11 @@ __device__ __forceinline__ void hs_store(float *lds, int base, const float *v, i
 
 /* Accumulate stage S from LDS, in the reference's fraction order.  nr[j]
  * is the LDS row of norm^2 values of cell j's block. */
-template <int S, int INMEM>
+template <int S, int MODE>
 __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const float *lds,
                                             int base, const float *nr0, const float *nr1,
                                             const float *nr2, const float *nr3, int rh_c0,
-                                            int rh_first, float *acc)
+                                            int rh_first, const double *rf, float *acc)
 {
     /* each normalised sub-harmonic power is rounded to FP32 before it is
      * added, as the reference adds stored FP32 planes
@@ -748,25 +775,38 @@ __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const fl
 #pragma unroll
     for (int k = 1; k < H; k += 2) {
         const int f = F0 + (k - 1) / 2;
-        const int lo = hs_round<INMEM>(rh_c0, k, S);
-        const int rs0 = hs_round<INMEM>(rh_first, k, S);
-        const int step = (HS_TPB / H) * k;   /* HS_TPB k / H, even */
-        const float *lp = lds + wl * 64 + (rs0 - lo);
+        const int lo = hs_span_lo<MODE>(a, rh_c0, k, S);
         float nn[4];
-        if (INMEM) {
-            const float *np = a.nsq_hb + rs0;
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                nn[j] = np[j * step];
-        } else {
+        if (MODE != HS_INMEM) {
             nn[0] = nr0[1 + f];
             nn[1] = nr1[1 + f];
             nn[2] = nr2[1 + f];
             nn[3] = nr3[1 + f];
         }
+        if (MODE == HS_FRAC) {
+            /* rint((2 rf) (k / H)): the reference's rounding steps, FP64 */
+            const double hf = (double) k / (double) H;
+            const int top = hs_frac_wl(k, H) * 64 - 1;
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-            acc[j] = acc[j] + lp[j * step] * nn[j];
+            for (int j = 0; j < 4; j++) {
+                int o = (int) __builtin_rint((2.0 * rf[j]) * hf) - lo;
+                o = o < 0 ? 0 : (o > top ? top : o);
+                acc[j] = acc[j] + lds[wl * 64 + o] * nn[j];
+            }
+        } else {
+            const int rs0 = hs_round<MODE>(rh_first + (MODE == HS_STD ? a.rh_off : 0), k, S);
+            const int step = (HS_TPB / H) * k;   /* HS_TPB k / H, even */
+            const float *lp = lds + wl * 64 + (rs0 - lo);
+            if (MODE == HS_INMEM) {
+                const float *np = a.nsq_hb + rs0;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    nn[j] = np[j * step];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[j] = acc[j] + lp[j * step] * nn[j];
+        }
         wl += hs_frac_wl(k, H);
     }
 }
@@ -783,7 +823,7 @@ __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const fl
  * 4).  A 32-harmonic search loads stage 5 in a third round.  Adding 256 to a fundamental half-bin adds an even
  * 256 k / 2^s to the rounded quotient, so one rounding per thread and
  * fraction serves all 4 cells. */
-template <int NSTAGES, int INMEM>
+template <int NSTAGES, int MODE>
 __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_lo, int c_hi)
 {
     __shared__ float s_norm[3 * 32];   /* norm^2 of blocks b0..b0+2 x fraction */
@@ -814,11 +854,11 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_l
         fund[j] = prow[t + HS_TPB * j];
     constexpr int NREG = hs_regs_upto(NSTAGES < 5 ? NSTAGES : 5);
     float v[NREG > 0 ? NREG : 1];
-    if (NSTAGES > 1) hs_load<1, INMEM>(a, v + hs_regs_upto(1), zi, wi, nrows, rh_c0, wv, lane);
-    if (NSTAGES > 2) hs_load<2, INMEM>(a, v + hs_regs_upto(2), zi, wi, nrows, rh_c0, wv, lane);
-    if (NSTAGES > 3) hs_load<3, INMEM>(a, v + hs_regs_upto(3), zi, wi, nrows, rh_c0, wv, lane);
-    if (NSTAGES > 4) hs_load<4, INMEM>(a, v + hs_regs_upto(4), zi, wi, nrows, rh_c0, wv, lane);
-    if (!INMEM && t < 3 * nfr) {
+    if (NSTAGES > 1) hs_load<1, MODE>(a, v + hs_regs_upto(1), zi, wi, nrows, rh_c0, wv, lane);
+    if (NSTAGES > 2) hs_load<2, MODE>(a, v + hs_regs_upto(2), zi, wi, nrows, rh_c0, wv, lane);
+    if (NSTAGES > 3) hs_load<3, MODE>(a, v + hs_regs_upto(3), zi, wi, nrows, rh_c0, wv, lane);
+    if (NSTAGES > 4) hs_load<4, MODE>(a, v + hs_regs_upto(4), zi, wi, nrows, rh_c0, wv, lane);
+    if (MODE != HS_INMEM && t < 3 * nfr) {
         const int bb = min(b0 + t / nfr, a.nblocks - 1);
         s_norm[(t / nfr) * 32 + t % nfr] = a.nsq[(size_t) bb * nfr + t % nfr];
     }
@@ -836,9 +876,21 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_l
         const int c = cell0 + t + HS_TPB * j;
         nb[j] = (c >= e1) + (c >= e2);
     }
+    /* HS_FRAC: each cell's start r + ii/2 in FP64 (src/accel_utils.c:1044) */
+    double rf[HS_CPT];
 #pragma unroll
     for (int j = 0; j < HS_CPT; j++) {
-        const float nn = INMEM ? a.nsq_hb[rh_first + HS_TPB * j] : s_norm[nb[j] * 32];
+        rf[j] = 0.0;
+        if (MODE == HS_FRAC) {
+            const int c = cell0 + t + HS_TPB * j;
+            const int b = min(b0 + nb[j], a.nblocks - 1);
+            rf[j] = a.bstart[b] + (double) (c - b * cu) * 0.5;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < HS_CPT; j++) {
+        const float nn = MODE == HS_INMEM ? a.nsq_hb[rh_first + HS_TPB * j]
+                                          : s_norm[nb[j] * 32];
         acc[j] = fund[j] * nn;
     }
 #define HS_EMIT(ST)                                                                    \
@@ -856,8 +908,8 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_l
     const float *nrow2 = s_norm + nb[2] * 32;
     const float *nrow3 = s_norm + nb[3] * 32;
 #define HS_STAGE(ST)                                                                   \
-    hs_consume4<ST, INMEM>(a, lds, hs_stage_base(ST), nrow0, nrow1, nrow2, nrow3, rh_c0, \
-                           rh_first, acc);                                             \
+    hs_consume4<ST, MODE>(a, lds, hs_stage_base(ST), nrow0, nrow1, nrow2, nrow3, rh_c0, \
+                          rh_first, rf, acc);                                         \
     HS_EMIT(ST)
     if (NSTAGES > 1) { HS_STAGE(1) }
     if (NSTAGES > 2) { HS_STAGE(2) }
@@ -868,16 +920,16 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_l
         __syncthreads();
         hs_store<4>(lds, 0, v + hs_regs_upto(4), wv, lane);
         __syncthreads();
-        hs_consume4<4, INMEM>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, rh_c0, rh_first, acc);
+        hs_consume4<4, MODE>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, rh_c0, rh_first, rf, acc);
         HS_EMIT(4)
     }
     if (NSTAGES > 5) {
         float v5[hs_stage_regs(5)];
-        hs_load<5, INMEM>(a, v5, zi, wi, nrows, rh_c0, wv, lane);
+        hs_load<5, MODE>(a, v5, zi, wi, nrows, rh_c0, wv, lane);
         __syncthreads();   /* stages 1-4 are done with LDS */
         hs_store<5>(lds, 0, v5, wv, lane);
         __syncthreads();
-        hs_consume4<5, INMEM>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, rh_c0, rh_first, acc);
+        hs_consume4<5, MODE>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, rh_c0, rh_first, rf, acc);
         HS_EMIT(5)
     }
 #undef HS_STAGE
@@ -1307,6 +1359,8 @@ int agpu_harmsum(const agpu_harmsum_args *a, long long c_lo, long long c_hi, voi
         return fail("agpu_harmsum: corr_uselen below 1024 half-bins");
     if (a->inmem && !a->nsq_hb)
         return fail("agpu_harmsum: in-memory path needs the per-half-bin norm table");
+    if (a->rh_off < 0 || a->rh_off > 1 || (a->inmem && (a->rh_off || a->bstart)))
+        return fail("agpu_harmsum: sub-harmonic offset is 0 or 1 half-bin, standard path only");
     const long long rh_end = a->rh0 + ncell;
     if (a->ntiles < ((rh_end - 1) >> AGPU_TILE_LOG2) + 2 || a->ntiles * AGPU_TILE > 0x7fffffffLL)
         return fail("agpu_harmsum: plane tiles must pad the half-bin range by one tile "
@@ -1331,9 +1385,14 @@ int agpu_harmsum(const agpu_harmsum_args *a, long long c_lo, long long c_hi, voi
 #define HS_CASE(NS)                                                                         \
     case NS:                                                                                \
         if (a->inmem)                                                                       \
-            hipLaunchKernelGGL((k_harmsum<NS, 1>), grid, dim3(HS_TPB), 0, st, *a, cl, ch);  \
+            hipLaunchKernelGGL((k_harmsum<NS, HS_INMEM>), grid, dim3(HS_TPB), 0, st, *a, cl,  \
+                               ch);                                                         \
+        else if (a->bstart)                                                                 \
+            hipLaunchKernelGGL((k_harmsum<NS, HS_FRAC>), grid, dim3(HS_TPB), 0, st, *a, cl,   \
+                               ch);                                                         \
         else                                                                                \
-            hipLaunchKernelGGL((k_harmsum<NS, 0>), grid, dim3(HS_TPB), 0, st, *a, cl, ch);  \
+            hipLaunchKernelGGL((k_harmsum<NS, HS_STD>), grid, dim3(HS_TPB), 0, st, *a, cl,    \
+                               ch);                                                         \
         break;
     switch (a->nstages) {
         HS_CASE(1)

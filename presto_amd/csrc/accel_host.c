@@ -82,6 +82,10 @@ static double now_ms(void)
 /* ---- index rules: src/accel_utils.c:36, 54-102 ---- */
 static int nearest_int(double x) { return (int) (x < 0 ? x - 0.5 : x + 0.5); }
 static double calc_required_r(double hf, double rfull) { return rint(ACCEL_RDR * rfull * hf) * ACCEL_DR; }
+/* First plane bin of the block that starts at startr: the fundamental's
+ * ffdot->rlo = floor(calc_required_r(1, startr)), src/accel_utils.c:1035-1037
+ * (startr itself when it is whole). */
+static long long fund_rlo(double startr) { return (long long) floor(calc_required_r(1.0, startr)); }
 static int calc_required_z(double hf, double zfull) { return nearest_int(ACCEL_RDZ * zfull * hf) * ACCEL_DZ; }
 static int calc_required_w(double hf, double wfull) { return nearest_int(ACCEL_RDW * wfull * hf) * ACCEL_DW; }
 
@@ -472,6 +476,11 @@ struct accel_search {
     agpu_hit *d_hits;
     unsigned *d_nhits;
     unsigned cap;
+    /* block start r, accumulated as src/accelsearch.c:189-224 does; a
+     * fractional -rlo makes the sub-harmonic lookups depend on it */
+    double *bstart;
+    double *d_bstart;     /* device copy, only for a start r off the half-bin grid */
+    int rh_off;           /* 2 startr - 2 fund_rlo(startr) when that is 0 or 1 */
     /* results */
     accel_hit *hits;
     long long nhits;
@@ -534,7 +543,7 @@ static int build_tables(accel_search *s, agpu_window **win_out, int **zmap_out,
             win[i].numdata = fund_nd;
         }
         for (int b = 0; b < g->nblocks; b++, i++) {
-            win[i].lobin = (long long) (g->rlo + (double) b * rstep) - fund_hw;
+            win[i].lobin = fund_rlo(s->bstart[b]) - fund_hw;
             win[i].numdata = fund_nd;
         }
     } else {
@@ -555,9 +564,9 @@ static int build_tables(accel_search *s, agpu_window **win_out, int **zmap_out,
             }
         }
         for (int b = 0; b < g->nblocks; b++) {
-            double startr = g->rlo + (double) b * rstep;
+            double startr = s->bstart[b];
             agpu_window *wb = win + (size_t) b * (s->nfrac + 1);
-            wb[0].lobin = (long long) startr - fund_hw;
+            wb[0].lobin = fund_rlo(startr) - fund_hw;
             wb[0].numdata = fund_nd;
             f = 0;
             for (int st = 1; st < g->numharmstages; st++) {
@@ -683,6 +692,51 @@ static int gen_templates(accel_search *s)
     return 0;
 }
 
+/* Start r of every block (the reference's accumulated startr) and how the
+ * standard path's sub-harmonic lookups relate to the fundamental's plane
+ * bins.  The reference computes them from startr + ii/2
+ * (src/accel_utils.c:1042-1047) while the fundamental's cell ii sits at
+ * fund_rlo(startr) + ii/2: the two differ by rh_off half-bins when startr
+ * lies on the half-bin grid, by a fraction otherwise (then the kernel
+ * evaluates the reference's FP64 expression per cell from d_bstart). */
+static int block_starts(accel_search *s)
+{
+    const accel_geom *g = &s->g;
+    const int rstep = g->corr_uselen / 2;
+    s->bstart = (double *) malloc(sizeof(double) * (size_t) (g->nblocks > 0 ? g->nblocks : 1));
+    if (!s->bstart) {
+        set_err("out of host memory for %d block starts", g->nblocks);
+        return -1;
+    }
+    double startr = g->rlo;
+    int ongrid = 1;
+    const long long r0 = fund_rlo(g->rlo);
+    s->rh_off = 0;
+    for (int b = 0; b < g->nblocks; b++) {
+        s->bstart[b] = startr;
+        if (fund_rlo(startr) != r0 + (long long) b * rstep) {
+            set_err("block %d of -rlo %.17g does not start %d bins after the previous one",
+                    b, g->rlo, rstep);
+            return -1;
+        }
+        const double d = 2.0 * startr - 2.0 * (double) fund_rlo(startr);
+        if ((d != 0.0 && d != 1.0) || (b > 0 && (int) d != s->rh_off))
+            ongrid = 0;
+        else
+            s->rh_off = (int) d;
+        startr += rstep;
+    }
+    if (!ongrid) {
+        s->rh_off = 0;
+        if (!g->inmem && g->numharmstages > 1) {
+            GPU(agpu_malloc((void **) &s->d_bstart, sizeof(double) * (size_t) g->nblocks));
+            GPU(agpu_h2d(s->d_bstart, s->bstart, sizeof(double) * (size_t) g->nblocks,
+                         s->stream));
+        }
+    }
+    return 0;
+}
+
 static int search_init(accel_search *s, int device)
 {
     accel_geom *g = &s->g;
@@ -699,7 +753,9 @@ static int search_init(accel_search *s, int device)
         return -1;
     }
     s->uselen = (s->L - 4 * s->hw) & ~1;
-    s->hb_end = (long long) (2.0 * g->rlo) + (long long) g->nblocks * g->corr_uselen;
+    if (block_starts(s))
+        return -1;
+    s->hb_end = 2 * fund_rlo(g->rlo) + (long long) g->nblocks * g->corr_uselen;
     /* one tile of padding past the last half-bin: the tail chunk of the
      * harmonic sum reads it instead of clamping indices */
     s->ntiles = (s->hb_end + AGPU_TILE - 1) / AGPU_TILE + 1;
@@ -766,6 +822,8 @@ void accel_search_destroy(accel_search *s)
     agpu_free(s->d_wmap);
     agpu_free(s->d_hits);
     agpu_free(s->d_nhits);
+    agpu_free(s->d_bstart);
+    free(s->bstart);
     for (int i = 0; i < 5; i++)
         if (s->ev[i])
             agpu_event_destroy(s->ev[i]);
@@ -816,7 +874,9 @@ static void fill_harmsum_args(const accel_search *s, agpu_harmsum_args *a)
     a->ntiles = s->ntiles;
     a->numz = g->numz;
     a->numw = s->numw1;
-    a->rh0 = (long long) (2.0 * g->rlo);
+    a->rh0 = 2 * fund_rlo(g->rlo);
+    a->rh_off = g->inmem ? 0 : s->rh_off;
+    a->bstart = g->inmem ? NULL : s->d_bstart;
     a->corr_uselen = g->corr_uselen;
     a->nblocks = g->nblocks;
     a->nstages = g->numharmstages;
@@ -844,7 +904,7 @@ int accel_search_launch(accel_search *s)
     GPU(agpu_block_norms(s->d_spec, g->numbins, g->lobin, s->d_win, s->nwin, s->maxn,
                          s->d_norm, s->d_nsq, s->stream));
     if (g->inmem)
-        GPU(agpu_inmem_nsq_table(s->d_nsq, s->npre, g->nblocks, (long long) (2.0 * g->rlo),
+        GPU(agpu_inmem_nsq_table(s->d_nsq, s->npre, g->nblocks, 2 * fund_rlo(g->rlo),
                                  g->corr_uselen, s->hb_end, s->d_nsq_hb, s->stream));
     GPU(agpu_event_record(s->ev[1], s->stream));
     GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw, s->uselen,
@@ -1069,7 +1129,6 @@ void accel_sigmas(const float *pow, long long n, int numharm, double numindep, f
 static int replay(accel_search *s)
 {
     const accel_geom *g = &s->g;
-    const int rstep = g->corr_uselen / 2;
     if (sort_hits(s->hits, s->nhits))
         return -1;
     float *sigs = (float *) malloc(sizeof(float) * (s->nhits ? s->nhits : 1));
@@ -1084,9 +1143,9 @@ static int replay(accel_search *s)
     for (long long i = 0; i < s->nhits; i++) {
         const accel_hit *h = &s->hits[i];
         int numharm = 1 << h->stage;
-        double startr = g->rlo + (double) h->block * rstep;
         float sig = sigs[i];
-        double rr = ((long long) startr + h->ii * (double) ACCEL_DR) / (double) numharm;
+        double rr = (fund_rlo(s->bstart[h->block]) + h->ii * (double) ACCEL_DR) /
+                    (double) numharm;
         double zz = (g->zlo + h->zi * (double) ACCEL_DZ) / (double) numharm;
         double ww = (g->wlo + h->wi * (double) ACCEL_DW) / (double) numharm;
         int added;

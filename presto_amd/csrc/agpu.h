@@ -57,7 +57,12 @@ typedef struct {
     const float *plane;      /* raw plane, tiled (AGPU_PLANE_IDX), column = absolute half-bin */
     long long ntiles;        /* tiles allocated (one past the last searched half-bin) */
     int numz, numw;          /* numw >= 1 */
-    long long rh0;           /* first fundamental half-bin (2*rlo) */
+    long long rh0;           /* first fundamental half-bin (2 * the first block's plane rlo) */
+    int rh_off;              /* standard path: sub-harmonic lookups are taken at
+                              * fundamental half-bin + rh_off (0 or 1) */
+    const double *bstart;    /* standard path, start r off the half-bin grid: each block's
+                              * start r [nblocks]; the lookups are then evaluated in FP64
+                              * per cell as the reference does */
     int corr_uselen;         /* half-bins per block */
     int nblocks;
     int nstages;

@@ -44,11 +44,37 @@ def test_geometry_matches_oracle(n, dt, zmax, numharm, wmax):
     assert list(g.numindep) == list(o.numindep)
 
 
+@pytest.mark.parametrize("kw", [
+    dict(rlo=3000.0, rhi=9000.0, zmax=25), dict(rlo=1234.5, rhi=5000.5, zmax=7, inmem=True),
+    dict(lobin=1500, dat_input=False, zmax=30), dict(lobin=1500, rlo=200.0, zmax=30),
+    dict(flo=20.0, fhi=300.0, zmax=13, wmax=35), dict(rhi=1e9, zmax=3),
+    dict(lobin=2000, fhi=120.0, zmax=1, numharm=32)])
+def test_geometry_options_match_oracle(kw):
+    """-rlo/-rhi/-flo/-fhi/-lobin and odd zmax/wmax (rounded up to the
+    next multiple of the step), src/accel_utils.c:1594-1730."""
+    kw = dict(dict(numharm=8), **kw)
+    g = A.geometry(A.make_params(1 << 15, 1 << 16, 5e-4, **kw))
+    o = O.make_obs(1 << 15, 1 << 16, 5e-4, **kw)
+    for f in ("numbins", "lobin", "highestbin", "maxkernlen", "corr_uselen", "fftlen",
+              "numharmstages", "numz", "numw", "inmem", "rlo", "rhi", "zlo", "zhi", "wlo",
+              "whi", "T"):
+        assert getattr(g, f) == getattr(o, f), f
+    assert g.zhi % 2 == 0 and g.zhi >= kw["zmax"]
+    assert list(g.powcut)[:g.numharmstages] == list(o.powcut)[:o.numharmstages]
+    assert list(g.numindep) == list(o.numindep)
+
+
 def test_geometry_errors():
     with pytest.raises(A.AccelError, match="power-of-two"):
         A.geometry(A.make_params(1 << 15, 1 << 16, 1e-3, numharm=3))
     with pytest.raises(A.AccelError):
         A.geometry(A.make_params(1 << 15, 1 << 16, 1e-3, flo=1e6))
+    with pytest.raises(A.AccelError, match="lobin"):
+        A.geometry(A.make_params(1 << 15, 1 << 16, 1e-3, lobin=1 << 15))
+    with pytest.raises(A.AccelError, match="below the low"):
+        A.geometry(A.make_params(1 << 15, 1 << 16, 1e-3, rlo=5000.0, rhi=4000.0))
+    with pytest.raises(A.AccelError, match="zmax"):
+        A.geometry(A.make_params(1 << 15, 1 << 16, 1e-3, zmax=-2))
 
 
 @pytest.mark.parametrize("pw,ns,nt", [(30.0, 1, 2e8), (60.0, 8, 1e7), (200.0, 16, 1e6),

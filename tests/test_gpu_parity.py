@@ -212,6 +212,61 @@ def test_edge_configurations(zmax, numharm, inmem):
     _compare_cands(gc, oc)
 
 
+def test_fft_input_with_lobin():
+    """A .fft input whose first bin is lobin 1500 (-lobin): the spectrum
+    holds bins [lobin, lobin + numbins) and every window reads it at
+    bin - lobin (src/accel_utils.c:993-1022); candidates keep absolute r.
+    rlo 6500 keeps the 1/4 subharmonic windows above lobin: below it the
+    reference zero-pads, the window median is 0 and its normalisation
+    infinite, so the powers there are inf/nan on both sides (and so is
+    every in-memory search with lobin > 8, whose plane is filled from
+    bin 8 up, src/accelsearch.c:166-181): no parity case."""
+    _say("case: .fft input, lobin 1500, rlo 6500, zmax 30 numharm 4, standard path")
+    N, dt, lobin = 1 << 16, 5e-4, 1500
+    x = synth.make_series(N, dt, 35, [dict(f0=77.7, z=4.0, duty=0.05, amp=0.4)])
+    spec = np.ascontiguousarray(A.spectrum_from_series(x, dt)[lobin:])
+    kw = dict(zmax=30, numharm=4, sigma=2.0, lobin=lobin, rlo=6500.0, dat_input=False,
+              force_standard=True)
+    p = A.make_params(len(spec), N, dt, **kw)
+    o = O.make_obs(len(spec), N, dt, **kw)
+    assert o.inmem == 0 and o.lobin == lobin and o.highestbin == len(spec) - 1
+    with _run_gpu(spec, p) as s:
+        gh, gc = s.hits(), s.candidates()
+    oh, oc, _ = O.accelsearch(o, spec)
+    assert np.isfinite(oh["pow"]).all()
+    _compare_hits(gh, oh, list(o.powcut))
+    _compare_cands(gc, oc)
+    # the injected pulsar (r = f0 T + z/2) as a 4-harmonic candidate
+    assert any(c["numharm"] == 4 and abs(c["r"] - (77.7 * N * dt + 2.0)) < 1.0 for c in gc)
+    _say(f"lobin: {len(gh)} hits, {len(gc)} candidates, identical")
+
+
+@pytest.mark.parametrize("rlo,rhi,zmax,inmem", [
+    (3000.0, 9000.0, 25, False), (1234.5, 5000.5, 7, True), (1234.5, 6000.0, 9, False),
+    (1234.3, 6000.0, 9, False), (1234.8, 7000.0, 11, False), (1234.75, 6000.0, 5, True)])
+def test_explicit_r_range_and_odd_zmax(rlo, rhi, zmax, inmem):
+    """-rlo/-rhi bound the search (src/accel_utils.c:1686-1730); an odd zmax
+    is rounded up to the next even value.  A fractional rlo starts each
+    block's plane at floor(calc_required_r(1, startr)) while the standard
+    path's sub-harmonic lookups are taken from startr + ii/2 itself
+    (src/accel_utils.c:1035-1047): half-bin starts (x.5) shift them by one
+    half-bin, other fractions by a fraction (the FP64 per-cell variant)."""
+    _say(f"case: rlo={rlo} rhi={rhi} zmax={zmax} inmem={inmem}")
+    fft, p, o = _case(1 << 16, 5e-4, 36, [dict(f0=77.7, z=3.0, duty=0.05, amp=0.4)],
+                      zmax=zmax, numharm=8, sigma=2.0, rlo=rlo, rhi=rhi,
+                      force_standard=not inmem)
+    assert o.inmem == int(inmem) and o.zhi == zmax + 1
+    with _run_gpu(fft, p) as s:
+        assert (s.geom.rlo, s.geom.highestbin, s.geom.zhi) == (o.rlo, o.highestbin, o.zhi)
+        gh, gc = s.hits(), s.candidates()
+    oh, oc, _ = O.accelsearch(o, fft)
+    _compare_hits(gh, oh, list(o.powcut))
+    _compare_cands(gc, oc)
+    assert len(gc) >= 1
+    # candidate r is the summed plane's r over numharm (src/accel_utils.c:1431)
+    assert all(rlo - 1 <= c["r"] * c["numharm"] <= rhi + 1 for c in gc)
+
+
 # ------------------------------------------------------------------ jerk (w) search
 @pytest.fixture(scope="module")
 def jerk_case():

@@ -208,6 +208,37 @@ def test_search_optimize_and_outputs(tmp_path):
     _say(f"ACCEL file: {len(fc)} candidates, harmonics tables complete")
 
 
+def test_optimize_with_lobin():
+    """Harmonic polishing of a .fft input whose first bin is lobin: the
+    reference starts the optimiser at r - lobin in the array's own bins,
+    evaluates harmonic k at k (r - lobin), and reports (r + lobin) k
+    (src/accel_utils.c:563-581)."""
+    _say("case: optimize, .fft input with lobin 1500, zmax 30 numharm 4")
+    N2, dt, lobin = 1 << 16, 5e-4, 1500
+    x = synth.make_series(N2, dt, 35, [dict(f0=77.7, z=4.0, duty=0.05, amp=0.4)])
+    spec = np.ascontiguousarray(A.spectrum_from_series(x, dt)[lobin:])
+    p = A.make_params(len(spec), N2, dt, zmax=30, numharm=4, sigma=2.0, lobin=lobin,
+                      rlo=6500.0, dat_input=False, force_standard=True)
+    with A.Search(p) as s:
+        s.set_spectrum(spec)
+        s.run()
+        cands = s.candidates()
+        assert len(cands) >= 2
+        s.optimize()
+        for i in range(2):
+            c = cands[i]
+            hr, hz, hw, pw, der = s.harmonics(i)
+            oro, ozo, ords, opw, _ = O.max_rz_arr_harmonics(
+                spec, c["r"] - lobin, c["z"], int(c["numharm"]))
+            res = _check_endpoint(spec, int(c["numharm"]), hr[0], hz[0], der, oro + lobin,
+                                  ozo, ords, lobin=lobin)
+            _say(f"  cand {i} (r {c['r']:.2f}): |dr| {res[0]:.2e} |dz| {res[1]:.2e} "
+                 f"objective {res[2]:.2e} derivs {res[3]:.2e} phase {res[4]:.2e}")
+            assert res[0] < 2e-3 and res[1] < 8e-3 and res[2] < 1e-3
+            assert res[3] < 1e-5 and res[4] < 1e-4
+            assert np.allclose(hr, hr[0] * np.arange(1, len(hr) + 1))
+
+
 def _jerk_signal(r, z, w, noise_amp, seed, numharm=1):
     us = np.arange(N, dtype=np.float64) / N
     sig = np.zeros(N)
