@@ -114,13 +114,14 @@ int main(int argc, char **argv)
             p.fhi = atof(NEXT());
             p.fhiP = 1;
         } else if (!strcmp(a, "-rlo")) {
+            /* -flo and -fhi carry defaults, so floP and fhiP are always set
+             * (src/accelsearch_cmd.c:49-61) and win over -rlo and -rhi
+             * (src/accel_utils.c:1686-1730): those two change nothing */
             p.rlo = atof(NEXT());
             p.rloP = 1;
-            p.floP = 0;
         } else if (!strcmp(a, "-rhi")) {
             p.rhi = atof(NEXT());
             p.rhiP = 1;
-            p.fhiP = 0;
         } else if (!strcmp(a, "-lobin"))
             p.lobin = atoll(NEXT());
         else if (!strcmp(a, "-inmem"))

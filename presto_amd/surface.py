@@ -190,8 +190,10 @@ def write_inf(root, N, dt, telescope="None (Artificial Data Set)"):
 
 
 def make_cmdline(infile, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0, fhi=10000.0,
-                 inmem=False, noharmremove=False, lobin=0):
-    """Cmdline with the clig defaults of clig/accelsearch_cmd.cli:13-48."""
+                 inmem=False, noharmremove=False, lobin=0, rlo=None, rhi=None):
+    """Cmdline with the clig defaults of clig/accelsearch_cmd.cli:13-48.
+    rlo / rhi clear floP / fhiP, which the command line itself never does
+    (an embedding caller can)."""
     c = Cmdline()
     c.ncpusP, c.ncpus = 1, 1
     c.lobinP, c.lobin = 1, lobin
@@ -202,6 +204,10 @@ def make_cmdline(infile, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0, fhi
     c.floP, c.flo = 1, flo
     c.fhiP, c.fhi = 1, fhi
     c.rloP = c.rhiP = 0
+    if rlo is not None:
+        c.floP, c.rloP, c.rlo = 0, 1, float(rlo)
+    if rhi is not None:
+        c.fhiP, c.rhiP, c.rhi = 0, 1, float(rhi)
     c.inmemP = 1 if inmem else 0
     c.noharmremoveP = 1 if noharmremove else 0
     c._argv = (_cp * 1)(infile.encode())

@@ -83,4 +83,11 @@ def test_cli_end_to_end_matches_oracle(tmp_path):
         assert int(f[4]) == int(c[2])
         assert abs(float(f[1]) - c[1]) <= 0.006
         assert abs(float(f[7].split("(")[0]) - c[3]) <= 0.5 / c[2] + 1e-6
-    assert "Number of bins in the time series" in open(acc).read()
+    first = open(acc).read()
+    assert "Number of bins in the time series" in first
+    # -flo/-fhi always carry their defaults, which win over -rlo/-rhi
+    # (src/accelsearch_cmd.c:49-61, src/accel_utils.c:1686-1730)
+    r = _run(["-zmax", "50", "-numharm", "8", "-sigma", "2.0", "-rlo", "5000", "-rhi", "6000",
+              "syn.dat"], tmp_path)
+    assert r.returncode == 0
+    assert open(acc).read() == first

@@ -44,15 +44,15 @@ def _handle_run(x, dt, tmp_path, tag, inf, **kw):
 
 
 def _compare_run(tmp_path, name, x, dt, zmax, numharm, wmax=None, flo=1.0, fhi=10000.0,
-                 expect_inmem=None):
+                 expect_inmem=None, rlo=None, rhi=None):
     root = _write_dat(tmp_path, name, x, dt)
     cmd = S.make_cmdline(root + ".dat", zmax=zmax, numharm=numharm, wmax=wmax, flo=flo,
-                         fhi=fhi)
+                         fhi=fhi, rlo=rlo, rhi=rhi)
     info, cands = S.drive(root + ".dat", cmd)
     if expect_inmem is not None:
         assert info["inmem"] == expect_inmem
     hc, hacc, g = _handle_run(x, dt, tmp_path, name, root + ".inf", zmax=zmax,
-                              numharm=numharm, wmax=wmax, flo=flo, fhi=fhi)
+                              numharm=numharm, wmax=wmax, flo=flo, fhi=fhi, rlo=rlo, rhi=rhi)
     assert info["inmem"] == g.inmem and info["corr_uselen"] == g.corr_uselen
     assert len(cands) == len(hc) > 0
     for a, b in zip(cands, hc):
@@ -89,6 +89,20 @@ def test_standard_run_equals_handle(tmp_path):
     info, cands = _compare_run(tmp_path, "std", x, dt, 200, 8, flo=4990.0, fhi=5000.0,
                                expect_inmem=0)
     assert any(abs(c["r"] / (N * dt) - 4995.37) < 0.05 for c in cands)
+
+
+@pytest.mark.parametrize("frac", [0.5, 0.3])
+def test_standard_run_fractional_rlo_equals_handle(tmp_path, frac):
+    """A start r off the whole bins, as an embedding caller can give
+    (floP 0, rloP 1): the reference's own block loop through the surface
+    and the handle agree to the byte."""
+    print(f"case: surface standard run, 2^23 samples, rlo 4990 Hz + {frac} bins", flush=True)
+    N, dt = 1 << 23, 64e-6
+    T = N * dt
+    x = synth.make_series_f32(N, dt, 72, [dict(f0=4995.37, z=30.0, duty=0.2, amp=0.08)])
+    info, cands = _compare_run(tmp_path, f"frac{frac}", x, dt, 200, 8, expect_inmem=0,
+                               rlo=np.floor(4990.0 * T) + frac, rhi=np.ceil(5000.0 * T))
+    assert any(abs(c["r"] / T - 4995.37) < 0.05 for c in cands)
 
 
 def test_jerk_run_equals_handle(tmp_path):
