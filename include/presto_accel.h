@@ -120,6 +120,12 @@ int accel_search_set_device_spectrum(accel_search *s, const void *d_fft);
  * candidate merge, sort_accelcands and eliminate_harmonics
  * (src/accelsearch.c:232-241). */
 int accel_search_run(accel_search *s);
+/* The .txtcand work file of an .fft input (src/accel_utils.c:1666-1684):
+ * opens path for writing ("w"); every later run then writes one line per
+ * candidate that the merge adds or replaces, in the reference's format
+ * and order (search_ffdotpows, src/accel_utils.c:1443-1447).  NULL closes
+ * the file.  Returns 0, or -1 if the file cannot be opened or closed. */
+int accel_search_set_txtcand(accel_search *s, const char *path);
 /* Only the device part (norms, correlation, harmonic sums); no hit replay.
  * Leaves the hit count on the device; accel_search_collect() finishes. */
 int accel_search_launch(accel_search *s);

@@ -71,6 +71,7 @@ _sig("accel_search_set_series", _i, _p, _p, _ll)
 _sig("accel_search_spectrum", _i, _p, _p, _ll)
 _sig("accel_dered_blocks", _i, C.c_long, _i, _i, _d, _d, _p, _p, _i)
 _sig("accel_search_run", _i, _p)
+_sig("accel_search_set_txtcand", _i, _p, C.c_char_p)
 _sig("accel_search_launch", _i, _p)
 _sig("accel_search_collect", _i, _p)
 _sig("accel_search_ncands", _i, _p)
@@ -130,6 +131,7 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_set_series", "accel_search_spectrum", "accel_dered_blocks",
             "accel_search_destroy", "accel_search_geom", "accel_search_set_spectrum",
             "accel_search_set_device_spectrum", "accel_search_run", "accel_search_launch",
+            "accel_search_set_txtcand",
             "accel_search_collect", "accel_search_ncands", "accel_search_cands",
             "accel_search_nhits", "accel_search_hits", "accel_search_plane_row",
             "accel_search_nwindows", "accel_search_norms", "accel_search_timings",
@@ -240,6 +242,11 @@ class Search:
 
     def run(self):
         if lib.accel_search_run(self.h):
+            raise _err()
+
+    def set_txtcand(self, path):
+        """Open (or with None close) the .txtcand work file of an .fft input."""
+        if lib.accel_search_set_txtcand(self.h, None if path is None else str(path).encode()):
             raise _err()
 
     def launch(self):

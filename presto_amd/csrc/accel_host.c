@@ -481,6 +481,7 @@ struct accel_search {
     double *bstart;
     double *d_bstart;     /* device copy, only for a start r off the half-bin grid */
     int rh_off;           /* 2 startr - 2 fund_rlo(startr) when that is 0 or 1 */
+    FILE *workfile;       /* .txtcand of an .fft input, or NULL */
     /* results */
     accel_hit *hits;
     long long nhits;
@@ -824,6 +825,8 @@ void accel_search_destroy(accel_search *s)
     agpu_free(s->d_nhits);
     agpu_free(s->d_bstart);
     free(s->bstart);
+    if (s->workfile)
+        fclose(s->workfile);
     for (int i = 0; i < 5; i++)
         if (s->ev[i])
             agpu_event_destroy(s->ev[i]);
@@ -1150,6 +1153,9 @@ static int replay(accel_search *s)
         double ww = (g->wlo + h->wi * (double) ACCEL_DW) / (double) numharm;
         int added;
         s->cands = insert_cand(s->cands, h->pow, sig, numharm, rr, zz, ww, &added);
+        if (added && s->workfile)   /* src/accel_utils.c:1443-1447 */
+            fprintf(s->workfile, "%-7.2f  %-7.4f  %-2d  %-14.4f  %-14.9f  %-10.4f %-10.4f\n",
+                    h->pow, sig, numharm, rr, rr / g->T, zz, ww);
     }
     free(sigs);
     int n = 0;
@@ -1546,6 +1552,24 @@ void accel_search_dev_view(const accel_search *s, accel_dev_view *v)
     v->numz = s->g.numz;
     v->numw1 = s->numw1;
     v->stream = s->stream;
+}
+
+int accel_search_set_txtcand(accel_search *s, const char *path)
+{
+    if (s->workfile && fclose(s->workfile)) {
+        s->workfile = NULL;
+        set_err("error closing the .txtcand file");
+        return -1;
+    }
+    s->workfile = NULL;
+    if (!path)
+        return 0;
+    s->workfile = fopen(path, "w");
+    if (!s->workfile) {
+        set_err("cannot open '%s' for writing", path);
+        return -1;
+    }
+    return 0;
 }
 
 int accel_search_plane_only(accel_search *s)

@@ -258,19 +258,22 @@ int main(int argc, char **argv)
     printf("  z = %.1f to %.1f Fourier bins drifted\n", g->zlo, g->zhi);
     if (g->numw)
         printf("  w = %.1f to %.1f Fourier-derivative bins drifted\n", g->wlo, g->whi);
-    if (accel_search_set_spectrum(s, fft) || accel_search_run(s)) {
-        fprintf(stderr, "accelsearch: %s\n", accel_last_error());
-        return 1;
-    }
-    const int ncand = accel_search_ncands(s);
-    printf("\nDone searching.  Now optimizing each candidate.\n\n");
-    char accelnm[4300], candnm[4400];
+    char accelnm[4300], candnm[4400], txtnm[4400];
     if (g->numw)
         snprintf(accelnm, sizeof(accelnm), "%s_ACCEL_%d_JERK_%d", root, (int) g->zhi,
                  (int) g->whi);
     else
         snprintf(accelnm, sizeof(accelnm), "%s_ACCEL_%d", root, (int) g->zhi);
     snprintf(candnm, sizeof(candnm), "%s.cand", accelnm);
+    snprintf(txtnm, sizeof(txtnm), "%s.txtcand", accelnm);
+    /* an .fft input gets the .txtcand work file (src/accel_utils.c:1666-1684) */
+    if ((!dat && accel_search_set_txtcand(s, txtnm)) || accel_search_set_spectrum(s, fft) ||
+        accel_search_run(s) || accel_search_set_txtcand(s, NULL)) {
+        fprintf(stderr, "accelsearch: %s\n", accel_last_error());
+        return 1;
+    }
+    const int ncand = accel_search_ncands(s);
+    printf("\nDone searching.  Now optimizing each candidate.\n\n");
     if (ncand) {
         if (accel_search_optimize(s) ||
             accel_search_write_outputs(s, accelnm, candnm, infpath)) {
