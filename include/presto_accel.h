@@ -45,6 +45,11 @@ typedef struct {
     int force_standard;  /* never pick the in-memory path (testing) */
     int dat_input;       /* the spectrum came from a .dat time series */
     int noharmremove;    /* -noharmremove: keep harmonically related candidates */
+    double nph;          /* -photon: the photon count (obs->nph, bin 0's real part,
+                          * src/accel_utils.c:1599-1605); every block is then
+                          * normalised by 1/sqrt(nph) (:1080-1086) instead of its
+                          * median.  0: block medians.  A .dat spectrum keeps bin 0
+                          * as de-reddened, (1, 0), instead of (1, 1) (:1606-1624) */
 } accel_params;
 
 /* Fills the command-line defaults: numharm 8, zmax 200, sigma 2.0,
@@ -62,6 +67,8 @@ typedef struct {
     float powcut[6];
     long long numindep[6];
     int nblocks;         /* blocks of the search loop (src/accelsearch.c:189) */
+    double nph;          /* photon normalisation in use (0 when lobin > 0,
+                          * src/accel_utils.c:1626-1628) */
 } accel_geom;
 
 /* Replaces the numeric part of create_accelobs (src/accel_utils.c:1455). */

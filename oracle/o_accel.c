@@ -61,6 +61,8 @@ int o_create_accelobs(o_obs *obs)
     if (obs->zmax % O_DZ)
         obs->zmax = (obs->zmax / O_DZ + 1) * O_DZ;
     obs->lobin = obs->lobin_arg;
+    if (obs->lobin > 0)
+        obs->nph = 0.0;
     if (obs->lobin > 0 && obs->lobin > obs->numbins - 1)
         return 1;
     if (obs->numharm != 1 && obs->numharm != 2 && obs->numharm != 4 &&
@@ -279,10 +281,13 @@ static void get_fourier_amplitudes(const o_obs *obs, const o_fcomplex *fft,
     }
 }
 
-/* src/accel_utils.c:1087-1099: 1/sqrt(median(|x|^2)/ln 2) with FP32 powers */
+/* src/accel_utils.c:1080-1099: 1/sqrt(nph) with -photon, else
+ * 1/sqrt(median(|x|^2)/ln 2) with FP32 powers */
 double o_block_norm(const o_obs *obs, const o_fcomplex *fft, long long lobin,
                     int numdata)
 {
+    if (obs->nph > 0.0)
+        return 1.0 / sqrt(obs->nph);
     o_fcomplex *data = (o_fcomplex *) malloc(sizeof(o_fcomplex) * numdata);
     float *powers = (float *) malloc(sizeof(float) * numdata);
     get_fourier_amplitudes(obs, fft, lobin, numdata, data);
@@ -345,7 +350,8 @@ o_ffdot *o_subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
         float pr = data[ii].r, pi = data[ii].i;
         powers[ii] = pr * pr + pi * pi;
     }
-    double norm = 1.0 / sqrt(o_median(powers, numdata) / log(2.0));
+    double norm = obs->nph > 0.0 ? 1.0 / sqrt(obs->nph)
+                                 : 1.0 / sqrt(o_median(powers, numdata) / log(2.0));
     free(powers);
 
     if (!fp64) {

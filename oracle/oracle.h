@@ -91,6 +91,10 @@ typedef struct {
     double T, rlo, rhi, dr, zlo, zhi, dz, wlo, whi, dw;
     float powcut[6];
     long long numindep[6];
+    /* -photon: the photon count whose 1/sqrt normalises every block
+     * (src/accel_utils.c:1080-1086); 0 selects block medians.  Set before
+     * o_create_accelobs, which clears it when lobin > 0 (:1626-1628). */
+    double nph;
 } o_obs;
 
 int o_create_accelobs(o_obs *obs);

@@ -40,7 +40,7 @@ class OObs(C.Structure):
         ("numz", _i), ("numw", _i), ("numbetween", _i), ("inmem", _i),
         ("T", _d), ("rlo", _d), ("rhi", _d), ("dr", _d), ("zlo", _d), ("zhi", _d),
         ("dz", _d), ("wlo", _d), ("whi", _d), ("dw", _d),
-        ("powcut", _f * 6), ("numindep", _ll * 6),
+        ("powcut", _f * 6), ("numindep", _ll * 6), ("nph", _d),
     ]
 
 
@@ -175,7 +175,7 @@ def prepare_dat(series, dt):
 
 def make_obs(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0,
              fhi=10000.0, inmem=False, force_standard=False, lobin=0, dat_input=True,
-             rlo=None, rhi=None):
+             rlo=None, rhi=None, nph=0.0):
     """Fill an o_obs the way accelsearch's command line does (flo/fhi default
     to 'given', clig/accelsearch_cmd.cli) and run o_create_accelobs."""
     o = OObs()
@@ -195,6 +195,7 @@ def make_obs(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0,
     o.force_standard = int(force_standard)
     o.lobin_arg = int(lobin)
     o.dat_input = int(dat_input)
+    o.nph = float(nph)
     rc = lib.o_create_accelobs(C.byref(o))
     if rc:
         raise ValueError(f"o_create_accelobs failed ({rc})")

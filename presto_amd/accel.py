@@ -24,7 +24,8 @@ class Params(C.Structure):
                 ("numharm", _i), ("zmax", _i), ("wmax", _i), ("wmaxP", _i),
                 ("sigma", _f), ("flo", _d), ("fhi", _d), ("rlo", _d), ("rhi", _d),
                 ("floP", _i), ("fhiP", _i), ("rloP", _i), ("rhiP", _i),
-                ("inmem", _i), ("force_standard", _i), ("dat_input", _i), ("noharmremove", _i)]
+                ("inmem", _i), ("force_standard", _i), ("dat_input", _i), ("noharmremove", _i),
+                ("nph", _d)]
 
 
 class Geom(C.Structure):
@@ -34,7 +35,8 @@ class Geom(C.Structure):
                 ("inmem", _i),
                 ("dt", _d), ("T", _d), ("rlo", _d), ("rhi", _d), ("dr", _d), ("zlo", _d),
                 ("zhi", _d), ("dz", _d), ("wlo", _d), ("whi", _d), ("dw", _d),
-                ("sigma", _f), ("powcut", _f * 6), ("numindep", _ll * 6), ("nblocks", _i)]
+                ("sigma", _f), ("powcut", _f * 6), ("numindep", _ll * 6), ("nblocks", _i),
+                ("nph", _d)]
 
 
 class Cand(C.Structure):
@@ -164,7 +166,7 @@ def _err():
 
 def make_params(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0,
                 fhi=10000.0, inmem=False, force_standard=False, lobin=0, dat_input=True,
-                rlo=None, rhi=None):
+                rlo=None, rhi=None, nph=0.0):
     """accelsearch command-line semantics (flo/fhi default to given)."""
     p = Params()
     lib.accel_params_default(C.byref(p))
@@ -181,6 +183,7 @@ def make_params(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1
     else:
         p.fhiP, p.fhi = 1, float(fhi)
     p.inmem, p.force_standard, p.dat_input = int(inmem), int(force_standard), int(dat_input)
+    p.nph = float(nph)
     return p
 
 

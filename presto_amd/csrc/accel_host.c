@@ -184,6 +184,7 @@ int accel_geom_create(const accel_params *p, accel_geom *g)
     g->N = (long long) p->N;
     g->numbins = p->numbins;
     g->lobin = p->lobin;
+    g->nph = (p->lobin > 0 || !(p->nph > 0.0)) ? 0.0 : p->nph;
     if (g->lobin > 0 && g->lobin > g->numbins - 1) {
         set_err("'lobin' is greater than the total number of frequencies in the data set");
         return -1;
@@ -738,6 +739,36 @@ static int block_starts(accel_search *s)
     return 0;
 }
 
+/* -photon: every window's norm is 1/sqrt(nph) (src/accel_utils.c:1080-1086),
+ * the same FP64 value and FP32 square that the median path produces */
+static int photon_norms(accel_search *s)
+{
+    const int n = s->nwin > 0 ? s->nwin : 1;
+    double *norm = (double *) malloc(sizeof(double) * (size_t) n);
+    float *nsq = (float *) malloc(sizeof(float) * (size_t) n);
+    if (!norm || !nsq) {
+        free(norm);
+        free(nsq);
+        set_err("out of host memory for %d window norms", n);
+        return -1;
+    }
+    const double nv = 1.0 / sqrt(s->g.nph);
+    for (int i = 0; i < n; i++) {
+        norm[i] = nv;
+        nsq[i] = (float) (nv * nv);
+    }
+    int rc = agpu_h2d(s->d_norm, norm, sizeof(double) * (size_t) n, s->stream) ||
+             agpu_h2d(s->d_nsq, nsq, sizeof(float) * (size_t) n, s->stream) ||
+             agpu_stream_sync(s->stream);
+    free(norm);
+    free(nsq);
+    if (rc) {
+        set_err("photon norms: %s", agpu_last_error());
+        return -1;
+    }
+    return 0;
+}
+
 static int search_init(accel_search *s, int device)
 {
     accel_geom *g = &s->g;
@@ -781,6 +812,8 @@ static int search_init(accel_search *s, int device)
     GPU(agpu_malloc((void **) &s->d_hits, sizeof(agpu_hit) * s->cap));
     GPU(agpu_malloc((void **) &s->d_nhits, sizeof(unsigned)));
     GPU(agpu_h2d(s->d_win, win, sizeof(agpu_window) * (size_t) (s->nwin > 0 ? s->nwin : 1), s->stream));
+    if (g->nph > 0.0 && photon_norms(s))
+        return -1;
     GPU(agpu_h2d(s->d_zmap, zmap, sizeof(int) * (size_t) nf * g->numz, s->stream));
     GPU(agpu_h2d(s->d_wmap, wmap, sizeof(int) * (size_t) nf * s->numw1, s->stream));
     free(win);
@@ -904,8 +937,9 @@ int accel_search_launch(accel_search *s)
     const double t0 = now_ms();
     GPU(agpu_memset(s->d_nhits, 0, sizeof(unsigned), s->stream));
     GPU(agpu_event_record(s->ev[0], s->stream));
-    GPU(agpu_block_norms(s->d_spec, g->numbins, g->lobin, s->d_win, s->nwin, s->maxn,
-                         s->d_norm, s->d_nsq, s->stream));
+    if (!(g->nph > 0.0))   /* -photon: the constant norms set at creation */
+        GPU(agpu_block_norms(s->d_spec, g->numbins, g->lobin, s->d_win, s->nwin, s->maxn,
+                             s->d_norm, s->d_nsq, s->stream));
     if (g->inmem)
         GPU(agpu_inmem_nsq_table(s->d_nsq, s->npre, g->nblocks, 2 * fund_rlo(g->rlo),
                                  g->corr_uselen, s->hb_end, s->d_nsq_hb, s->stream));
@@ -1848,13 +1882,14 @@ int accel_search_set_series(accel_search *s, const float *series, long long n)
         s->dmaxn = maxn;
     }
     /* realfft (src/accel_utils.c:1547), deredden (:1561), bin 0 := 1+1i
-     * (:1621-1624), all on the device */
+     * (:1621-1624; left as de-reddened under -photon), all on the device */
     static const accel_fcomplex one = {1.0f, 1.0f};
     GPU(agpu_h2d(s->d_spec, series, sizeof(float) * (size_t) n, s->stream));
     GPU(agpu_realfft((const float *) s->d_spec, n, s->d_spec, s->d_fftwork, s->stream));
     GPU(agpu_deredden(s->d_spec, g->numbins, s->d_dblk, s->ndblk, s->dmaxn, s->d_dmean,
                       s->stream));
-    GPU(agpu_h2d(s->d_spec, &one, sizeof(one), s->stream));
+    if (!(s->p.nph > 0.0))
+        GPU(agpu_h2d(s->d_spec, &one, sizeof(one), s->stream));
     return 0;
 }
 
@@ -1871,7 +1906,7 @@ long long accel_search_footprint(const accel_params *p)
     if (accel_geom_create(p, &g))
         return -1;
     const long long numw1 = g.numw ? g.numw : 1, nt = numw1 * g.numz;
-    const long long hb_end = (long long) (2.0 * g.rlo) + (long long) g.nblocks * g.corr_uselen;
+    const long long hb_end = 2 * fund_rlo(g.rlo) + (long long) g.nblocks * g.corr_uselen;
     const long long ntiles = (hb_end + AGPU_TILE - 1) / AGPU_TILE + 1;
     const int nfrac = (1 << (g.numharmstages - 1)) - 1;
     long long bytes = 4LL * nt * ntiles * AGPU_ROWLEN;        /* raw plane */

@@ -695,8 +695,8 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
         input_shorts = strcmp(suffix, "sdat") == 0;
         free(suffix);
     }
-    if (cmd->photonP || cmd->locpowP) {
-        sf_fail(1, "-photon and -locpow normalisations are not built in this library");
+    if (cmd->locpowP) {
+        sf_fail(1, "the -locpow normalisation is not built in this library");
         return;
     }
     if (cmd->noharmpolishP) {
@@ -785,9 +785,17 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
     if (cmd->wmaxP && cmd->wmax % ACCEL_DW)
         cmd->wmax = (cmd->wmax / ACCEL_DW + 1) * ACCEL_DW;
     obs->N = (long long) idata->N;
-    obs->nph = 0.0f;
     obs->norm_type = 0;
-    printf("Normalizing powers using median-blocks%s.\n\n", cmd->medianP ? "" : " (default)");
+    if (cmd->photonP) {
+        /* bin 0's real part: the mapped .fft's, or 1 for a .dat series,
+         * whose de-reddening sets bin 0 to (1, 0) (src/misc_utils.c:943-944)
+         * and which then keeps it (src/accel_utils.c:1599-1605) */
+        obs->nph = obs->dat_input ? 1.0 : (obs->numbins > 0 ? obs->fft[0].r : 0.0);
+        printf("Normalizing powers using %.0f photons.\n\n", obs->nph);
+    } else {
+        obs->nph = 0.0;
+        printf("Normalizing powers using median-blocks%s.\n\n", cmd->medianP ? "" : " (default)");
+    }
 
     accel_params p;
     accel_params_default(&p);
@@ -811,6 +819,7 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
     p.inmem = cmd->inmemP;
     p.dat_input = obs->dat_input;
     p.noharmremove = cmd->noharmremoveP;
+    p.nph = obs->nph;
     accel_geom g;
     if (accel_geom_create(&p, &g)) {
         char m[1100];
@@ -826,6 +835,7 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
         return;
     }
     obs->lobin = g.lobin;
+    obs->nph = g.nph;   /* 0 when lobin > 0, src/accel_utils.c:1626-1628 */
     obs->numharmstages = g.numharmstages;
     obs->dz = g.dz;
     obs->numz = g.numz;
@@ -1177,8 +1187,17 @@ ffdotpows *subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
     f->fullrlo = fullrlo;
     f->pub.rinds = shi->rinds;
     f->pub.zinds = shi->zinds;
-    if (agpu_block_norm1(st->v.d_spec, st->v.numbins, st->v.lobin, lobin, numdata, st->d_norm,
-                         st->d_nsq, st->v.stream) ||
+    int nrc;
+    if (st->obs->nph > 0.0) {   /* -photon, src/accel_utils.c:1080-1086 */
+        const double nv = 1.0 / sqrt(st->obs->nph);
+        const float nsq = (float) (nv * nv);
+        nrc = agpu_h2d(st->d_norm, &nv, sizeof(nv), st->v.stream) ||
+              agpu_h2d(st->d_nsq, &nsq, sizeof(nsq), st->v.stream);
+    } else {
+        nrc = agpu_block_norm1(st->v.d_spec, st->v.numbins, st->v.lobin, lobin, numdata,
+                               st->d_norm, st->d_nsq, st->v.stream);
+    }
+    if (nrc ||
         agpu_sf_extract(st->v.d_plane, st->v.nrows, st->v.ntiles, st->v.numz, woff, zoff, numws,
                         numzs, numrs, hb0, st->d_nsq, f->d_pow, st->v.stream)) {
         sf_fail(-1, "subharm_fderivs_vol: %s", agpu_last_error());

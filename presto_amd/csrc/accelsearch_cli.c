@@ -3,18 +3,22 @@
  * options of clig/accelsearch_cmd.cli:13-48) over libpresto_accel.
  *
  *   accelsearch [-numharm n] [-zmax z] [-wmax w] [-sigma s] [-flo f] [-fhi f]
- *               [-rlo r] [-rhi r] [-lobin b] [-inmem] [-median]
- *               [-zaplist file] [-baryv v] [-noharmremove] [-ncpus n]
- *               [-device d] file.{dat,sdat,fft}
+ *               [-rlo r] [-rhi r] [-lobin b] [-inmem] [-median] [-photon]
+ *               [-zaplist file] [-baryv v] [-noharmremove] [-otheropt]
+ *               [-ncpus n] [-device d] file.{dat,sdat,fft}
  *
  * Reads <root>.inf (N, dt, telescope) and the data file; for time series:
- * GPU real FFT, de-reddening and bin 0 := 1+1i, optional birdie zapping;
- * then the search, the FP64 refinement and the outputs
- * <root>_ACCEL_<zmax>[_JERK_<wmax>] (text, .inf appended) and .cand.
- * Normalisations other than the block median (-photon, -locpow), the
- * unpolished optimiser (-noharmpolish) and -otheropt are not built and
- * are refused.  -ncpus is accepted and ignored (the work runs on the GPU).
+ * GPU real FFT, de-reddening and bin 0 := 1+1i (left as de-reddened under
+ * -photon), optional birdie zapping; then the search, the FP64 refinement
+ * and the outputs <root>_ACCEL_<zmax>[_JERK_<wmax>] (text, .inf appended),
+ * .cand and, for an .fft input, .txtcand.  The local-power normalisation
+ * (-locpow) and the unpolished optimiser (-noharmpolish) are not built and
+ * are refused.  -otheropt selects the reference's transposed-plane variants,
+ * which form the same sums: it changes nothing here.  -ncpus is accepted
+ * and ignored (the work runs on the GPU).
  */
+#include <limits.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -25,9 +29,35 @@ static void usage(void)
 {
     fprintf(stderr,
             "usage: accelsearch [-numharm n] [-zmax z] [-wmax w] [-sigma s] [-flo Hz] [-fhi Hz]\n"
-            "                   [-rlo bin] [-rhi bin] [-lobin bin] [-inmem] [-median]\n"
-            "                   [-zaplist file] [-baryv v] [-noharmremove] [-ncpus n]\n"
-            "                   [-device d] infile.{dat,sdat,fft}\n");
+            "                   [-rlo bin] [-rhi bin] [-lobin bin] [-inmem] [-median] [-photon]\n"
+            "                   [-zaplist file] [-baryv v] [-noharmremove] [-otheropt]\n"
+            "                   [-ncpus n] [-device d] infile.{dat,sdat,fft}\n");
+}
+
+/* The option ranges of clig/accelsearch_cmd.cli:13-44, checked as the
+ * generated parser does (src/accelsearch_cmd.c:246-335, 640-770). */
+static void check_int(const char *opt, long long v, long long lo, long long hi)
+{
+    if (v > hi) {
+        fprintf(stderr, "accelsearch: parameter 1 of option `%s' greater than max=%lld\n", opt, hi);
+        exit(EXIT_FAILURE);
+    }
+    if (v < lo) {
+        fprintf(stderr, "accelsearch: parameter 1 of option `%s' smaller than min=%lld\n", opt, lo);
+        exit(EXIT_FAILURE);
+    }
+}
+
+static void check_real(const char *opt, double v, double lo, double hi)
+{
+    if (v > hi) {
+        fprintf(stderr, "accelsearch: parameter 1 of option `%s' greater than max=%f\n", opt, hi);
+        exit(EXIT_FAILURE);
+    }
+    if (v < lo) {
+        fprintf(stderr, "accelsearch: parameter 1 of option `%s' smaller than min=%f\n", opt, lo);
+        exit(EXIT_FAILURE);
+    }
 }
 
 typedef struct {
@@ -92,54 +122,69 @@ int main(int argc, char **argv)
     accel_params_default(&p);
     const char *infile = NULL, *zaplist = NULL;
     double baryv = 0.0;
-    int device = 0;
+    int device = 0, photon = 0;
     p.floP = 1;
     p.fhiP = 1;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
 #define NEXT() (i + 1 < argc ? argv[++i] : (usage(), exit(1), ""))
-        if (!strcmp(a, "-numharm"))
+        if (!strcmp(a, "-numharm")) {
             p.numharm = atoi(NEXT());
-        else if (!strcmp(a, "-zmax"))
+            check_int(a, p.numharm, 1, 32);
+        } else if (!strcmp(a, "-zmax")) {
             p.zmax = atoi(NEXT());
-        else if (!strcmp(a, "-wmax")) {
+            check_int(a, p.zmax, 0, 1200);
+        } else if (!strcmp(a, "-wmax")) {
             p.wmax = atoi(NEXT());
             p.wmaxP = 1;
-        } else if (!strcmp(a, "-sigma"))
+            check_int(a, p.wmax, 0, 4000);
+        } else if (!strcmp(a, "-sigma")) {
             p.sigma = (float) atof(NEXT());
-        else if (!strcmp(a, "-flo")) {
+            check_real(a, p.sigma, 1.0, 30.0);
+        } else if (!strcmp(a, "-flo")) {
             p.flo = atof(NEXT());
             p.floP = 1;
+            check_real(a, p.flo, 0.0, HUGE_VAL);
         } else if (!strcmp(a, "-fhi")) {
             p.fhi = atof(NEXT());
             p.fhiP = 1;
+            check_real(a, p.fhi, 0.0, HUGE_VAL);
         } else if (!strcmp(a, "-rlo")) {
             /* -flo and -fhi carry defaults, so floP and fhiP are always set
              * (src/accelsearch_cmd.c:49-61) and win over -rlo and -rhi
              * (src/accel_utils.c:1686-1730): those two change nothing */
             p.rlo = atof(NEXT());
             p.rloP = 1;
+            check_real(a, p.rlo, 0.0, HUGE_VAL);
         } else if (!strcmp(a, "-rhi")) {
             p.rhi = atof(NEXT());
             p.rhiP = 1;
-        } else if (!strcmp(a, "-lobin"))
+            check_real(a, p.rhi, 0.0, HUGE_VAL);
+        } else if (!strcmp(a, "-lobin")) {
             p.lobin = atoll(NEXT());
-        else if (!strcmp(a, "-inmem"))
+            check_int(a, p.lobin, 0, LLONG_MAX);
+        } else if (!strcmp(a, "-inmem"))
             p.inmem = 1;
         else if (!strcmp(a, "-median"))
             ;
+        else if (!strcmp(a, "-photon"))
+            photon = 1;
+        else if (!strcmp(a, "-otheropt"))
+            ;   /* the transposed-plane variants form the same sums
+                 * (src/accel_utils.c:1236-1251, 1284-1309, 1364-1409) */
         else if (!strcmp(a, "-zaplist"))
             zaplist = NEXT();
-        else if (!strcmp(a, "-baryv"))
+        else if (!strcmp(a, "-baryv")) {
             baryv = atof(NEXT());
+            check_real(a, baryv, -0.1, 0.1);
+        }
         else if (!strcmp(a, "-noharmremove"))
             p.noharmremove = 1;
         else if (!strcmp(a, "-ncpus"))
-            (void) NEXT();
+            check_int(a, atoi(NEXT()), 1, INT_MAX);
         else if (!strcmp(a, "-device"))
             device = atoi(NEXT());
-        else if (!strcmp(a, "-photon") || !strcmp(a, "-locpow") || !strcmp(a, "-otheropt") ||
-                 !strcmp(a, "-noharmpolish")) {
+        else if (!strcmp(a, "-locpow") || !strcmp(a, "-noharmpolish")) {
             fprintf(stderr, "accelsearch: option %s is not built in this implementation\n", a);
             return 1;
         } else if (a[0] == '-') {
@@ -211,8 +256,10 @@ int main(int argc, char **argv)
             fprintf(stderr, "\naccelsearch: de-reddening failed: %s\n", accel_last_error());
             return 1;
         }
-        fft[0].r = 1.0f;
-        fft[0].i = 1.0f;
+        if (!photon) {
+            fft[0].r = 1.0f;
+            fft[0].i = 1.0f;
+        }
         printf("done.\n\n");
         p.N = (double) n;
         p.dat_input = 1;
@@ -226,6 +273,12 @@ int main(int argc, char **argv)
     }
     p.dt = inf.dt;
     p.numbins = numbins;
+    /* src/accel_utils.c:1599-1605: bin 0's real part (mapped .fft, or the
+     * de-reddened .dat spectrum) counts the photons */
+    if (photon) {
+        p.nph = numbins > 0 ? fft[0].r : 0.0;
+        printf("Normalizing powers using %.0f photons.\n\n", p.nph);
+    }
     if (zaplist && dat) {
         /* src/accelsearch.c:87-110 */
         const double T = p.N * p.dt;

@@ -35,8 +35,22 @@ def test_cli_rejects_bad_arguments(tmp_path):
     assert os.path.exists(CLI)
     r = _run(["-bogus", "x.dat"], tmp_path)
     assert r.returncode == 1 and "unknown option" in r.stderr
-    r = _run(["-photon", "x.dat"], tmp_path)
+    r = _run(["-locpow", "x.dat"], tmp_path)
     assert r.returncode == 1 and "not built" in r.stderr
+    # the ranges of clig/accelsearch_cmd.cli:13-44, with the generated
+    # parser's messages (src/accelsearch_cmd.c:246-335)
+    for args, msg in [(["-zmax", "1300"], "`-zmax' greater than max=1200"),
+                      (["-numharm", "64"], "`-numharm' greater than max=32"),
+                      (["-numharm", "0"], "`-numharm' smaller than min=1"),
+                      (["-wmax", "4020"], "`-wmax' greater than max=4000"),
+                      (["-sigma", "0.5"], "`-sigma' smaller than min=1.000000"),
+                      (["-sigma", "31"], "`-sigma' greater than max=30.000000"),
+                      (["-baryv", "0.2"], "`-baryv' greater than max=0.100000"),
+                      (["-lobin", "-3"], "`-lobin' smaller than min=0"),
+                      (["-flo", "-1"], "`-flo' smaller than min=0.000000"),
+                      (["-ncpus", "0"], "`-ncpus' smaller than min=1")]:
+        r = _run(args + ["x.dat"], tmp_path)
+        assert r.returncode == 1 and msg in r.stderr, (args, r.stderr)
     r = _run(["x.txt"], tmp_path)
     assert r.returncode == 1 and "must be an '.fft' or '.[s]dat' file" in r.stderr
     r = _run(["missing.dat"], tmp_path)

@@ -267,6 +267,68 @@ def test_explicit_r_range_and_odd_zmax(rlo, rhi, zmax, inmem):
     assert all(rlo - 1 <= c["r"] * c["numharm"] <= rhi + 1 for c in gc)
 
 
+def _poisson_counts(N, dt, seed, f0, fdot, mean, frac, width=0.04):
+    """Photon counts per bin of a pulsed source: a Gaussian pulse of the
+    given phase width carrying `frac` of the mean rate."""
+    t = np.arange(N) * dt
+    ph = np.mod(f0 * t + 0.5 * fdot * t * t, 1.0)
+    d = np.minimum(ph, 1.0 - ph)
+    prof = np.exp(-0.5 * (d / width) ** 2)
+    rate = mean * ((1.0 - frac) + frac * prof / prof.mean())
+    return np.random.default_rng(seed).poisson(rate).astype(np.float32)
+
+
+@pytest.mark.parametrize("inmem", [False, True])
+def test_photon_fft_input(inmem):
+    """-photon on an .fft of photon counts: nph is bin 0's real part (the
+    total count, src/accel_utils.c:1599-1605) and every block is normalised
+    by 1/sqrt(nph) instead of its median (:1080-1086)."""
+    _say(f"case: -photon, .fft of Poisson counts, zmax 20 numharm 8, inmem={inmem}")
+    N, dt = 1 << 16, 5e-4
+    x = _poisson_counts(N, dt, 38, 41.7, 0.02, 3.0, 0.08)
+    spec = A.realfft_gpu(x)
+    nph = float(spec[0].real)
+    assert abs(nph - x.sum()) < 1e-3 * x.sum()
+    kw = dict(zmax=20, numharm=8, sigma=2.0, dat_input=False, force_standard=not inmem,
+              nph=nph)
+    p = A.make_params(len(spec), N, dt, **kw)
+    o = O.make_obs(len(spec), N, dt, **kw)
+    assert o.inmem == int(inmem) and o.nph == nph
+    with _run_gpu(spec, p) as s:
+        assert s.geom.nph == nph
+        assert np.all(s.norms() == 1.0 / np.sqrt(nph))
+        gh, gc = s.hits(), s.candidates()
+    oh, oc, _ = O.accelsearch(o, spec)
+    worst = _compare_hits(gh, oh, list(o.powcut))
+    _compare_cands(gc, oc)
+    assert worst < 1e-4 and len(gc) >= 1
+    r_exp = 41.7 * N * dt + 0.02 * (N * dt) ** 2 / 2
+    assert any(abs(c["r"] - r_exp) < 3.0 for c in gc)
+    _say(f"photon: {len(gh)} hits, {len(gc)} candidates, identical")
+
+
+def test_photon_dat_series_keeps_bin0():
+    """-photon on a .dat series: the de-reddened bin 0 (1, 0) is kept, not
+    set to (1, 1) (src/accel_utils.c:1599-1624), so nph is 1."""
+    _say("case: -photon, .dat series through the device preparation")
+    N, dt = 1 << 16, 5e-4
+    x = synth.make_series_f32(N, dt, 39, [dict(f0=41.7, z=3.0, duty=0.05, amp=0.3)])
+    p = A.make_params(N // 2, N, dt, zmax=20, numharm=4, sigma=2.0, nph=1.0)
+    with A.Search(p) as s:
+        s.set_series(x)
+        spec = s.spectrum()
+        s.run()
+        gh, gc = s.hits(), s.candidates()
+    assert spec[0] == np.complex64(1.0 + 0.0j)
+    ref, _ = O.deredden(A.realfft_gpu(x), 6, 200, 0.5 / dt, N * dt)
+    assert np.array_equal(spec.view(np.uint32), ref.view(np.uint32))
+    o = O.make_obs(len(spec), N, dt, zmax=20, numharm=4, sigma=2.0, nph=1.0)
+    oh, oc, _ = O.accelsearch(o, spec)
+    _compare_hits(gh, oh, list(o.powcut))
+    _compare_cands(gc, oc)
+    assert len(gc) >= 1
+
+
 # ------------------------------------------------------------------ jerk (w) search
 @pytest.fixture(scope="module")
 def jerk_case():

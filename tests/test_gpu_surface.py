@@ -105,19 +105,24 @@ def test_standard_run_fractional_rlo_equals_handle(tmp_path, frac):
     assert any(abs(c["r"] / T - 4995.37) < 0.05 for c in cands)
 
 
-def test_fft_input_cli_equals_surface(tmp_path):
+@pytest.mark.parametrize("photon", [False, True])
+def test_fft_input_cli_equals_surface(tmp_path, photon):
     """An .fft input: the accelsearch command (handle path) writes the
     same .txtcand work file (one line per candidate the merge adds or
     replaces, src/accel_utils.c:1443-1447, 1666-1684), ACCEL text and .cand
-    bytes as the reference's block loop through the surface."""
+    bytes as the reference's block loop through the surface; with -photon
+    both normalise by bin 0's count (src/accel_utils.c:1599-1605)."""
     import subprocess
-    print("case: .fft input, CLI vs surface, 2^17 samples, zmax 20, numharm 4", flush=True)
+    print(f"case: .fft input, CLI vs surface, 2^17 samples, zmax 20, numharm 4, "
+          f"photon {photon}", flush=True)
     N, dt = 1 << 17, 64e-6
     x = synth.make_series_f32(N, dt, 75, [dict(f0=123.456, z=6.0, duty=0.05, amp=0.3),
                                           dict(f0=377.7, z=-10.0, duty=0.08, amp=0.25)])
+    if photon:   # photon counts of the same pulsed signal
+        x = np.random.default_rng(75).poisson(np.maximum(2.0 + x, 0.0)).astype(np.float32)
     A.realfft_gpu(x).tofile(tmp_path / "ffti.fft")
     S.write_inf(str(tmp_path / "ffti"), N, dt)
-    cmd = S.make_cmdline(str(tmp_path / "ffti.fft"), zmax=20, numharm=4)
+    cmd = S.make_cmdline(str(tmp_path / "ffti.fft"), zmax=20, numharm=4, photon=photon)
     info, cands = S.drive(str(tmp_path / "ffti.fft"), cmd)
     assert len(cands) >= 2
     names = ("ffti_ACCEL_20", "ffti_ACCEL_20.cand", "ffti_ACCEL_20.txtcand")
@@ -127,9 +132,10 @@ def test_fft_input_cli_equals_surface(tmp_path):
     for n in names:
         os.remove(tmp_path / n)
     cli = os.path.join(os.path.dirname(A.__file__), "accelsearch")
-    r = subprocess.run([cli, "-zmax", "20", "-numharm", "4", "ffti.fft"], cwd=tmp_path,
-                       capture_output=True, text=True, timeout=300)
+    r = subprocess.run([cli, "-zmax", "20", "-numharm", "4"] + (["-photon"] if photon else []) +
+                       ["ffti.fft"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
+    assert ("Normalizing powers using" in r.stdout and "photons" in r.stdout) == photon
     for n in names:
         assert open(tmp_path / n, "rb").read() == surf[n], n
 

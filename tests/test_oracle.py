@@ -234,6 +234,34 @@ def test_fundamental_plane_is_normalised_direct_correlation(small_case):
     banks.close()
 
 
+@pytest.mark.parametrize("numharm,harm", [(1, 1), (4, 3)])
+def test_photon_plane_is_direct_correlation_over_nph(small_case, numharm, harm):
+    """-photon (src/accel_utils.c:1080-1086): every block is normalised by
+    1/sqrt(nph), whatever its median, so the plane is the raw correlation
+    power over nph; lobin > 0 switches it off (:1626-1628)."""
+    fft, obs0 = small_case
+    nph = 7.25
+    obs = O.make_obs(len(fft), 8192, 1e-3, zmax=20, numharm=4, sigma=2.0,
+                     force_standard=True, nph=nph)
+    assert obs.nph == nph
+    assert O.make_obs(len(fft), 8192, 1e-3, zmax=20, numharm=4, lobin=3, nph=nph).nph == 0.0
+    assert O.block_norm(obs, fft, 100, 512) == 1.0 / np.sqrt(nph)
+    banks = O.SubharmBanks(obs, fp64=True)
+    startr = obs.rlo + 2 * (obs.corr_uselen // 2)
+    lastr = startr + obs.corr_uselen // 2 - 0.5
+    plane, meta = O.subharm_fderivs_vol(numharm, harm, startr, lastr, banks, obs, fft,
+                                        fp64=True)
+    rng = np.random.default_rng(2)
+    nr = obs.corr_uselen if numharm == 1 else int(obs.corr_uselen * harm / numharm)
+    for _ in range(30):
+        zi = int(rng.integers(meta["numzs"]))
+        ri = int(rng.integers(nr))
+        z = meta["zlo"] + 2 * zi
+        expect = _raw_power(fft, obs.numbins, 2 * meta["rlo"] + ri, z) / nph
+        assert abs(plane[0, zi, ri] - expect) <= 1e-9 * max(expect, 1.0), (zi, ri)
+    banks.close()
+
+
 @pytest.mark.parametrize("numharm,harm", [(2, 1), (4, 1), (4, 3)])
 def test_subharmonic_plane_is_normalised_direct_correlation(small_case, numharm, harm):
     """The standard path's sub-harmonic block equals norm_h^2 times the raw
