@@ -50,6 +50,9 @@ typedef struct {
                           * normalised by 1/sqrt(nph) (:1080-1086) instead of its
                           * median.  0: block medians.  A .dat spectrum keeps bin 0
                           * as de-reddened, (1, 0), instead of (1, 1) (:1606-1624) */
+    int noharmpolish;    /* -noharmpolish: refine every harmonic on its own, without
+                          * polishing (src/accel_utils.c:582-622, max_rz_arr /
+                          * max_rzw_arr) */
 } accel_params;
 
 /* Fills the command-line defaults: numharm 8, zmax 200, sigma 2.0,

@@ -12,6 +12,7 @@
  *                                 restated from its published algorithm)
  *   o_max_rz_arr[_harmonics]      src/maximize_rz.c:103-228
  *   o_max_rzw_arr_harmonics       src/maximize_rzw.c:84-191
+ *   o_max_rzw_arr                 src/maximize_rzw.c:3-55 (-noharmpolish)
  *
  * Arithmetic follows the reference: FP32 spectrum and FP32 response taps,
  * products accumulated in FP64 and stored to an FP32 complex after every
@@ -562,4 +563,45 @@ void o_max_rzw_arr_harmonics(const o_fcomplex *data, long numdata, int numharm, 
         powers[k] = derivs[k].pow;
     }
     free(powsum);
+}
+
+/* src/maximize_rzw.c:3-55: one harmonic, no polishing.  The 9 x 9 x 24
+ * grid of r from (int) rin - 1 in steps of 1/8, z = zin +- DZ and
+ * w = win +- DW in quarter steps; the first maximum of the raw power in
+ * (w, z, r) order; derivatives at it with the local power of the start. */
+double o_max_rzw_arr(const o_fcomplex *data, long numdata, double rin, double zin, double win,
+                     double *rout, double *zout, double *wout, o_rderivs *derivs)
+{
+    const int interpfac = 4, nb = interpfac * O_RDR;
+    const int numr = 3 * interpfac * O_RDR, numz = 2 * interpfac + 1, numw = numz;
+    const float locpow = (float) o_localpower3d(data, numdata, rin, zin, win);
+    const long startbin = (long) (int) rin - 1;
+    const double zlo = zin - O_DZ, zhi = zin + O_DZ, wlo = win - O_DW, whi = win + O_DW;
+    const double dz = (zhi - zlo) / (numz - 1), dw = (whi - wlo) / (numw - 1);
+    const double maxz = (fabs(zlo) < fabs(zhi)) ? zhi : zlo;
+    const double maxw = (fabs(wlo) < fabs(whi)) ? whi : wlo;
+    const int vkhw = o_w_resp_halfwidth(maxz, maxw, 0);
+    double *row = (double *) malloc(sizeof(double) * numr);
+    float maxpow = 0.0f;
+    int rind = 0, zind = 0, wind = 0;
+    for (int ii = 0; ii < numw; ii++)
+        for (int jj = 0; jj < numz; jj++) {
+            rzw_vol_row(data, numdata, nb, startbin, zlo + jj * dz, wlo + ii * dw, vkhw, 0, numr,
+                        row);
+            for (int kk = 0; kk < numr; kk++) {
+                const float p = (float) row[kk];
+                if (p > maxpow) {
+                    maxpow = p;
+                    wind = ii;
+                    zind = jj;
+                    rind = kk;
+                }
+            }
+        }
+    free(row);
+    *rout = startbin + rind * O_DR / (double) interpfac;
+    *zout = zin - O_DZ + zind * O_DZ / (double) interpfac;
+    *wout = win - O_DW + wind * O_DW / (double) interpfac;
+    o_derivs3d(data, numdata, *rout, *zout, *wout, locpow, derivs);
+    return maxpow;
 }

@@ -207,6 +207,8 @@ long o_max_rz_arr_harmonics(const o_fcomplex *data, long numdata, int numharm, d
 void o_max_rzw_arr_harmonics(const o_fcomplex *data, long numdata, int numharm, double rin,
                              double zin, double win, double *rout, double *zout, double *wout,
                              o_rderivs *derivs, double *powers);
+double o_max_rzw_arr(const o_fcomplex *data, long numdata, double rin, double zin, double win,
+                     double *rout, double *zout, double *wout, o_rderivs *derivs);
 
 #ifdef __cplusplus
 }

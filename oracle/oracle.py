@@ -343,6 +343,8 @@ _sig("o_max_rz_arr_harmonics", C.c_long, _cf, C.c_long, _i, _d, _d, C.POINTER(_d
      C.POINTER(_d), C.POINTER(ORDerivs), _dp)
 _sig("o_max_rzw_arr_harmonics", None, _cf, C.c_long, _i, _d, _d, _d, C.POINTER(_d),
      C.POINTER(_d), C.POINTER(_d), C.POINTER(ORDerivs), _dp)
+_sig("o_max_rzw_arr", _d, _cf, C.c_long, _d, _d, _d, C.POINTER(_d), C.POINTER(_d),
+     C.POINTER(_d), C.POINTER(ORDerivs))
 
 
 def _c64(data):
@@ -406,6 +408,15 @@ def max_rz_arr_harmonics(data, r, z, numharm):
     nev = lib.o_max_rz_arr_harmonics(d, len(d), numharm, r, z, C.byref(ro), C.byref(zo),
                                      rds, pw)
     return ro.value, zo.value, list(rds), pw, nev
+
+
+def max_rzw_arr(data, r, z, w):
+    """[maxpow, rmax, zmax, wmax, rderivs] as src/maximize_rzw.c:3-55."""
+    d = _c64(data)
+    ro, zo, wo, rd = _d(), _d(), _d(), ORDerivs()
+    p = lib.o_max_rzw_arr(d, len(d), r, z, w, C.byref(ro), C.byref(zo), C.byref(wo),
+                          C.byref(rd))
+    return p, ro.value, zo.value, wo.value, rd
 
 
 def max_rzw_arr_harmonics(data, r, z, w, numharm):

@@ -25,7 +25,7 @@ class Params(C.Structure):
                 ("sigma", _f), ("flo", _d), ("fhi", _d), ("rlo", _d), ("rhi", _d),
                 ("floP", _i), ("fhiP", _i), ("rloP", _i), ("rhiP", _i),
                 ("inmem", _i), ("force_standard", _i), ("dat_input", _i), ("noharmremove", _i),
-                ("nph", _d)]
+                ("nph", _d), ("noharmpolish", _i)]
 
 
 class Geom(C.Structure):
@@ -166,7 +166,7 @@ def _err():
 
 def make_params(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0,
                 fhi=10000.0, inmem=False, force_standard=False, lobin=0, dat_input=True,
-                rlo=None, rhi=None, nph=0.0):
+                rlo=None, rhi=None, nph=0.0, noharmpolish=False):
     """accelsearch command-line semantics (flo/fhi default to given)."""
     p = Params()
     lib.accel_params_default(C.byref(p))
@@ -184,6 +184,7 @@ def make_params(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1
         p.fhiP, p.fhi = 1, float(fhi)
     p.inmem, p.force_standard, p.dat_input = int(inmem), int(force_standard), int(dat_input)
     p.nph = float(nph)
+    p.noharmpolish = int(noharmpolish)
     return p
 
 

@@ -494,6 +494,8 @@ struct accel_search {
     int nopt;
     int *opt_nh;
     double *opt_r, *opt_z, *opt_w;    /* fundamental optimum; r includes lobin */
+    double *opt_h;                    /* [nopt][3][ACCEL_MAXHARM]: every harmonic's
+                                       * r (with lobin), z, w */
     double *opt_pows;                 /* [nopt][ACCEL_MAXHARM] */
     accel_rderivs *opt_derivs;        /* [nopt][ACCEL_MAXHARM] */
     refine_ws *ws;                    /* refinement device buffers */
@@ -873,6 +875,7 @@ void accel_search_destroy(accel_search *s)
     free(s->opt_w);
     free(s->opt_pows);
     free(s->opt_derivs);
+    free(s->opt_h);
     if (s->ws) {
         ws_free(s->ws);
         free(s->ws);
@@ -1250,6 +1253,8 @@ static void clear_opt(accel_search *s)
     free(s->opt_w);
     free(s->opt_pows);
     free(s->opt_derivs);
+    free(s->opt_h);
+    s->opt_h = NULL;
     s->opt_nh = NULL;
     s->opt_r = s->opt_z = s->opt_w = s->opt_pows = NULL;
     s->opt_derivs = NULL;
@@ -1439,15 +1444,15 @@ static int ws_reserve(refine_ws *w, size_t scr_bytes)
 /* Runs max_rz_arr_harmonics (win == NULL) or max_rzw_arr_harmonics for n
  * starts on the device spectrum d_spec, in chunks of at most REFINE_CHUNK
  * candidates (one workgroup each), with the buffers of w. */
-static int refine_device(refine_ws *w, const void *d_spec, long long numdata, int n,
+static int refine_device(refine_ws *w, const void *d_spec, long long numdata, int raw, int n,
                          const int *nh, const double *rin, const double *zin, const double *win,
                          double *rout, double *zout, double *wout, double *pows,
                          accel_rderivs *derivs, void *stream)
 {
     int cap = 0;
     for (int i = 0; i < n; i++) {
-        if (nh[i] < 1 || nh[i] > ACCEL_MAXHARM) {
-            set_err("refinement: numharm %d out of range 1..%d", nh[i], ACCEL_MAXHARM);
+        if (nh[i] < 1 || nh[i] > ACCEL_MAXHARM || (raw && nh[i] != 1)) {
+            set_err("refinement: numharm %d out of range 1..%d", nh[i], raw ? 1 : ACCEL_MAXHARM);
             return -1;
         }
         if (win) {
@@ -1491,6 +1496,7 @@ static int refine_device(refine_ws *w, const void *d_spec, long long numdata, in
         a.nevals = w->d_nev;
         a.scratch = w->d_scr;
         a.npts_cap = cap;
+        a.raw = raw;
         if (agpu_memset(w->d_der, 0, sizeof(double) * 7 * mhx, stream) ||
             agpu_memset(w->d_pow, 0, sizeof(double) * mhx, stream) ||
             (win ? agpu_refine_rzw(&a, stream) : agpu_refine_rz(&a, stream)) ||
@@ -1512,6 +1518,66 @@ static int refine_device(refine_ws *w, const void *d_spec, long long numdata, in
             }
     }
     return 0;
+}
+
+/* -noharmpolish (src/accel_utils.c:582-622): harmonic k of each candidate
+ * refined on its own from (k r - lobin, k z, k w), max_rz_arr /
+ * max_rzw_arr, all harmonics of all candidates in one launch */
+static int refine_unpolished(accel_search *s, int n, const int *nh, const double *rin,
+                             const double *zin, const double *win)
+{
+    const accel_geom *g = &s->g;
+    int nv = 0;
+    for (int i = 0; i < n; i++)
+        nv += nh[i];
+    int *vnh = (int *) malloc(sizeof(int) * nv);
+    double *v = (double *) malloc(sizeof(double) * 6 * (size_t) nv);
+    double *vpow = (double *) calloc((size_t) nv * ACCEL_MAXHARM, sizeof(double));
+    accel_rderivs *vder = (accel_rderivs *) calloc((size_t) nv * ACCEL_MAXHARM,
+                                                   sizeof(accel_rderivs));
+    if (!vnh || !v || !vpow || !vder) {
+        free(vnh);
+        free(v);
+        free(vpow);
+        free(vder);
+        set_err("refinement: out of host memory");
+        return -1;
+    }
+    double *vr = v, *vz = v + nv, *vw = v + 2 * nv, *vro = v + 3 * nv, *vzo = v + 4 * nv,
+           *vwo = v + 5 * nv;
+    int j = 0;
+    for (int i = 0; i < n; i++) {
+        const double r = rin[i] + (double) g->lobin;
+        for (int k = 0; k < nh[i]; k++, j++) {
+            vnh[j] = 1;
+            vr[j] = r * (k + 1) - (double) g->lobin;
+            vz[j] = zin[i] * (k + 1);
+            vw[j] = win[i] * (k + 1);
+        }
+    }
+    int rc = refine_device(s->ws, s->d_spec, g->numbins, 1, nv, vnh, vr, vz,
+                           g->numw ? vw : NULL, vro, vzo, vwo, vpow, vder, s->stream);
+    if (rc == 0) {
+        j = 0;
+        for (int i = 0; i < n; i++) {
+            double *h = s->opt_h + (size_t) i * 3 * ACCEL_MAXHARM;
+            for (int k = 0; k < nh[i]; k++, j++) {
+                h[k] = vro[j] + (double) g->lobin;
+                h[ACCEL_MAXHARM + k] = vzo[j];
+                h[2 * ACCEL_MAXHARM + k] = g->numw ? vwo[j] : 0.0;
+                s->opt_pows[(size_t) i * ACCEL_MAXHARM + k] = vpow[(size_t) j * ACCEL_MAXHARM];
+                s->opt_derivs[(size_t) i * ACCEL_MAXHARM + k] = vder[(size_t) j * ACCEL_MAXHARM];
+            }
+            s->opt_r[i] = h[0];
+            s->opt_z[i] = h[ACCEL_MAXHARM];
+            s->opt_w[i] = h[2 * ACCEL_MAXHARM];
+        }
+    }
+    free(vnh);
+    free(v);
+    free(vpow);
+    free(vder);
+    return rc;
 }
 
 int accel_search_optimize(accel_search *s)
@@ -1542,12 +1608,24 @@ int accel_search_optimize(accel_search *s)
      * max_rz_arr_harmonics */
     if (!s->ws)
         s->ws = (refine_ws *) calloc(1, sizeof(refine_ws));
-    int rc = refine_device(s->ws, s->d_spec, g->numbins, n, nh, rin, zin, g->numw ? win : NULL,
-                           s->opt_r, s->opt_z, s->opt_w, s->opt_pows, s->opt_derivs, s->stream);
+    s->opt_h = (double *) calloc((size_t) n * 3 * ACCEL_MAXHARM, sizeof(double));
+    int rc = s->p.noharmpolish ? refine_unpolished(s, n, nh, rin, zin, win)
+                               : refine_device(s->ws, s->d_spec, g->numbins, 0, n, nh, rin, zin,
+                                               g->numw ? win : NULL, s->opt_r, s->opt_z,
+                                               s->opt_w, s->opt_pows, s->opt_derivs, s->stream);
     if (rc == 0) {
         for (i = 0; i < n; i++) {
             s->opt_nh[i] = nh[i];
+            if (s->p.noharmpolish)
+                continue;
+            /* src/accel_utils.c:577-581 */
             s->opt_r[i] += (double) g->lobin;
+            double *h = s->opt_h + (size_t) i * 3 * ACCEL_MAXHARM;
+            for (int k = 0; k < nh[i]; k++) {
+                h[k] = s->opt_r[i] * (k + 1);
+                h[ACCEL_MAXHARM + k] = s->opt_z[i] * (k + 1);
+                h[2 * ACCEL_MAXHARM + k] = g->numw ? s->opt_w[i] * (k + 1) : 0.0;
+            }
         }
         s->nopt = n;
     } else {
@@ -1560,7 +1638,7 @@ int accel_search_optimize(accel_search *s)
     return rc;
 }
 
-int accel_search_refine_dev(accel_search *s, int n, const int *nh, const double *rin,
+int accel_search_refine_dev(accel_search *s, int raw, int n, const int *nh, const double *rin,
                             const double *zin, const double *win, double *rout, double *zout,
                             double *wout, double *pows, accel_rderivs *derivs)
 {
@@ -1570,8 +1648,8 @@ int accel_search_refine_dev(accel_search *s, int n, const int *nh, const double 
         set_err("refinement: out of host memory");
         return -1;
     }
-    return refine_device(s->ws, s->d_spec, s->g.numbins, n, nh, rin, zin, win, rout, zout, wout,
-                         pows, derivs, s->stream);
+    return refine_device(s->ws, s->d_spec, s->g.numbins, raw, n, nh, rin, zin, win, rout, zout,
+                         wout, pows, derivs, s->stream);
 }
 
 void accel_search_dev_view(const accel_search *s, accel_dev_view *v)
@@ -1626,13 +1704,13 @@ int accel_search_cand_harmonics(const accel_search *s, int i, double *hirs, doub
     int nh = s->opt_nh[i];
     int m = nh < nmax ? nh : nmax;
     for (int k = 0; k < m; k++) {
-        /* src/accel_utils.c:577-581 */
+        const double *h = s->opt_h + (size_t) i * 3 * ACCEL_MAXHARM;
         if (hirs)
-            hirs[k] = s->opt_r[i] * (k + 1);
+            hirs[k] = h[k];
         if (hizs)
-            hizs[k] = s->opt_z[i] * (k + 1);
+            hizs[k] = h[ACCEL_MAXHARM + k];
         if (hiws)
-            hiws[k] = s->g.numw ? s->opt_w[i] * (k + 1) : 0.0;
+            hiws[k] = h[2 * ACCEL_MAXHARM + k];
         if (pows)
             pows[k] = s->opt_pows[(size_t) i * ACCEL_MAXHARM + k];
         if (derivs)
@@ -1680,7 +1758,7 @@ int accel_max_rz_harmonics(const accel_fcomplex *spec, long long numdata, int n,
     }
     refine_ws w;
     memset(&w, 0, sizeof(w));
-    rc = refine_device(&w, d_spec, numdata, n, numharm, rin, zin, NULL, rout, zout, NULL, pows,
+    rc = refine_device(&w, d_spec, numdata, 0, n, numharm, rin, zin, NULL, rout, zout, NULL, pows,
                        derivs, stream);
     ws_free(&w);
 done:
@@ -1710,7 +1788,7 @@ int accel_max_rzw_harmonics(const accel_fcomplex *spec, long long numdata, int n
     }
     refine_ws w;
     memset(&w, 0, sizeof(w));
-    rc = refine_device(&w, d_spec, numdata, n, numharm, rin, zin, win, rout, zout, wout, pows,
+    rc = refine_device(&w, d_spec, numdata, 0, n, numharm, rin, zin, win, rout, zout, wout, pows,
                        derivs, stream);
     ws_free(&w);
 done:

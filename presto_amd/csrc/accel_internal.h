@@ -31,8 +31,9 @@ ACCEL_HIDDEN void accel_search_dev_view(const accel_search *s, accel_dev_view *v
 ACCEL_HIDDEN int accel_search_plane_only(accel_search *s);
 /* Harmonic-polished refinement of n candidates on the handle's resident
  * spectrum (rin relative to lobin; win NULL for rz); pows and derivs are
- * [n][ACCEL_MAXHARM]. */
-ACCEL_HIDDEN int accel_search_refine_dev(accel_search *s, int n, const int *nh,
+ * [n][ACCEL_MAXHARM].  raw: single harmonics without polishing
+ * (max_rz_arr / max_rzw_arr; every nh must be 1). */
+ACCEL_HIDDEN int accel_search_refine_dev(accel_search *s, int raw, int n, const int *nh,
                                          const double *rin, const double *zin,
                                          const double *win, double *rout, double *zout,
                                          double *wout, double *pows, accel_rderivs *derivs);

@@ -699,11 +699,7 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
         sf_fail(1, "the -locpow normalisation is not built in this library");
         return;
     }
-    if (cmd->noharmpolishP) {
-        sf_fail(1, "-noharmpolish is not built in this library");
-        return;
-    }
-    obs->use_harmonic_polishing = 1;
+    obs->use_harmonic_polishing = !cmd->noharmpolishP;   /* src/accel_utils.c:1490-1493 */
     if (accel_readinf(idata, obs->rootfilenm)) {
         sf_fail(-1, "%s", accel_last_error());
         return;
@@ -820,6 +816,7 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
     p.dat_input = obs->dat_input;
     p.noharmremove = cmd->noharmremoveP;
     p.nph = obs->nph;
+    p.noharmpolish = !obs->use_harmonic_polishing;
     accel_geom g;
     if (accel_geom_create(&p, &g)) {
         char m[1100];
@@ -1654,7 +1651,8 @@ GSList *eliminate_harmonics(GSList *cands, int *numcands)
     return cands;
 }
 
-/* optimize_accelcand, src/accel_utils.c:552-626 (harmonic polishing) */
+/* optimize_accelcand, src/accel_utils.c:552-626: harmonic polishing, or
+ * (-noharmpolish) each harmonic refined on its own */
 void optimize_accelcand(accelcand *cand, accelobs *obs)
 {
     sf_state *st = state_of(obs);
@@ -1669,21 +1667,56 @@ void optimize_accelcand(accelcand *cand, accelobs *obs)
                 ACCEL_MAXHARM);
         return;
     }
-    double pows[ACCEL_MAXHARM];
-    rderivs der[ACCEL_MAXHARM];
-    double r = 0.0, z = 0.0, w = 0.0;
-    const double rin = cand->r - (double) obs->lobin, zin = cand->z, win = cand->w;
-    if (accel_search_refine_dev(st->s, 1, &nh, &rin, &zin, obs->numw ? &win : NULL, &r, &z, &w,
-                                pows, der)) {
-        sf_fail(-1, "optimize_accelcand: %s", accel_last_error());
-        return;
-    }
-    for (int i = 0; i < nh; i++) {
-        cand->pows[i] = pows[i];
-        cand->derivs[i] = der[i];
-        cand->hirs[i] = (r + obs->lobin) * (i + 1);
-        cand->hizs[i] = z * (i + 1);
-        cand->hiws[i] = obs->numw ? w * (i + 1) : 0.0;
+    if (!obs->use_harmonic_polishing) {
+        /* src/accel_utils.c:582-622: max_rz_arr / max_rzw_arr per harmonic */
+        int one[ACCEL_MAXHARM];
+        double v[6][ACCEL_MAXHARM];
+        double *vp = (double *) calloc((size_t) nh * ACCEL_MAXHARM, sizeof(double));
+        rderivs *vd = (rderivs *) calloc((size_t) nh * ACCEL_MAXHARM, sizeof(rderivs));
+        if (!vp || !vd) {
+            free(vp);
+            free(vd);
+            sf_fail(-1, "optimize_accelcand: out of host memory");
+            return;
+        }
+        for (int i = 0; i < nh; i++) {
+            one[i] = 1;
+            v[0][i] = cand->r * (i + 1) - (double) obs->lobin;
+            v[1][i] = cand->z * (i + 1);
+            v[2][i] = cand->w * (i + 1);
+        }
+        int rc = accel_search_refine_dev(st->s, 1, nh, one, v[0], v[1],
+                                         obs->numw ? v[2] : NULL, v[3], v[4], v[5], vp, vd);
+        for (int i = 0; i < nh && !rc; i++) {
+            cand->pows[i] = vp[i * ACCEL_MAXHARM];
+            cand->derivs[i] = vd[i * ACCEL_MAXHARM];
+            cand->hirs[i] = v[3][i] + obs->lobin;
+            cand->hizs[i] = v[4][i];
+            cand->hiws[i] = obs->numw ? v[5][i] : 0.0;
+        }
+        free(vp);
+        free(vd);
+        if (rc) {
+            sf_fail(-1, "optimize_accelcand: %s", accel_last_error());
+            return;
+        }
+    } else {
+        double pows[ACCEL_MAXHARM];
+        rderivs der[ACCEL_MAXHARM];
+        double r = 0.0, z = 0.0, w = 0.0;
+        const double rin = cand->r - (double) obs->lobin, zin = cand->z, win = cand->w;
+        if (accel_search_refine_dev(st->s, 0, 1, &nh, &rin, &zin, obs->numw ? &win : NULL, &r,
+                                    &z, &w, pows, der)) {
+            sf_fail(-1, "optimize_accelcand: %s", accel_last_error());
+            return;
+        }
+        for (int i = 0; i < nh; i++) {
+            cand->pows[i] = pows[i];
+            cand->derivs[i] = der[i];
+            cand->hirs[i] = (r + obs->lobin) * (i + 1);
+            cand->hizs[i] = z * (i + 1);
+            cand->hiws[i] = obs->numw ? w * (i + 1) : 0.0;
+        }
     }
     cand->sigma = (float) accel_candidate_sigma(cand->power, nh,
                                                 (double) obs->numindep[twon_to_index(nh)]);

@@ -4,7 +4,7 @@
  *
  *   accelsearch [-numharm n] [-zmax z] [-wmax w] [-sigma s] [-flo f] [-fhi f]
  *               [-rlo r] [-rhi r] [-lobin b] [-inmem] [-median] [-photon]
- *               [-zaplist file] [-baryv v] [-noharmremove] [-otheropt]
+ *               [-zaplist file] [-baryv v] [-noharmremove] [-noharmpolish] [-otheropt]
  *               [-ncpus n] [-device d] file.{dat,sdat,fft}
  *
  * Reads <root>.inf (N, dt, telescope) and the data file; for time series:
@@ -12,8 +12,7 @@
  * -photon), optional birdie zapping; then the search, the FP64 refinement
  * and the outputs <root>_ACCEL_<zmax>[_JERK_<wmax>] (text, .inf appended),
  * .cand and, for an .fft input, .txtcand.  The local-power normalisation
- * (-locpow) and the unpolished optimiser (-noharmpolish) are not built and
- * are refused.  -otheropt selects the reference's transposed-plane variants,
+ * (-locpow) is not built and is refused.  -otheropt selects the reference's transposed-plane variants,
  * which form the same sums: it changes nothing here.  -ncpus is accepted
  * and ignored (the work runs on the GPU).
  */
@@ -30,8 +29,8 @@ static void usage(void)
     fprintf(stderr,
             "usage: accelsearch [-numharm n] [-zmax z] [-wmax w] [-sigma s] [-flo Hz] [-fhi Hz]\n"
             "                   [-rlo bin] [-rhi bin] [-lobin bin] [-inmem] [-median] [-photon]\n"
-            "                   [-zaplist file] [-baryv v] [-noharmremove] [-otheropt]\n"
-            "                   [-ncpus n] [-device d] infile.{dat,sdat,fft}\n");
+            "                   [-zaplist file] [-baryv v] [-noharmremove] [-noharmpolish]\n"
+            "                   [-otheropt] [-ncpus n] [-device d] infile.{dat,sdat,fft}\n");
 }
 
 /* The option ranges of clig/accelsearch_cmd.cli:13-44, checked as the
@@ -184,7 +183,9 @@ int main(int argc, char **argv)
             check_int(a, atoi(NEXT()), 1, INT_MAX);
         else if (!strcmp(a, "-device"))
             device = atoi(NEXT());
-        else if (!strcmp(a, "-locpow") || !strcmp(a, "-noharmpolish")) {
+        else if (!strcmp(a, "-noharmpolish"))
+            p.noharmpolish = 1;
+        else if (!strcmp(a, "-locpow")) {
             fprintf(stderr, "accelsearch: option %s is not built in this implementation\n", a);
             return 1;
         } else if (a[0] == '-') {

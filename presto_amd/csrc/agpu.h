@@ -220,6 +220,12 @@ typedef struct {
     /* jerk refinement only: ncand x 26 x npts_cap complex FP64 scratch */
     void *scratch;
     int npts_cap;
+    /* 1: one harmonic per candidate without polishing (-noharmpolish):
+     * max_rz_arr (src/maximize_rz.c:103-136: the raw power, not over the
+     * local power) or max_rzw_arr (src/maximize_rzw.c:3-55: the 9 x 9 x 24
+     * grid of zin +- DZ, win +- DW, raw power, derivatives with the start's
+     * local power, powers = the grid maximum); numharm must be 1 */
+    int raw;
 } agpu_refine_args;
 
 /* max_rz_arr_harmonics per candidate */

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rz(agpu_refine_args a)
         const int n = k + 1;
         double lp = localpower_wg(spec, a.numdata, rin * n, zin * n, red);
         if (threadIdx.x == 0)
-            s_lp[k] = (float) lp;
+            s_lp[k] = a.raw ? 1.0f : (float) lp;
     }
     __syncthreads();
     rf_ctx c{spec, a.numdata, s_lp, H, 0, red, 0};
@@ -678,20 +678,25 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
     __syncthreads();
     long long hhlobin = 0;
     int bad = 0;
+    /* grid half-width in peak widths: 2 (harmonics), 1 (max_rzw_arr) */
+    const int numwid = a.raw ? 1 : 2, ng = 2 * numwid * 4 + 1;
+    float locpow0 = 0.0f;
     for (int hind = 0; hind < H && !bad; hind++) {
         const int n = H - hind;
         const double rh = rin * n, zh = zin * n, wh = win * n;
         const double hfrac = n / (double) H;
         const float locpow =
             (float) localpower3d_wg(spec, numdata, rh, zh, wh, red, fa, fb, a.npts_cap, &bad);
+        if (hind == 0)
+            locpow0 = locpow;
         double rh_int;
         modf(rh, &rh_int);
         const long long lobin = (long long) rh_int - 1;
         if (hind == 0)
             hhlobin = lobin;
-        double zlo = zh - 2 * hfrac * 2.0, zhi = zh + 2 * hfrac * 2.0;
-        double wlo = wh - 2 * hfrac * 20.0, whi = wh + 2 * hfrac * 20.0;
-        const double dz = (zhi - zlo) / (RZW_NZ - 1), dw = (whi - wlo) / (RZW_NZ - 1);
+        double zlo = zh - numwid * hfrac * 2.0, zhi = zh + numwid * hfrac * 2.0;
+        double wlo = wh - numwid * hfrac * 20.0, whi = wh + numwid * hfrac * 20.0;
+        const double dz = (zhi - zlo) / (ng - 1), dw = (whi - wlo) / (ng - 1);
         const double maxz = (fabs(zlo) < fabs(zhi)) ? zhi : zlo;
         const double maxw = (fabs(wlo) < fabs(whi)) ? whi : wlo;
         const int khw = wr_halfwidth(maxz, maxw, 0);
@@ -706,8 +711,8 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
         if (bad)
             break;
         int npts_cur = -1;   /* npts of the Q tables in scratch */
-        for (int wi = 0; wi < RZW_NZ && !bad; wi++) {
-            for (int zi = 0; zi < RZW_NZ; zi++) {
+        for (int wi = 0; wi < ng && !bad; wi++) {
+            for (int zi = 0; zi < ng; zi++) {
                 const double z = zlo + zi * dz, w = wlo + wi * dw;
                 if (fabs(w) < 1e-4) {
                     /* z-response template: direct sums with Fresnel taps,
@@ -783,13 +788,15 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
                     }
                     wg_sum_vec(part, vol, red4);
                 }
-                /* powsum += |vol|^2 / locpow (src/maximize_rzw.c:145-153);
-                 * the reference's volume is FP32 */
+                /* powsum += |vol|^2 / locpow (src/maximize_rzw.c:145-153),
+                 * the raw |vol|^2 for max_rzw_arr (:24-37); the reference's
+                 * volume is FP32 */
                 if (threadIdx.x < RZW_NR) {
                     const int kk = threadIdx.x;
                     const float vr = (float) vol[kk].x, vi = (float) vol[kk].y;
                     float *ps = &powsum[(wi * RZW_NZ + zi) * RZW_NR + kk];
-                    *ps += ((double) vr * vr + (double) vi * vi) / locpow;
+                    const double pw = (double) vr * vr + (double) vi * vi;
+                    *ps += a.raw ? pw : pw / locpow;
                 }
                 __syncthreads();
             }
@@ -803,7 +810,8 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
         }
         return;
     }
-    /* argmax in (w, z, r) order, first maximum wins (src/maximize_rzw.c:160-173) */
+    /* argmax in (w, z, r) order, first maximum wins (src/maximize_rzw.c:160-173);
+     * cells outside an ng x ng grid stay 0 and never win */
     float maxpow = 0.0f;
     int best = 0;
     for (int i = 0; i < RZW_NZ * RZW_NZ * RZW_NR; i++)
@@ -815,19 +823,20 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
               rix = best % RZW_NR;
     const double nh = H;
     const double rout = (hhlobin + rix * dr) / nh;
-    const double zout = ((zin * nh) - 2 * 2.0 + zind * 2.0 / 4.0) / nh;
-    const double wout = ((win * nh) - 2 * 20.0 + wind * 20.0 / 4.0) / nh;
+    const double zout = ((zin * nh) - numwid * 2.0 + zind * 2.0 / 4.0) / nh;
+    const double wout = ((win * nh) - numwid * 20.0 + wind * 20.0 / 4.0) / nh;
     for (int k = 0; k < H; k++) {
         const int hh = k + 1;
         const double rr = rout * hh, zz = zout * hh, ww = wout * hh;
-        const float lp =
-            (float) localpower3d_wg(spec, numdata, rr, zz, ww, red, fa, fb, a.npts_cap, &bad);
+        const float lp = a.raw ? locpow0
+                               : (float) localpower3d_wg(spec, numdata, rr, zz, ww, red, fa, fb,
+                                                         a.npts_cap, &bad);
         double d7[7];
         derivs3d_wg(spec, numdata, rr, zz, ww, lp, d7, red, fa, fb, a.npts_cap, &bad);
         if (bad)
             break;
         if (threadIdx.x == 0) {
-            a.powers[(size_t) ci * RF_MAXH + k] = d7[0];
+            a.powers[(size_t) ci * RF_MAXH + k] = a.raw ? (double) maxpow : d7[0];
             for (int j = 0; j < 7; j++)
                 a.derivs[((size_t) ci * RF_MAXH + k) * 7 + j] = d7[j];
         }

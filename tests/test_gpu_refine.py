@@ -308,3 +308,73 @@ def test_jerk_search_optimize_and_outputs(tmp_path):
         assert abs(float(f[10].split("(")[0]) - c["w"]) <= 20.0 / c["numharm"] + 1e-6
     head = open(acc).readline()
     assert "FFT 'w'" in head
+
+
+def test_unpolished_rz_optimize_matches_oracle():
+    """-noharmpolish (src/accel_utils.c:582-622): harmonic k of a candidate
+    is refined on its own from (k r - lobin, k z) by max_rz_arr, the raw
+    power as the objective; the endpoints, powers and derivatives of each
+    harmonic against the oracle's max_rz_arr from the same start."""
+    _say("case: -noharmpolish rz optimize, N=2^16 zmax 100 numharm 8")
+    dt = 64e-6
+    x = synth.make_series(1 << 16, dt, 57, [dict(f0=61.3, z=9.0, duty=0.05, amp=0.35),
+                                            dict(f0=377.7, z=-7.0, duty=0.08, amp=0.3)])
+    fft = A.spectrum_from_series(x, dt)
+    p = A.make_params(len(fft), len(x), dt, zmax=100, numharm=8, sigma=2.0, noharmpolish=True)
+    with A.Search(p) as s:
+        s.set_spectrum(fft)
+        s.run()
+        cands = s.candidates()
+        assert len(cands) >= 2
+        s.optimize()
+        worst = [0.0] * 4
+        for i in range(2):
+            c = cands[i]
+            hr, hz, hw, pw, der = s.harmonics(i)
+            assert len(hr) == c["numharm"]
+            for k in range(int(c["numharm"])):
+                opw, oro, ozo, ord_ = O.max_rz_arr(fft, c["r"] * (k + 1) - p.lobin,
+                                                   c["z"] * (k + 1))
+                lp = O.localpower3d(fft, hr[k] - p.lobin, hz[k], 0.0)
+                od = O.derivs3d(fft, hr[k] - p.lobin, hz[k], 0.0, lp)
+                e = [abs(hr[k] - (oro + p.lobin)), abs(hz[k] - ozo),
+                     abs(pw[k] - opw) / opw, abs(der[k].pow - od.pow) / max(od.pow, 1e-30)]
+                worst = [max(a, b) for a, b in zip(worst, e)]
+        _say(f"  worst |dr| {worst[0]:.2e} |dz| {worst[1]:.2e} power {worst[2]:.2e} "
+             f"derivs {worst[3]:.2e}")
+        assert worst[0] < 2e-3 and worst[1] < 8e-3 and worst[2] < 1e-3 and worst[3] < 1e-5
+        # not the polished multiples
+        hr, _, _, _, _ = s.harmonics(0)
+        assert len(hr) > 1 and not np.allclose(hr, hr[0] * np.arange(1, len(hr) + 1),
+                                              rtol=0, atol=1e-9)
+
+
+def test_unpolished_rzw_matches_oracle():
+    """-noharmpolish jerk refinement, max_rzw_arr (src/maximize_rzw.c:3-55):
+    the same grid point as the oracle, the raw grid maximum as the power
+    and the start's local power in the derivatives."""
+    _say("case: GPU unpolished jerk refinement vs oracle")
+    dt = 64e-6
+    x = synth.make_series(1 << 16, dt, 41, [dict(f0=211.9, z=8.0, w=40.0, duty=0.06, amp=0.5)])
+    fft = A.spectrum_from_series(x, dt)
+    p = A.make_params(len(fft), len(x), dt, zmax=40, wmax=60, numharm=2, sigma=2.0,
+                      force_standard=True, noharmpolish=True)
+    with A.Search(p) as s:
+        s.set_spectrum(fft)
+        s.run()
+        cands = s.candidates()
+        assert len(cands) >= 1
+        s.optimize()
+        for i in range(min(2, len(cands))):
+            c = cands[i]
+            hr, hz, hw, pw, der = s.harmonics(i)
+            for k in range(int(c["numharm"])):
+                opw, oro, ozo, owo, ord_ = O.max_rzw_arr(fft, c["r"] * (k + 1) - p.lobin,
+                                                         c["z"] * (k + 1), c["w"] * (k + 1))
+                _say(f"  cand {i} harmonic {k + 1}: gpu ({hr[k]:.4f}, {hz[k]:.3f}, "
+                     f"{hw[k]:.2f}) oracle ({oro + p.lobin:.4f}, {ozo:.3f}, {owo:.2f})")
+                assert abs(hr[k] - (oro + p.lobin)) < 1e-9
+                assert abs(hz[k] - ozo) < 1e-9 and abs(hw[k] - owo) < 1e-9
+                assert abs(pw[k] - opw) <= 2e-5 * opw
+                assert abs(der[k].locpow - ord_.locpow) <= 2e-5 * ord_.locpow
+                assert abs(der[k].pow - ord_.pow) <= 2e-5 * ord_.pow

@@ -105,16 +105,17 @@ def test_standard_run_fractional_rlo_equals_handle(tmp_path, frac):
     assert any(abs(c["r"] / T - 4995.37) < 0.05 for c in cands)
 
 
-@pytest.mark.parametrize("photon", [False, True])
-def test_fft_input_cli_equals_surface(tmp_path, photon):
+@pytest.mark.parametrize("photon,nopolish", [(False, False), (True, False), (False, True)])
+def test_fft_input_cli_equals_surface(tmp_path, photon, nopolish):
     """An .fft input: the accelsearch command (handle path) writes the
     same .txtcand work file (one line per candidate the merge adds or
     replaces, src/accel_utils.c:1443-1447, 1666-1684), ACCEL text and .cand
     bytes as the reference's block loop through the surface; with -photon
-    both normalise by bin 0's count (src/accel_utils.c:1599-1605)."""
+    both normalise by bin 0's count (src/accel_utils.c:1599-1605), with
+    -noharmpolish both refine each harmonic on its own (:582-622)."""
     import subprocess
     print(f"case: .fft input, CLI vs surface, 2^17 samples, zmax 20, numharm 4, "
-          f"photon {photon}", flush=True)
+          f"photon {photon}, noharmpolish {nopolish}", flush=True)
     N, dt = 1 << 17, 64e-6
     x = synth.make_series_f32(N, dt, 75, [dict(f0=123.456, z=6.0, duty=0.05, amp=0.3),
                                           dict(f0=377.7, z=-10.0, duty=0.08, amp=0.25)])
@@ -122,7 +123,8 @@ def test_fft_input_cli_equals_surface(tmp_path, photon):
         x = np.random.default_rng(75).poisson(np.maximum(2.0 + x, 0.0)).astype(np.float32)
     A.realfft_gpu(x).tofile(tmp_path / "ffti.fft")
     S.write_inf(str(tmp_path / "ffti"), N, dt)
-    cmd = S.make_cmdline(str(tmp_path / "ffti.fft"), zmax=20, numharm=4, photon=photon)
+    cmd = S.make_cmdline(str(tmp_path / "ffti.fft"), zmax=20, numharm=4, photon=photon,
+                         noharmpolish=nopolish)
     info, cands = S.drive(str(tmp_path / "ffti.fft"), cmd)
     assert len(cands) >= 2
     names = ("ffti_ACCEL_20", "ffti_ACCEL_20.cand", "ffti_ACCEL_20.txtcand")
@@ -133,7 +135,8 @@ def test_fft_input_cli_equals_surface(tmp_path, photon):
         os.remove(tmp_path / n)
     cli = os.path.join(os.path.dirname(A.__file__), "accelsearch")
     r = subprocess.run([cli, "-zmax", "20", "-numharm", "4"] + (["-photon"] if photon else []) +
-                       ["ffti.fft"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+                       (["-noharmpolish"] if nopolish else []) + ["ffti.fft"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert ("Normalizing powers using" in r.stdout and "photons" in r.stdout) == photon
     for n in names:
@@ -245,15 +248,15 @@ def test_block_planes_tables_and_adds(tmp_path, wmax):
 
 
 def test_refusals_return_errors(tmp_path):
-    print("case: surface refusals (photon, bad suffix)", flush=True)
+    print("case: surface refusals (locpow, bad suffix)", flush=True)
     N, dt = 1 << 14, 64e-6
     x = synth.make_series_f32(N, dt, 75)
     root = _write_dat(tmp_path, "ref", x, dt)
     obs, idata = S.AccelObs(), S.InfoData()
     cmd = S.make_cmdline(root + ".dat")
-    cmd.photonP = 1
+    cmd.locpowP = 1
     S.lib.create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), 1)
-    with pytest.raises(A.AccelError, match="photon"):
+    with pytest.raises(A.AccelError, match="locpow"):
         S._check()
     cmd = S.make_cmdline(root + ".txt")
     S.lib.create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), 1)

@@ -93,6 +93,33 @@ def test_max_rzw_arr_harmonics_lands_on_nearest_grid_point(numharm, z, w):
     assert 0.97 <= sum(pw) / (numharm * THEO_POW) <= 1.01
 
 
+@pytest.mark.parametrize("z,w", [(0.0, 0.0), (10.0, 60.0), (-20.0, -100.0)])
+def test_max_rzw_arr_single_harmonic_grid(z, w):
+    """The unpolished jerk optimiser (src/maximize_rzw.c:3-55) searches the
+    9 x 9 x 24 grid r = (int) rin - 1 + k/8, zin +- 2, win +- 20 in steps of
+    1/2 and 5: a noise-free injection comes back within half a step of
+    the grid point nearest to it, with its full power, and the returned
+    power is the raw (not locally normalised) grid maximum."""
+    r = R_HARM + 0.37
+    ft = make_signal(r, z, 0.0, 0, w=w)
+    zin, win = z + 0.6, w + 7.0
+    pw, rmax, zmax, wmax, rd = O.max_rzw_arr(ft, r + 0.1, zin, win)
+    # on the grid
+    assert abs((rmax - (int(r + 0.1) - 1)) * 8 - round((rmax - (int(r + 0.1) - 1)) * 8)) < 1e-9
+    assert abs((zmax - zin) * 2 - round((zmax - zin) * 2)) < 1e-9 and abs(zmax - zin) <= 2.0
+    assert abs((wmax - win) / 5 - round((wmax - win) / 5)) < 1e-9 and abs(wmax - win) <= 20.0
+    assert abs(rmax - r) <= 0.5 / 8 + 1e-9
+    assert abs(zmax - z) <= 0.25 + 1e-9 + 0.1
+    assert abs(wmax - w) <= 2.5 + 1e-9 + 2.0
+    assert 0.97 <= pw / THEO_POW <= 1.01
+    # the derivatives use the local power at the start, not at the peak
+    lp0 = O.localpower3d(ft, r + 0.1, zin, win)
+    assert abs(rd.locpow - np.float32(lp0)) <= 1e-6 * lp0
+    # the same single-harmonic peak as the polished search's (one harmonic)
+    r1, z1, w1, _, _ = O.max_rzw_arr_harmonics(ft, r + 0.1, zin, win, 1)
+    assert abs(r1 - rmax) <= 1.0 / 8 + 1e-9 and abs(z1 - zmax) <= 0.5 + 1e-9
+
+
 def test_calc_props_of_a_pure_sinusoid():
     """Middleditch, Deich & Kulkarni (1993): a constant-amplitude signal
     over the whole run has purity 1 and centroid 1/2; its phase at the peak
