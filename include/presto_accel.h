@@ -367,6 +367,45 @@ long long accel_server_memory(accel_server *sv);
 /* Finishes every queued request, then stops the workers. */
 void accel_server_destroy(accel_server *sv);
 
+/* ---- DM trials from a filterbank: subband de-dispersion ---- */
+
+/* prepsubband's parameters for its topocentric path
+ * (clig/prepsubband_cmd.cli: -nsub, -lodm, -dmstep, -numdms, -downsamp,
+ * -numout, with -nobary and -noclip) and the filterbank's (.inf freq,
+ * chan_wid, dt; spectra_per_subint of the raw reader). */
+typedef struct {
+    int numchan;            /* channels, in ascending frequency */
+    double dt;              /* sample time (s) */
+    double lofreq;          /* centre of the lowest channel (MHz) */
+    double chanwidth;       /* channel width (MHz), > 0 */
+    int spectra_per_subint; /* spectra per raw block */
+    int nsub;               /* -nsub: divides numchan */
+    int downsamp;           /* -downsamp: divides spectra_per_subint */
+    int numdms;             /* -numdms */
+    double lodm, dmstep;    /* -lodm, -dmstep */
+    long long numout;       /* -numout: samples per DM trial */
+} accel_dedisp_params;
+
+/* The plan of src/prepsubband.c:255-306, 364-387: channel delays inside
+ * each subband (idispdt[numchan], samples of dt), subband offsets of each
+ * DM (offsets[numdms][nsub], samples of downsamp dt), blocksperread, and
+ * the samples the reference writes from nspec spectra before it pads
+ * (*nwritten).  Any output may be NULL.  -1 when the channels do not split
+ * into subbands, or when a delay exceeds the reference's working blocks
+ * (src/backend_common.c:785-789, src/prepsubband.c:937-943). */
+int accel_dedisp_plan(const accel_dedisp_params *p, long long nspec, int *idispdt, int *offsets,
+                      int *blocksperread, long long *nwritten);
+/* The DM trials prepsubband -nobary -noclip writes from a filterbank
+ * (no mask, no ignored channels): raw [nspec][numchan] FP32 and out
+ * [numdms][numout] both in device memory on `device`; the samples after
+ * the written ones hold the running mean of the first trial's
+ * (src/prepsubband.c:399-431, 667-718), returned in *padval. */
+int accel_dedisperse_dev(const accel_dedisp_params *p, const float *d_raw, long long nspec,
+                         float *d_out, double *padval, int device);
+/* The same from and to host memory; returns the written count or -1. */
+long long accel_dedisperse(const accel_dedisp_params *p, const float *raw, long long nspec,
+                           float *out, double *padval, int device);
+
 const char *accel_last_error(void);
 
 #ifdef __cplusplus

@@ -428,3 +428,39 @@ def max_rzw_arr_harmonics(data, r, z, w, numharm):
     lib.o_max_rzw_arr_harmonics(d, len(d), numharm, r, z, w, C.byref(ro), C.byref(zo),
                                 C.byref(wo), rds, pw)
     return ro.value, zo.value, wo.value, list(rds), pw
+
+
+# ---------------------------------------------------------------- de-dispersion
+_sig("o_delay_from_dm", _d, _d, _d)
+_sig("o_dedisp_plan", _d, _i, _d, _d, _d, _i, _i, _i, _i, _d, _d, C.POINTER(_i),
+     C.POINTER(_i), C.POINTER(_i))
+_sig("o_prepsubband", _ll, C.POINTER(_f), _ll, _i, _d, _d, _d, _i, _i, _i, _i, _d, _d, _ll,
+     C.POINTER(_f), C.POINTER(_d))
+
+
+def dedisp_plan(numchan, dt, lofreq, chanwidth, spectra_per_subint, nsub, downsamp, numdms,
+                lodm, dmstep):
+    """(avgdm, idispdt[numchan], offsets[numdms][nsub], blocksperread) as
+    src/prepsubband.c:255-306, 364-387 compute them."""
+    idd = (_i * numchan)()
+    off = (_i * (numdms * nsub))()
+    bpr = _i()
+    avg = lib.o_dedisp_plan(numchan, dt, lofreq, chanwidth, spectra_per_subint, nsub, downsamp,
+                            numdms, lodm, dmstep, idd, off, C.byref(bpr))
+    return avg, np.array(idd[:]), np.array(off[:]).reshape(numdms, nsub), bpr.value
+
+
+def prepsubband(raw, dt, lofreq, chanwidth, spectra_per_subint, nsub, downsamp, numdms, lodm,
+                dmstep, numout):
+    """prepsubband -nobary -noclip over raw[nspec][numchan] (float32,
+    ascending frequency): (out[numdms][numout], points written, pad value)."""
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    nspec, numchan = raw.shape
+    out = np.zeros((numdms, numout), dtype=np.float32)
+    pad = _d()
+    n = lib.o_prepsubband(raw.ctypes.data_as(C.POINTER(_f)), nspec, numchan, dt, lofreq,
+                          chanwidth, spectra_per_subint, nsub, downsamp, numdms, lodm, dmstep,
+                          numout, out.ctypes.data_as(C.POINTER(_f)), C.byref(pad))
+    if n < 0:
+        raise ValueError("prepsubband: delays exceed the working blocks")
+    return out, n, pad.value

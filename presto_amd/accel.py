@@ -153,7 +153,7 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_server_submit", "accel_server_wait", "accel_server_cands",
             "accel_server_props", "accel_server_memory", "accel_server_destroy",
             "accel_server_timing",
-            "accel_last_error"]
+            "accel_last_error", "accel_dedisp_plan", "accel_dedisperse_dev", "accel_dedisperse"]
 
 
 class AccelError(RuntimeError):
@@ -563,6 +563,47 @@ def zap_birdies(fft, N, lobins, hibins):
                              hi.ctypes.data, len(lo)):
         raise AccelError("zapping failed (birdie outside the spectrum)")
     return out
+
+
+class DedispParams(C.Structure):
+    _fields_ = [("numchan", _i), ("dt", _d), ("lofreq", _d), ("chanwidth", _d),
+                ("spectra_per_subint", _i), ("nsub", _i), ("downsamp", _i), ("numdms", _i),
+                ("lodm", _d), ("dmstep", _d), ("numout", _ll)]
+
+
+_sig("accel_dedisp_plan", _i, _p, _ll, _p, _p, _p, _p)
+_sig("accel_dedisperse_dev", _i, _p, _p, _ll, _p, _p, _i)
+_sig("accel_dedisperse", _ll, _p, _p, _ll, _p, _p, _i)
+
+
+def dedisp_params(numchan, dt, lofreq, chanwidth, spectra_per_subint, nsub, downsamp, numdms,
+                  lodm, dmstep, numout):
+    return DedispParams(numchan, dt, lofreq, chanwidth, spectra_per_subint, nsub, downsamp,
+                        numdms, lodm, dmstep, numout)
+
+
+def dedisp_plan(p, nspec):
+    """(idispdt[numchan], offsets[numdms][nsub], blocksperread, written)."""
+    idd = np.zeros(p.numchan, dtype=np.int32)
+    off = np.zeros((p.numdms, p.nsub), dtype=np.int32)
+    bpr, nw = C.c_int(), C.c_longlong()
+    if lib.accel_dedisp_plan(C.byref(p), int(nspec), idd.ctypes.data, off.ctypes.data,
+                             C.byref(bpr), C.byref(nw)):
+        raise _err()
+    return idd, off, bpr.value, nw.value
+
+
+def dedisperse(p, raw, device=0):
+    """DM trials of raw[nspec][numchan] (float32): (out[numdms][numout],
+    written, pad value)."""
+    raw = np.ascontiguousarray(raw, dtype=np.float32)
+    out = np.zeros((p.numdms, p.numout), dtype=np.float32)
+    pad = C.c_double()
+    n = lib.accel_dedisperse(C.byref(p), raw.ctypes.data, raw.shape[0], out.ctypes.data,
+                             C.byref(pad), int(device))
+    if n < 0:
+        raise _err()
+    return out, n, pad.value
 
 
 def realfft_gpu(x, device=0):

@@ -17,9 +17,9 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-I", CSRC, "-I", os.path.join(ROOT, "include")]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall",
            "-I", CSRC, "-I", os.path.join(ROOT, "include")]
-HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip", "fft_gpu.hip", "surface_gpu.hip"]
+HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip", "fft_gpu.hip", "surface_gpu.hip", "dedisp_gpu.hip"]
 C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c", "accel_output.c", "accel_batch.c",
-         "accel_server.c", "accel_surface.c"]
+         "accel_server.c", "accel_surface.c", "accel_dedisp.c"]
 
 
 def run(cmd):

@@ -234,6 +234,18 @@ int agpu_refine_rz(const agpu_refine_args *args, void *stream);
  * exceeds npts_cap gets rout = NaN, nevals = -1 */
 int agpu_refine_rzw(const agpu_refine_args *args, void *stream);
 
+/* Subband de-dispersion (dedisp_gpu.hip).  raw: [nspec][numchan] FP32,
+ * ascending frequency; ds: [nsub][nds]; out: [numdms][numout].  maxdelay /
+ * maxoff bound idispdt / offsets, which are >= 0. */
+int agpu_dedisp_subbands(const float *d_raw, long long nspec, int numchan, const int *d_idispdt,
+                         int maxdelay, int nsub, int downsamp, long long nds, float *d_ds,
+                         void *stream);
+int agpu_dedisp_dms(const float *d_ds, long long nds, int nsub, const int *d_offsets,
+                    int maxoff, int numdms, long long nwrite, long long numout, float *d_out,
+                    void *stream);
+int agpu_dedisp_fill(float *d_out, int numdms, long long numout, long long from, float value,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,3 +121,23 @@ def config5_requests(n, seed_arrivals=5, seed0=5_000_000, rate=40.0):
             ps = [dict(f0=float(r.uniform(20, 500)), z=float(r.uniform(-0.5, 0.5) * zmax),
                        w=float(r.uniform(-0.5, 0.5) * wmax), duty=0.05, amp=0.05)]
         yield i, float(t[i]), N, 64e-6, zmax, wmax, nh, seed0 + i, ps
+
+
+def filterbank(nspec, numchan, dt, lofreq, chanwidth, seed, pulsars=()):
+    """float32 spectra [nspec][numchan] in ascending frequency: N(0,1)
+    noise from PCG64(seed) plus dispersed pulsars (dicts with f0, dm, duty,
+    amp): in channel c the pulse train arrives dm / (0.000241 f_c^2) s
+    later than at the top channel (src/dispersion.c:34-43)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = rng.standard_normal((nspec, numchan), dtype=np.float32)
+    t = np.arange(nspec, dtype=np.float64) * dt
+    ftop = lofreq + (numchan - 1) * chanwidth
+    for p in pulsars:
+        sig = p.get("duty", 0.05) / (2.0 * np.sqrt(2.0 * np.log(2.0)))
+        for c in range(numchan):
+            f = lofreq + c * chanwidth
+            delay = p["dm"] / (0.000241 * f * f) - p["dm"] / (0.000241 * ftop * ftop)
+            ph = p["f0"] * (t - delay)
+            d = ph - np.floor(ph + 0.5)
+            x[:, c] += (p["amp"] * np.exp(-0.5 * (d / sig) ** 2)).astype(np.float32)
+    return x
