@@ -216,6 +216,65 @@ def serving_section(nreq, rate, threads, budget_gib, workers):
             "service_p50_ms": float(np.percentile([svc[i] for i in ok], 50)) if ok else None}
 
 
+def dedisp_section(numdms=64):
+    """DM trials from a filterbank held in HBM: 128 channels (1200-1455
+    MHz), 2^21 + spectra at dt 64 us with a dispersed pulsar (DM 45),
+    prepsubband -nsub 16 -lodm 0 -dmstep 1.5 -numdms 64 -numout 2^21; both
+    kernels timed with HIP events; then every trial searched from device
+    memory (zmax 50, numharm 8) and the DM at which the pulsar is strongest."""
+    from presto_amd import accel as A, synth
+    numchan, dt, lofreq, cw, B = 128, 64e-6, 1200.0, 2.0, 8192
+    numout = 1 << 21
+    nspec = numout + 4 * B
+    nspec -= nspec % B
+    raw = synth.filterbank(nspec, numchan, dt, lofreq, cw, 61,
+                           [dict(f0=91.7, dm=45.0, duty=0.04, amp=0.05)])
+    p = A.dedisp_params(numchan, dt, lofreq, cw, B, 16, 1, numdms, 0.0, 1.5, numout)
+    idd, off, bpr, nw = A.dedisp_plan(p, nspec)
+    ms = []
+    with A.DeviceBuffer(raw.nbytes) as d_raw, \
+            A.DeviceBuffer(4 * numdms * numout) as d_out:
+        d_raw.upload(raw)
+        for k in range(4):
+            _, m = A.dedisperse_dev(p, d_raw.ptr, nspec, d_out.ptr)
+            if k:
+                ms.append(m)
+        sub_ms = float(np.mean([m[0] for m in ms]))
+        dm_ms = float(np.mean([m[1] for m in ms]))
+        nds = nw + int(off.max())
+        # algorithmic bytes: the spectra the subbands read once and the
+        # subbands written; the subbands read once and the trials written
+        sub_bytes = 4.0 * (nds * numchan + nds * 16)
+        dm_bytes = 4.0 * (nds * 16 + numdms * nw)
+        sp = A.make_params(numout // 2, numout, dt, zmax=50, numharm=8, sigma=2.0)
+        best = []
+        t0 = time.perf_counter()
+        with A.Search(sp) as s:
+            for d in range(numdms):
+                s.set_device_series(d_out.ptr + d * numout * 4, numout)
+                s.run()
+                c = s.candidates()
+                near = [x["sigma"] for x in c
+                        if abs(x["r"] / (numout * dt) - 91.7) < 0.02 * 91.7]
+                best.append(max(near) if near else 0.0)
+        search_s = time.perf_counter() - t0
+    del raw
+    return {"workload": "128-channel filterbank (1200-1455 MHz), %d spectra at dt 64 us with "
+                        "a pulsar at DM 45, prepsubband -nsub 16 -lodm 0 -dmstep 1.5 -numdms "
+                        "%d -numout 2^21; every trial searched from device memory (zmax 50, "
+                        "numharm 8)" % (nspec, numdms),
+            "written": nw, "blocksperread": bpr,
+            "kernel_ms": {"subbands": sub_ms, "dm_sums": dm_ms},
+            "roofline": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0,
+                         "subbands_achieved": sub_bytes / sub_ms / 1e6,
+                         "dm_sums_achieved": dm_bytes / dm_ms / 1e6,
+                         "subbands_frac": sub_bytes / sub_ms / 1e6 / 8000.0,
+                         "dm_sums_frac": dm_bytes / dm_ms / 1e6 / 8000.0},
+            "dm_trials_per_s_dedispersion": numdms / ((sub_ms + dm_ms) / 1e3),
+            "dm_trials_per_s_dedisp_and_search": numdms / (search_s + (sub_ms + dm_ms) / 1e3),
+            "best_dm": 1.5 * int(np.argmax(best)), "best_sigma": float(max(best))}
+
+
 def jerk_section(n):
     """The jerk workload: n spectra of 2^22 samples (dt 64 us), zmax 200,
     wmax 300, numharm 8, each with one binary-like signal (average z in
@@ -297,6 +356,8 @@ def main():
     ap.add_argument("--serve-budget-gib", type=int, default=190,
                     help="device memory for requests; allocations slow down sharply "
                          "above ~200 GB in use on a 288 GB MI355X")
+    ap.add_argument("--dedisp-dms", type=int, default=64,
+                    help="DM trials of the filterbank section (0 skips it)")
     ap.add_argument("--jerk", type=int, default=64,
                     help="spectra of the full-size jerk section (0: skip)")
     ap.add_argument("--cpu-threads", type=int,
@@ -418,6 +479,9 @@ def main():
     }
     if args.batch_trials > 0:
         line["batched"] = batched_section(args.batch_trials, args.cpu_threads)
+    if args.dedisp_dms > 0:
+        _stage("filterbank de-dispersion workload")
+        line["dedispersion"] = dedisp_section(args.dedisp_dms)
     if args.jerk > 0:
         line["jerk"] = jerk_section(args.jerk)
     if args.serve_requests > 0:

@@ -17,6 +17,8 @@
 #ifndef PRESTO_ACCEL_H
 #define PRESTO_ACCEL_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -112,6 +114,9 @@ int accel_search_set_spectrum(accel_search *s, const accel_fcomplex *fft);
  * .dat: realfft, deredden and bin 0 := 1+1i (src/accel_utils.c:1507-1562,
  * 1621-1624). */
 int accel_search_set_series(accel_search *s, const float *series, long long n);
+/* The same for a series already in device memory on the handle's device
+ * (a DM trial of accel_dedisperse_dev): no host round trip. */
+int accel_search_set_device_series(accel_search *s, const float *d_series, long long n);
 /* Copies the first n bins of the search's current spectrum to the host. */
 int accel_search_spectrum(const accel_search *s, accel_fcomplex *out, long long n);
 /* The median blocks of deredden (src/misc_utils.c:940-1046): starts and
@@ -399,12 +404,20 @@ int accel_dedisp_plan(const accel_dedisp_params *p, long long nspec, int *idispd
  * (no mask, no ignored channels): raw [nspec][numchan] FP32 and out
  * [numdms][numout] both in device memory on `device`; the samples after
  * the written ones hold the running mean of the first trial's
- * (src/prepsubband.c:399-431, 667-718), returned in *padval. */
+ * (src/prepsubband.c:399-431, 667-718), returned in *padval.  ms (or
+ * NULL): device time of the subband kernel and of the DM-sum kernel. */
 int accel_dedisperse_dev(const accel_dedisp_params *p, const float *d_raw, long long nspec,
-                         float *d_out, double *padval, int device);
+                         float *d_out, double *padval, float *ms, int device);
 /* The same from and to host memory; returns the written count or -1. */
 long long accel_dedisperse(const accel_dedisp_params *p, const float *raw, long long nspec,
                            float *out, double *padval, int device);
+/* Device buffers for the device-resident path (filterbank in, DM trials
+ * out, accel_search_set_device_series): no counterpart in the reference,
+ * whose stages exchange files. */
+void *accel_device_alloc(size_t bytes, int device);
+int accel_device_free(void *d);
+int accel_device_upload(void *d_dst, const void *src, size_t bytes);
+int accel_device_download(void *dst, const void *d_src, size_t bytes);
 
 const char *accel_last_error(void);
 

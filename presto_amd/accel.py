@@ -153,7 +153,9 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_server_submit", "accel_server_wait", "accel_server_cands",
             "accel_server_props", "accel_server_memory", "accel_server_destroy",
             "accel_server_timing",
-            "accel_last_error", "accel_dedisp_plan", "accel_dedisperse_dev", "accel_dedisperse"]
+            "accel_last_error", "accel_search_set_device_series", "accel_dedisp_plan",
+            "accel_dedisperse_dev", "accel_dedisperse", "accel_device_alloc", "accel_device_free",
+            "accel_device_upload", "accel_device_download"]
 
 
 class AccelError(RuntimeError):
@@ -239,6 +241,11 @@ class Search:
         if lib.accel_search_spectrum(self.h, out.ctypes.data, len(out)):
             raise _err()
         return out
+
+    def set_device_series(self, dptr, n):
+        """A time series already in device memory (e.g. one DM trial)."""
+        if lib.accel_search_set_device_series(self.h, C.c_void_p(int(dptr)), int(n)):
+            raise _err()
 
     def set_device_spectrum(self, dptr):
         if lib.accel_search_set_device_spectrum(self.h, C.c_void_p(int(dptr))):
@@ -572,7 +579,12 @@ class DedispParams(C.Structure):
 
 
 _sig("accel_dedisp_plan", _i, _p, _ll, _p, _p, _p, _p)
-_sig("accel_dedisperse_dev", _i, _p, _p, _ll, _p, _p, _i)
+_sig("accel_search_set_device_series", _i, _p, _p, _ll)
+_sig("accel_dedisperse_dev", _i, _p, _p, _ll, _p, _p, _p, _i)
+_sig("accel_device_alloc", _p, C.c_size_t, _i)
+_sig("accel_device_free", _i, _p)
+_sig("accel_device_upload", _i, _p, _p, C.c_size_t)
+_sig("accel_device_download", _i, _p, _p, C.c_size_t)
 _sig("accel_dedisperse", _ll, _p, _p, _ll, _p, _p, _i)
 
 
@@ -604,6 +616,50 @@ def dedisperse(p, raw, device=0):
     if n < 0:
         raise _err()
     return out, n, pad.value
+
+
+class DeviceBuffer:
+    """Device memory from the library (accel_device_alloc), freed on close."""
+
+    def __init__(self, nbytes, device=0):
+        self.ptr = lib.accel_device_alloc(int(nbytes), int(device))
+        if not self.ptr:
+            raise _err()
+        self.nbytes = int(nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes or lib.accel_device_upload(self.ptr, arr.ctypes.data,
+                                                               arr.nbytes):
+            raise _err()
+
+    def download(self, arr):
+        if arr.nbytes > self.nbytes or lib.accel_device_download(arr.ctypes.data, self.ptr,
+                                                                 arr.nbytes):
+            raise _err()
+        return arr
+
+    def close(self):
+        if self.ptr:
+            lib.accel_device_free(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def dedisperse_dev(p, d_raw, nspec, d_out, device=0):
+    """DM trials from a device filterbank into a device buffer: (pad value,
+    [subband kernel ms, DM-sum kernel ms])."""
+    pad = C.c_double()
+    ms = (C.c_float * 2)()
+    if lib.accel_dedisperse_dev(C.byref(p), C.c_void_p(int(d_raw)), int(nspec),
+                                C.c_void_p(int(d_out)), C.byref(pad), ms, int(device)):
+        raise _err()
+    return pad.value, [ms[0], ms[1]]
 
 
 def realfft_gpu(x, device=0):

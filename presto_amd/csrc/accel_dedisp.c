@@ -124,7 +124,7 @@ int accel_dedisp_plan(const accel_dedisp_params *p, long long nspec, int *idispd
 }
 
 int accel_dedisperse_dev(const accel_dedisp_params *p, const float *d_raw, long long nspec,
-                         float *d_out, double *padval, int device)
+                         float *d_out, double *padval, float *ms, int device)
 {
     int bpr;
     long long nw;
@@ -136,7 +136,8 @@ int accel_dedisperse_dev(const accel_dedisp_params *p, const float *d_raw, long 
     float *row0 = (float *) malloc(sizeof(float) * (size_t) (nw > 0 ? nw : 1));
     void *stream = NULL;
     int *d_idd = NULL, *d_off = NULL;
-    float *d_ds = NULL;
+    float *d_ds = NULL, *d_rawT = NULL;
+    void *ev[3] = {NULL, NULL, NULL};
     int rc = -1;
     if (!idd || !off || !row0) {
         accel_set_error("accel_dedisperse: out of host memory");
@@ -148,18 +149,38 @@ int accel_dedisperse_dev(const accel_dedisp_params *p, const float *d_raw, long 
         maxd = idd[c] > maxd ? idd[c] : maxd;
     for (int i = 0; i < nd * nsub; i++)
         maxo = off[i] > maxo ? off[i] : maxo;
+    int span = 0;
+    for (int d0 = 0; d0 < nd; d0 += AGPU_DD_GROUP)
+        for (int s = 0; s < nsub; s++) {
+            int lo = off[(size_t) d0 * nsub + s], hi = lo;
+            for (int d = d0 + 1; d < nd && d < d0 + AGPU_DD_GROUP; d++) {
+                const int v = off[(size_t) d * nsub + s];
+                lo = v < lo ? v : lo;
+                hi = v > hi ? v : hi;
+            }
+            span = hi - lo > span ? hi - lo : span;
+        }
     /* downsampled subband samples the written outputs read */
     const long long nds = nw > 0 ? nw + maxo : 0;
     if (agpu_init(device) || agpu_stream_create(&stream) ||
+        agpu_event_create(&ev[0]) || agpu_event_create(&ev[1]) || agpu_event_create(&ev[2]) ||
         agpu_malloc((void **) &d_idd, sizeof(int) * (size_t) numchan) ||
         agpu_malloc((void **) &d_off, sizeof(int) * (size_t) nd * nsub) ||
         agpu_malloc((void **) &d_ds, sizeof(float) * (size_t) nsub * (size_t) (nds > 0 ? nds : 1)) ||
+        agpu_malloc((void **) &d_rawT,
+                    sizeof(float) * (size_t) numchan *
+                        (size_t) (nds > 0 ? agpu_dedisp_ncols(nds, p->downsamp, maxd) : 1)) ||
         agpu_h2d(d_idd, idd, sizeof(int) * (size_t) numchan, stream) ||
         agpu_h2d(d_off, off, sizeof(int) * (size_t) nd * nsub, stream) ||
-        agpu_dedisp_subbands(d_raw, nspec, numchan, d_idd, maxd, nsub, p->downsamp, nds, d_ds,
-                             stream) ||
-        agpu_dedisp_dms(d_ds, nds, nsub, d_off, maxo, nd, nw, p->numout, d_out, stream) ||
-        (nw > 0 && agpu_d2h(row0, d_out, sizeof(float) * (size_t) nw, stream))) {
+        agpu_event_record(ev[0], stream) ||
+        agpu_dedisp_subbands(d_raw, nspec, numchan, d_idd, maxd, nsub, p->downsamp, nds, d_rawT,
+                             d_ds, stream) ||
+        agpu_event_record(ev[1], stream) ||
+        agpu_dedisp_dms(d_ds, nds, nsub, d_off, maxo, span, nd, nw, p->numout, d_out, stream) ||
+        agpu_event_record(ev[2], stream) ||
+        (nw > 0 && agpu_d2h(row0, d_out, sizeof(float) * (size_t) nw, stream)) ||
+        (ms && (agpu_event_elapsed(&ms[0], ev[0], ev[1]) ||
+                agpu_event_elapsed(&ms[1], ev[1], ev[2])))) {
         accel_set_error("accel_dedisperse: %s", agpu_last_error());
         goto done;
     }
@@ -182,7 +203,11 @@ int accel_dedisperse_dev(const accel_dedisp_params *p, const float *d_raw, long 
 done:
     if (stream)
         agpu_stream_sync(stream);
+    for (int i = 0; i < 3; i++)
+        if (ev[i])
+            agpu_event_destroy(ev[i]);
     agpu_free(d_ds);
+    agpu_free(d_rawT);
     agpu_free(d_off);
     agpu_free(d_idd);
     if (stream)
@@ -211,7 +236,7 @@ long long accel_dedisperse(const accel_dedisp_params *p, const float *raw, long 
         accel_set_error("accel_dedisperse: %s", agpu_last_error());
         goto done;
     }
-    if (accel_dedisperse_dev(p, d_raw, nspec, d_out, padval, device))
+    if (accel_dedisperse_dev(p, d_raw, nspec, d_out, padval, NULL, device))
         goto done;
     if (outb && agpu_d2h(out, d_out, outb, stream)) {
         accel_set_error("accel_dedisperse: %s", agpu_last_error());
@@ -224,4 +249,37 @@ done:
     if (stream)
         agpu_stream_destroy(stream);
     return rc;
+}
+
+void *accel_device_alloc(size_t bytes, int device)
+{
+    void *p = NULL;
+    if (agpu_init(device) || agpu_malloc(&p, bytes ? bytes : 4)) {
+        accel_set_error("accel_device_alloc: %s", agpu_last_error());
+        return NULL;
+    }
+    return p;
+}
+
+int accel_device_free(void *p)
+{
+    return agpu_free(p);
+}
+
+int accel_device_upload(void *d_dst, const void *src, size_t bytes)
+{
+    if (agpu_h2d(d_dst, src, bytes, NULL)) {
+        accel_set_error("accel_device_upload: %s", agpu_last_error());
+        return -1;
+    }
+    return 0;
+}
+
+int accel_device_download(void *dst, const void *d_src, size_t bytes)
+{
+    if (agpu_d2h(dst, d_src, bytes, NULL)) {
+        accel_set_error("accel_device_download: %s", agpu_last_error());
+        return -1;
+    }
+    return 0;
 }

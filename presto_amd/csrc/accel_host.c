@@ -1913,7 +1913,7 @@ done:
     return rc;
 }
 
-int accel_search_set_series(accel_search *s, const float *series, long long n)
+static int set_series(accel_search *s, const float *series, long long n, int on_device)
 {
     const accel_geom *g = &s->g;
     if (n < 4 || (n & (n - 1)) || n / 2 != g->numbins || g->lobin != 0) {
@@ -1962,13 +1962,26 @@ int accel_search_set_series(accel_search *s, const float *series, long long n)
     /* realfft (src/accel_utils.c:1547), deredden (:1561), bin 0 := 1+1i
      * (:1621-1624; left as de-reddened under -photon), all on the device */
     static const accel_fcomplex one = {1.0f, 1.0f};
-    GPU(agpu_h2d(s->d_spec, series, sizeof(float) * (size_t) n, s->stream));
+    if (on_device)
+        GPU(agpu_copy_d2d(s->d_spec, series, sizeof(float) * (size_t) n, s->stream));
+    else
+        GPU(agpu_h2d(s->d_spec, series, sizeof(float) * (size_t) n, s->stream));
     GPU(agpu_realfft((const float *) s->d_spec, n, s->d_spec, s->d_fftwork, s->stream));
     GPU(agpu_deredden(s->d_spec, g->numbins, s->d_dblk, s->ndblk, s->dmaxn, s->d_dmean,
                       s->stream));
     if (!(s->p.nph > 0.0))
         GPU(agpu_h2d(s->d_spec, &one, sizeof(one), s->stream));
     return 0;
+}
+
+int accel_search_set_series(accel_search *s, const float *series, long long n)
+{
+    return set_series(s, series, n, 0);
+}
+
+int accel_search_set_device_series(accel_search *s, const float *d_series, long long n)
+{
+    return set_series(s, d_series, n, 1);
 }
 
 int accel_search_spectrum(const accel_search *s, accel_fcomplex *out, long long n)

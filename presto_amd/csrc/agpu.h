@@ -237,12 +237,18 @@ int agpu_refine_rzw(const agpu_refine_args *args, void *stream);
 /* Subband de-dispersion (dedisp_gpu.hip).  raw: [nspec][numchan] FP32,
  * ascending frequency; ds: [nsub][nds]; out: [numdms][numout].  maxdelay /
  * maxoff bound idispdt / offsets, which are >= 0. */
+/* the spectra the subbands read: the length of rawT's rows */
+long long agpu_dedisp_ncols(long long nds, int downsamp, int maxdelay);
+/* transposes the spectra read into d_rawT [numchan][ncols], then sums */
 int agpu_dedisp_subbands(const float *d_raw, long long nspec, int numchan, const int *d_idispdt,
-                         int maxdelay, int nsub, int downsamp, long long nds, float *d_ds,
-                         void *stream);
+                         int maxdelay, int nsub, int downsamp, long long nds, float *d_rawT,
+                         float *d_ds, void *stream);
+/* span: the largest spread of one subband's offsets over AGPU_DD_GROUP
+ * consecutive DMs (groups from DM 0); it selects the LDS kernel */
+#define AGPU_DD_GROUP 8
 int agpu_dedisp_dms(const float *d_ds, long long nds, int nsub, const int *d_offsets,
-                    int maxoff, int numdms, long long nwrite, long long numout, float *d_out,
-                    void *stream);
+                    int maxoff, int span, int numdms, long long nwrite, long long numout,
+                    float *d_out, void *stream);
 int agpu_dedisp_fill(float *d_out, int numdms, long long numout, long long from, float value,
                      void *stream);
 

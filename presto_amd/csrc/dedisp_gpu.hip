@@ -33,9 +33,33 @@
         }                                                                         \
     } while (0)
 
-/* one thread per (v, s); lanes run along v, so the channel loop walks one
- * row segment of raw at a time and its cache lines serve the next channels */
-__global__ __launch_bounds__(DD_TPB) void k_dd_subbands(const float *raw, int numchan,
+/* raw [nspec][numchan] -> rawT [numchan][ncols] for the first ncols
+ * spectra, through 64 x 64 LDS tiles (both sides coalesced) */
+__global__ __launch_bounds__(DD_TPB) void k_dd_transpose(const float *raw, int numchan,
+                                                         long long ncols, float *rawT)
+{
+    __shared__ float tile[64][65];
+    const long long t0 = (long long) blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   /* 64 x 4 */
+    for (int r = ty; r < 64; r += 4) {
+        const long long t = t0 + r;
+        const int c = c0 + tx;
+        if (t < ncols && c < numchan)
+            tile[r][tx] = raw[t * numchan + c];
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int c = c0 + r;
+        const long long t = t0 + tx;
+        if (t < ncols && c < numchan)
+            rawT[(long long) c * ncols + t] = tile[tx][r];
+    }
+}
+
+/* one thread per (v, s) on the channel-major copy; lanes run along v, so
+ * every channel's read is one coalesced run */
+__global__ __launch_bounds__(DD_TPB) void k_dd_subbands(const float *rawT, long long ncols,
                                                         const int *idispdt, int cps, int nsub,
                                                         int downsamp, long long nds, float *ds)
 {
@@ -51,7 +75,7 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_subbands(const float *raw, int nu
         float sub = 0.0f;
         for (int j = 0; j < cps; j++) {
             const int c = c0 + j;
-            sub = sub + raw[(t + idispdt[c]) * numchan + c];
+            sub = sub + rawT[(long long) c * ncols + t + idispdt[c]];
         }
         if (downsamp == 1) {
             ds[(long long) s * nds + v] = sub;
@@ -88,6 +112,57 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_dms(const float *ds, long long nd
     out[(long long) d * numout + u] = acc;
 }
 
+/* The same sums through LDS: a workgroup takes 256 output samples of a
+ * group of AGPU_DD_GROUP consecutive DMs.  For each subband it loads the ds run
+ * that the group's offsets reach, [u0 + lo_s, u0 + 256 + hi_s), once
+ * (coalesced), and every DM of the group reads its shifted window from
+ * LDS: one global read per sample and subband instead of one per DM. */
+__global__ __launch_bounds__(DD_TPB) void k_dd_dms_lds(const float *ds, long long nds, int nsub,
+                                                       const int *offsets, int numdms,
+                                                       long long nwrite, long long numout,
+                                                       long long ntiles, int ngroups, int width,
+                                                       float *out)
+{
+#pragma clang fp contract(off)
+    extern __shared__ float lds[];   /* [nsub][width] */
+    __shared__ int lo[256];
+    const long long bid = blockIdx.x;
+    const int xcd = (int) (bid & 7);
+    const long long local = bid >> 3;
+    const int g = (int) (local % ngroups);
+    const long long tile = (local / ngroups) * 8 + xcd;
+    if (tile >= ntiles)
+        return;
+    const int d0 = g * AGPU_DD_GROUP, d1 = min(d0 + AGPU_DD_GROUP, numdms);
+    const long long u0 = tile * DD_TPB;
+    for (int si = threadIdx.x; si < nsub; si += DD_TPB) {
+        int m = offsets[(long long) d0 * nsub + si];
+        for (int d = d0 + 1; d < d1; d++)
+            m = min(m, offsets[(long long) d * nsub + si]);
+        lo[si] = m;
+    }
+    __syncthreads();
+    for (int si = 0; si < nsub; si++) {
+        const long long base = u0 + lo[si];
+        const float *src = ds + (long long) si * nds;
+        for (int i = threadIdx.x; i < width; i += DD_TPB) {
+            const long long k = base + i;
+            lds[si * width + i] = k < nds ? src[k] : 0.0f;
+        }
+    }
+    __syncthreads();
+    const long long u = u0 + threadIdx.x;
+    if (u >= nwrite)
+        return;
+    for (int d = d0; d < d1; d++) {
+        const int *off = offsets + (long long) d * nsub;
+        float acc = -0.0f;
+        for (int si = 0; si < nsub; si++)
+            acc = acc + lds[si * width + threadIdx.x + (off[si] - lo[si])];
+        out[(long long) d * numout + u] = acc;
+    }
+}
+
 __global__ __launch_bounds__(DD_TPB) void k_dd_fill(float *out, int numdms, long long numout,
                                                     long long from, float value)
 {
@@ -101,30 +176,39 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_fill(float *out, int numdms, long
 
 extern "C" {
 
+long long agpu_dedisp_ncols(long long nds, int downsamp, int maxdelay)
+{
+    return nds > 0 ? nds * downsamp + maxdelay : 0;
+}
+
 int agpu_dedisp_subbands(const float *d_raw, long long nspec, int numchan, const int *d_idispdt,
-                         int maxdelay, int nsub, int downsamp, long long nds, float *d_ds,
-                         void *stream)
+                         int maxdelay, int nsub, int downsamp, long long nds, float *d_rawT,
+                         float *d_ds, void *stream)
 {
     if (numchan <= 0 || nsub <= 0 || numchan % nsub || downsamp < 1 || nds < 0 || maxdelay < 0)
         return agpu_set_error("agpu_dedisp_subbands: bad geometry"), -1;
-    /* the last sample read: (nds D - 1) + maxdelay */
-    if (nds && nds * downsamp - 1 + maxdelay >= nspec)
+    /* the spectra read: [0, nds D + maxdelay) */
+    const long long ncols = agpu_dedisp_ncols(nds, downsamp, maxdelay);
+    if (ncols > nspec)
         return agpu_set_error("agpu_dedisp_subbands: reads past the filterbank"), -1;
     if (nds == 0)
         return 0;
-    const long long nb = (nds + DD_TPB - 1) / DD_TPB;
-    if (nb > 0x7fffffffLL || nsub > 65535)
+    const long long nb = (nds + DD_TPB - 1) / DD_TPB, nt = (ncols + 63) / 64;
+    if (nb > 0x7fffffffLL || nt > 0x7fffffffLL || nsub > 65535 || numchan > 65535 * 64)
         return agpu_set_error("agpu_dedisp_subbands: grid too large"), -1;
+    hipLaunchKernelGGL(k_dd_transpose, dim3((unsigned) nt, (unsigned) ((numchan + 63) / 64)),
+                       dim3(DD_TPB), 0, (hipStream_t) stream, d_raw, numchan, ncols, d_rawT);
+    DD_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_dd_subbands, dim3((unsigned) nb, (unsigned) nsub), dim3(DD_TPB), 0,
-                       (hipStream_t) stream, d_raw, numchan, d_idispdt, numchan / nsub, nsub,
+                       (hipStream_t) stream, d_rawT, ncols, d_idispdt, numchan / nsub, nsub,
                        downsamp, nds, d_ds);
     DD_CHECK(hipGetLastError());
     return 0;
 }
 
 int agpu_dedisp_dms(const float *d_ds, long long nds, int nsub, const int *d_offsets,
-                    int maxoff, int numdms, long long nwrite, long long numout, float *d_out,
-                    void *stream)
+                    int maxoff, int span, int numdms, long long nwrite, long long numout,
+                    float *d_out, void *stream)
 {
     if (nsub <= 0 || numdms <= 0 || nwrite < 0 || nwrite > numout || maxoff < 0)
         return agpu_set_error("agpu_dedisp_dms: bad geometry"), -1;
@@ -136,8 +220,22 @@ int agpu_dedisp_dms(const float *d_ds, long long nds, int nsub, const int *d_off
     const long long nblk = ((ntiles + 7) / 8) * 8 * numdms;
     if (nblk > 0x7fffffffLL)
         return agpu_set_error("agpu_dedisp_dms: grid too large"), -1;
-    hipLaunchKernelGGL(k_dd_dms, dim3((unsigned) nblk), dim3(DD_TPB), 0, (hipStream_t) stream,
-                       d_ds, nds, nsub, d_offsets, numdms, nwrite, numout, ntiles, d_out);
+    /* LDS path when every group's runs fit: width = 256 + the largest spread
+     * of one subband's offsets over AGPU_DD_GROUP consecutive DMs */
+    const int width = DD_TPB + span;
+    if (span >= 0 && span <= maxoff && nsub <= 256 &&
+        (long long) nsub * width * (long long) sizeof(float) <= 64 * 1024) {
+        const int ngroups = (numdms + AGPU_DD_GROUP - 1) / AGPU_DD_GROUP;
+        const long long nblk2 = ((ntiles + 7) / 8) * 8 * ngroups;
+        hipLaunchKernelGGL(k_dd_dms_lds, dim3((unsigned) nblk2), dim3(DD_TPB),
+                           (size_t) nsub * width * sizeof(float), (hipStream_t) stream, d_ds,
+                           nds, nsub, d_offsets, numdms, nwrite, numout, ntiles, ngroups, width,
+                           d_out);
+    } else {
+        hipLaunchKernelGGL(k_dd_dms, dim3((unsigned) nblk), dim3(DD_TPB), 0,
+                           (hipStream_t) stream, d_ds, nds, nsub, d_offsets, numdms, nwrite,
+                           numout, ntiles, d_out);
+    }
     DD_CHECK(hipGetLastError());
     return 0;
 }

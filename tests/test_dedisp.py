@@ -24,6 +24,9 @@ CASES = [
     (64, 256e-6, 1200.0, 4.0, 1024, 8, 2, 9, 20.0, 5.0),
     (128, 128e-6, 1300.0, 2.0, 2048, 32, 4, 7, 50.0, 2.5),
     (32, 512e-6, 400.0, 1.0, 512, 32, 1, 5, 10.0, 1.0),
+    # one channel per subband, offsets spread by thousands of samples per
+    # group of 8 DMs: the DM sums take the direct (non-LDS) kernel
+    (64, 256e-6, 300.0, 1.0, 1024, 64, 1, 10, 0.0, 20.0),
 ]
 
 
@@ -159,3 +162,32 @@ def test_dm_trials_feed_the_search():
     print("best sigma per DM:", ["%.1f" % b for b in best], flush=True)
     assert int(np.argmax(best)) == 6 and best[6] > 5.0
     assert best[0] < best[6]
+
+
+@pytest.mark.gpu
+def test_device_resident_trials_equal_host_path():
+    """The filterbank uploaded once, its DM trials written to device memory
+    and each searched from there (accel_search_set_device_series): the
+    trials and the candidates equal the host path's."""
+    print("case: device-resident DM trials into the search", flush=True)
+    numchan, dt, lofreq, cw, B = 64, 256e-6, 1200.0, 4.0, 2048
+    numout, nspec = 1 << 16, 36 * 2048
+    raw = _fb(numchan, dt, lofreq, cw, nspec, 12, [dict(f0=23.9, dm=40.0, duty=0.04,
+                                                         amp=0.08)])
+    p = A.dedisp_params(numchan, dt, lofreq, cw, B, 16, 1, 9, 0.0, 10.0, numout)
+    host, n, pad = A.dedisperse(p, raw)
+    sp = A.make_params(numout // 2, numout, dt, zmax=20, numharm=4, sigma=2.0)
+    with A.DeviceBuffer(raw.nbytes) as d_raw, A.DeviceBuffer(host.nbytes) as d_out:
+        d_raw.upload(raw)
+        dpad, ms = A.dedisperse_dev(p, d_raw.ptr, nspec, d_out.ptr)
+        assert dpad == pad and ms[0] > 0 and ms[1] > 0
+        dev = d_out.download(np.zeros_like(host))
+        assert np.array_equal(dev.view(np.uint32), host.view(np.uint32))
+        with A.Search(sp) as s:
+            for d in (0, 4, 8):
+                s.set_device_series(d_out.ptr + d * numout * 4, numout)
+                s.run()
+                c_dev = s.candidates().copy()
+                s.set_series(host[d])
+                s.run()
+                assert np.array_equal(c_dev, s.candidates())
