@@ -301,6 +301,22 @@ double o_block_norm(const o_obs *obs, const o_fcomplex *fft, long long lobin,
     return norm;
 }
 
+void o_corr_loc_pow(const float *powers, int n, float *loc)
+{
+    const int khw = 10 + 5;                       /* NUMLOCPOWAVG / 2 + DELTAAVGBINS */
+    const double scale = (2.0 / n) * (float) ((n / 2) / 20);
+    for (int i = khw; i < n - khw; i++) {
+        double s = 0.0;
+        for (int j = 5 + 1; j <= khw; j++)
+            s += (double) powers[i + j] + (double) powers[i - j];
+        loc[i] = (float) (scale * s);
+    }
+    for (int i = 0; i < khw; i++) {
+        loc[i] = loc[khw];
+        loc[n - 1 - i] = loc[n - 1 - khw];
+    }
+}
+
 /* src/accel_utils.c:1024-1198 */
 o_ffdot *o_subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double fullrhi,
                                o_shi *shi, const o_obs *obs, const o_fcomplex *fft,
@@ -352,6 +368,23 @@ o_ffdot *o_subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
     }
     double norm = obs->nph > 0.0 ? 1.0 / sqrt(obs->nph)
                                  : 1.0 / sqrt(o_median(powers, numdata) / log(2.0));
+    /* -locpow: per-bin factors invsqrtf(loc) applied in FP32 first, the
+     * scalar norm is then 1 (src/accel_utils.c:1100-1115) */
+    if (obs->nph <= 0.0 && obs->norm_type == 1) {
+        for (int ii = 0; ii < numdata; ii++) {
+            float pr = data[ii].r, pi = data[ii].i;
+            powers[ii] = pr * pr + pi * pi;
+        }
+        float *loc = (float *) malloc(sizeof(float) * numdata);
+        o_corr_loc_pow(powers, numdata, loc);
+        for (int ii = 0; ii < numdata; ii++) {
+            float nl = o_invsqrtf(loc[ii]);
+            data[ii].r *= nl;
+            data[ii].i *= nl;
+        }
+        free(loc);
+        norm = 1.0;
+    }
     free(powers);
 
     if (!fp64) {

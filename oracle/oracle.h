@@ -95,7 +95,16 @@ typedef struct {
      * (src/accel_utils.c:1080-1086); 0 selects block medians.  Set before
      * o_create_accelobs, which clears it when lobin > 0 (:1626-1628). */
     double nph;
+    /* -locpow: 1 normalises every bin of a window by invsqrtf of its
+     * double-tophat local power (src/accel_utils.c:1100-1115); 0 medians */
+    int norm_type;
 } o_obs;
+
+/* corr_loc_pow (src/corr_routines.c:309-366) for one window's FP32 powers:
+ * (2/n) floor((n/2)/20) times the sum of the powers 6..15 bins either
+ * side, the first and last 15 values set to their neighbours'.  The
+ * reference sums with an FP32 FFT; this sums exactly, then rounds. */
+void o_corr_loc_pow(const float *powers, int n, float *loc);
 
 int o_create_accelobs(o_obs *obs);
 

@@ -40,7 +40,7 @@ class OObs(C.Structure):
         ("numz", _i), ("numw", _i), ("numbetween", _i), ("inmem", _i),
         ("T", _d), ("rlo", _d), ("rhi", _d), ("dr", _d), ("zlo", _d), ("zhi", _d),
         ("dz", _d), ("wlo", _d), ("whi", _d), ("dw", _d),
-        ("powcut", _f * 6), ("numindep", _ll * 6), ("nph", _d),
+        ("powcut", _f * 6), ("numindep", _ll * 6), ("nph", _d), ("norm_type", _i),
     ]
 
 
@@ -175,7 +175,7 @@ def prepare_dat(series, dt):
 
 def make_obs(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0,
              fhi=10000.0, inmem=False, force_standard=False, lobin=0, dat_input=True,
-             rlo=None, rhi=None, nph=0.0):
+             rlo=None, rhi=None, nph=0.0, locpow=False):
     """Fill an o_obs the way accelsearch's command line does (flo/fhi default
     to 'given', clig/accelsearch_cmd.cli) and run o_create_accelobs."""
     o = OObs()
@@ -196,6 +196,7 @@ def make_obs(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0,
     o.lobin_arg = int(lobin)
     o.dat_input = int(dat_input)
     o.nph = float(nph)
+    o.norm_type = 1 if locpow else 0
     rc = lib.o_create_accelobs(C.byref(o))
     if rc:
         raise ValueError(f"o_create_accelobs failed ({rc})")
@@ -464,3 +465,13 @@ def prepsubband(raw, dt, lofreq, chanwidth, spectra_per_subint, nsub, downsamp, 
     if n < 0:
         raise ValueError("prepsubband: delays exceed the working blocks")
     return out, n, pad.value
+
+
+_sig("o_corr_loc_pow", None, C.POINTER(_f), _i, C.POINTER(_f))
+
+
+def corr_loc_pow(powers):
+    p = np.ascontiguousarray(powers, dtype=np.float32)
+    out = np.zeros_like(p)
+    lib.o_corr_loc_pow(p.ctypes.data_as(C.POINTER(_f)), len(p), out.ctypes.data_as(C.POINTER(_f)))
+    return out

@@ -242,6 +242,8 @@ typedef struct sf_state {
     long long rlen;
     double *d_norm;
     float *d_nsq;
+    void *d_wdata;            /* -locpow: the normalised window */
+    size_t wcap;
     agpu_sf_hit *d_hits;
     unsigned *d_count;
     unsigned cap;
@@ -281,6 +283,8 @@ static void state_release(sf_state *st)
         agpu_free(st->d_norm);
     if (st->d_nsq)
         agpu_free(st->d_nsq);
+    if (st->d_wdata)
+        agpu_free(st->d_wdata);
     if (st->d_hits)
         agpu_free(st->d_hits);
     if (st->d_count)
@@ -695,8 +699,8 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
         input_shorts = strcmp(suffix, "sdat") == 0;
         free(suffix);
     }
-    if (cmd->locpowP) {
-        sf_fail(1, "the -locpow normalisation is not built in this library");
+    if (cmd->locpowP && !cmd->photonP && cmd->wmaxP) {
+        sf_fail(1, "the -locpow normalisation is built for -wmax 0 searches only");
         return;
     }
     obs->use_harmonic_polishing = !cmd->noharmpolishP;   /* src/accel_utils.c:1490-1493 */
@@ -788,6 +792,10 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
          * and which then keeps it (src/accel_utils.c:1599-1605) */
         obs->nph = obs->dat_input ? 1.0 : (obs->numbins > 0 ? obs->fft[0].r : 0.0);
         printf("Normalizing powers using %.0f photons.\n\n", obs->nph);
+    } else if (cmd->locpowP) {   /* src/accel_utils.c:1611-1613 */
+        obs->nph = 0.0;
+        obs->norm_type = 1;
+        printf("Normalizing powers using local-power determination.\n\n");
     } else {
         obs->nph = 0.0;
         printf("Normalizing powers using median-blocks%s.\n\n", cmd->medianP ? "" : " (default)");
@@ -1185,6 +1193,30 @@ ffdotpows *subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
     f->pub.rinds = shi->rinds;
     f->pub.zinds = shi->zinds;
     int nrc;
+    if (st->obs->nph <= 0.0 && st->obs->norm_type == 1) {
+        /* -locpow (src/accel_utils.c:1100-1188): each bin has its own
+         * factor, so the plane is correlated from the normalised window */
+        const size_t wb = sizeof(fcomplex) * (size_t) numdata;
+        if (wb > st->wcap) {
+            if (st->d_wdata)
+                agpu_free(st->d_wdata);
+            st->d_wdata = NULL;
+            st->wcap = 0;
+            if (agpu_malloc(&st->d_wdata, wb)) {
+                sf_fail(-1, "subharm_fderivs_vol: %s", agpu_last_error());
+                return &f->pub;
+            }
+            st->wcap = wb;
+        }
+        if (numws != 1 ||
+            agpu_sf_locpow_norm(st->v.d_spec, st->v.numbins, st->v.lobin, lobin, numdata,
+                                st->d_wdata, st->v.stream) ||
+            agpu_sf_locpow_plane(st->d_wdata, numdata, binoffset, zlo, numzs, numrs, f->d_pow,
+                                 st->v.stream))
+            sf_fail(-1, "subharm_fderivs_vol: %s",
+                    numws != 1 ? "-locpow with w rows" : agpu_last_error());
+        return &f->pub;
+    }
     if (st->obs->nph > 0.0) {   /* -photon, src/accel_utils.c:1080-1086 */
         const double nv = 1.0 / sqrt(st->obs->nph);
         const float nsq = (float) (nv * nv);

@@ -12,7 +12,8 @@
  * -photon), optional birdie zapping; then the search, the FP64 refinement
  * and the outputs <root>_ACCEL_<zmax>[_JERK_<wmax>] (text, .inf appended),
  * .cand and, for an .fft input, .txtcand.  The local-power normalisation
- * (-locpow) is not built and is refused.  -otheropt selects the reference's transposed-plane variants,
+ * (-locpow) needs a correlation per window, which the accel.h surface
+ * (create_accelobs and the reference's driver) does; this command refuses it.  -otheropt selects the reference's transposed-plane variants,
  * which form the same sums: it changes nothing here.  -ncpus is accepted
  * and ignored (the work runs on the GPU).
  */
@@ -186,7 +187,8 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "-noharmpolish"))
             p.noharmpolish = 1;
         else if (!strcmp(a, "-locpow")) {
-            fprintf(stderr, "accelsearch: option %s is not built in this implementation\n", a);
+            fprintf(stderr, "accelsearch: option %s runs through the accel.h surface "
+                            "(create_accelobs), not this command\n", a);
             return 1;
         } else if (a[0] == '-') {
             fprintf(stderr, "accelsearch: unknown option %s\n", a);

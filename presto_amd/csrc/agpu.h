@@ -162,6 +162,14 @@ typedef struct {
 int agpu_sf_threshold(const float *d_pow, long long n, float powcut, agpu_sf_hit *d_hits,
                       unsigned *d_count, unsigned cap, void *stream);
 int agpu_copy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+/* -locpow: the window's numdata bins from lobin, each scaled by the
+ * inverse square root of its local power (surface_gpu.hip) */
+int agpu_sf_locpow_norm(const void *d_spec, long long numbins, long long lobin_file,
+                        long long lobin, int numdata, void *d_wdata, void *stream);
+/* -locpow block plane [numzs][numrs] from a window's locally normalised
+ * bins (numdata float2), z rows zlo + 2 i (surface_gpu.hip) */
+int agpu_sf_locpow_plane(const void *d_wdata, int numdata, int binoffset, int zlo, int numzs,
+                         int numrs, float *d_out, void *stream);
 
 /* Fused overlap-save correlation: raw |corr|^2 plane rows for templates
  * [0, nt) (row = template), half-bins [0, hb_end), segment length L = 4096,

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include "agpu.h"
+#include "dev_resp.h"
 
 #define SF_TPB 256
 
@@ -224,6 +225,113 @@ int agpu_sf_threshold(const float *d_pow, long long n, float powcut, agpu_sf_hit
         nb = 4096;
     hipLaunchKernelGGL(k_sf_threshold, dim3((unsigned) nb), dim3(SF_TPB), 0,
                        (hipStream_t) stream, d_pow, n, powcut, d_hits, d_count, cap);
+    SF_CHECK(hipGetLastError());
+    return 0;
+}
+
+/* -locpow window normalisation (src/accel_utils.c:1100-1115 with
+ * corr_loc_pow, src/corr_routines.c:309-370): the window's bins
+ * spec[lobin - lobin_file + i] (zero outside the spectrum), their FP32
+ * powers p, the local power loc(i) = (2/n) ((n/2)/20) sum_{j=6..15}
+ * (p[c+j] + p[c-j]) with c clamped to [15, n-16] (the reference's copies at
+ * both ends), accumulated in FP64, and the bin scaled by the
+ * 0x5f3759df inverse square root of loc with one Newton step
+ * (src/misc_utils.c:291-302). */
+__device__ static float2 sf_bin(const float2 *spec, long long numbins, long long b)
+{
+    return (b >= 0 && b < numbins) ? spec[b] : make_float2(0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(SF_TPB) void k_sf_locpow_norm(const float2 *spec, long long numbins,
+                                                           long long first, int n, float2 *wdata)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * SF_TPB + threadIdx.x;
+    if (i >= n)
+        return;
+    const int c = i < 15 ? 15 : (i > n - 16 ? n - 16 : i);
+    double s = 0.0;
+    for (int j = 6; j <= 15; j++) {
+        const float2 a = sf_bin(spec, numbins, first + c + j);
+        const float2 b = sf_bin(spec, numbins, first + c - j);
+        const float pa = a.x * a.x + a.y * a.y, pb = b.x * b.x + b.y * b.y;
+        s += (double) pa + (double) pb;
+    }
+    const double scale = (2.0 / n) * (float) ((n / 2) / 20);
+    const float loc = (float) (scale * s);
+    const float y = __int_as_float(0x5f3759df - (__float_as_int(loc) >> 1));
+    const float nl = y * (1.5f - 0.5f * loc * y * y);
+    const float2 d = sf_bin(spec, numbins, first + i);
+    wdata[i] = make_float2(d.x * nl, d.y * nl);
+}
+
+/* The block plane of one -locpow window (src/accel_utils.c:1100-1188):
+ * the window's numdata bins, each already scaled by invsqrtf of its local
+ * power, spread to half-bins and correlated with the z row's response
+ * (gen_z_response, numbetween 2, its own 2 numbetween z_resp_halfwidth(z)
+ * taps); cell kk is |sum|^2 at spread index kk + 2 binoffset.  The
+ * reference gets the same sums from FFTs; here they are direct, with FP64
+ * accumulation of the FP32 products.  One workgroup per (z row, 256 cells);
+ * the row's taps are made in LDS. */
+__global__ __launch_bounds__(SF_TPB) void k_sf_locpow_plane(const float2 *wdata, int numdata,
+                                                            int binoffset, int zlo, int numrs,
+                                                            float *out)
+{
+    extern __shared__ float2 taps[];
+    const int zi = blockIdx.y;
+    const double z = (double) zlo + 2.0 * zi;
+    const int nk = 2 * 2 * zr_halfwidth(z, 0);
+    for (int m = threadIdx.x; m < nk; m += SF_TPB)
+        taps[m] = resp_tap_nb(0.0, 2, z, nk, m);
+    __syncthreads();
+    const int kk = blockIdx.x * SF_TPB + threadIdx.x;
+    if (kk >= numrs)
+        return;
+    const int j = kk + 2 * binoffset;
+    double ar = 0.0, ai = 0.0;
+    /* taps m in [-nk/2, nk/2) with j + m even and inside the window */
+    int m0 = -nk / 2;
+    if ((j + m0) & 1)
+        m0++;
+    for (int m = m0; m < nk / 2; m += 2) {
+        const int i = (j + m) >> 1;
+        if (i < 0 || i >= numdata)
+            continue;
+        const float2 d = wdata[i], q = taps[nk / 2 + m];
+        ar += (double) d.x * q.x + (double) d.y * q.y;
+        ai += (double) d.y * q.x - (double) d.x * q.y;
+    }
+    const float fr = (float) ar, fi = (float) ai;
+    out[(long long) zi * numrs + kk] = fr * fr + fi * fi;
+}
+
+int agpu_sf_locpow_norm(const void *d_spec, long long numbins, long long lobin_file,
+                        long long lobin, int numdata, void *d_wdata, void *stream)
+{
+    if (numdata < 31)
+        return agpu_set_error("agpu_sf_locpow_norm: window shorter than 31 bins"), -1;
+    hipLaunchKernelGGL(k_sf_locpow_norm, dim3((unsigned) ((numdata + SF_TPB - 1) / SF_TPB)),
+                       dim3(SF_TPB), 0, (hipStream_t) stream, (const float2 *) d_spec, numbins,
+                       lobin - lobin_file, numdata, (float2 *) d_wdata);
+    SF_CHECK(hipGetLastError());
+    return 0;
+}
+
+int agpu_sf_locpow_plane(const void *d_wdata, int numdata, int binoffset, int zlo, int numzs,
+                         int numrs, float *d_out, void *stream)
+{
+    if (numdata <= 0 || numzs <= 0 || numrs <= 0 || binoffset < 0)
+        return agpu_set_error("agpu_sf_locpow_plane: bad geometry"), -1;
+    /* the widest row's taps */
+    const double zmax = fabs((double) zlo) > fabs((double) zlo + 2.0 * (numzs - 1))
+                            ? fabs((double) zlo) : fabs((double) zlo + 2.0 * (numzs - 1));
+    const int nkmax = 2 * 2 * ((int) (0.5 * 1.1 * zmax) + 16);
+    const size_t lds = sizeof(float2) * (size_t) nkmax;
+    if (lds > 64 * 1024)
+        return agpu_set_error("agpu_sf_locpow_plane: response too long for LDS"), -1;
+    dim3 grid((unsigned) ((numrs + SF_TPB - 1) / SF_TPB), (unsigned) numzs);
+    hipLaunchKernelGGL(k_sf_locpow_plane, grid, dim3(SF_TPB), lds, (hipStream_t) stream,
+                       (const float2 *) d_wdata, numdata, binoffset, zlo, numrs, d_out);
     SF_CHECK(hipGetLastError());
     return 0;
 }

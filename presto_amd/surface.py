@@ -191,7 +191,7 @@ def write_inf(root, N, dt, telescope="None (Artificial Data Set)"):
 
 def make_cmdline(infile, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0, fhi=10000.0,
                  inmem=False, noharmremove=False, lobin=0, rlo=None, rhi=None, photon=False,
-                 noharmpolish=False):
+                 noharmpolish=False, locpow=False):
     """Cmdline with the clig defaults of clig/accelsearch_cmd.cli:13-48.
     rlo / rhi clear floP / fhiP, which the command line itself never does
     (an embedding caller can)."""
@@ -211,6 +211,7 @@ def make_cmdline(infile, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0, fhi
         c.fhiP, c.rhiP, c.rhi = 0, 1, float(rhi)
     c.inmemP = 1 if inmem else 0
     c.photonP = 1 if photon else 0
+    c.locpowP = 1 if locpow else 0
     c.noharmpolishP = 1 if noharmpolish else 0
     c.noharmremoveP = 1 if noharmremove else 0
     c._argv = (_cp * 1)(infile.encode())

@@ -248,13 +248,12 @@ def test_block_planes_tables_and_adds(tmp_path, wmax):
 
 
 def test_refusals_return_errors(tmp_path):
-    print("case: surface refusals (locpow, bad suffix)", flush=True)
+    print("case: surface refusals (locpow with wmax, bad suffix)", flush=True)
     N, dt = 1 << 14, 64e-6
     x = synth.make_series_f32(N, dt, 75)
     root = _write_dat(tmp_path, "ref", x, dt)
     obs, idata = S.AccelObs(), S.InfoData()
-    cmd = S.make_cmdline(root + ".dat")
-    cmd.locpowP = 1
+    cmd = S.make_cmdline(root + ".dat", wmax=20, locpow=True)
     S.lib.create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), 1)
     with pytest.raises(A.AccelError, match="locpow"):
         S._check()
@@ -262,3 +261,51 @@ def test_refusals_return_errors(tmp_path):
     S.lib.create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), 1)
     with pytest.raises(A.AccelError, match="must be an"):
         S._check()
+
+
+@pytest.mark.parametrize("inmem", [True, False])
+def test_locpow_run_equals_oracle(tmp_path, inmem):
+    """-locpow: every block is normalised bin by bin by its running local
+    power (corr_loc_pow) and correlated on the GPU from the normalised
+    window; the candidate list of the reference's block loop through the
+    surface matches the oracle's.  The GPU sums in FP64 and the oracle
+    through FP32 FFTs, so powers agree to 1e-4 relative and only
+    candidates within 0.05 sigma of the threshold may differ."""
+    if inmem:
+        N, dt, kw = 1 << 17, 64e-6, dict(zmax=50, numharm=8)
+        pul = [dict(f0=123.456, z=12.0, duty=0.05, amp=0.3)]
+    else:
+        N, dt = 1 << 23, 64e-6
+        kw = dict(zmax=200, numharm=4, flo=4995.0, fhi=5005.0)
+        pul = [dict(f0=5001.3, z=20.0, duty=0.2, amp=0.12)]
+    print(f"case: surface -locpow run, inmem={inmem}, N={N}", flush=True)
+    sigma = 3.0
+    x = synth.make_series_f32(N, dt, 77, pul)
+    root = _write_dat(tmp_path, f"lp{int(inmem)}", x, dt)
+    cmd = S.make_cmdline(root + ".dat", sigma=sigma, locpow=True, **kw)
+    info, cands = S.drive(root + ".dat", cmd, optimize=False, outputs=False)
+    assert info["inmem"] == int(inmem)
+    with A.Search(A.make_params(N // 2, N, dt, **kw)) as s:
+        s.set_series(x)
+        spec = s.spectrum()
+    okw = dict(kw)
+    obs = O.make_obs(len(spec), N, dt, sigma=sigma, locpow=True, **okw)
+    _, oc, _ = O.accelsearch(obs, spec)
+    obs0 = O.make_obs(len(spec), N, dt, sigma=sigma, **okw)
+    _, oc0, _ = O.accelsearch(obs0, spec)
+    assert len(oc) > 0 and len(cands) > 0
+    ours = {(c["numharm"], round(c["r"], 6), round(c["z"], 6)): c for c in cands}
+    ref = {(int(c[2]), round(c[3], 6), round(c[4], 6)): c for c in oc}
+    for k, c in ours.items():
+        if k in ref:
+            assert abs(c["power"] - ref[k][0]) <= 1e-4 * ref[k][0]
+        else:
+            assert c["sigma"] < sigma + 0.05, k
+    for k, c in ref.items():
+        if k not in ours:
+            assert c[1] < sigma + 0.05, k
+    # the local-power normalisation changed the powers
+    ref0 = {(int(c[2]), round(c[3], 6), round(c[4], 6)): c[0] for c in oc0}
+    common = [k for k in ref if k in ref0]
+    assert any(abs(ref[k][0] - ref0[k]) > 1e-3 * ref0[k] for k in common) or \
+        set(ref) != set(ref0)

@@ -450,3 +450,67 @@ def test_deredden_whitens_noise():
     pw = np.abs(fft[1000:]) ** 2
     assert abs(np.mean(pw) - 1.0) < 0.05
     assert fft[0] == np.complex64(1 + 1j)
+
+
+def test_corr_loc_pow_constant_and_delta():
+    """corr_loc_pow (src/corr_routines.c:309-370): for constant powers P and
+    n a multiple of 40 the local power is P exactly (20 taps times the
+    normalisation 2/n (n/2)/20); a single unit spike at k reaches exactly
+    the bins 6 <= |i - k| <= 15 (the double top-hat with its 5-bin gap
+    around the centre bin), each with weight 2/n (n/2)/20; the 15 bins at
+    each end copy the nearest interior value."""
+    n = 400
+    loc = O.corr_loc_pow(np.full(n, 3.0, np.float32))
+    assert np.all(loc == np.float32(3.0))
+    n = 401
+    p = np.zeros(n, np.float32)
+    k = 200
+    p[k] = 1.0
+    loc = O.corr_loc_pow(p)
+    w = np.float32((2.0 / n) * np.float32((n // 2) // 20))
+    d = np.abs(np.arange(n) - k)
+    want = np.where((d >= 6) & (d <= 15), w, np.float32(0.0))
+    assert np.array_equal(loc, want)
+    # ends: a spike near the start shows in bin 15 and in every bin below
+    p = np.zeros(n, np.float32)
+    p[25] = 1.0
+    loc = O.corr_loc_pow(p)
+    assert loc[15] == w and np.all(loc[:15] == loc[15])
+    p = np.zeros(n, np.float32)
+    p[n - 26] = 1.0
+    loc = O.corr_loc_pow(p)
+    assert loc[n - 16] == w and np.all(loc[n - 15:] == loc[n - 16])
+
+
+def test_locpow_flat_spectrum_plane_ratio():
+    """-locpow on a flat-power spectrum (|X| = c in every bin): every bin's
+    local power is the same, L = c^2 (2/n) 20 floor(n/40) for the window's
+    n bins, so the locally normalised window is X times one constant, and
+    the fundamental's block plane is the median-normalised one
+    (X sqrt(ln 2) / c) times c^2 / (L ln 2), to the 0x5f3759df inverse
+    square root's accuracy (below 0.2 percent in amplitude), and that
+    ratio is the same in every cell.  A factor applied to the wrong bins,
+    a dropped tap or a wrong normalisation count fails one of the two."""
+    import ctypes as C
+    n = 1 << 14
+    rng = np.random.default_rng(5)
+    spec = (3.0 * np.exp(1j * rng.uniform(0, 2 * np.pi, n))).astype(np.complex64)
+    kw = dict(zmax=20, numharm=1, sigma=1.0, flo=1.0, fhi=400.0)
+    obs1 = O.make_obs(n, 2 * n, 1e-3, locpow=True, **kw)
+    obs0 = O.make_obs(n, 2 * n, 1e-3, **kw)
+    banks = O.SubharmBanks(obs0, fp64=True)
+    try:
+        shi = banks.shi(0, 1).value
+        kern = C.cast(C.c_void_p(shi + 32), C.POINTER(C.c_void_p))[0]
+        fftlen = C.cast(C.c_void_p(kern + 8), C.POINTER(C.c_int))[0]
+        nd = fftlen // 2
+        p1, _ = O.subharm_fderivs_vol(1, 1, 2000.0, 2400.0, banks, obs1, spec, fp64=True)
+        p0, _ = O.subharm_fderivs_vol(1, 1, 2000.0, 2400.0, banks, obs0, spec, fp64=True)
+    finally:
+        banks.close()
+    r = np.median(p1 / p0)
+    # the same factor in every cell (up to the FP32 rounding of |X| = 3)
+    assert np.max(np.abs(p1 - r * p0)) <= 1e-6 * p0.max()
+    L = 9.0 * (2.0 / nd) * 20 * ((nd // 2) // 20)
+    want = 9.0 / (L * np.log(2.0))
+    assert abs(r - want) <= 4e-3 * want
