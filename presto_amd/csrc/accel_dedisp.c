@@ -167,9 +167,10 @@ int accel_dedisperse_dev(const accel_dedisp_params *p, const float *d_raw, long 
         agpu_malloc((void **) &d_idd, sizeof(int) * (size_t) numchan) ||
         agpu_malloc((void **) &d_off, sizeof(int) * (size_t) nd * nsub) ||
         agpu_malloc((void **) &d_ds, sizeof(float) * (size_t) nsub * (size_t) (nds > 0 ? nds : 1)) ||
-        agpu_malloc((void **) &d_rawT,
-                    sizeof(float) * (size_t) numchan *
-                        (size_t) (nds > 0 ? agpu_dedisp_ncols(nds, p->downsamp, maxd) : 1)) ||
+        (agpu_dedisp_needs_transpose(numchan, nsub, p->downsamp, maxd) &&
+         agpu_malloc((void **) &d_rawT,
+                     sizeof(float) * (size_t) numchan *
+                         (size_t) (nds > 0 ? agpu_dedisp_ncols(nds, p->downsamp, maxd) : 1))) ||
         agpu_h2d(d_idd, idd, sizeof(int) * (size_t) numchan, stream) ||
         agpu_h2d(d_off, off, sizeof(int) * (size_t) nd * nsub, stream) ||
         agpu_event_record(ev[0], stream) ||

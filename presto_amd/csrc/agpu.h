@@ -247,13 +247,19 @@ int agpu_refine_rzw(const agpu_refine_args *args, void *stream);
  * maxoff bound idispdt / offsets, which are >= 0. */
 /* the spectra the subbands read: the length of rawT's rows */
 long long agpu_dedisp_ncols(long long nds, int downsamp, int maxdelay);
-/* transposes the spectra read into d_rawT [numchan][ncols], then sums */
+/* 1 when the subband sums need the channel-major copy d_rawT (a
+ * subband wider than 64 channels, or an LDS tile of 256 output samples
+ * plus maxdelay rows that does not fit); 0: they read d_raw through LDS
+ * tiles and d_rawT may be NULL */
+int agpu_dedisp_needs_transpose(int numchan, int nsub, int downsamp, int maxdelay);
+/* the subband sums; through d_rawT [numchan][ncols] when the transpose is
+ * needed */
 int agpu_dedisp_subbands(const float *d_raw, long long nspec, int numchan, const int *d_idispdt,
                          int maxdelay, int nsub, int downsamp, long long nds, float *d_rawT,
                          float *d_ds, void *stream);
 /* span: the largest spread of one subband's offsets over AGPU_DD_GROUP
  * consecutive DMs (groups from DM 0); it selects the LDS kernel */
-#define AGPU_DD_GROUP 8
+#define AGPU_DD_GROUP 16
 int agpu_dedisp_dms(const float *d_ds, long long nds, int nsub, const int *d_offsets,
                     int maxoff, int span, int numdms, long long nwrite, long long numout,
                     float *d_out, void *stream);

@@ -1,7 +1,9 @@
 /*
  * dedisp_gpu.hip -- subband de-dispersion of a filterbank held in HBM
  * (prepsubband's data path, src/prepsubband.c:918-1040 with
- * src/dispersion.c:164-224).  Two kernels:
+ * src/dispersion.c:164-224).  Two stages (k_dd_subbands_tile does the
+ * first straight from the filterbank through LDS when a tile fits, else
+ * k_dd_transpose + k_dd_subbands):
  *
  *   k_dd_subbands  sub(t, s) = sum over the channels c of subband s, in
  *                  ascending order, of raw(t + idispdt[c], c); then the
@@ -20,6 +22,7 @@
 #include "agpu.h"
 
 #define DD_TPB 256
+#define DD_UNROLL 8
 
 #define DD_CHECK(call)                                                            \
     do {                                                                          \
@@ -86,6 +89,90 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_subbands(const float *rawT, long 
     ds[(long long) s * nds + v] = 0.0f + ftmp / (float) downsamp;
 }
 
+/* The subband sums straight from the spectrum-major filterbank: a
+ * workgroup takes DD_TV output samples of spw consecutive subbands
+ * (gc = spw cps channels), loads the rows [v0 D, v0 D + DD_TV D + maxdelay)
+ * of those channels into LDS once (each row's gc channels are one
+ * contiguous run), and sums from LDS in the same order as k_dd_subbands.
+ * The row stride gc + 1 keeps the lanes of one channel on distinct banks. */
+#define DD_TV 256
+__global__ __launch_bounds__(DD_TPB) void k_dd_subbands_tile(const float *raw, long long nspec,
+                                                             int numchan, const int *idispdt,
+                                                             int cps, int spw, int nsub,
+                                                             int downsamp, int maxdelay,
+                                                             long long nds, float *ds)
+{
+#pragma clang fp contract(off)
+    extern __shared__ float tile[];   /* [rows][gc + 1] */
+    __shared__ int idd[64];
+    const int gc = spw * cps, stride = gc + 1;
+    const int s0 = blockIdx.y * spw, c0 = s0 * cps;
+    const long long v0 = (long long) blockIdx.x * DD_TV;
+    const long long t0 = v0 * downsamp;
+    const int rows = DD_TV * downsamp + maxdelay;
+    for (int j = threadIdx.x; j < gc; j += DD_TPB)
+        idd[j] = idispdt[c0 + j];
+    /* DD_UNROLL independent loads in flight per thread before their LDS
+     * stores */
+    const int n = rows * gc;
+    for (int i0 = threadIdx.x; i0 < n; i0 += DD_TPB * DD_UNROLL) {
+        float val[DD_UNROLL];
+#pragma unroll
+        for (int u = 0; u < DD_UNROLL; u++) {
+            const int i = i0 + u * DD_TPB;
+            const int r = i / gc, j = i - r * gc;
+            const long long t = t0 + r;
+            val[u] = (i < n && t < nspec) ? raw[t * numchan + c0 + j] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < DD_UNROLL; u++) {
+            const int i = i0 + u * DD_TPB;
+            const int r = i / gc, j = i - r * gc;
+            if (i < n)
+                tile[r * stride + j] = val[u];
+        }
+    }
+    __syncthreads();
+    const long long v = v0 + threadIdx.x;
+    if (v >= nds)
+        return;
+    const int vl = threadIdx.x;
+    for (int si = 0; si < spw; si++) {
+        const int jb = si * cps;
+        float ftmp = 0.0f;
+        for (int k = 0; k < downsamp; k++) {
+            const int r0 = vl * downsamp + k;
+            float sub = 0.0f;
+            for (int j = 0; j < cps; j++)
+                sub = sub + tile[(r0 + idd[jb + j]) * stride + jb + j];
+            if (downsamp == 1) {
+                ftmp = sub;
+                break;
+            }
+            ftmp = ftmp + sub;
+        }
+        ds[(long long) (s0 + si) * nds + v] = downsamp == 1 ? ftmp : 0.0f + ftmp / (float) downsamp;
+    }
+}
+
+/* subbands per workgroup for k_dd_subbands_tile, 0 when the tile does not
+ * fit (the transpose path then runs) */
+static int dd_tile_spw(int numchan, int nsub, int downsamp, int maxdelay)
+{
+    const int cps = numchan / nsub;
+    if (cps > 64 || downsamp > 16)
+        return 0;
+    for (int spw = nsub; spw >= 1; spw--) {
+        if (nsub % spw || spw * cps > 64)
+            continue;
+        const long long rows = (long long) DD_TV * downsamp + maxdelay;
+        if (rows * (spw * cps + 1) * (long long) sizeof(float) <= 64 * 1024 &&
+            (spw * cps >= 16 || spw == nsub))
+            return spw;
+    }
+    return 0;
+}
+
 /* one workgroup = 256 output samples of one DM; the DMs of a time tile
  * are consecutive workgroups of one XCD (ids are dealt round-robin to the
  * 8 XCDs), so the tile's ds rows are read from HBM once per XCD */
@@ -112,11 +199,13 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_dms(const float *ds, long long nd
     out[(long long) d * numout + u] = acc;
 }
 
-/* The same sums through LDS: a workgroup takes 256 output samples of a
- * group of AGPU_DD_GROUP consecutive DMs.  For each subband it loads the ds run
- * that the group's offsets reach, [u0 + lo_s, u0 + 256 + hi_s), once
- * (coalesced), and every DM of the group reads its shifted window from
- * LDS: one global read per sample and subband instead of one per DM. */
+/* The same sums through LDS: a workgroup takes DD_DMT output samples (two
+ * per thread) of a group of AGPU_DD_GROUP consecutive DMs.  For each
+ * subband it loads the ds run that the group's offsets reach,
+ * [u0 + lo_s, u0 + DD_DMT + hi_s), once (coalesced), and every DM of the
+ * group reads its shifted window from LDS: one global read per sample and
+ * subband instead of one per DM. */
+#define DD_DMT (2 * DD_TPB)
 __global__ __launch_bounds__(DD_TPB) void k_dd_dms_lds(const float *ds, long long nds, int nsub,
                                                        const int *offsets, int numdms,
                                                        long long nwrite, long long numout,
@@ -134,7 +223,7 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_dms_lds(const float *ds, long lon
     if (tile >= ntiles)
         return;
     const int d0 = g * AGPU_DD_GROUP, d1 = min(d0 + AGPU_DD_GROUP, numdms);
-    const long long u0 = tile * DD_TPB;
+    const long long u0 = tile * DD_DMT;
     for (int si = threadIdx.x; si < nsub; si += DD_TPB) {
         int m = offsets[(long long) d0 * nsub + si];
         for (int d = d0 + 1; d < d1; d++)
@@ -142,24 +231,40 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_dms_lds(const float *ds, long lon
         lo[si] = m;
     }
     __syncthreads();
-    for (int si = 0; si < nsub; si++) {
-        const long long base = u0 + lo[si];
-        const float *src = ds + (long long) si * nds;
-        for (int i = threadIdx.x; i < width; i += DD_TPB) {
-            const long long k = base + i;
-            lds[si * width + i] = k < nds ? src[k] : 0.0f;
+    /* DD_UNROLL independent loads in flight per thread before their LDS
+     * stores */
+    const int n = nsub * width;
+    for (int i0 = threadIdx.x; i0 < n; i0 += DD_TPB * DD_UNROLL) {
+        float val[DD_UNROLL];
+#pragma unroll
+        for (int u = 0; u < DD_UNROLL; u++) {
+            const int i = i0 + u * DD_TPB;
+            const int si = i / width;
+            const long long k = i < n ? u0 + lo[si] + (i - si * width) : nds;
+            val[u] = k < nds ? ds[(long long) si * nds + k] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < DD_UNROLL; u++) {
+            const int i = i0 + u * DD_TPB;
+            if (i < n)
+                lds[i] = val[u];
         }
     }
     __syncthreads();
-    const long long u = u0 + threadIdx.x;
-    if (u >= nwrite)
-        return;
+    const int x0 = threadIdx.x, x1 = threadIdx.x + DD_TPB;
+    const bool w0 = u0 + x0 < nwrite, w1 = u0 + x1 < nwrite;
     for (int d = d0; d < d1; d++) {
         const int *off = offsets + (long long) d * nsub;
-        float acc = -0.0f;
-        for (int si = 0; si < nsub; si++)
-            acc = acc + lds[si * width + threadIdx.x + (off[si] - lo[si])];
-        out[(long long) d * numout + u] = acc;
+        float a0 = -0.0f, a1 = -0.0f;
+        for (int si = 0; si < nsub; si++) {
+            const int b = si * width + (off[si] - lo[si]);
+            a0 = a0 + lds[b + x0];
+            a1 = a1 + lds[b + x1];
+        }
+        if (w0)
+            out[(long long) d * numout + u0 + x0] = a0;
+        if (w1)
+            out[(long long) d * numout + u0 + x1] = a1;
     }
 }
 
@@ -181,6 +286,11 @@ long long agpu_dedisp_ncols(long long nds, int downsamp, int maxdelay)
     return nds > 0 ? nds * downsamp + maxdelay : 0;
 }
 
+int agpu_dedisp_needs_transpose(int numchan, int nsub, int downsamp, int maxdelay)
+{
+    return dd_tile_spw(numchan, nsub, downsamp, maxdelay) == 0;
+}
+
 int agpu_dedisp_subbands(const float *d_raw, long long nspec, int numchan, const int *d_idispdt,
                          int maxdelay, int nsub, int downsamp, long long nds, float *d_rawT,
                          float *d_ds, void *stream)
@@ -193,6 +303,22 @@ int agpu_dedisp_subbands(const float *d_raw, long long nspec, int numchan, const
         return agpu_set_error("agpu_dedisp_subbands: reads past the filterbank"), -1;
     if (nds == 0)
         return 0;
+    const int spw = dd_tile_spw(numchan, nsub, downsamp, maxdelay);
+    if (spw) {
+        const long long nbt = (nds + DD_TV - 1) / DD_TV;
+        if (nbt > 0x7fffffffLL || nsub / spw > 65535)
+            return agpu_set_error("agpu_dedisp_subbands: grid too large"), -1;
+        const int cps = numchan / nsub;
+        const size_t lds = sizeof(float) * (size_t) (DD_TV * downsamp + maxdelay) *
+                           (size_t) (spw * cps + 1);
+        hipLaunchKernelGGL(k_dd_subbands_tile, dim3((unsigned) nbt, (unsigned) (nsub / spw)),
+                           dim3(DD_TPB), lds, (hipStream_t) stream, d_raw, nspec, numchan,
+                           d_idispdt, cps, spw, nsub, downsamp, maxdelay, nds, d_ds);
+        DD_CHECK(hipGetLastError());
+        return 0;
+    }
+    if (!d_rawT)
+        return agpu_set_error("agpu_dedisp_subbands: this geometry needs the transposed copy"), -1;
     const long long nb = (nds + DD_TPB - 1) / DD_TPB, nt = (ncols + 63) / 64;
     if (nb > 0x7fffffffLL || nt > 0x7fffffffLL || nsub > 65535 || numchan > 65535 * 64)
         return agpu_set_error("agpu_dedisp_subbands: grid too large"), -1;
@@ -220,16 +346,19 @@ int agpu_dedisp_dms(const float *d_ds, long long nds, int nsub, const int *d_off
     const long long nblk = ((ntiles + 7) / 8) * 8 * numdms;
     if (nblk > 0x7fffffffLL)
         return agpu_set_error("agpu_dedisp_dms: grid too large"), -1;
-    /* LDS path when every group's runs fit: width = 256 + the largest spread
-     * of one subband's offsets over AGPU_DD_GROUP consecutive DMs */
-    const int width = DD_TPB + span;
-    if (span >= 0 && span <= maxoff && nsub <= 256 &&
-        (long long) nsub * width * (long long) sizeof(float) <= 64 * 1024) {
+    /* LDS path when every group's runs fit: width = DD_DMT + the largest
+     * spread of one subband's offsets over AGPU_DD_GROUP consecutive DMs */
+    const int width = DD_DMT + span;
+    const long long ldsb = (long long) nsub * width * (long long) sizeof(float);
+    if (span >= 0 && span <= maxoff && nsub <= 256 && ldsb <= 64 * 1024) {
         const int ngroups = (numdms + AGPU_DD_GROUP - 1) / AGPU_DD_GROUP;
-        const long long nblk2 = ((ntiles + 7) / 8) * 8 * ngroups;
+        const long long ntl = (nwrite + DD_DMT - 1) / DD_DMT;
+        const long long nblk2 = ((ntl + 7) / 8) * 8 * ngroups;
+        if (nblk2 > 0x7fffffffLL)
+            return agpu_set_error("agpu_dedisp_dms: grid too large"), -1;
         hipLaunchKernelGGL(k_dd_dms_lds, dim3((unsigned) nblk2), dim3(DD_TPB),
-                           (size_t) nsub * width * sizeof(float), (hipStream_t) stream, d_ds,
-                           nds, nsub, d_offsets, numdms, nwrite, numout, ntiles, ngroups, width,
+                           (size_t) ldsb, (hipStream_t) stream, d_ds,
+                           nds, nsub, d_offsets, numdms, nwrite, numout, ntl, ngroups, width,
                            d_out);
     } else {
         hipLaunchKernelGGL(k_dd_dms, dim3((unsigned) nblk), dim3(DD_TPB), 0,

@@ -25,7 +25,7 @@ CASES = [
     (128, 128e-6, 1300.0, 2.0, 2048, 32, 4, 7, 50.0, 2.5),
     (32, 512e-6, 400.0, 1.0, 512, 32, 1, 5, 10.0, 1.0),
     # one channel per subband, offsets spread by thousands of samples per
-    # group of 8 DMs: the DM sums take the direct (non-LDS) kernel
+    # group of 16 DMs: the DM sums take the direct (non-LDS) kernel
     (64, 256e-6, 300.0, 1.0, 1024, 64, 1, 10, 0.0, 20.0),
 ]
 
