@@ -324,6 +324,11 @@ typedef struct {
     int ncands;                   /* out: candidates found */
     int status;                   /* out: 0 or -1 */
     float gpu_ms;                 /* out: device time of its search */
+    /* a time series already in device memory instead of fft (fft NULL):
+     * nseries floats, prepared on the device as accel_search_set_device_series
+     * does; it must stay valid until accel_batch_run returns */
+    const float *d_series;
+    long long nseries;
 } accel_batch_item;
 
 typedef struct accel_batch accel_batch;

@@ -353,7 +353,8 @@ class Search:
 class BatchItem(C.Structure):
     _fields_ = [("params", Params), ("fft", _p), ("optimize", _i), ("accelnm", C.c_char_p),
                 ("candnm", C.c_char_p), ("inf_path", C.c_char_p), ("cands", _p),
-                ("maxcands", _i), ("ncands", _i), ("status", _i), ("gpu_ms", _f)]
+                ("maxcands", _i), ("ncands", _i), ("status", _i), ("gpu_ms", _f),
+                ("d_series", _p), ("nseries", _ll)]
 
 
 _sig("accel_batch_create", _p, _i)
@@ -407,6 +408,31 @@ class Batch:
             it.accelnm = acc.encode() if acc else None
             it.candnm = cand.encode() if cand else None
             it.inf_path = None
+            it.cands = out.ctypes.data
+            it.maxcands = maxcands
+        rc = lib.accel_batch_run(self.h, items, n)
+        if rc:
+            bad = [i for i in range(n) if items[i].status]
+            raise AccelError(f"batch items {bad[:8]} failed: {lib.accel_last_error().decode()}")
+        return ([o[:min(it.ncands, maxcands)].copy() for o, it in zip(outs, items)],
+                [it.gpu_ms for it in items])
+
+    def run_device(self, params, series_ptrs, nseries, optimize=False, maxcands=4096):
+        """Pipelined searches of time series already in device memory (one
+        device pointer of nseries floats per job, e.g. the DM trials of
+        dedisperse_dev), all with the same params.  Returns the candidate
+        arrays and per-job device times, as run()."""
+        n = len(series_ptrs)
+        items = (BatchItem * n)()
+        outs = []
+        for it, ptr in zip(items, series_ptrs):
+            out = np.zeros(maxcands, dtype=CAND_DTYPE)
+            outs.append(out)
+            it.params = params
+            it.fft = None
+            it.d_series = int(ptr)
+            it.nseries = int(nseries)
+            it.optimize = int(optimize)
             it.cands = out.ctypes.data
             it.maxcands = maxcands
         rc = lib.accel_batch_run(self.h, items, n)

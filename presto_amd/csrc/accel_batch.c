@@ -99,13 +99,16 @@ int accel_batch_run(accel_batch *b, accel_batch_item *items, int n)
     for (int i = 0; i < n; i++) {
         accel_batch_item *it = &items[i];
         it->status = 0;
-        if (it->params.numbins <= 0 || !it->fft) {
+        if (it->params.numbins <= 0 || (!it->fft && !it->d_series)) {
             it->status = -1;
             rc = -1;
             continue;
         }
         pooled *q = get_handle(b, &it->params, previ);
-        if (!q || accel_search_set_spectrum(q->s, it->fft) || accel_search_launch(q->s)) {
+        if (!q ||
+            (it->fft ? accel_search_set_spectrum(q->s, it->fft)
+                     : accel_search_set_device_series(q->s, it->d_series, it->nseries)) ||
+            accel_search_launch(q->s)) {
             it->status = -1;
             rc = -1;
             continue;

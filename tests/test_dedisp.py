@@ -167,8 +167,9 @@ def test_dm_trials_feed_the_search():
 @pytest.mark.gpu
 def test_device_resident_trials_equal_host_path():
     """The filterbank uploaded once, its DM trials written to device memory
-    and each searched from there (accel_search_set_device_series): the
-    trials and the candidates equal the host path's."""
+    and each searched from there (accel_search_set_device_series, one by one
+    and as a pipelined batch): the trials and the candidates equal the host
+    path's."""
     print("case: device-resident DM trials into the search", flush=True)
     numchan, dt, lofreq, cw, B = 64, 256e-6, 1200.0, 4.0, 2048
     numout, nspec = 1 << 16, 36 * 2048
@@ -191,3 +192,15 @@ def test_device_resident_trials_equal_host_path():
                 s.set_series(host[d])
                 s.run()
                 assert np.array_equal(c_dev, s.candidates())
+        # every trial through the pipelined batch, straight from device memory
+        ref = []
+        with A.Search(sp) as s:
+            for d in range(9):
+                s.set_series(host[d])
+                s.run()
+                ref.append(s.candidates().copy())
+        with A.Batch() as b:
+            cands, gms = b.run_device(sp, [d_out.ptr + d * numout * 4 for d in range(9)], numout)
+        assert sum(len(c) for c in ref) > 0 and all(m > 0 for m in gms)
+        for d in range(9):
+            assert np.array_equal(cands[d], ref[d]), d
