@@ -115,6 +115,26 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_subbands_tile(const float *raw, l
     /* DD_UNROLL independent loads in flight per thread before their LDS
      * stores */
     const int n = rows * gc;
+    if (DD_TPB % gc == 0) {
+        /* whole rows per pass: each thread keeps one channel, no division */
+        const int rpp = DD_TPB / gc, j = threadIdx.x % gc;
+        const float *src = raw + c0 + j;
+        for (int r0 = threadIdx.x / gc; r0 < rows; r0 += rpp * DD_UNROLL) {
+            float val[DD_UNROLL];
+#pragma unroll
+            for (int u = 0; u < DD_UNROLL; u++) {
+                const int r = r0 + u * rpp;
+                const long long t = t0 + r;
+                val[u] = (r < rows && t < nspec) ? src[t * numchan] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < DD_UNROLL; u++) {
+                const int r = r0 + u * rpp;
+                if (r < rows)
+                    tile[r * stride + j] = val[u];
+            }
+        }
+    } else
     for (int i0 = threadIdx.x; i0 < n; i0 += DD_TPB * DD_UNROLL) {
         float val[DD_UNROLL];
 #pragma unroll

@@ -24,6 +24,9 @@ CASES = [
     (64, 256e-6, 1200.0, 4.0, 1024, 8, 2, 9, 20.0, 5.0),
     (128, 128e-6, 1300.0, 2.0, 2048, 32, 4, 7, 50.0, 2.5),
     (32, 512e-6, 400.0, 1.0, 512, 32, 1, 5, 10.0, 1.0),
+    # 3 channels per subband: LDS tiles 48 channels wide, staged element by
+    # element (48 does not divide the 256 threads)
+    (48, 256e-6, 1200.0, 4.0, 1024, 16, 1, 6, 0.0, 10.0),
     # one channel per subband, offsets spread by thousands of samples per
     # group of 16 DMs: the DM sums take the direct (non-LDS) kernel
     (64, 256e-6, 300.0, 1.0, 1024, 64, 1, 10, 0.0, 20.0),
