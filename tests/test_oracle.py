@@ -171,8 +171,10 @@ def test_extended_sigma_branch():
 
 # ---------------------------------------------------------------- geometry
 @pytest.mark.parametrize("n,dt,zmax,numharm,wmax,expect", [
-    # numbins, rlo, highestbin, numz, numw, maxkernlen,
-    # fftlen, corr_uselen, blocks, in-memory
+    # the reference's geometry for these configurations (src/accel_utils.c
+    # create_accelobs, src/accelsearch.c block loop): numbins, rlo,
+    # highestbin, numz, numw, maxkernlen, fftlen, corr_uselen, blocks,
+    # in-memory
     (2**20, 64e-6, 50, 8, None, (524288, 67, 524287, 51, 0, 172, 2048, 1876, 558, 1)),
     (2**23, 64e-6, 200, 16, None, (4194304, 536, 4194303, 201, 0, 504, 2048, 1544, 5432, 0)),
     (2**21, 256e-6, 200, 16, None, (1048576, 536, 1048575, 201, 0, 504, 2048, 1544, 1357, 1)),
