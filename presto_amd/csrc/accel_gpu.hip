@@ -1184,20 +1184,40 @@ int agpu_event_elapsed(float *ms, void *start, void *stop)
     return 0;
 }
 
+/* Records a HIP failure in g_err and jumps to the function's cleanup. */
+#define AGPU_TRY(call)                                                            \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            snprintf(g_err, sizeof(g_err), "%s:%d %s: %s", __FILE__, __LINE__,    \
+                     #call, hipGetErrorString(e_));                               \
+            goto cleanup;                                                         \
+        }                                                                         \
+    } while (0)
+
 int agpu_gen_templates(const int *zs, const int *ws, const int *numkern, int nt, int L,
                        void *d_T, void *stream)
 {
     hipStream_t st = (hipStream_t) stream;
     if (nt <= 0 || L <= 0 || nt > 65535)
         return fail("agpu_gen_templates: bad template count or length");
+    int rc = -1;
     int nkmax = 0, nw = 0, nptsmax = 0;
+    /* scratch comes from agpu_malloc: inside a request loop's arena no
+     * hipMalloc / hipFree (and no device-wide wait) happens here */
+    int *d_z = NULL, *d_w = NULL, *d_nk = NULL, *d_np = NULL, *d_tl = NULL;
+    double2 *d_resp = NULL;
     int *npts = (int *) malloc(sizeof(int) * nt);
     int *tlist = (int *) malloc(sizeof(int) * nt);
+    if (!npts || !tlist) {
+        snprintf(g_err, sizeof(g_err), "agpu_gen_templates: out of host memory");
+        goto cleanup;
+    }
     for (int t = 0; t < nt; t++) {
         if (numkern[t] <= 0 || numkern[t] % 4 || numkern[t] > L) {
-            free(npts);
-            free(tlist);
-            return fail("agpu_gen_templates: kernel width must be a positive multiple of 4 <= L");
+            snprintf(g_err, sizeof(g_err),
+                     "agpu_gen_templates: kernel width must be a positive multiple of 4 <= L");
+            goto cleanup;
         }
         if (numkern[t] > nkmax)
             nkmax = numkern[t];
@@ -1214,47 +1234,53 @@ int agpu_gen_templates(const int *zs, const int *ws, const int *numkern, int nt,
         }
     }
     if (nptsmax * (long long) sizeof(float) > 60 * 1024) {
-        free(npts);
-        free(tlist);
-        return fail("agpu_gen_templates: jerk chirp longer than 15360 samples");
+        snprintf(g_err, sizeof(g_err), "agpu_gen_templates: jerk chirp longer than 15360 samples");
+        goto cleanup;
     }
-    int *d_z = NULL, *d_w = NULL, *d_nk = NULL, *d_np = NULL, *d_tl = NULL;
-    double2 *d_resp = NULL;
-    AGPU_CHECK(hipMalloc(&d_z, sizeof(int) * nt));
-    AGPU_CHECK(hipMalloc(&d_w, sizeof(int) * nt));
-    AGPU_CHECK(hipMalloc(&d_nk, sizeof(int) * nt));
-    AGPU_CHECK(hipMalloc(&d_np, sizeof(int) * nt));
-    AGPU_CHECK(hipMalloc(&d_tl, sizeof(int) * (nw > 0 ? nw : 1)));
-    AGPU_CHECK(hipMalloc(&d_resp, sizeof(double2) * (size_t) nt * nkmax));
-    AGPU_CHECK(hipMemcpyAsync(d_z, zs, sizeof(int) * nt, hipMemcpyHostToDevice, st));
-    AGPU_CHECK(hipMemcpyAsync(d_w, ws, sizeof(int) * nt, hipMemcpyHostToDevice, st));
-    AGPU_CHECK(hipMemcpyAsync(d_nk, numkern, sizeof(int) * nt, hipMemcpyHostToDevice, st));
-    AGPU_CHECK(hipMemcpyAsync(d_np, npts, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    if (agpu_malloc((void **) &d_z, sizeof(int) * nt) ||
+        agpu_malloc((void **) &d_w, sizeof(int) * nt) ||
+        agpu_malloc((void **) &d_nk, sizeof(int) * nt) ||
+        agpu_malloc((void **) &d_np, sizeof(int) * nt) ||
+        agpu_malloc((void **) &d_tl, sizeof(int) * (nw > 0 ? nw : 1)) ||
+        agpu_malloc((void **) &d_resp, sizeof(double2) * (size_t) nt * nkmax))
+        goto cleanup;
+    AGPU_TRY(hipMemcpyAsync(d_z, zs, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    AGPU_TRY(hipMemcpyAsync(d_w, ws, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    AGPU_TRY(hipMemcpyAsync(d_nk, numkern, sizeof(int) * nt, hipMemcpyHostToDevice, st));
+    AGPU_TRY(hipMemcpyAsync(d_np, npts, sizeof(int) * nt, hipMemcpyHostToDevice, st));
     if (nw)
-        AGPU_CHECK(hipMemcpyAsync(d_tl, tlist, sizeof(int) * nw, hipMemcpyHostToDevice, st));
-    /* z responses for every template; jerk templates then overwrite theirs */
-    dim3 g1((nkmax + 255) / 256, nt);
-    hipLaunchKernelGGL(k_resp_z, g1, dim3(256), 0, st, d_z, d_nk, nkmax, d_resp);
-    AGPU_CHECK(hipGetLastError());
-    if (nw) {
-        hipLaunchKernelGGL(k_resp_w, dim3(nw), dim3(256), sizeof(float) * nptsmax, st, d_z, d_w,
-                           d_nk, d_np, d_tl, nkmax, d_resp);
-        AGPU_CHECK(hipGetLastError());
+        AGPU_TRY(hipMemcpyAsync(d_tl, tlist, sizeof(int) * nw, hipMemcpyHostToDevice, st));
+    {
+        /* z responses for every template; jerk templates then overwrite theirs */
+        dim3 g1((nkmax + 255) / 256, nt);
+        hipLaunchKernelGGL(k_resp_z, g1, dim3(256), 0, st, d_z, d_nk, nkmax, d_resp);
+        AGPU_TRY(hipGetLastError());
+        if (nw) {
+            hipLaunchKernelGGL(k_resp_w, dim3(nw), dim3(256), sizeof(float) * nptsmax, st, d_z,
+                               d_w, d_nk, d_np, d_tl, nkmax, d_resp);
+            AGPU_TRY(hipGetLastError());
+        }
+        dim3 g2((L + 255) / 256, nt);
+        hipLaunchKernelGGL(k_template_dft, g2, dim3(256), 0, st, d_resp, d_nk, nkmax, L,
+                           (float2 *) d_T);
+        AGPU_TRY(hipGetLastError());
     }
-    dim3 g2((L + 255) / 256, nt);
-    hipLaunchKernelGGL(k_template_dft, g2, dim3(256), 0, st, d_resp, d_nk, nkmax, L,
-                       (float2 *) d_T);
-    AGPU_CHECK(hipGetLastError());
-    AGPU_CHECK(hipStreamSynchronize(st));
-    AGPU_CHECK(hipFree(d_z));
-    AGPU_CHECK(hipFree(d_w));
-    AGPU_CHECK(hipFree(d_nk));
-    AGPU_CHECK(hipFree(d_np));
-    AGPU_CHECK(hipFree(d_tl));
-    AGPU_CHECK(hipFree(d_resp));
+    rc = 0;
+cleanup:
+    /* the scratch may be returned only once the kernels that read it ended */
+    if (hipStreamSynchronize(st) != hipSuccess && rc == 0) {
+        snprintf(g_err, sizeof(g_err), "agpu_gen_templates: stream synchronisation failed");
+        rc = -1;
+    }
+    agpu_free(d_z);
+    agpu_free(d_w);
+    agpu_free(d_nk);
+    agpu_free(d_np);
+    agpu_free(d_tl);
+    agpu_free(d_resp);
     free(npts);
     free(tlist);
-    return 0;
+    return rc;
 }
 
 int agpu_block_norms(const void *d_spec, long long numbins, long long lobin_file,

@@ -937,6 +937,10 @@ int accel_search_launch(accel_search *s)
 {
     const accel_geom *g = &s->g;
     agpu_harmsum_args a;
+    if (!s->d_hits) {
+        set_err("search handle has no hit buffer (a regrowth failed); destroy it");
+        return -1;
+    }
     const double t0 = now_ms();
     GPU(agpu_memset(s->d_nhits, 0, sizeof(unsigned), s->stream));
     GPU(agpu_event_record(s->ev[0], s->stream));
@@ -1214,10 +1218,17 @@ int accel_search_collect(accel_search *s)
     double t0 = now_ms();
     GPU(agpu_d2h(&nh, s->d_nhits, sizeof(unsigned), s->stream));
     if (nh > s->cap) {
-        /* too many crossings for the buffer: grow it and sum again */
+        /* too many crossings for the buffer: grow it and sum again.  The
+         * handle never keeps a freed buffer: on a failed allocation it has
+         * no hit buffer (cap 0) and refuses to launch until rebuilt. */
         GPU(agpu_free(s->d_hits));
-        s->cap = nh + nh / 4 + 1024;
-        GPU(agpu_malloc((void **) &s->d_hits, sizeof(agpu_hit) * s->cap));
+        s->d_hits = NULL;
+        s->cap = 0;
+        const unsigned cap = nh + nh / 4 + 1024;
+        void *d_new = NULL;
+        GPU(agpu_malloc(&d_new, sizeof(agpu_hit) * cap));
+        s->d_hits = (agpu_hit *) d_new;
+        s->cap = cap;
         agpu_harmsum_args a;
         GPU(agpu_memset(s->d_nhits, 0, sizeof(unsigned), s->stream));
         fill_harmsum_args(s, &a);

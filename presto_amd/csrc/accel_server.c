@@ -171,6 +171,9 @@ static void *worker(void *arg)
         accel_search *s = sv->hd[hi].s;
         accel_params p = sv->hd[hi].p;
         const long long hid = sv->hd[hi].id;
+        /* accel_server_submit may move the request array once the lock is
+         * released: work on a copy taken under the lock */
+        srv_req local = *r;
         pthread_mutex_unlock(&sv->mu);
         /* the current device is per host thread */
         agpu_init(sv->device);
@@ -178,11 +181,16 @@ static void *worker(void *arg)
         if (!s)
             s = accel_search_create(&p, sv->device);
         double t1 = now_s();
-        srv_req local = *r;   /* the request array may move while unlocked */
         if (s)
             run_request(sv, &local, s);
         else
             snprintf(local.err, sizeof(local.err), "%s", accel_last_error());
+        /* a handle whose run failed may hold a half-rebuilt state: it is
+         * destroyed rather than cached */
+        if (s && local.status != 0) {
+            accel_search_destroy(s);
+            s = NULL;
+        }
         double t2 = now_s();
         pthread_mutex_lock(&sv->mu);
         /* the handle array may have moved too: find this handle again */

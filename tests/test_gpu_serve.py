@@ -64,3 +64,27 @@ def test_request_larger_than_the_budget_fails_cleanly():
         st, _, _ = sv.wait(t)
         assert st == -1
         assert sv.memory() == 0
+
+
+def test_request_array_growth_while_workers_run():
+    """More requests than the server's first request-array allocation (64),
+    submitted while workers are busy, so the array is reallocated under
+    running requests; each result equals the same request served alone."""
+    print("case: 150 small requests (24 distinct) through 4 workers", flush=True)
+    distinct = []
+    for i in range(24):
+        N, dt = (1 << 14) if i % 2 else (1 << 15), 1e-3
+        ps = [dict(f0=31.7 + 2.1 * i, z=float(i % 5) - 2.0, duty=0.08, amp=0.5)] if i % 4 else []
+        x = synth.make_series(N, dt, 7000 + i, ps)
+        p = A.make_params(N // 2, N, dt, zmax=(0, 10, 20)[i % 3], numharm=(2, 4)[i % 2], sigma=2.0)
+        distinct.append((p, x))
+    alone = _serve(distinct, 1, 16 << 30, together=False)
+    order = [k % len(distinct) for k in range(150)]
+    with A.Server(budget_bytes=16 << 30, workers=4) as sv:
+        tickets = [sv.submit(*distinct[k]) for k in order]
+        for k, t in zip(order, tickets):
+            st, _, _ = sv.wait(t)
+            assert st == 0
+            ca, pa = alone[k]
+            assert np.array_equal(sv.candidates(t), ca)
+            assert [bytes(v) for v in sv.props(t)] == [bytes(v) for v in pa]
