@@ -424,6 +424,105 @@ int accel_device_free(void *d);
 int accel_device_upload(void *d_dst, const void *src, size_t bytes);
 int accel_device_download(void *dst, const void *d_src, size_t bytes);
 
+/* ---- sifting candidates across DM trials ---- */
+
+/* The sifting of PRESTO's python/presto/sifting.py:14-1206 (driven as in
+ * examplescripts/ACCEL_sift.py) over ACCEL files, on the host.  A sift set
+ * holds candidates in lists: the good list, the duplicates and eight
+ * reject lists (numbered in the order of the reference's report). */
+typedef struct accel_sift accel_sift;
+enum {
+    ACCEL_SIFT_GOOD = 0,
+    ACCEL_SIFT_DUPLICATE,
+    ACCEL_SIFT_BAD_KNOWNBIRDS,
+    ACCEL_SIFT_BAD_LONGPERIOD,
+    ACCEL_SIFT_BAD_SHORTPERIOD,
+    ACCEL_SIFT_BAD_THRESHOLD,
+    ACCEL_SIFT_BAD_HARMPOWCUTOFF,
+    ACCEL_SIFT_BAD_ROGUEHARMPOW,
+    ACCEL_SIFT_BAD_HARMONIC,
+    ACCEL_SIFT_BAD_DMPROBLEM,
+    ACCEL_SIFT_NLISTS
+};
+
+/* The rules of accel_sift_reject (Candlist.reject_*,
+ * python/presto/sifting.py:504-647); a candidate failing several goes to
+ * the first in this order. */
+#define ACCEL_SIFT_R_LONGPERIOD 1u
+#define ACCEL_SIFT_R_SHORTPERIOD 2u
+#define ACCEL_SIFT_R_KNOWNBIRDS 4u
+#define ACCEL_SIFT_R_THRESHOLD 8u
+#define ACCEL_SIFT_R_HARMPOWCUTOFF 16u
+#define ACCEL_SIFT_R_ROGUEHARMPOW 32u
+#define ACCEL_SIFT_R_DEFAULT 0x3fu   /* Candlist.default_rejection */
+
+/* The sifting module's thresholds (python/presto/sifting.py:14-36). */
+typedef struct {
+    double r_err;             /* bins: one signal when |r1 - r2| < r_err (1.1) */
+    double long_period;       /* s (15.0) */
+    double short_period;      /* s (0.0005) */
+    double sigma_threshold;   /* (6.0) */
+    double c_pow_threshold;   /* coherent power (100.0) */
+    double harm_pow_cutoff;   /* (8.0) */
+    const double *birds_f;    /* known birdies: (Hz, half width Hz) pairs */
+    int nbirds_f;
+    const double *birds_p;    /* (ms, half width ms) pairs */
+    int nbirds_p;
+} accel_sift_params;
+
+/* One candidate (Candidate, python/presto/sifting.py:135-171). */
+typedef struct {
+    char path[512];           /* ACCEL file it came from */
+    char dmstr[32];           /* DM text of the file name ("30.00") */
+    char note[512];           /* why it was set aside */
+    int candnum, numharm, nhits, pad;
+    double sigma, ipow, cpow, r, z, T, snr;
+} accel_sift_cand;
+
+/* A sift set; removed candidates are kept in their lists only when
+ * track_bad / track_dupes are set. */
+accel_sift *accel_sift_create(int track_bad, int track_dupes);
+void accel_sift_destroy(accel_sift *s);
+/* Adds a candidate to the good list: harm_pows / harm_phases numharm
+ * values (or NULL), hits nhits (DM, S/N, sigma) triples. */
+int accel_sift_add(accel_sift *s, const accel_sift_cand *c, const double *harm_pows,
+                   const double *harm_phases, const double *hits, int nhits);
+/* candlist_from_candfile (python/presto/sifting.py:1075-1176): the
+ * candidates of one ACCEL file, re-scored from their refined harmonics
+ * (sigma for one trial, S/N, the file's DM as their one hit).  Returns the
+ * number read or -1. */
+int accel_sift_read(accel_sift *s, const char *accel_path);
+/* parse_power (python/presto/sifting.py:124-132) */
+double accel_sift_parse_power(const char *txt);
+/* Candlist.reject_* / default_rejection: moves the good candidates that
+ * fail one of `rules`; returns how many. */
+int accel_sift_reject(accel_sift *s, const accel_sift_params *p, unsigned rules);
+/* Candlist.remove_duplicate_candidates (python/presto/sifting.py:660-721):
+ * candidates within r_err of each other become hits of the one with the
+ * highest sigma; the list ends sorted by sigma.  Returns how many moved. */
+int accel_sift_remove_duplicates(accel_sift *s, const accel_sift_params *p);
+/* Candlist.remove_DM_problems (python/presto/sifting.py:829-919): fewer
+ * than min_num_dms hits, the strongest hit at DM <= low_dm_cutoff, or no
+ * two hits at neighbouring DMs of the list dms. */
+int accel_sift_remove_dm_problems(accel_sift *s, int min_num_dms, const double *dms, int ndms,
+                                  double low_dm_cutoff);
+/* Candlist.remove_harmonics (python/presto/sifting.py:723-827) */
+int accel_sift_remove_harmonics(accel_sift *s, const accel_sift_params *p);
+int accel_sift_sort_sigma(accel_sift *s);
+/* Candlist.extend: moves every candidate of `from` into s. */
+int accel_sift_extend(accel_sift *s, accel_sift *from);
+/* The number of candidates in a list, and one of them with its harmonic
+ * powers and phases (numharm values each, or NULL) and up to maxhits hits
+ * (triples); returns its hit count or -1. */
+int accel_sift_count(const accel_sift *s, int list);
+int accel_sift_get(const accel_sift *s, int list, int i, accel_sift_cand *c, double *harm_pows,
+                   double *harm_phases, double *hits, int maxhits);
+/* Candlist.to_file, write_cand_report and print_cand_summary
+ * (python/presto/sifting.py:921-1071); path NULL writes to stdout. */
+int accel_sift_write(const accel_sift *s, const char *path);
+int accel_sift_write_report(const accel_sift *s, const char *path);
+int accel_sift_write_summary(const accel_sift *s, const char *path);
+
 const char *accel_last_error(void);
 
 #ifdef __cplusplus

@@ -155,7 +155,12 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_server_timing",
             "accel_last_error", "accel_search_set_device_series", "accel_dedisp_plan",
             "accel_dedisperse_dev", "accel_dedisperse", "accel_device_alloc", "accel_device_free",
-            "accel_device_upload", "accel_device_download"]
+            "accel_device_upload", "accel_device_download",
+            "accel_sift_create", "accel_sift_destroy", "accel_sift_add", "accel_sift_read",
+            "accel_sift_parse_power", "accel_sift_reject", "accel_sift_remove_duplicates",
+            "accel_sift_remove_dm_problems", "accel_sift_remove_harmonics",
+            "accel_sift_sort_sigma", "accel_sift_extend", "accel_sift_count", "accel_sift_get",
+            "accel_sift_write", "accel_sift_write_report", "accel_sift_write_summary"]
 
 
 class AccelError(RuntimeError):

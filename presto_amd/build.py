@@ -19,7 +19,7 @@ C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall",
            "-I", CSRC, "-I", os.path.join(ROOT, "include")]
 HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip", "fft_gpu.hip", "surface_gpu.hip", "dedisp_gpu.hip"]
 C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c", "accel_output.c", "accel_batch.c",
-         "accel_server.c", "accel_surface.c", "accel_dedisp.c"]
+         "accel_server.c", "accel_surface.c", "accel_dedisp.c", "accel_sift.c"]
 
 
 def run(cmd):
@@ -51,6 +51,13 @@ def build(verbose=False):
     tmp = CLI + ".tmp"
     run(["gcc", "-o", tmp, cli_obj, "-L", HERE, "-lpresto_accel", "-Wl,-rpath,$ORIGIN", "-lm"])
     os.replace(tmp, CLI)
+    # accel_sift: the sifting flow over a directory's ACCEL files
+    obj = os.path.join(BUILD, "accel_sift_cli.o")
+    run(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, "accel_sift_cli.c"), "-o", obj])
+    out = os.path.join(HERE, "accel_sift")
+    run(["gcc", "-o", out + ".tmp", obj, "-L", HERE, "-lpresto_accel", "-Wl,-rpath,$ORIGIN",
+         "-lm"])
+    os.replace(out + ".tmp", out)
     # realfft, rednoise, zapbirds: one source, three commands
     for tool in ("realfft", "rednoise", "zapbirds"):
         obj = os.path.join(BUILD, tool + ".o")

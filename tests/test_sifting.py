@@ -1,5 +1,6 @@
-"""The ACCEL sifter (presto_amd/sifting.py, the rules of the reference's
-python/presto/sifting.py) on the CPU:
+"""The ACCEL sifter (the C sifter accel_sift_* behind presto_amd/sift.py and
+the accel_sift command; the rules of the reference's
+python/presto/sifting.py and examplescripts/ACCEL_sift.py) on the CPU:
 
 * parsing of ACCEL files written by this package's accelsearch (fixtures
   under tests/golden/sift/, written by tests/golden/make_sift_fixtures.py),
@@ -8,9 +9,9 @@ python/presto/sifting.py) on the CPU:
   tail with 2 numharm degrees of freedom at twice the summed power);
 * every rejection rule and each of the duplicate, DM-problem and harmonic
   passes on hand-built candidate lists whose outcome the rules fix;
-* the examplescripts/ACCEL_sift.py flow over the six-DM fixture set: the
-  pulsar survives with its three DM hits, the interference line that
-  peaks at DM 0 goes to the DM-problem list.
+* the examplescripts/ACCEL_sift.py flow (the accel_sift command) over the
+  six-DM fixture set: the pulsar survives with its three DM hits, the
+  interference line that peaks at DM 0 goes to the DM-problem list.
 
 On the GPU, the fixture set is written again by the accelsearch command
 and must equal the committed files byte for byte."""
@@ -18,13 +19,13 @@ import glob
 import os
 import re
 import shutil
-from operator import attrgetter
+import subprocess
 
 import numpy as np
 import pytest
 from scipy import stats
 
-S = pytest.importorskip("presto_amd.sifting")
+S = pytest.importorskip("presto_amd.sift")
 HERE = os.path.dirname(os.path.abspath(__file__))
 SIFT = os.path.join(HERE, "golden", "sift")
 T_FIX = 262144 * 0.000256
@@ -180,16 +181,12 @@ def test_remove_harmonics():
 def test_accel_sift_flow_on_fixture(tmp_path, capsys):
     for fn in glob.glob(os.path.join(SIFT, "*")):
         shutil.copy(fn, tmp_path)
-    cwd = os.getcwd()
-    os.chdir(tmp_path)
-    try:
-        from presto_amd import accel_sift
-        assert accel_sift.main(["-o", "candlist.txt"]) == 0
-        out = open("candlist.txt").read().splitlines()
-        S.sigma_threshold = 4.0
-        cands = S.read_candidates(sorted(glob.glob("*ACCEL_*0")), track=True)
-    finally:
-        os.chdir(cwd)
+    cli = os.path.join(os.path.dirname(S.__file__), "accel_sift")
+    r = subprocess.run([cli, "-o", "candlist.txt"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = open(tmp_path / "candlist.txt").read().splitlines()
+    S.sigma_threshold = 4.0
+    cands = S.read_candidates(sorted(glob.glob(str(tmp_path / "*ACCEL_*0"))), track=True)
     assert out[0] == ("#" + "file:candnum".center(66) + "DM".center(9) + "SNR".center(8) +
                       "sigma".center(8) + "numharm".center(9) + "ipow".center(9) +
                       "cpow".center(9) + "P(ms)".center(14) + "r".center(12) +
