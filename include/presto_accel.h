@@ -109,8 +109,7 @@ const accel_geom *accel_search_geom(const accel_search *s);
 /* Copies the spectrum (numbins complex values, first one at bin lobin) to
  * HBM.  Replaces the mmap / .dat read of create_accelobs. */
 int accel_search_set_spectrum(accel_search *s, const accel_fcomplex *fft);
-/* Takes a time series of n samples (n a power of two, n/2 = numbins,
- * lobin 0) and prepares its spectrum on the GPU as accelsearch prepares a
+/* Takes a time series of n samples (n even, n/2 = numbins, lobin 0) and prepares its spectrum on the GPU as accelsearch prepares a
  * .dat: realfft, deredden and bin 0 := 1+1i (src/accel_utils.c:1507-1562,
  * 1621-1624). */
 int accel_search_set_series(accel_search *s, const float *series, long long n);
@@ -175,14 +174,14 @@ long long accel_search_plane_cells(const accel_search *s);
 int accel_search_windows(const accel_search *s, long long *lobin, int *numdata, int n);
 
 /* realfft(data, n, -1) (src/fastffts.c:163-243) of a host series on the
- * GPU: n a power of two; out receives n/2 complex bins, out[0] =
+ * GPU: n even (any such length); out receives n/2 complex bins, out[0] =
  * (DC, Nyquist).  Replaces the FFT of create_accelobs's .dat input
  * (src/accel_utils.c:1547). */
 int accel_realfft_forward(const float *series, long long n, accel_fcomplex *out, int device);
 
 /* realfft(data, n, +1) (src/fastffts.c:163-243): the inverse of the above,
- * scaled by 2/n so that it returns the original series; on the GPU, n a
- * power of two.  Replaces the in-core inverse of realfft -inv
+ * scaled by 2/n so that it returns the original series; on the GPU, n
+ * even.  Replaces the in-core inverse of realfft -inv
  * (src/realfft.c:331). */
 int accel_realfft_inverse(const accel_fcomplex *spec, long long n, float *out, int device);
 

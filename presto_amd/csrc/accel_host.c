@@ -1814,14 +1814,14 @@ int accel_realfft_forward(const float *series, long long n, accel_fcomplex *out,
 {
     void *stream = NULL, *d_x = NULL, *d_out = NULL, *d_work = NULL;
     int rc = -1;
-    if (n < 4 || (n & (n - 1))) {
-        set_err("accel_realfft_forward: length %lld is not a power of two >= 4", n);
+    if (n < 4 || (n & 1)) {
+        set_err("accel_realfft_forward: length %lld is not even and >= 4", n);
         return -1;
     }
     if (agpu_init(device) || agpu_stream_create(&stream) ||
         agpu_malloc(&d_x, sizeof(float) * (size_t) n) ||
         agpu_malloc(&d_out, sizeof(accel_fcomplex) * (size_t) (n / 2)) ||
-        agpu_malloc(&d_work, sizeof(accel_fcomplex) * (size_t) (3 * n / 4)) ||
+        agpu_malloc(&d_work, sizeof(accel_fcomplex) * (size_t) agpu_realfft_work(n)) ||
         agpu_h2d(d_x, series, sizeof(float) * (size_t) n, stream) ||
         agpu_realfft((const float *) d_x, n, d_out, d_work, stream) ||
         agpu_d2h(out, d_out, sizeof(accel_fcomplex) * (size_t) (n / 2), stream)) {
@@ -1845,13 +1845,13 @@ int accel_realfft_inverse(const accel_fcomplex *spec, long long n, float *out, i
 {
     void *stream = NULL, *d_x = NULL, *d_work = NULL;
     int rc = -1;
-    if (n < 4 || (n & (n - 1))) {
-        set_err("accel_realfft_inverse: length %lld is not a power of two >= 4", n);
+    if (n < 4 || (n & 1)) {
+        set_err("accel_realfft_inverse: length %lld is not even and >= 4", n);
         return -1;
     }
     if (agpu_init(device) || agpu_stream_create(&stream) ||
         agpu_malloc(&d_x, sizeof(float) * (size_t) n) ||
-        agpu_malloc(&d_work, sizeof(accel_fcomplex) * (size_t) (3 * n / 4)) ||
+        agpu_malloc(&d_work, sizeof(accel_fcomplex) * (size_t) agpu_realfft_work(n)) ||
         agpu_h2d(d_x, spec, sizeof(float) * (size_t) n, stream) ||
         agpu_realfft_inverse(d_x, n, (float *) d_x, d_work, stream) ||
         agpu_d2h(out, d_x, sizeof(float) * (size_t) n, stream)) {
@@ -1927,9 +1927,9 @@ done:
 static int set_series(accel_search *s, const float *series, long long n, int on_device)
 {
     const accel_geom *g = &s->g;
-    if (n < 4 || (n & (n - 1)) || n / 2 != g->numbins || g->lobin != 0) {
+    if (n < 4 || (n & 1) || n / 2 != g->numbins || g->lobin != 0) {
         set_err("accel_search_set_series: %lld samples do not give the %lld-bin spectrum "
-                "of this search (power-of-two length, lobin 0)", n, g->numbins);
+                "of this search (even length, lobin 0)", n, g->numbins);
         return -1;
     }
     if (!s->own_spec) {
@@ -1937,7 +1937,7 @@ static int set_series(accel_search *s, const float *series, long long n, int on_
         s->own_spec = 1;
     }
     if (!s->d_fftwork)
-        GPU(agpu_malloc(&s->d_fftwork, sizeof(accel_fcomplex) * (size_t) (3 * n / 4)));
+        GPU(agpu_malloc(&s->d_fftwork, sizeof(accel_fcomplex) * (size_t) agpu_realfft_work(n)));
     if (!s->d_dblk) {
         /* src/accel_utils.c:1561: deredden(fft, numbins, 6, 200, 0.5/dt, dt N) */
         const double T = g->dt * (double) n;
@@ -2018,7 +2018,7 @@ long long accel_search_footprint(const accel_params *p)
     if (g.inmem)
         bytes += 4LL * ntiles * AGPU_TILE;                    /* per-half-bin norm^2 */
     bytes += 24LL * HIT_CAP0;                                 /* hit buffer */
-    bytes += 12LL * g.numbins;                                /* FFT work of series input */
+    bytes += 8LL * agpu_realfft_work(2 * g.numbins);          /* FFT work of series input */
     bytes += 64LL << 20;                                      /* refinement, tables, slack */
     if (g.numw)   /* jerk refinement scratch for a full chunk of candidates */
         bytes += 16LL * 26 * REFINE_CHUNK * rzw_npts_bound(1, g.zhi, g.whi);

@@ -202,8 +202,12 @@ int agpu_inmem_nsq_table(const float *d_nsq, int npre, int nblocks, long long rh
 int agpu_deredden(void *d_spec, long long numbins, const agpu_window *d_blk, int nblk,
                   int maxn, float *d_mean, void *stream);
 
-/* Forward real FFT of n (a power of two) FP32 samples in PRESTO's packing
- * into d_out (n/2 complex); d_work holds 3n/4 complex values. */
+/* Complex values of work agpu_realfft and agpu_realfft_inverse need for
+ * an n-sample series. */
+long long agpu_realfft_work(long long n);
+/* Forward real FFT of n (even) FP32 samples in PRESTO's packing
+ * into d_out (n/2 complex); d_work holds agpu_realfft_work(n) complex
+ * values. */
 int agpu_realfft(const float *d_x, long long n, void *d_out, void *d_work, void *stream);
 /* Its inverse (realfft with isign = +1, scaled by 2/n): n/2 complex bins in
  * PRESTO's packing to n samples; d_X may be d_out; the same d_work size. */

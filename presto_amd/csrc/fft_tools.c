@@ -7,7 +7,8 @@
  *
  * The transforms and the de-reddening run on the GPU (accel_realfft_forward,
  * accel_realfft_inverse, accel_deredden_gpu); time series lengths must be
- * powers of two.  -disk, -mem and -tmpdir are accepted and have no effect
+ * even, as the reference's realfft requires.  -disk, -mem and -tmpdir are
+ * accepted and have no effect
  * (every transform is in-core, in device memory); zapbirds' interactive
  * mode (-in / -out without -zap) is not built.
  */
@@ -58,7 +59,8 @@ static const char *split(const char *in, char *root, size_t n)
     return dot + 1;
 }
 
-static int is_pow2(long long n) { return n >= 4 && !(n & (n - 1)); }
+/* src/fastffts.c:173-176: realfft takes even lengths */
+static int even_len(long long n) { return n >= 4 && !(n & 1); }
 
 #if defined(TOOL_REALFFT)
 
@@ -148,9 +150,8 @@ int main(int argc, char **argv)
             fprintf(stderr, "realfft: cannot read '%s'\n", argv[fi]);
             return 1;
         }
-        if (!is_pow2(n)) {
-            fprintf(stderr, "realfft: '%s' holds %lld floats; the GPU transform takes a power "
-                            "of two\n", argv[fi], n);
+        if (!even_len(n)) {
+            fprintf(stderr, "\nrealfft() arrays lengths must be evenly divisible by 2.\n\n");
             return 1;
         }
         printf("\nPerforming in-core %s FFT on data:\n", isign == -1 ? "forward" : "inverse");
