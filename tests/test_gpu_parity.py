@@ -9,6 +9,10 @@ Bars (from the task statement for this path):
   * identical candidate lists: same numharm, r within 1e-3 bins, z within
     0.05, sigma within 1e-3.
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -376,10 +380,16 @@ def test_repeat_runs_are_deterministic(std_case):
 
 
 # ------------------------------------------------------------------ full size
+CONFIG2_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                               "config2_candidates.json")
+
+
 def test_config2_full_size_properties():
-    """2^23 samples, zmax 200, numharm 16 (standard path): the 3 injected
-    pulsars are found, and plane spot checks and the first blocks' hits
-    agree with the oracle."""
+    """2^23 samples, zmax 200, numharm 16 (standard path): the full
+    candidate list equals the oracle's (tests/golden/config2_candidates.json,
+    written by tests/golden/make_config2_candidates.py over all 5432
+    blocks), the 3 injected pulsars are found, and plane spot checks and
+    the first blocks' hits agree with the oracle."""
     _say("case: config-2 full size")
     x, ps = synth.config2_series()
     dt = 64e-6
@@ -387,10 +397,17 @@ def test_config2_full_size_properties():
     p = A.make_params(len(fft), len(x), dt, zmax=200, numharm=16, sigma=2.0)
     o = O.make_obs(len(fft), len(x), dt, zmax=200, numharm=16, sigma=2.0)
     assert o.inmem == 0
+    fix = json.load(open(CONFIG2_FIXTURE))
+    same_input = hashlib.sha256(fft.tobytes()).hexdigest() == fix["spectrum_sha256"]
     with A.Search(p) as s:
         s.set_spectrum(fft)
         s.run()
         gc, gh = s.candidates(), s.hits()
+        want = np.array([[c["power"], c["sigma"], c["numharm"], c["r"], c["z"], c["w"]]
+                         for c in fix["candidates"]])
+        _say(f"full candidate list: {len(gc)} here, {len(want)} from the oracle "
+             f"(input spectrum {'identical' if same_input else 'DIFFERENT'})")
+        _compare_cands(gc, want)
         T = len(x) * dt
         _say("injected: " + ", ".join(f"f0={q['f0']:.2f} z={q['z']:.1f} amp={q['amp']:.3f}"
                                        for q in ps))
