@@ -179,6 +179,21 @@ int accel_search_windows(const accel_search *s, long long *lobin, int *numdata, 
  * (src/accel_utils.c:1547). */
 int accel_realfft_forward(const float *series, long long n, accel_fcomplex *out, int device);
 
+/* accel_realfft_forward on device buffers that allocates nothing: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  d_series (n floats) and d_out (n/2 complex) are
+ * device memory on `device`; the work runs on `stream` (a hipStream_t,
+ * NULL for the default stream) and is complete when that stream's work
+ * is; the call itself does not wait.  The FFT's work buffer is a 24 MiB
+ * array each device holds from its first call until the process ends: an
+ * n whose work does not fit it is refused (powers of two fit up to 2^22).
+ * Calls that overlap on one
+ * device share that buffer and race: serialise them.  The results equal
+ * accel_realfft_forward's bit for bit. */
+int accel_realfft_forward_noalloc(const float *d_series, long long n, accel_fcomplex *d_out,
+                                  void *stream, int device);
+
 /* realfft(data, n, +1) (src/fastffts.c:163-243): the inverse of the above,
  * scaled by 2/n so that it returns the original series; on the GPU, n
  * even.  Replaces the in-core inverse of realfft -inv

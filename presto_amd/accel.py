@@ -155,7 +155,7 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_server_timing",
             "accel_last_error", "accel_search_set_device_series", "accel_dedisp_plan",
             "accel_dedisperse_dev", "accel_dedisperse", "accel_device_alloc", "accel_device_free",
-            "accel_device_upload", "accel_device_download",
+            "accel_device_upload", "accel_device_download", "accel_realfft_forward_noalloc",
             "accel_sift_create", "accel_sift_destroy", "accel_sift_add", "accel_sift_read",
             "accel_sift_parse_power", "accel_sift_reject", "accel_sift_remove_duplicates",
             "accel_sift_remove_dm_problems", "accel_sift_remove_harmonics",
@@ -617,6 +617,7 @@ _sig("accel_device_free", _i, _p)
 _sig("accel_device_upload", _i, _p, _p, C.c_size_t)
 _sig("accel_device_download", _i, _p, _p, C.c_size_t)
 _sig("accel_dedisperse", _ll, _p, _p, _ll, _p, _p, _i)
+_sig("accel_realfft_forward_noalloc", _i, _p, _ll, _p, _p, _i)
 
 
 def dedisp_params(numchan, dt, lofreq, chanwidth, spectra_per_subint, nsub, downsamp, numdms,

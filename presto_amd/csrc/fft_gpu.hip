@@ -20,8 +20,10 @@
  * series, pairs (k, M-k) per thread, in place.
  */
 #include <hip/hip_runtime.h>
+#include <pthread.h>
 #include <stdio.h>
 #include "agpu.h"
+#include "accel_internal.h"
 
 #define FFT_TPB 256
 
@@ -530,5 +532,69 @@ int agpu_realfft(const float *d_x, long long n, void *d_out, void *d_work, void 
     /* X into d_out: in place (pairs per thread) or from the ping-pong buffer */
     hipLaunchKernelGGL(k_realfft_post, blocks_for(M / 2 + 1), dim3(FFT_TPB), 0, st, Z, a, M);
     FFT_CHECK(hipGetLastError());
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* The forward real FFT without allocation (accel_realfft_forward_noalloc). */
+/* Its work buffer is this file-scope device array; each device holds its  */
+/* own copy, found once per device with hipGetSymbolAddress.               */
+
+#define FFT_SCRATCH_COMPLEX (3LL << 20)   /* 24 MiB: the work of a 2^22-sample series */
+static __device__ __align__(256) float2 scratch[FFT_SCRATCH_COMPLEX];
+
+#define FFT_MAX_DEVICES 64
+static float2 *g_scratch_dev[FFT_MAX_DEVICES];
+static pthread_mutex_t g_scratch_mu = PTHREAD_MUTEX_INITIALIZER;
+
+/* The device address of `scratch` on `device` (which must be current). */
+static float2 *scratch_address(int device)
+{
+    if (device < 0 || device >= FFT_MAX_DEVICES)
+        return NULL;
+    pthread_mutex_lock(&g_scratch_mu);
+    float2 *ptr = g_scratch_dev[device];
+    if (!ptr) {
+        void *p = NULL;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e == hipSuccess && p) {
+            ptr = (float2 *) p;
+            g_scratch_dev[device] = ptr;
+        } else {
+            char m[256];
+            snprintf(m, sizeof(m), "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            agpu_set_error(m);
+        }
+    }
+    pthread_mutex_unlock(&g_scratch_mu);
+    return ptr;
+}
+
+extern "C" int accel_realfft_forward_noalloc(const float *d_series, long long n,
+                                             accel_fcomplex *d_out, void *stream, int device)
+{
+    if (n < 4 || (n & 1)) {
+        accel_set_error("accel_realfft_forward_noalloc: length %lld is not even and >= 4", n);
+        return -1;
+    }
+    if (agpu_realfft_work(n) > FFT_SCRATCH_COMPLEX) {
+        accel_set_error("accel_realfft_forward_noalloc: a series of %lld samples needs %lld "
+                        "bytes of work, more than the %lld-byte scratch", n,
+                        8 * agpu_realfft_work(n), 8 * FFT_SCRATCH_COMPLEX);
+        return -1;
+    }
+    if (!d_series || !d_out) {
+        accel_set_error("accel_realfft_forward_noalloc: null device buffer");
+        return -1;
+    }
+    if (agpu_init(device)) {
+        accel_set_error("accel_realfft_forward_noalloc: %s", agpu_last_error());
+        return -1;
+    }
+    float2 *work = scratch_address(device);
+    if (!work || agpu_realfft(d_series, n, d_out, work, stream)) {
+        accel_set_error("accel_realfft_forward_noalloc: %s", agpu_last_error());
+        return -1;
+    }
     return 0;
 }
