@@ -217,10 +217,14 @@ static void init_subharminfo(int numharm, int harmnum, int zmax, int wmax, o_shi
     shi->numkern_wdim = (shi->wmax / O_DW) * 2 + 1;
     shi->numkern = shi->numkern_zdim * shi->numkern_wdim;
     shi->kern = (o_kernel *) malloc(sizeof(o_kernel) * shi->numkern);
-    for (int ii = 0; ii < shi->numkern_wdim; ii++)
-        for (int jj = 0; jj < shi->numkern_zdim; jj++)
-            init_kernel(-shi->zmax + jj * O_DZ, -shi->wmax + ii * O_DW, fftlen,
-                        &shi->kern[ii * shi->numkern_zdim + jj], fp64);
+    /* kernels are independent (the FFT plan cache is locked): threads
+     * only shorten the oracle's set-up */
+#pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < shi->numkern; k++) {
+        const int ii = k / shi->numkern_zdim, jj = k % shi->numkern_zdim;
+        init_kernel(-shi->zmax + jj * O_DZ, -shi->wmax + ii * O_DW, fftlen, &shi->kern[k],
+                    fp64);
+    }
 }
 
 o_shi **o_create_subharminfos(const o_obs *obs, int fp64)
@@ -775,6 +779,16 @@ static o_cand *search_ffdotpows(const o_ffdot *ffdot, int numharm, const o_obs *
 int o_accelsearch(const o_obs *obs, const o_fcomplex *fft, int fp64, int max_blocks,
                   o_result *res)
 {
+    return o_accelsearch_range(obs, fft, fp64, 0, max_blocks, res);
+}
+
+/* The block loop over blocks [first_block, first_block + max_blocks) only
+ * (max_blocks < 0: to the end).  Block numbers stay those of the whole
+ * search; on the in-memory path the earlier blocks' fundamentals are still
+ * written into the plane, since later sub-harmonic sums read them. */
+int o_accelsearch_range(const o_obs *obs, const o_fcomplex *fft, int fp64, int first_block,
+                        int max_blocks, o_result *res)
+{
     memset(res, 0, sizeof(*res));
     o_shi **shis = o_create_subharminfos(obs, fp64);
     int rstep = (int) (obs->corr_uselen * O_DR);
@@ -799,9 +813,20 @@ int o_accelsearch(const o_obs *obs, const o_fcomplex *fft, int fp64, int max_blo
     double startr = obs->rlo;
     int block = 0;
     while (startr + rstep < obs->highestbin) {
-        if (max_blocks >= 0 && block >= max_blocks)
+        if (max_blocks >= 0 && block >= first_block + max_blocks)
             break;
         double nextr = startr + rstep, lastr = nextr - O_DR;
+        if (block < first_block) {
+            if (obs->inmem && obs->numharmstages > 1) {
+                o_ffdot *f = o_subharm_fderivs_vol(1, 1, startr, lastr, &shis[0][0], obs, fft,
+                                                   fp64);
+                o_fund_to_ffdotplane(f, obs, plane, dplane);
+                o_free_ffdot(f);
+            }
+            startr = nextr;
+            block++;
+            continue;
+        }
         o_ffdot *fund = o_subharm_fderivs_vol(1, 1, startr, lastr, &shis[0][0], obs, fft, fp64);
         cands = search_ffdotpows(fund, 1, obs, cands, block, res);
         if (obs->numharmstages > 1) {

@@ -8,6 +8,7 @@
  */
 #include <math.h>
 #include <stdlib.h>
+#include <time.h>
 #include <string.h>
 #include <pthread.h>
 #include "oracle.h"
@@ -66,28 +67,32 @@ static fftplan *get_plan(long n)
 }
 
 /* One Stockham pass: radix R, Ns = product of earlier radices.
- * y[(j/Ns)*Ns*R + j%Ns + r*Ns] = DFT_R over r of x[j + r*N/R]*w^(r*(j%Ns)). */
+ * y[(j/Ns)*Ns*R + j%Ns + r*Ns] = DFT_R over r of x[j + r*N/R]*w^(r*(j%Ns)).
+ * j runs as (b, jm) = (j/Ns, j%Ns); the twiddle index jm*r*step stays
+ * below n, so no reduction modulo n is needed. */
 #define STOCKHAM_IMPL(NAME, T, TC, TWF)                                          \
 static void NAME(TC *x, TC *y, long n, int isign, const fftplan *p)              \
 {                                                                                \
     long Ns = 1;                                                                 \
     TC *in = x, *out = y;                                                        \
     const TC *tw = (const TC *) p->TWF;                                          \
+    const T sgn = (isign < 0) ? (T) 1 : (T) -1;                                  \
     for (int f = 0; f < p->nf; f++) {                                            \
         int R = p->fac[f];                                                       \
-        long NR = n / R, step = n / (Ns * R);                                    \
+        long NR = n / R, step = n / (Ns * R), nb = NR / Ns;                      \
         TC v[64], u[64];                                                         \
-        for (long j = 0; j < NR; j++) {                                          \
-            long jm = j % Ns;                                                    \
+        for (long b = 0; b < nb; b++)                                            \
+        for (long jm = 0; jm < Ns; jm++) {                                       \
+            const long j = b * Ns + jm;                                          \
             for (int r = 0; r < R; r++) {                                        \
                 TC a = in[j + r * NR];                                           \
                 if (jm && r) {                                                   \
-                    long k = (jm * r * step) % n;                                \
-                    T wr = tw[k].r, wi = (isign < 0) ? tw[k].i : -tw[k].i;       \
-                    TC b;                                                        \
-                    b.r = a.r * wr - a.i * wi;                                   \
-                    b.i = a.r * wi + a.i * wr;                                   \
-                    a = b;                                                       \
+                    const long k = jm * r * step;                                \
+                    T wr = tw[k].r, wi = sgn * tw[k].i;                          \
+                    TC c;                                                        \
+                    c.r = a.r * wr - a.i * wi;                                   \
+                    c.i = a.r * wi + a.i * wr;                                   \
+                    a = c;                                                       \
                 }                                                                \
                 v[r] = a;                                                        \
             }                                                                    \
@@ -115,14 +120,14 @@ static void NAME(TC *x, TC *y, long n, int isign, const fftplan *p)             
                     T sr = 0, si = 0;                                            \
                     for (int r = 0; r < R; r++) {                                \
                         long k = ((long) q * r % R) * base;                      \
-                        T wr = tw[k].r, wi = (isign < 0) ? tw[k].i : -tw[k].i;   \
+                        T wr = tw[k].r, wi = sgn * tw[k].i;                      \
                         sr += v[r].r * wr - v[r].i * wi;                         \
                         si += v[r].r * wi + v[r].i * wr;                         \
                     }                                                            \
                     u[q].r = sr; u[q].i = si;                                    \
                 }                                                                \
             }                                                                    \
-            long idxD = (j / Ns) * Ns * R + jm;                                  \
+            const long idxD = b * Ns * R + jm;                                   \
             for (int r = 0; r < R; r++)                                          \
                 out[idxD + r * Ns] = u[r];                                       \
         }                                                                        \
@@ -318,4 +323,37 @@ error:
     return -1;
 #undef RD
 #undef WR
+}
+
+/* Nanoseconds per in-place forward transform of n complex floats, the
+ * median of `reps` timed batches of 64 transforms on one thread: the
+ * figure reported beside the CPU baseline, so that this FFT can be set
+ * against FFTW's. */
+double o_time_cfft_ns(long n, int reps)
+{
+    o_fcomplex *x = (o_fcomplex *) malloc(sizeof(o_fcomplex) * n);
+    double *t = (double *) malloc(sizeof(double) * (reps > 0 ? reps : 1));
+    for (long i = 0; i < n; i++) {
+        x[i].r = (float) ((i * 7919) % 101) / 101.0f;
+        x[i].i = (float) ((i * 104729) % 97) / 97.0f;
+    }
+    o_cfft_f(x, n, -1);   /* plan made before timing */
+    for (int k = 0; k < reps; k++) {
+        struct timespec a, b;
+        clock_gettime(CLOCK_MONOTONIC, &a);
+        for (int j = 0; j < 64; j++)
+            o_cfft_f(x, n, (j & 1) ? 1 : -1);
+        clock_gettime(CLOCK_MONOTONIC, &b);
+        t[k] = ((b.tv_sec - a.tv_sec) * 1e9 + (b.tv_nsec - a.tv_nsec)) / 64.0;
+    }
+    for (int i = 1; i < reps; i++)   /* insertion sort for the median */
+        for (int j = i; j > 0 && t[j - 1] > t[j]; j--) {
+            double tmp = t[j];
+            t[j] = t[j - 1];
+            t[j - 1] = tmp;
+        }
+    double med = reps > 0 ? t[reps / 2] : 0.0;
+    free(x);
+    free(t);
+    return med;
 }

@@ -67,6 +67,7 @@ float o_invsqrtf(float x);
 
 /* ---- FFTs: o_fft.c ---- */
 void o_cfft_f(o_fcomplex *data, long n, int isign);
+double o_time_cfft_ns(long n, int reps);
 void o_cfft_d(o_dcomplex *data, long n, int isign);
 void o_realfft_f(float *data, long n, int isign);
 long o_deredden(o_fcomplex *fft, long numbins, int startwidth, int endwidth,
@@ -172,6 +173,8 @@ typedef struct {
 /* The block loop of src/accelsearch.c:161-228.  max_blocks < 0: all. */
 int o_accelsearch(const o_obs *obs, const o_fcomplex *fft, int fp64, int max_blocks,
                   o_result *res);
+int o_accelsearch_range(const o_obs *obs, const o_fcomplex *fft, int fp64, int first_block,
+                        int max_blocks, o_result *res);
 void o_free_result(o_result *res);
 void o_set_threads(int n);
 long long o_prepare_dat(float *series, long long n, double dt);

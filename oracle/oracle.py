@@ -96,6 +96,8 @@ _sig("o_realfft_f", None, _fp, C.c_long, _i)
 _sig("o_deredden", C.c_long, _cf, C.c_long, _i, _i, _d, _d)
 _sig("o_create_accelobs", _i, C.POINTER(OObs))
 _sig("o_accelsearch", _i, C.POINTER(OObs), _cf, _i, _i, C.POINTER(OResult))
+_sig("o_time_cfft_ns", C.c_double, C.c_long, _i)
+_sig("o_accelsearch_range", _i, C.POINTER(OObs), _cf, _i, _i, _i, C.POINTER(OResult))
 _sig("o_free_result", None, C.POINTER(OResult))
 _sig("o_set_threads", None, _i)
 _sig("o_result_ncands", _i, C.POINTER(OResult))
@@ -203,11 +205,14 @@ def make_obs(numbins, N, dt, zmax=200, numharm=8, sigma=2.0, wmax=None, flo=1.0,
     return o
 
 
-def accelsearch(obs, fft, fp64=False, max_blocks=-1):
-    """Run the reference block loop; returns (hits structured array, cands)."""
+def accelsearch(obs, fft, fp64=False, max_blocks=-1, first_block=0):
+    """Run the reference block loop over blocks [first_block, first_block +
+    max_blocks) (max_blocks < 0: to the end); returns (hits structured
+    array, cands, blocks run)."""
     fft = np.ascontiguousarray(fft, dtype=np.complex64)
     res = OResult()
-    lib.o_accelsearch(C.byref(obs), fft, int(fp64), int(max_blocks), C.byref(res))
+    lib.o_accelsearch_range(C.byref(obs), fft, int(fp64), int(first_block), int(max_blocks),
+                            C.byref(res))
     n = res.nhits
     hits = np.zeros(n, dtype=[("block", "i4"), ("stage", "i4"), ("wi", "i4"),
                               ("zi", "i4"), ("ri", "i4"), ("pow", "f4"), ("sigma", "f4")])

@@ -100,19 +100,36 @@ def config4_trials(n=64, seed0=4000):
             (r_avg, z_avg, w)
 
 
+def smooth_even(n, primes=(2, 3, 5, 7)):
+    """The even length nearest to n (ties to the smaller) whose prime
+    factors are all in primes: the lengths real FFTs handle with small
+    radices."""
+    def smooth(m):
+        for p in primes:
+            while m % p == 0:
+                m //= p
+        return m == 1
+    n = max(2, int(round(n)))
+    for d in range(n):
+        for m in (n - d, n + d):
+            if m >= 2 and m % 2 == 0 and smooth(m):
+                return m
+    return 2
+
+
 def config5_requests(n, seed_arrivals=5, seed0=5_000_000, rate=40.0):
     """The serving workload: n requests with Poisson arrival times (rate per
     second, PCG64(seed_arrivals)); per request (PCG64(seed0 + i)): a length
-    log-uniform in 2^20..2^24 samples, rounded to a power of two (the GPU
-    real FFT takes powers of two), dt 64 us, zmax from {0, 50, 200}, wmax
-    from {0, 100} (0: no jerk search), numharm from {8, 16}, and a pulsar
-    in 20 % of them.  Yields (i, t_arrival, N, dt, zmax, wmax, numharm,
-    seed, pulsars)."""
+    log-uniform in 2^20..2^24 samples, rounded to the nearest even length
+    whose prime factors are 2, 3, 5 and 7 (smooth_even), dt 64 us, zmax from
+    {0, 50, 200}, wmax from {0, 100} (0: no jerk search), numharm from
+    {8, 16}, and a pulsar in 20 % of them.  Yields (i, t_arrival, N, dt,
+    zmax, wmax, numharm, seed, pulsars)."""
     ra = np.random.Generator(np.random.PCG64(seed_arrivals))
     t = np.cumsum(ra.exponential(1.0 / rate, n))
     for i in range(n):
         r = np.random.Generator(np.random.PCG64(seed0 + i))
-        N = 1 << int(round(r.uniform(20.0, 24.0)))
+        N = smooth_even(2.0 ** r.uniform(20.0, 24.0))
         zmax = int(r.choice([0, 50, 200]))
         wmax = int(r.choice([0, 100]))
         nh = int(r.choice([8, 16]))
