@@ -460,10 +460,12 @@ __device__ __forceinline__ void dft16(float2 *v)
 #define LP(i) ((i) + ((i) >> 4))
 #define FFT_LDS (4096 + 256)
 
-/* Input v[r] = x[j + 256 r]; output v[r] = X[j + 256 r]. tw1/tw2 hold the
- * inverse-sign pass twiddles W_256^{r (j%16)} and W_4096^{r j}. */
+/* Input v[r] = x[j + 256 r]; output v[r] = X[j + 256 r].  tw1l is the
+ * LDS table W_256^{r m} at [16 r + m] (m = j % 16, the first inter-pass
+ * twiddles); tw2 holds this thread's second inter-pass twiddles
+ * W_4096^{r j}.  Twiddles are the inverse-sign (S = +1) values. */
 template <int S>
-__device__ __forceinline__ void fft4096(float2 *v, float2 *lds, const float2 *tw1,
+__device__ __forceinline__ void fft4096(float2 *v, float2 *lds, const float2 *tw1l,
                                         const float2 *tw2)
 {
     const int j = threadIdx.x;
@@ -478,7 +480,7 @@ __device__ __forceinline__ void fft4096(float2 *v, float2 *lds, const float2 *tw
     __syncthreads();
 #pragma unroll
     for (int r = 1; r < 16; r++) {
-        float2 w = tw1[r];
+        float2 w = tw1l[16 * r + (j & 15)];
         v[r] = cmul(v[r], make_float2(w.x, S * w.y));
     }
     dft16<S>(v);
@@ -509,7 +511,11 @@ __device__ __forceinline__ void fft4096(float2 *v, float2 *lds, const float2 *tw
 /* Fused correlation for segments of L = 4096 half-bins.  Block id ->
  * (segment, template group); groups is a multiple of 8 so each XCD (block
  * ids are dealt round-robin over the 8 XCDs) touches one residue class of
- * template groups and keeps those templates in its own L2. */
+ * template groups and keeps those templates in its own L2.  The forward
+ * FFT of a segment is shared by the group's templates.  The first
+ * inter-pass twiddles come from a 2 KiB LDS table (16 distinct values per
+ * pass and thread), the second stay in registers: 196 VGPRs, two waves
+ * per SIMD. */
 __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long numbins,
                                                   long long lobin_file, const float2 *T,
                                                   int nt, int hw, int uselen, long long hb_end,
@@ -517,6 +523,7 @@ __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long 
                                                   long long seg0)
 {
     __shared__ float2 lds[FFT_LDS];
+    __shared__ float2 tw1l[256];
     const int j = threadIdx.x;
     const int g = blockIdx.x % groups;
     const long long seg = seg0 + blockIdx.x / groups;
@@ -526,13 +533,15 @@ __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long 
     if (t0 >= nt)
         return;
     const int t1 = min(nt, t0 + tpg);
-
-    float2 tw1[16], tw2[16];
+    {
+        double s, c;
+        sincospi(2.0 * (double) ((j >> 4) * (j & 15)) / 256.0, &s, &c);
+        tw1l[j] = make_float2((float) c, (float) s);
+    }
+    float2 tw2[16];
 #pragma unroll
     for (int r = 1; r < 16; r++) {
         double s, c;
-        sincospi(2.0 * (double) (r * (j & 15)) / 256.0, &s, &c);
-        tw1[r] = make_float2((float) c, (float) s);
         sincospi(2.0 * (double) (r * j) / 4096.0, &s, &c);
         tw2[r] = make_float2((float) c, (float) s);
     }
@@ -547,7 +556,8 @@ __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long 
                 v[r] = spec[bin];
         }
     }
-    fft4096<-1>(v, lds, tw1, tw2);
+    __syncthreads();
+    fft4096<-1>(v, lds, tw1l, tw2);
     float2 D[16];
 #pragma unroll
     for (int r = 0; r < 16; r++)
@@ -558,7 +568,7 @@ __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long 
 #pragma unroll
         for (int r = 0; r < 16; r++)
             v[r] = cmul(D[r], Tt[j + 256 * r]);
-        fft4096<1>(v, lds, tw1, tw2);
+        fft4096<1>(v, lds, tw1l, tw2);
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int q = j + 256 * r - 2 * hw;

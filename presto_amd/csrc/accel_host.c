@@ -793,7 +793,10 @@ static int search_init(accel_search *s, int device)
     /* one tile of padding past the last half-bin: the tail chunk of the
      * harmonic sum reads it instead of clamping indices */
     s->ntiles = (s->hb_end + AGPU_TILE - 1) / AGPU_TILE + 1;
-    s->groups = 16;
+    /* template groups of about 26 (the forward FFT of a segment is shared
+     * by a group), a multiple of 8 so that each XCD keeps one residue
+     * class of templates in its L2 */
+    s->groups = 8 * (s->nt + 103 >= 208 ? (s->nt + 103) / 208 : 1);
     s->nseg = (s->hb_end + s->uselen - 1) / s->uselen;
     agpu_window *win = NULL;
     int *zmap = NULL, *wmap = NULL;
@@ -1392,6 +1395,7 @@ static int rzw_npts_bound(int numharm, double zin, double win)
  * freeing device memory waits for the whole device, which would stop a
  * refinement from overlapping the next search. */
 #define REFINE_CHUNK 256
+#define REFINE_RZW_BUDGET ((size_t) 3 << 30)
 struct refine_ws {
     int ready;
     size_t scr_bytes;
@@ -1476,13 +1480,24 @@ static int refine_device(refine_ws *w, const void *d_spec, long long numdata, in
         set_err("refinement: jerk chirp of %d points is too long", cap);
         return -1;
     }
-    const int chunk = n < REFINE_CHUNK ? n : REFINE_CHUNK;
-    if (ws_reserve(w, win ? (size_t) 16 * 26 * (size_t) cap * chunk : 0))
+    int chunk = n < REFINE_CHUNK ? n : REFINE_CHUNK;
+    if (win) {
+        /* the jerk refinement's scratch grows with the chirp bound: keep a
+         * chunk within REFINE_RZW_BUDGET bytes */
+        const size_t per = agpu_refine_rzw_scratch(1, cap);
+        const int fit = (int) (REFINE_RZW_BUDGET / (per ? per : 1));
+        if (chunk > fit)
+            chunk = fit > 0 ? fit : 1;
+    }
+    if (ws_reserve(w, win ? agpu_refine_rzw_scratch(chunk, cap) : 0))
         return -1;
     int nev[REFINE_CHUNK];
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int m = (n - c0) < chunk ? n - c0 : chunk;
         const size_t md = sizeof(double) * (size_t) m, mhx = (size_t) m * ACCEL_MAXHARM;
+        int maxh = 1;
+        for (int i = 0; i < m; i++)
+            maxh = nh[c0 + i] > maxh ? nh[c0 + i] : maxh;
         if (agpu_h2d(w->d_nh, nh + c0, sizeof(int) * m, stream) ||
             agpu_h2d(w->d_rin, rin + c0, md, stream) ||
             agpu_h2d(w->d_zin, zin + c0, md, stream) ||
@@ -1510,7 +1525,7 @@ static int refine_device(refine_ws *w, const void *d_spec, long long numdata, in
         a.raw = raw;
         if (agpu_memset(w->d_der, 0, sizeof(double) * 7 * mhx, stream) ||
             agpu_memset(w->d_pow, 0, sizeof(double) * mhx, stream) ||
-            (win ? agpu_refine_rzw(&a, stream) : agpu_refine_rz(&a, stream)) ||
+            (win ? agpu_refine_rzw(&a, stream, maxh) : agpu_refine_rz(&a, stream)) ||
             agpu_d2h(rout + c0, w->d_rout, md, stream) ||
             agpu_d2h(zout + c0, w->d_zout, md, stream) ||
             (wout && agpu_d2h(wout + c0, w->d_wout, md, stream)) ||

@@ -229,7 +229,7 @@ typedef struct {
     double *derivs;
     double *powers;
     int *nevals;
-    /* jerk refinement only: ncand x 26 x npts_cap complex FP64 scratch */
+    /* jerk refinement only: agpu_refine_rzw_scratch(ncand, npts_cap) bytes */
     void *scratch;
     int npts_cap;
     /* 1: one harmonic per candidate without polishing (-noharmpolish):
@@ -242,9 +242,12 @@ typedef struct {
 
 /* max_rz_arr_harmonics per candidate */
 int agpu_refine_rz(const agpu_refine_args *args, void *stream);
-/* max_rzw_arr_harmonics per candidate; a candidate whose chirp length
- * exceeds npts_cap gets rout = NaN, nevals = -1 */
-int agpu_refine_rzw(const agpu_refine_args *args, void *stream);
+/* max_rzw_arr_harmonics per candidate, spread over many workgroups per
+ * candidate; maxharm: the largest numharm of the candidates.  scratch must
+ * hold agpu_refine_rzw_scratch(ncand, npts_cap) bytes.  A candidate whose
+ * chirp length exceeds npts_cap gets rout = NaN, nevals = -1. */
+size_t agpu_refine_rzw_scratch(int ncand, int npts_cap);
+int agpu_refine_rzw(const agpu_refine_args *args, void *stream, int maxharm);
 
 /* Subband de-dispersion (dedisp_gpu.hip).  raw: [nspec][numchan] FP32,
  * ascending frequency; ds: [nsub][nds]; out: [numdms][numout].  maxdelay /
@@ -269,6 +272,22 @@ int agpu_dedisp_dms(const float *d_ds, long long nds, int nsub, const int *d_off
                     float *d_out, void *stream);
 int agpu_dedisp_fill(float *d_out, int numdms, long long numout, long long from, float value,
                      void *stream);
+
+
+/* prepfold's fold of a time series for one candidate (fold_gpu.hip):
+ * records of 1024 points, npart sub-integrations of rpp records each;
+ * part_end[p]: the phase added since the fold start at the end of part p;
+ * rawfolds: npart x proflen FP64, zeroed by the caller; recsums: per
+ * record the sum and the sum of squares of its points. */
+typedef struct {
+    const float *series;
+    double dt, f, fd, fdd, phi_start;
+    int npart, rpp, proflen;
+    const double *part_end;
+    double *rawfolds;
+    double *recsums;
+} agpu_fold_args;
+int agpu_fold(const agpu_fold_args *args, void *stream);
 
 #ifdef __cplusplus
 }

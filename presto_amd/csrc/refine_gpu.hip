@@ -562,7 +562,10 @@ __device__ static void derivs3d_wg(const float2 *spec, long long numdata, double
 #define RZW_NZ 17    /* 2 * numwid * interpfac + 1 */
 #define RZW_NB 8     /* interpfac * ACCEL_RDR */
 #define RZW_NC 7     /* data windows c = -1 .. 5 */
-#define RZW_SCR (RZW_NR + 2)   /* scratch rows per candidate: Q tables + 2 FFT buffers */
+#define RZW_NLP 20   /* interpolations of a local power (NUMLOCPOWAVG) */
+#define RZW_NPT 25   /* per candidate: the local power's 20 plus the 5 of the derivatives */
+#define RZW_NV 21    /* distinct chirp lengths of one grid: powers of two up to 2^20 */
+#define RZW_CELLS (RZW_NZ * RZW_NZ * RZW_NR)
 
 /* Output j (in 1/8 bins from lobin) correlates the data bins
  * [c - khw, c + khw) around lobin, c = ceil(j / 8).  j in [-8, 47]: a
@@ -575,8 +578,7 @@ __device__ static int rzw_window(int j)
 }
 
 /* Workgroup sum of RZW_NR complex values per thread (wave shuffles, then
- * one LDS pass over the 4 waves); lane 0 of wave 0 ... every thread gets
- * the result in out. */
+ * one LDS pass over the 4 waves); every thread gets the result in out. */
 __device__ static void wg_sum_vec(double2 *v, double2 *out, double2 *red4)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -602,208 +604,522 @@ __device__ static void wg_sum_vec(double2 *v, double2 *out, double2 *red4)
     __syncthreads();
 }
 
-__global__ __launch_bounds__(RF_TPB) void k_refine_rzw(agpu_refine_args a)
+/* max_rzw_arr_harmonics (src/maximize_rzw.c:84-191) is spread over many
+ * workgroups per candidate, harmonic by harmonic, from highest to lowest
+ * as the reference loops:
+ *   k_rzw_points  the 20 interpolations of the harmonic's local power (one
+ *                 workgroup each);
+ *   k_rzw_setup   the local power, lobin, the (z, w) grid, the r indices
+ *                 and the distinct chirp lengths of the grid;
+ *   k_rzw_qtab    per chirp length, the tables Q_j(n) = i^n
+ *                 e^{-2 pi i j n / (8 npts)} S_c(n), S_c the DFT of the data
+ *                 window [c - khw, c + khw) around lobin (c = ceil(j / 8) in
+ *                 [-1, 5]): with R the FFT of the zero-padded FP32 chirp x,
+ *                 the reference's correlation sum_k d_k conj(R(k - r_j))
+ *                 regroups exactly into sum_n x_n Q_j(n);
+ *   k_rzw_grid    one workgroup per (w, z) grid point: the 24 correlations
+ *                 and their powers over the local power.
+ * Then k_rzw_pick sums the powers over the harmonics in the reference's
+ * order (an FP32 sum) and takes the first maximum in (w, z, r) order, and
+ * per harmonic of the result k_rzw_points and k_rzw_derivs give the local
+ * power, the power and the 5-point derivatives.  Every value is formed
+ * exactly as by one workgroup running the whole loop: same expressions,
+ * same fixed-order workgroup sums. */
+typedef struct {
+    int bad, H;
+    long long hhlobin;
+    float locpow0, maxpow;
+    double rout, zout, wout;
+    /* the current harmonic */
+    int khw, numkern, ng, nvar;
+    long long lobin;
+    double zlo, dz, wlo, dw;
+    float locpow;
+    int npts[RZW_NV];
+    int rind[RZW_NR];
+} rzw_state;
+
+static size_t rzw_align(size_t b) { return (b + 255) & ~(size_t) 255; }
+
+/* scratch of a launch for ncand candidates, by part */
+struct rzw_layout {
+    size_t st, pts, terms, q, fft, total;
+};
+
+static rzw_layout rzw_layout_of(int ncand, int npts_cap)
+{
+    rzw_layout l;
+    size_t o = 0;
+    l.st = o;
+    o += rzw_align(sizeof(rzw_state) * (size_t) ncand);
+    l.pts = o;
+    o += rzw_align(sizeof(double2) * RZW_NPT * (size_t) ncand);
+    l.terms = o;
+    o += rzw_align(sizeof(double) * (size_t) RF_MAXH * RZW_CELLS * ncand);
+    l.q = o;
+    o += rzw_align(sizeof(double2) * RZW_NR * 2 * (size_t) npts_cap * ncand);
+    l.fft = o;
+    o += rzw_align(sizeof(double2) * 2 * (size_t) npts_cap * RZW_NPT * ncand);
+    l.total = o;
+    return l;
+}
+
+struct rzw_ptrs {
+    rzw_state *st;
+    double2 *pts;
+    double *terms;
+    double2 *q, *fft;
+};
+
+__device__ static rzw_ptrs rzw_ptrs_of(const agpu_refine_args &a, size_t o_st, size_t o_pts,
+                                       size_t o_terms, size_t o_q, size_t o_fft)
+{
+    char *b = (char *) a.scratch;
+    rzw_ptrs p;
+    p.st = (rzw_state *) (b + o_st);
+    p.pts = (double2 *) (b + o_pts);
+    p.terms = (double *) (b + o_terms);
+    p.q = (double2 *) (b + o_q);
+    p.fft = (double2 *) (b + o_fft);
+    return p;
+}
+
+#define RZW_LAYOUT_ARGS size_t o_st, size_t o_pts, size_t o_terms, size_t o_q, size_t o_fft
+#define RZW_PTRS rzw_ptrs_of(a, o_st, o_pts, o_terms, o_q, o_fft)
+
+/* the FFT buffers of slot k of candidate c */
+__device__ static double2 *rzw_fa(const rzw_ptrs &P, const agpu_refine_args &a, int c, int k)
+{
+    return P.fft + ((size_t) c * RZW_NPT + k) * 2 * (size_t) a.npts_cap;
+}
+
+__global__ __launch_bounds__(RF_TPB) void k_rzw_init(agpu_refine_args a, RZW_LAYOUT_ARGS)
+{
+    const rzw_ptrs P = RZW_PTRS;
+    const int c = blockIdx.x;
+    rzw_state *st = P.st + c;
+    double *terms = P.terms + (size_t) c * RF_MAXH * RZW_CELLS;
+    for (int i = threadIdx.x; i < RF_MAXH * RZW_CELLS; i += RF_TPB)
+        terms[i] = 0.0;
+    if (threadIdx.x == 0) {
+        st->bad = 0;
+        st->H = a.numharm[c];
+        st->locpow0 = 0.0f;
+        st->maxpow = 0.0f;
+    }
+}
+
+/* mode 0: the local-power interpolations of harmonic n = H - hind at the
+ * start (rin, zin, win) n; mode 1: those of harmonic hind + 1 of the
+ * result (slots 0-19, unless raw) and its 5 derivative interpolations
+ * (slots 20-24, src/characteristics.c:140-192).  One workgroup per
+ * (candidate, slot); each point is formed by the loop arithmetic of
+ * get_localpower3d / get_derivs3d (src/characteristics.c:79-136). */
+__global__ __launch_bounds__(RF_TPB) void k_rzw_points(agpu_refine_args a, RZW_LAYOUT_ARGS,
+                                                       int hind, int mode)
 {
     __shared__ double2 red[RF_TPB];
+    const rzw_ptrs P = RZW_PTRS;
+    const int c = blockIdx.x / RZW_NPT, slot = blockIdx.x % RZW_NPT;
+    rzw_state *st = P.st + c;
+    const int H = st->H;
+    if (st->bad || hind >= H)
+        return;
+    if (mode == 0 && slot >= RZW_NLP)
+        return;
+    if (mode == 1 && slot < RZW_NLP && a.raw)
+        return;
+    double r, z, w;
+    if (mode == 0) {
+        const int n = H - hind;
+        r = a.rin[c] * n;
+        z = a.zin[c] * n;
+        w = a.win[c] * n;
+    } else {
+        const int hh = hind + 1;
+        r = st->rout * hh;
+        z = st->zout * hh;
+        w = st->wout * hh;
+    }
+    const float2 *spec = (const float2 *) a.spec;
+    double2 *fa = rzw_fa(P, a, c, slot), *fb = fa + a.npts_cap;
+    int bad = 0;
+    double2 v = make_double2(0.0, 0.0);
+    int have = 0;
+    if (slot < RZW_NLP) {
+        const int khw = wr_halfwidth(z, w, 0);
+        double lo1 = r - 5 - 10, hi1 = lo1 + 10;
+        double lo2 = r + 5, hi2 = lo2 + 10;
+        const double nd = (double) a.numdata;
+        lo1 = lo1 < 0.0 ? 0.0 : (lo1 > nd ? nd : lo1);
+        lo2 = lo2 < 0.0 ? 0.0 : (lo2 > nd ? nd : lo2);
+        hi1 = hi1 < 0.0 ? 0.0 : (hi1 > nd ? nd : hi1);
+        hi2 = hi2 < 0.0 ? 0.0 : (hi2 > nd ? nd : hi2);
+        /* point slot of the two loops, stepped as they step */
+        int k = 0;
+        for (double f = lo1; f < hi1; f += 1.0, k++)
+            if (k == slot) {
+                v = rzw_interp_wg(spec, a.numdata, f, z, w, khw, red, fa, fb, a.npts_cap, &bad);
+                have = 1;
+            }
+        for (double f = lo2; f < hi2 && !have; f += 1.0, k++)
+            if (k == slot) {
+                v = rzw_interp_wg(spec, a.numdata, f, z, w, khw, red, fa, fb, a.npts_cap, &bad);
+                have = 1;
+            }
+    } else {
+        const double h = 0.003, twoh = 0.006;
+        const int khw = wr_halfwidth(z, w, 1);
+        double f = r - twoh;
+        for (int k = 0; k < 5; k++, f += h)
+            if (k == slot - RZW_NLP)
+                v = rzw_interp_wg(spec, a.numdata, f, z, w, khw, red, fa, fb, a.npts_cap, &bad);
+        have = 1;
+    }
+    if (threadIdx.x == 0) {
+        P.pts[(size_t) c * RZW_NPT + slot] = v;
+        if (bad)
+            st->bad = 1;
+    }
+}
+
+/* local power of the points of slots 0-19 (the loops' sum, over 20) */
+__device__ static double rzw_locpow_sum(const double2 *pts, double r, long long numdata)
+{
+    double lo1 = r - 5 - 10, hi1 = lo1 + 10;
+    double lo2 = r + 5, hi2 = lo2 + 10;
+    const double nd = (double) numdata;
+    lo1 = lo1 < 0.0 ? 0.0 : (lo1 > nd ? nd : lo1);
+    lo2 = lo2 < 0.0 ? 0.0 : (lo2 > nd ? nd : lo2);
+    hi1 = hi1 < 0.0 ? 0.0 : (hi1 > nd ? nd : hi1);
+    hi2 = hi2 < 0.0 ? 0.0 : (hi2 > nd ? nd : hi2);
+    double sum = 0.0;
+    int k = 0;
+    for (double f = lo1; f < hi1; f += 1.0, k++) {
+        const double2 v = pts[k];
+        sum += v.x * v.x + v.y * v.y;
+    }
+    for (double f = lo2; f < hi2; f += 1.0, k++) {
+        const double2 v = pts[k];
+        sum += v.x * v.x + v.y * v.y;
+    }
+    return sum / 20.0;
+}
+
+/* harmonic n = H - hind: local power, lobin, grid, r indices, chirp lengths */
+__global__ __launch_bounds__(RF_TPB) void k_rzw_setup(agpu_refine_args a, RZW_LAYOUT_ARGS,
+                                                      int hind)
+{
+    const rzw_ptrs P = RZW_PTRS;
+    const int c = blockIdx.x;
+    rzw_state *st = P.st + c;
+    const int H = st->H;
+    if (st->bad || hind >= H || threadIdx.x != 0)
+        return;
+    const int n = H - hind;
+    const double rh = a.rin[c] * n, zh = a.zin[c] * n, wh = a.win[c] * n;
+    const double hfrac = n / (double) H;
+    const double dr = 1.0 / RZW_NB;
+    const int numwid = a.raw ? 1 : 2, ng = 2 * numwid * 4 + 1;
+    const float locpow = (float) rzw_locpow_sum(P.pts + (size_t) c * RZW_NPT, rh, a.numdata);
+    if (hind == 0)
+        st->locpow0 = locpow;
+    double rh_int;
+    modf(rh, &rh_int);
+    const long long lobin = (long long) rh_int - 1;
+    if (hind == 0)
+        st->hhlobin = lobin;
+    const long long hhlobin = st->hhlobin;
+    const double zlo = zh - numwid * hfrac * 2.0, zhi = zh + numwid * hfrac * 2.0;
+    const double wlo = wh - numwid * hfrac * 20.0, whi = wh + numwid * hfrac * 20.0;
+    const double dz = (zhi - zlo) / (ng - 1), dw = (whi - wlo) / (ng - 1);
+    const double maxz = (fabs(zlo) < fabs(zhi)) ? zhi : zlo;
+    const double maxw = (fabs(wlo) < fabs(whi)) ? whi : wlo;
+    const int khw = wr_halfwidth(maxz, maxw, 0);
+    const int numkern = 2 * RZW_NB * khw;
+    int bad = 0;
+    for (int kk = 0; kk < RZW_NR; kk++) {
+        const int rk = (int) llround(((hhlobin + kk * dr) * hfrac - lobin) * RZW_NB);
+        st->rind[kk] = rk;
+        if (rk < -RZW_NB || rzw_window(rk) > RZW_NC - 2)
+            bad = 1;
+    }
+    /* the distinct chirp lengths of the grid, ascending */
+    int nvar = 0;
+    for (int wi = 0; wi < ng && !bad; wi++)
+        for (int zi = 0; zi < ng; zi++) {
+            const double z = zlo + zi * dz, w = wlo + wi * dw;
+            if (fabs(w) < 1e-4)
+                continue;
+            const int npts = (int) next2_dev(6 * wr_halfwidth(z, w, 0) + 200 + numkern / RZW_NB);
+            if (npts > a.npts_cap) {
+                bad = 1;
+                break;
+            }
+            int at = 0;
+            while (at < nvar && st->npts[at] < npts)
+                at++;
+            if (at < nvar && st->npts[at] == npts)
+                continue;
+            for (int q = nvar; q > at; q--)
+                st->npts[q] = st->npts[q - 1];
+            st->npts[at] = npts;
+            nvar++;
+        }
+    st->nvar = nvar;
+    st->khw = khw;
+    st->numkern = numkern;
+    st->ng = ng;
+    st->lobin = lobin;
+    st->zlo = zlo;
+    st->dz = dz;
+    st->wlo = wlo;
+    st->dw = dw;
+    st->locpow = locpow;
+    if (bad)
+        st->bad = 1;
+}
+
+/* offset (complex values) of chirp-length variant v's Q tables */
+__device__ static size_t rzw_qoff(const rzw_state *st, int v)
+{
+    size_t o = 0;
+    for (int u = 0; u < v; u++)
+        o += (size_t) RZW_NR * st->npts[u];
+    return o;
+}
+
+__global__ __launch_bounds__(RF_TPB) void k_rzw_qtab(agpu_refine_args a, RZW_LAYOUT_ARGS,
+                                                     int hind)
+{
+    const rzw_ptrs P = RZW_PTRS;
+    const int c = blockIdx.x / RZW_NV, v = blockIdx.x % RZW_NV;
+    const rzw_state *st = P.st + c;
+    if (st->bad || hind >= st->H || v >= st->nvar)
+        return;
+    const float2 *spec = (const float2 *) a.spec;
+    const long long numdata = a.numdata, lobin = st->lobin;
+    const int khw = st->khw, npts = st->npts[v];
+    double2 *fa = rzw_fa(P, a, c, v), *fb = fa + a.npts_cap;
+    double2 *Q = P.q + (size_t) c * RZW_NR * 2 * a.npts_cap + rzw_qoff(st, v);
+    const double2 *D = window_spectrum(spec, numdata, lobin, -1 - khw, khw - 1, npts, fa, fb);
+    for (int nn = threadIdx.x; nn < npts; nn += RF_TPB) {
+        double2 S[RZW_NC];
+        S[0] = make_double2(D[nn].x, -D[nn].y);
+        for (int cc = 0; cc < RZW_NC - 1; cc++) {
+            double2 e0 = window_dft(spec, numdata, lobin, cc - 1 - khw, cc - khw, nn, npts);
+            double2 e1 = window_dft(spec, numdata, lobin, khw + cc - 1, khw + cc, nn, npts);
+            S[cc + 1] = make_double2(S[cc].x - e0.x + e1.x, S[cc].y - e0.y + e1.y);
+        }
+        const double2 ph = ipow(nn);
+        for (int kk = 0; kk < RZW_NR; kk++) {
+            const int j = st->rind[kk];
+            const int cw = rzw_window(j);
+            const long long q = ((long long) j * nn) % (RZW_NB * (long long) npts);
+            double sv, cv;
+            sincospi(-2.0 * (double) q / (RZW_NB * (double) npts), &sv, &cv);
+            const double er = ph.x * cv - ph.y * sv, ei = ph.x * sv + ph.y * cv;
+            const double2 Sc = S[cw + 1];
+            Q[(size_t) kk * npts + nn] =
+                make_double2(er * Sc.x - ei * Sc.y, er * Sc.y + ei * Sc.x);
+        }
+    }
+}
+
+/* one (w, z) grid point of harmonic n = H - hind: powsum's term
+ * |vol|^2 / locpow (src/maximize_rzw.c:145-153), the raw |vol|^2 for
+ * max_rzw_arr (:24-37); the reference's volume is FP32 */
+__global__ __launch_bounds__(RF_TPB) void k_rzw_grid(agpu_refine_args a, RZW_LAYOUT_ARGS,
+                                                     int hind)
+{
     __shared__ double2 red4[(RF_TPB / 64) * RZW_NR];
     __shared__ double2 vol[RZW_NR];
-    __shared__ float powsum[RZW_NZ * RZW_NZ * RZW_NR];
-    __shared__ int rind[RZW_NR];
-    const int ci = blockIdx.x;
-    const int H = a.numharm[ci];
+    const rzw_ptrs P = RZW_PTRS;
+    const int c = blockIdx.x / (RZW_NZ * RZW_NZ), g = blockIdx.x % (RZW_NZ * RZW_NZ);
+    const int wi = g / RZW_NZ, zi = g % RZW_NZ;
+    const rzw_state *st = P.st + c;
+    if (st->bad || hind >= st->H || wi >= st->ng || zi >= st->ng)
+        return;
     const float2 *spec = (const float2 *) a.spec;
-    const long long numdata = a.numdata;
-    const double rin = a.rin[ci], zin = a.zin[ci], win = a.win[ci];
-    /* scratch of this candidate: Q tables [RZW_NR][cap], two FFT buffers */
-    double2 *Q = (double2 *) a.scratch + (size_t) ci * RZW_SCR * a.npts_cap;
-    double2 *fa = Q + (size_t) RZW_NR * a.npts_cap, *fb = fa + a.npts_cap;
-    const double dr = 1.0 / RZW_NB;
-    for (int i = threadIdx.x; i < RZW_NZ * RZW_NZ * RZW_NR; i += RF_TPB)
-        powsum[i] = 0.0f;
-    __syncthreads();
-    long long hhlobin = 0;
-    int bad = 0;
-    /* grid half-width in peak widths: 2 (harmonics), 1 (max_rzw_arr) */
-    const int numwid = a.raw ? 1 : 2, ng = 2 * numwid * 4 + 1;
-    float locpow0 = 0.0f;
-    for (int hind = 0; hind < H && !bad; hind++) {
-        const int n = H - hind;
-        const double rh = rin * n, zh = zin * n, wh = win * n;
-        const double hfrac = n / (double) H;
-        const float locpow =
-            (float) localpower3d_wg(spec, numdata, rh, zh, wh, red, fa, fb, a.npts_cap, &bad);
-        if (hind == 0)
-            locpow0 = locpow;
-        double rh_int;
-        modf(rh, &rh_int);
-        const long long lobin = (long long) rh_int - 1;
-        if (hind == 0)
-            hhlobin = lobin;
-        double zlo = zh - numwid * hfrac * 2.0, zhi = zh + numwid * hfrac * 2.0;
-        double wlo = wh - numwid * hfrac * 20.0, whi = wh + numwid * hfrac * 20.0;
-        const double dz = (zhi - zlo) / (ng - 1), dw = (whi - wlo) / (ng - 1);
-        const double maxz = (fabs(zlo) < fabs(zhi)) ? zhi : zlo;
-        const double maxw = (fabs(wlo) < fabs(whi)) ? whi : wlo;
-        const int khw = wr_halfwidth(maxz, maxw, 0);
-        const int numkern = 2 * RZW_NB * khw;
-        if (threadIdx.x < RZW_NR)
-            rind[threadIdx.x] =
-                (int) llround(((hhlobin + threadIdx.x * dr) * hfrac - lobin) * RZW_NB);
-        __syncthreads();
+    const long long numdata = a.numdata, lobin = st->lobin;
+    const int khw = st->khw, numkern = st->numkern;
+    const double z = st->zlo + zi * st->dz, w = st->wlo + wi * st->dw;
+    double2 part[RZW_NR];
+    if (fabs(w) < 1e-4) {
+        /* z-response template: direct sums with Fresnel taps, window
+         * [ceil(j/8) - khw, ceil(j/8) + khw) around lobin */
+        for (int kk = 0; kk < RZW_NR; kk++) {
+            const int j = st->rind[kk];
+            const int cw = rzw_window(j);
+            double2 acc = make_double2(0.0, 0.0);
+            for (int m = cw - khw + (int) threadIdx.x; m < cw + khw; m += RF_TPB) {
+                const long long k = lobin + m;
+                if (k < 0 || k >= numdata)
+                    continue;
+                const int idx = numkern / 2 + m * RZW_NB - j;
+                const float2 q = resp_tap_nb(0.0, RZW_NB, z, numkern, idx);
+                const float2 d = spec[k];
+                acc.x += (double) d.x * q.x + (double) d.y * q.y;
+                acc.y += (double) d.y * q.x - (double) q.y * d.x;
+            }
+            part[kk] = acc;
+        }
+    } else {
+        const int npts = (int) next2_dev(6 * wr_halfwidth(z, w, 0) + 200 + numkern / RZW_NB);
+        int v = 0;
+        while (v < st->nvar && st->npts[v] != npts)
+            v++;
+        if (v == st->nvar)
+            return;   /* not reached: setup listed every length of the grid */
+        const double2 *Q = P.q + (size_t) c * RZW_NR * 2 * a.npts_cap + rzw_qoff(st, v);
         for (int kk = 0; kk < RZW_NR; kk++)
-            if (rind[kk] < -RZW_NB || rzw_window(rind[kk]) > RZW_NC - 2)
-                bad = 1;
-        if (bad)
-            break;
-        int npts_cur = -1;   /* npts of the Q tables in scratch */
-        for (int wi = 0; wi < ng && !bad; wi++) {
-            for (int zi = 0; zi < ng; zi++) {
-                const double z = zlo + zi * dz, w = wlo + wi * dw;
-                if (fabs(w) < 1e-4) {
-                    /* z-response template: direct sums with Fresnel taps,
-                     * window [ceil(j/8) - khw, ceil(j/8) + khw) around lobin */
-                    double2 part[RZW_NR];
-                    for (int kk = 0; kk < RZW_NR; kk++) {
-                        const int j = rind[kk];
-                        const int c = rzw_window(j);
-                        double2 acc = make_double2(0.0, 0.0);
-                        for (int m = c - khw + (int) threadIdx.x; m < c + khw; m += RF_TPB) {
-                            const long long k = lobin + m;
-                            if (k < 0 || k >= numdata)
-                                continue;
-                            const int idx = numkern / 2 + m * RZW_NB - j;
-                            const float2 q = resp_tap_nb(0.0, RZW_NB, z, numkern, idx);
-                            const float2 d = spec[k];
-                            acc.x += (double) d.x * q.x + (double) d.y * q.y;
-                            acc.y += (double) d.y * q.x - (double) q.y * d.x;
-                        }
-                        part[kk] = acc;
-                    }
-                    wg_sum_vec(part, vol, red4);
-                } else {
-                    const int npts =
-                        (int) next2_dev(6 * wr_halfwidth(z, w, 0) + 200 + numkern / RZW_NB);
-                    if (npts > a.npts_cap) {
-                        bad = 1;
-                        break;
-                    }
-                    if (npts != npts_cur) {
-                        /* Q_kk(n) = i^n e^{-2 pi i j n / (8 npts)} S_c(n), S_c the
-                         * DFT of the data window [c - khw, c + khw) around lobin,
-                         * c = ceil(j / 8) in [-1, 5] */
-                        const double2 *D = window_spectrum(spec, numdata, lobin, -1 - khw,
-                                                           khw - 1, npts, fa, fb);
-                        for (int nn = threadIdx.x; nn < npts; nn += RF_TPB) {
-                            double2 S[RZW_NC];
-                            S[0] = make_double2(D[nn].x, -D[nn].y);
-                            for (int c = 0; c < RZW_NC - 1; c++) {
-                                double2 e0 = window_dft(spec, numdata, lobin, c - 1 - khw,
-                                                        c - khw, nn, npts);
-                                double2 e1 = window_dft(spec, numdata, lobin, khw + c - 1,
-                                                        khw + c, nn, npts);
-                                S[c + 1] = make_double2(S[c].x - e0.x + e1.x,
-                                                        S[c].y - e0.y + e1.y);
-                            }
-                            const double2 ph = ipow(nn);
-                            for (int kk = 0; kk < RZW_NR; kk++) {
-                                const int j = rind[kk];
-                                const int c = rzw_window(j);
-                                const long long q = ((long long) j * nn) % (RZW_NB * (long long) npts);
-                                double sv, cv;
-                                sincospi(-2.0 * (double) q / (RZW_NB * (double) npts), &sv, &cv);
-                                const double er = ph.x * cv - ph.y * sv, ei = ph.x * sv + ph.y * cv;
-                                const double2 Sc = S[c + 1];
-                                Q[(size_t) kk * a.npts_cap + nn] =
-                                    make_double2(er * Sc.x - ei * Sc.y, er * Sc.y + ei * Sc.x);
-                            }
-                        }
-                        __syncthreads();
-                        npts_cur = npts;
-                    }
-                    double2 part[RZW_NR];
-                    for (int kk = 0; kk < RZW_NR; kk++)
-                        part[kk] = make_double2(0.0, 0.0);
-                    for (int nn = threadIdx.x; nn < npts; nn += RF_TPB) {
-                        const double x = chirp_x(nn, npts, 0.0, z, w);
-                        for (int kk = 0; kk < RZW_NR; kk++) {
-                            const double2 q = Q[(size_t) kk * a.npts_cap + nn];
-                            part[kk].x += x * q.x;
-                            part[kk].y += x * q.y;
-                        }
-                    }
-                    wg_sum_vec(part, vol, red4);
-                }
-                /* powsum += |vol|^2 / locpow (src/maximize_rzw.c:145-153),
-                 * the raw |vol|^2 for max_rzw_arr (:24-37); the reference's
-                 * volume is FP32 */
-                if (threadIdx.x < RZW_NR) {
-                    const int kk = threadIdx.x;
-                    const float vr = (float) vol[kk].x, vi = (float) vol[kk].y;
-                    float *ps = &powsum[(wi * RZW_NZ + zi) * RZW_NR + kk];
-                    const double pw = (double) vr * vr + (double) vi * vi;
-                    *ps += a.raw ? pw : pw / locpow;
-                }
-                __syncthreads();
+            part[kk] = make_double2(0.0, 0.0);
+        for (int nn = threadIdx.x; nn < npts; nn += RF_TPB) {
+            const double x = chirp_x(nn, npts, 0.0, z, w);
+            for (int kk = 0; kk < RZW_NR; kk++) {
+                const double2 q = Q[(size_t) kk * npts + nn];
+                part[kk].x += x * q.x;
+                part[kk].y += x * q.y;
             }
         }
-        __syncthreads();
     }
-    if (bad) {
-        if (threadIdx.x == 0) {
-            a.rout[ci] = NAN;
-            a.nevals[ci] = -1;
-        }
+    wg_sum_vec(part, vol, red4);
+    if (threadIdx.x < RZW_NR) {
+        const int kk = threadIdx.x;
+        const float vr = (float) vol[kk].x, vi = (float) vol[kk].y;
+        const double pw = (double) vr * vr + (double) vi * vi;
+        P.terms[((size_t) c * RF_MAXH + hind) * RZW_CELLS + (wi * RZW_NZ + zi) * RZW_NR + kk] =
+            a.raw ? pw : pw / st->locpow;
+    }
+}
+
+/* powsum in the reference's harmonic order, first maximum in (w, z, r)
+ * order (src/maximize_rzw.c:160-173); cells outside an ng x ng grid stay
+ * 0 and never win */
+__global__ __launch_bounds__(RF_TPB) void k_rzw_pick(agpu_refine_args a, RZW_LAYOUT_ARGS)
+{
+    __shared__ float powsum[RZW_CELLS];
+    const rzw_ptrs P = RZW_PTRS;
+    const int c = blockIdx.x;
+    rzw_state *st = P.st + c;
+    if (st->bad)
         return;
+    const int H = st->H;
+    const double *terms = P.terms + (size_t) c * RF_MAXH * RZW_CELLS;
+    for (int i = threadIdx.x; i < RZW_CELLS; i += RF_TPB) {
+        float ps = 0.0f;
+        for (int hind = 0; hind < H; hind++)
+            ps += terms[(size_t) hind * RZW_CELLS + i];
+        powsum[i] = ps;
     }
-    /* argmax in (w, z, r) order, first maximum wins (src/maximize_rzw.c:160-173);
-     * cells outside an ng x ng grid stay 0 and never win */
+    __syncthreads();
+    if (threadIdx.x != 0)
+        return;
     float maxpow = 0.0f;
     int best = 0;
-    for (int i = 0; i < RZW_NZ * RZW_NZ * RZW_NR; i++)
+    for (int i = 0; i < RZW_CELLS; i++)
         if (powsum[i] > maxpow) {
             maxpow = powsum[i];
             best = i;
         }
+    const int numwid = a.raw ? 1 : 2;
     const int wind = best / (RZW_NZ * RZW_NR), zind = (best / RZW_NR) % RZW_NZ,
               rix = best % RZW_NR;
-    const double nh = H;
-    const double rout = (hhlobin + rix * dr) / nh;
-    const double zout = ((zin * nh) - numwid * 2.0 + zind * 2.0 / 4.0) / nh;
-    const double wout = ((win * nh) - numwid * 20.0 + wind * 20.0 / 4.0) / nh;
-    for (int k = 0; k < H; k++) {
-        const int hh = k + 1;
-        const double rr = rout * hh, zz = zout * hh, ww = wout * hh;
-        const float lp = a.raw ? locpow0
-                               : (float) localpower3d_wg(spec, numdata, rr, zz, ww, red, fa, fb,
-                                                         a.npts_cap, &bad);
-        double d7[7];
-        derivs3d_wg(spec, numdata, rr, zz, ww, lp, d7, red, fa, fb, a.npts_cap, &bad);
-        if (bad)
-            break;
-        if (threadIdx.x == 0) {
-            a.powers[(size_t) ci * RF_MAXH + k] = a.raw ? (double) maxpow : d7[0];
-            for (int j = 0; j < 7; j++)
-                a.derivs[((size_t) ci * RF_MAXH + k) * 7 + j] = d7[j];
-        }
-    }
-    if (threadIdx.x == 0) {
-        a.rout[ci] = bad ? NAN : rout;
-        a.zout[ci] = zout;
-        a.wout[ci] = wout;
-        a.nevals[ci] = bad ? -1 : 0;
-    }
+    const double nh = H, dr = 1.0 / RZW_NB;
+    const double zin = a.zin[c], win = a.win[c];
+    st->rout = (st->hhlobin + rix * dr) / nh;
+    st->zout = ((zin * nh) - numwid * 2.0 + zind * 2.0 / 4.0) / nh;
+    st->wout = ((win * nh) - numwid * 20.0 + wind * 20.0 / 4.0) / nh;
+    st->maxpow = maxpow;
 }
 
-int agpu_refine_rzw(const agpu_refine_args *args, void *stream)
+/* harmonic k + 1 of the result: local power (or the start's, raw), power
+ * and the 5-point derivatives (src/characteristics.c:140-192) */
+__global__ void k_rzw_derivs(agpu_refine_args a, RZW_LAYOUT_ARGS, int k)
+{
+    const rzw_ptrs P = RZW_PTRS;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.ncand)
+        return;
+    const rzw_state *st = P.st + c;
+    if (st->bad || k >= st->H)
+        return;
+    const double PI = 3.14159265358979323846, TWOPI = 6.28318530717958647692;
+    const double twoh = 0.006, twohsqrd = 0.000036;
+    const int hh = k + 1;
+    const double2 *pts = P.pts + (size_t) c * RZW_NPT;
+    const float lp = a.raw ? st->locpow0
+                           : (float) rzw_locpow_sum(pts, st->rout * hh, a.numdata);
+    const double locpow = lp;
+    double pwr[5], phs[5];
+    for (int q = 0; q < 5; q++) {
+        const double2 v = pts[RZW_NLP + q];
+        pwr[q] = v.x * v.x + v.y * v.y;
+        double t = atan2(v.y, v.x);
+        phs[q] = t > 0.0 ? t : t + TWOPI;
+    }
+    for (int q = 0; q < 4; q++)
+        if (fabs(phs[q + 1] - phs[q]) > PI)
+            phs[q + 1] -= TWOPI;
+    double d7[7];
+    d7[0] = pwr[2];
+    d7[1] = phs[2];
+    d7[2] = (pwr[3] - pwr[1]) / twoh;
+    d7[3] = (phs[3] - phs[1]) / twoh;
+    d7[4] = (pwr[4] - 2.0 * pwr[2] + pwr[0]) / twohsqrd;
+    d7[5] = (phs[4] - 2.0 * phs[2] + phs[0]) / twohsqrd;
+    d7[6] = locpow;
+    a.powers[(size_t) c * RF_MAXH + k] = a.raw ? (double) st->maxpow : d7[0];
+    for (int j = 0; j < 7; j++)
+        a.derivs[((size_t) c * RF_MAXH + k) * 7 + j] = d7[j];
+}
+
+__global__ void k_rzw_done(agpu_refine_args a, RZW_LAYOUT_ARGS)
+{
+    const rzw_ptrs P = RZW_PTRS;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.ncand)
+        return;
+    const rzw_state *st = P.st + c;
+    a.rout[c] = st->bad ? NAN : st->rout;
+    a.zout[c] = st->zout;
+    a.wout[c] = st->wout;
+    a.nevals[c] = st->bad ? -1 : 0;
+}
+
+size_t agpu_refine_rzw_scratch(int ncand, int npts_cap)
+{
+    return rzw_layout_of(ncand, npts_cap).total;
+}
+
+int agpu_refine_rzw(const agpu_refine_args *args, void *stream, int maxharm)
 {
     if (args->ncand <= 0)
         return 0;
-    if (args->ncand > 1 << 20 || args->npts_cap <= 0) {
-        agpu_set_error("agpu_refine_rzw: bad candidate count or scratch size");
+    if (args->ncand > 1 << 16 || args->npts_cap <= 0 || maxharm < 1 || maxharm > RF_MAXH) {
+        agpu_set_error("agpu_refine_rzw: bad candidate count, harmonic count or scratch size");
         return -2;
     }
-    hipLaunchKernelGGL(k_refine_rzw, dim3(args->ncand), dim3(RF_TPB), 0, (hipStream_t) stream,
-                       *args);
+    const agpu_refine_args a = *args;
+    const rzw_layout l = rzw_layout_of(a.ncand, a.npts_cap);
+    hipStream_t st = (hipStream_t) stream;
+    const int m = a.ncand, nb1 = (m + 63) / 64;
+#define RZW_L l.st, l.pts, l.terms, l.q, l.fft
+    hipLaunchKernelGGL(k_rzw_init, dim3(m), dim3(RF_TPB), 0, st, a, RZW_L);
+    for (int hind = 0; hind < maxharm; hind++) {
+        hipLaunchKernelGGL(k_rzw_points, dim3(m * RZW_NPT), dim3(RF_TPB), 0, st, a, RZW_L, hind, 0);
+        hipLaunchKernelGGL(k_rzw_setup, dim3(m), dim3(64), 0, st, a, RZW_L, hind);
+        hipLaunchKernelGGL(k_rzw_qtab, dim3(m * RZW_NV), dim3(RF_TPB), 0, st, a, RZW_L, hind);
+        hipLaunchKernelGGL(k_rzw_grid, dim3(m * RZW_NZ * RZW_NZ), dim3(RF_TPB), 0, st, a, RZW_L,
+                           hind);
+    }
+    hipLaunchKernelGGL(k_rzw_pick, dim3(m), dim3(RF_TPB), 0, st, a, RZW_L);
+    for (int k = 0; k < maxharm; k++) {
+        hipLaunchKernelGGL(k_rzw_points, dim3(m * RZW_NPT), dim3(RF_TPB), 0, st, a, RZW_L, k, 1);
+        hipLaunchKernelGGL(k_rzw_derivs, dim3(nb1), dim3(64), 0, st, a, RZW_L, k);
+    }
+    hipLaunchKernelGGL(k_rzw_done, dim3(nb1), dim3(64), 0, st, a, RZW_L);
+#undef RZW_L
     RF_CHECK(hipGetLastError());
     return 0;
 }
