@@ -383,9 +383,10 @@ int accel_server_wait(accel_server *sv, int ticket, double *latency_ms, double *
 int accel_server_cands(accel_server *sv, int ticket, accel_cand *out, int n);
 int accel_server_props(accel_server *sv, int ticket, accel_fourierprops *out, int n);
 /* Where a finished request's time went (ms): queued, evicting cached
- * handles, creating its handle, preparation + search + refinement, the
- * whole service; then its device-memory footprint (bytes). */
-int accel_server_timing(accel_server *sv, int ticket, double *ms6);
+ * handles, creating its handle, preparation (upload, real FFT,
+ * de-reddening), search, refinement, the last three together, the whole
+ * service; then its device-memory footprint (bytes). */
+int accel_server_timing(accel_server *sv, int ticket, double *ms9);
 /* Device memory held by live handles (running and cached). */
 long long accel_server_memory(accel_server *sv);
 /* Finishes every queued request, then stops the workers. */
@@ -538,6 +539,41 @@ int accel_sift_write_report(const accel_sift *s, const char *path);
 int accel_sift_write_summary(const accel_sift *s, const char *path);
 
 const char *accel_last_error(void);
+
+
+/* ---- the .cand consumer: prepfold -accelcand on a .dat ---- */
+
+/* foldstats (include/presto.h:261-269), member for member. */
+typedef struct {
+    double numdata, data_avg, data_var, numprof, prof_avg, prof_var, redchi;
+} accel_foldstats;
+
+/* get_rzw_cand (src/get_candidates.c:27-37): record candnum (from 1) of a
+ * .cand file of 88-byte fourierprops.  Returns 0, or -1 when the file or
+ * the record is missing. */
+int accel_read_rzw_cand(const char *candfile, int candnum, accel_fourierprops *out);
+/* The start frequency and derivatives prepfold folds a candidate with
+ * (src/prepfold_pipeline.c:759-767): the .cand holds the average r, z
+ * over an observation of T seconds. */
+void accel_cand_fold_params(const accel_fourierprops *c, double T, double *f, double *fd,
+                            double *fdd);
+/* prepfold's default profile length for a period of 1/f s sampled every
+ * dt s (src/prepfold_pipeline.c:345-358): one bin per sample, at most 64. */
+int accel_fold_proflen(double f, double dt);
+/* Folds n samples (host memory) on the GPU as prepfold folds a .dat
+ * (src/prepfold_pipeline.c:514-551, 1375-1594; src/fold.c:46-107,
+ * 443-653): records of 1024 points, the number of records rounded down to
+ * a multiple of npart, every point drizzled over the profile bins its phase
+ * interval covers, a sub-integration's profile holding the full turns of
+ * phase completed during it.  rawfolds: npart x proflen; parts: npart
+ * statistics as fold() leaves them; total: the summed profile's (its
+ * redchi = sum (prof - avg)^2 / (prof_var (proflen - 1)), prof_var =
+ * data_var numdata / proflen, src/fold.c:627-640).  FP64 phases and sums;
+ * the reference keeps phases in long double.  Returns the number of points
+ * folded, or -1. */
+long long accel_fold(const float *series, long long n, double dt, double f, double fd,
+                     double fdd, int npart, int proflen, double *rawfolds,
+                     accel_foldstats *parts, accel_foldstats *total, int device);
 
 #ifdef __cplusplus
 }

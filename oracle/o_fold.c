@@ -1,0 +1,136 @@
+/*
+ * o_fold.c -- oracle restatement of prepfold's fold of a .dat time series
+ * for an ACCEL candidate (prepfold -accelcand, topocentric float data):
+ * src/fold.c:46-107 (add_to_prof, the "drizzling" of finite-duration
+ * samples into profile bins through a buffer that is dumped once a full
+ * turn of phase has been added), src/fold.c:443-653 (fold(), flags 0, in
+ * long double as there), src/prepfold_pipeline.c:514-551 and 1375-1594
+ * (records of SUBSBLOCKLEN = 1024 points, npart sub-integrations of
+ * reads_per_part records, buffers and phases carried across calls), and
+ * src/prepfold_pipeline.c:750-767 (the start f, fdot, fdotdot of a
+ * candidate's average r, z, w).  Test infrastructure only (see oracle.h).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+#define O_SUBSBLOCKLEN 1024
+#define O_TEST_ONE(a) (fabs((a) - 1.0) < 1e-12)
+
+/* src/fold.c:46-107 */
+static void add_to_prof(double *prof, double *buffer, int n, double lophase,
+                        double deltaphase, double dataval, double *phaseadded)
+{
+    const double binw = 1.0 / n;
+    const double valperphs = dataval / deltaphase;
+    double curbin = lophase * n;
+    while (deltaphase > 1e-12) {
+        int ib = (int) floor(curbin + 1e-12);
+        double dphs = ((ib + 1.0) - curbin) * binw;
+        ib %= n;
+        if (dphs > deltaphase)
+            dphs = deltaphase;
+        const double room = 1.0 - *phaseadded;
+        double part = room > dphs ? dphs : room;
+        buffer[ib] += part * valperphs;
+        *phaseadded += part;
+        if (O_TEST_ONE(*phaseadded)) {
+            for (int k = 0; k < n; k++) {
+                prof[k] += buffer[k];
+                buffer[k] = 0.0;
+            }
+            part = dphs - room;
+            buffer[ib] += part * valperphs;
+            *phaseadded = part;
+        }
+        deltaphase -= dphs;
+        curbin += dphs * n;
+    }
+}
+
+/* src/fold.c:443-653 with flags = 0 (no delays, no on/off pairs) and the
+ * standard drizzling; stats accumulate across calls as there */
+static double fold_block(const float *data, int numdata, double dt, double tlo, double *prof,
+                         int numprof, double startphs, double *buffer, double *phaseadded,
+                         double fo, double fdot, double fdotdot, o_foldstats *st)
+{
+    long double phase, phasenext = 0.0, deltaphase, T, Tnext, lophase, hiphase;
+    double dtmp;
+    fdot /= 2.0;
+    fdotdot /= 6.0;
+    st->numprof = (double) numprof;
+    st->data_var *= (st->numdata - 1.0);
+    T = tlo;
+    phase = T * (T * (T * fdotdot + fdot) + fo) + startphs;
+    lophase = (phase < 0.0) ? 1.0 + modf((double) phase, &dtmp) : modf((double) phase, &dtmp);
+    for (int ii = 0; ii < numdata; ii++) {
+        Tnext = tlo + (ii + 1) * dt;
+        phasenext = Tnext * (Tnext * (Tnext * fdotdot + fdot) + fo) + startphs;
+        deltaphase = phasenext - phase;
+        add_to_prof(prof, buffer, numprof, (double) lophase, (double) deltaphase, data[ii],
+                    phaseadded);
+        hiphase = lophase + deltaphase;
+        lophase = hiphase - (int) hiphase;
+        phase = phasenext;
+        st->numdata += 1.0;
+        const double dev = data[ii] - st->data_avg;
+        st->data_avg += dev / st->numdata;
+        st->data_var += dev * (data[ii] - st->data_avg);
+    }
+    st->prof_avg = 0.0;
+    for (int k = 0; k < numprof; k++)
+        st->prof_avg += prof[k];
+    st->prof_avg /= numprof;
+    st->redchi = 0.0;
+    for (int k = 0; k < numprof; k++) {
+        dtmp = prof[k] - st->prof_avg;
+        st->redchi += dtmp * dtmp;
+    }
+    st->data_var /= (st->numdata - 1.0);
+    st->prof_var = st->data_var * st->numdata / numprof;
+    st->redchi /= (st->prof_var * (numprof - 1));
+    phasenext = (phasenext < 0.0) ? 1.0 + phasenext - (int) phasenext
+                                  : phasenext - (int) phasenext;
+    return (double) phasenext;
+}
+
+/* src/prepfold_pipeline.c:750-767 */
+void o_cand_fold_params(double r, double z, double w, double T, double *f, double *fd,
+                        double *fdd)
+{
+    const double z0 = z - 0.5 * w;
+    const double r0 = r - 0.5 * z0 - w / 6.0;
+    *f = r0 / T;
+    *fd = z0 / (T * T);
+    *fdd = w / (T * T * T);
+}
+
+/* The fold of fold_timeseries() for a .dat of n floats: the records used
+ * (numrec = n / 1024 rounded down to a multiple of npart), every record
+ * folded at its own start time, one profile and statistics per part.
+ * rawfolds: npart x proflen (zeroed here); stats: npart.  Returns the
+ * number of points folded. */
+long long o_fold_dat(const float *series, long long n, double dt, double f, double fd,
+                     double fdd, int npart, int proflen, double *rawfolds, o_foldstats *stats)
+{
+    const long long numrec0 = n / O_SUBSBLOCKLEN;
+    const long long rpp = numrec0 / npart;
+    double *buffer = (double *) calloc(proflen, sizeof(double));
+    double phaseadded = 0.0;
+    memset(rawfolds, 0, sizeof(double) * (size_t) npart * proflen);
+    memset(stats, 0, sizeof(o_foldstats) * (size_t) npart);
+    const double proftime = O_SUBSBLOCKLEN * dt;
+    long long done = 0;
+    for (int ii = 0; ii < npart; ii++) {
+        const double parttime = ii * rpp * proftime;
+        for (long long jj = 0; jj < rpp; jj++) {
+            const double t0 = parttime + jj * proftime;
+            fold_block(series + done, O_SUBSBLOCKLEN, dt, t0, rawfolds + (size_t) ii * proflen,
+                       proflen, 0.0, buffer, &phaseadded, f, fd, fdd, &stats[ii]);
+            done += O_SUBSBLOCKLEN;
+        }
+    }
+    free(buffer);
+    return done;
+}

@@ -225,4 +225,14 @@ double o_max_rzw_arr(const o_fcomplex *data, long numdata, double rin, double zi
 #ifdef __cplusplus
 }
 #endif
+
+/* prepfold's fold of a .dat for an ACCEL candidate (o_fold.c) */
+typedef struct {
+    double numdata, data_avg, data_var, numprof, prof_avg, prof_var, redchi;
+} o_foldstats;
+void o_cand_fold_params(double r, double z, double w, double T, double *f, double *fd,
+                        double *fdd);
+long long o_fold_dat(const float *series, long long n, double dt, double f, double fd,
+                     double fdd, int npart, int proflen, double *rawfolds, o_foldstats *stats);
+
 #endif

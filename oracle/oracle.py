@@ -480,3 +480,32 @@ def corr_loc_pow(powers):
     out = np.zeros_like(p)
     lib.o_corr_loc_pow(p.ctypes.data_as(C.POINTER(_f)), len(p), out.ctypes.data_as(C.POINTER(_f)))
     return out
+
+
+class OFoldStats(C.Structure):
+    _fields_ = [(n, _d) for n in ("numdata", "data_avg", "data_var", "numprof", "prof_avg",
+                                  "prof_var", "redchi")]
+
+
+_sig("o_cand_fold_params", None, _d, _d, _d, _d, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d))
+_sig("o_fold_dat", _ll, C.POINTER(_f), _ll, _d, _d, _d, _d, _i, _i, C.POINTER(_d),
+     C.POINTER(OFoldStats))
+
+
+def cand_fold_params(r, z, w, T):
+    """prepfold's start f, fdot, fdotdot of a candidate's average r, z, w."""
+    f, fd, fdd = _d(), _d(), _d()
+    lib.o_cand_fold_params(r, z, w, T, C.byref(f), C.byref(fd), C.byref(fdd))
+    return f.value, fd.value, fdd.value
+
+
+def fold_dat(series, dt, f, fd, fdd, npart=64, proflen=64):
+    """prepfold's fold of a .dat: (rawfolds [npart][proflen], per-part
+    stats as dicts, points folded)."""
+    x = np.ascontiguousarray(series, dtype=np.float32)
+    prof = np.zeros((npart, proflen), dtype=np.float64)
+    st = (OFoldStats * npart)()
+    n = lib.o_fold_dat(x.ctypes.data_as(C.POINTER(_f)), len(x), dt, f, fd, fdd, npart, proflen,
+                       prof.ctypes.data_as(C.POINTER(_d)), st)
+    stats = [{k: getattr(s, k) for k, _ in OFoldStats._fields_} for s in st]
+    return prof, stats, int(n)

@@ -160,7 +160,8 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_sift_parse_power", "accel_sift_reject", "accel_sift_remove_duplicates",
             "accel_sift_remove_dm_problems", "accel_sift_remove_harmonics",
             "accel_sift_sort_sigma", "accel_sift_extend", "accel_sift_count", "accel_sift_get",
-            "accel_sift_write", "accel_sift_write_report", "accel_sift_write_summary"]
+            "accel_sift_write", "accel_sift_write_report", "accel_sift_write_summary",
+            "accel_read_rzw_cand", "accel_cand_fold_params", "accel_fold_proflen", "accel_fold"]
 
 
 class AccelError(RuntimeError):
@@ -506,11 +507,13 @@ class Server:
         return lib.accel_server_memory(self.h)
 
     def timing(self, ticket):
-        """dict of queued, evict, create, work, service (ms) and footprint bytes"""
-        v = np.zeros(6)
+        """dict of queued, evict, create, prep, search, refine, work, service
+        (ms) and footprint bytes"""
+        v = np.zeros(9)
         if lib.accel_server_timing(self.h, ticket, v.ctypes.data):
             raise AccelError("request not finished")
-        return dict(zip(("queued", "evict", "create", "work", "service", "bytes"), v))
+        return dict(zip(("queued", "evict", "create", "prep", "search", "refine", "work",
+                         "service", "bytes"), v))
 
     def close(self):
         if self.h:
@@ -759,3 +762,52 @@ def spectrum_from_series(x, dt):
     if lib.accel_prepare_dat_spectrum(out.ctypes.data, n2, float(len(x)), float(dt)) < 0:
         raise AccelError("deredden failed (spectrum too short)")
     return out
+
+
+class FoldStats(C.Structure):
+    """foldstats (include/presto.h:261-269)"""
+    _fields_ = [(n, _d) for n in ("numdata", "data_avg", "data_var", "numprof", "prof_avg",
+                                  "prof_var", "redchi")]
+
+
+_sig("accel_read_rzw_cand", _i, C.c_char_p, _i, C.POINTER(FourierProps))
+_sig("accel_cand_fold_params", None, C.POINTER(FourierProps), _d, C.POINTER(_d), C.POINTER(_d),
+     C.POINTER(_d))
+_sig("accel_fold_proflen", _i, _d, _d)
+_sig("accel_fold", _ll, _p, _ll, _d, _d, _d, _d, _i, _i, _p, C.POINTER(FoldStats),
+     C.POINTER(FoldStats), _i)
+
+
+def read_rzw_cand(candfile, candnum):
+    """Record candnum (from 1) of a .cand file (get_rzw_cand)."""
+    out = FourierProps()
+    if lib.accel_read_rzw_cand(str(candfile).encode(), int(candnum), C.byref(out)):
+        raise _err()
+    return out
+
+
+def cand_fold_params(props, T):
+    """prepfold's start f, fdot, fdotdot for a candidate's average r, z, w."""
+    f, fd, fdd = _d(), _d(), _d()
+    lib.accel_cand_fold_params(C.byref(props), T, C.byref(f), C.byref(fd), C.byref(fdd))
+    return f.value, fd.value, fdd.value
+
+
+def fold_proflen(f, dt):
+    return lib.accel_fold_proflen(f, dt)
+
+
+def fold(series, dt, f, fd=0.0, fdd=0.0, npart=64, proflen=None, device=0):
+    """prepfold's fold of a time series on the GPU: (rawfolds [npart][proflen],
+    per-part stats, total stats, points folded); stats are FoldStats."""
+    x = np.ascontiguousarray(series, dtype=np.float32)
+    if proflen is None:
+        proflen = fold_proflen(f, dt)
+    prof = np.zeros((npart, proflen), dtype=np.float64)
+    parts = (FoldStats * npart)()
+    total = FoldStats()
+    n = lib.accel_fold(x.ctypes.data, len(x), dt, f, fd, fdd, npart, proflen, prof.ctypes.data,
+                       parts, C.byref(total), device)
+    if n < 0:
+        raise _err()
+    return prof, list(parts), total, int(n)

@@ -17,9 +17,10 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-I", CSRC, "-I", os.path.join(ROOT, "include")]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall",
            "-I", CSRC, "-I", os.path.join(ROOT, "include")]
-HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip", "fft_gpu.hip", "surface_gpu.hip", "dedisp_gpu.hip"]
+HIP_SRC = ["accel_gpu.hip", "refine_gpu.hip", "fft_gpu.hip", "surface_gpu.hip", "dedisp_gpu.hip",
+           "fold_gpu.hip"]
 C_SRC = ["accel_host.c", "accel_stats.c", "accel_dered.c", "accel_output.c", "accel_batch.c",
-         "accel_server.c", "accel_surface.c", "accel_dedisp.c", "accel_sift.c"]
+         "accel_server.c", "accel_surface.c", "accel_dedisp.c", "accel_sift.c", "accel_fold.c"]
 
 
 def run(cmd):
@@ -55,6 +56,13 @@ def build(verbose=False):
     obj = os.path.join(BUILD, "accel_sift_cli.o")
     run(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, "accel_sift_cli.c"), "-o", obj])
     out = os.path.join(HERE, "accel_sift")
+    run(["gcc", "-o", out + ".tmp", obj, "-L", HERE, "-lpresto_accel", "-Wl,-rpath,$ORIGIN",
+         "-lm"])
+    os.replace(out + ".tmp", out)
+    # accelfold: prepfold -accelcand on a .dat
+    obj = os.path.join(BUILD, "accelfold_cli.o")
+    run(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, "accelfold_cli.c"), "-o", obj])
+    out = os.path.join(HERE, "accelfold")
     run(["gcc", "-o", out + ".tmp", obj, "-L", HERE, "-lpresto_accel", "-Wl,-rpath,$ORIGIN",
          "-lm"])
     os.replace(out + ".tmp", out)

@@ -1,0 +1,181 @@
+/*
+ * accel_fold.c -- the consumer of the .cand files: prepfold -accelcand on
+ * a .dat (src/prepfold_pipeline.c:740-767, 1375-1594; src/fold.c;
+ * src/get_candidates.c:27-37).  A candidate's record is read, its average
+ * r, z, w become the start f, fdot, fdotdot of the fold, and the series is
+ * folded on the GPU (fold_gpu.hip) into npart sub-integration profiles
+ * with fold()'s statistics.  The period / period-derivative search that
+ * prepfold runs afterwards is not built.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "presto_accel.h"
+#include "accel_internal.h"
+#include "agpu.h"
+
+#define FOLD_REC 1024   /* SUBSBLOCKLEN (include/presto.h:58) */
+
+int accel_read_rzw_cand(const char *candfile, int candnum, accel_fourierprops *out)
+{
+    if (candnum < 1) {
+        accel_set_error("candidate numbers start at 1 (got %d)", candnum);
+        return -1;
+    }
+    FILE *f = fopen(candfile, "rb");
+    if (!f) {
+        accel_set_error("cannot open '%s'", candfile);
+        return -1;
+    }
+    int rc = -1;
+    if (fseek(f, (long) (candnum - 1) * (long) sizeof(accel_fourierprops), SEEK_SET) == 0 &&
+        fread(out, sizeof(accel_fourierprops), 1, f) == 1)
+        rc = 0;
+    else
+        accel_set_error("'%s' has no candidate %d", candfile, candnum);
+    fclose(f);
+    return rc;
+}
+
+void accel_cand_fold_params(const accel_fourierprops *c, double T, double *f, double *fd,
+                            double *fdd)
+{
+    /* the .cand holds the average r and z; the fold wants start values */
+    const double z0 = c->z - 0.5 * c->w;
+    const double r0 = c->r - 0.5 * z0 - c->w / 6.0;
+    *f = r0 / T;
+    *fd = z0 / (T * T);
+    *fdd = c->w / (T * T * T);
+}
+
+int accel_fold_proflen(double f, double dt)
+{
+    long n = (long) (1.0 / f / dt + 0.5);
+    return (int) (n > 64 ? 64 : (n < 2 ? 2 : n));
+}
+
+static double phase_at(double t, double f, double fd, double fdd)
+{
+    return t * (t * (t * (fdd / 6.0) + fd / 2.0) + f);
+}
+
+static void profile_stats(const double *prof, int n, double numdata, double data_avg,
+                          double data_var, accel_foldstats *st)
+{
+    st->numdata = numdata;
+    st->data_avg = data_avg;
+    st->data_var = data_var;
+    st->numprof = n;
+    double avg = 0.0;
+    for (int k = 0; k < n; k++)
+        avg += prof[k];
+    avg /= n;
+    double chi = 0.0;
+    for (int k = 0; k < n; k++)
+        chi += (prof[k] - avg) * (prof[k] - avg);
+    st->prof_avg = avg;
+    st->prof_var = data_var * numdata / n;
+    st->redchi = st->prof_var > 0.0 ? chi / (st->prof_var * (n - 1)) : 0.0;
+}
+
+long long accel_fold(const float *series, long long n, double dt, double f, double fd,
+                     double fdd, int npart, int proflen, double *rawfolds,
+                     accel_foldstats *parts, accel_foldstats *total, int device)
+{
+    if (npart < 1 || proflen < 2 || dt <= 0.0 || !(f > 0.0)) {
+        accel_set_error("fold: npart %d, proflen %d, dt %g, f %g out of range", npart, proflen,
+                        dt, f);
+        return -1;
+    }
+    /* records of 1024 points, a whole number per part
+     * (src/prepfold_pipeline.c:536-551) */
+    const long long rpp = n / FOLD_REC / npart;
+    if (rpp < 1) {
+        accel_set_error("fold: %lld points make fewer than %d records of %d", n, npart,
+                        FOLD_REC);
+        return -1;
+    }
+    if (rpp > 0x7fffffffLL / npart) {
+        accel_set_error("fold: too many records");
+        return -1;
+    }
+    const long long nrec = rpp * npart, used = nrec * FOLD_REC;
+    double *part_end = (double *) malloc(sizeof(double) * npart);
+    double *recsums = (double *) malloc(sizeof(double) * 2 * (size_t) nrec);
+    const double proftime = FOLD_REC * dt;
+    const double phi0 = phase_at(0.0, f, fd, fdd);
+    for (int p = 0; p < npart; p++) {
+        /* end time of the part's last point: its record's start + 1024 dt */
+        const double tlo = (double) ((long long) p * rpp) * proftime + (rpp - 1) * proftime;
+        part_end[p] = phase_at(tlo + FOLD_REC * dt, f, fd, fdd) - phi0;
+    }
+    void *stream = NULL, *d_x = NULL, *d_end = NULL, *d_prof = NULL, *d_sums = NULL;
+    const size_t pbytes = sizeof(double) * (size_t) npart * proflen;
+    long long rc = -1;
+    if (agpu_init(device) || agpu_stream_create(&stream) ||
+        agpu_malloc(&d_x, sizeof(float) * (size_t) used) ||
+        agpu_malloc(&d_end, sizeof(double) * npart) || agpu_malloc(&d_prof, pbytes) ||
+        agpu_malloc(&d_sums, sizeof(double) * 2 * (size_t) nrec) ||
+        agpu_h2d(d_x, series, sizeof(float) * (size_t) used, stream) ||
+        agpu_h2d(d_end, part_end, sizeof(double) * npart, stream) ||
+        agpu_memset(d_prof, 0, pbytes, stream)) {
+        accel_set_error("fold: %s", agpu_last_error());
+        goto done;
+    }
+    agpu_fold_args a;
+    memset(&a, 0, sizeof(a));
+    a.series = (const float *) d_x;
+    a.dt = dt;
+    a.f = f;
+    a.fd = fd;
+    a.fdd = fdd;
+    a.phi_start = phi0;
+    a.npart = npart;
+    a.rpp = (int) rpp;
+    a.proflen = proflen;
+    a.part_end = (const double *) d_end;
+    a.rawfolds = (double *) d_prof;
+    a.recsums = (double *) d_sums;
+    if (agpu_fold(&a, stream) || agpu_d2h(rawfolds, d_prof, pbytes, stream) ||
+        agpu_d2h(recsums, d_sums, sizeof(double) * 2 * (size_t) nrec, stream) ||
+        agpu_stream_sync(stream)) {
+        accel_set_error("fold: %s", agpu_last_error());
+        goto done;
+    }
+    /* statistics: per part, then of the summed profile */
+    double all1 = 0.0, all2 = 0.0;
+    double *sum = (double *) calloc(proflen, sizeof(double));
+    for (int p = 0; p < npart; p++) {
+        double s1 = 0.0, s2 = 0.0;
+        for (long long r = (long long) p * rpp; r < (long long) (p + 1) * rpp; r++) {
+            s1 += recsums[2 * r];
+            s2 += recsums[2 * r + 1];
+        }
+        all1 += s1;
+        all2 += s2;
+        const double m = (double) (rpp * FOLD_REC);
+        const double avg = s1 / m, var = (s2 - s1 * avg) / (m - 1.0);
+        if (parts)
+            profile_stats(rawfolds + (size_t) p * proflen, proflen, m, avg, var, &parts[p]);
+        for (int k = 0; k < proflen; k++)
+            sum[k] += rawfolds[(size_t) p * proflen + k];
+    }
+    if (total) {
+        const double m = (double) used;
+        const double avg = all1 / m, var = (all2 - all1 * avg) / (m - 1.0);
+        profile_stats(sum, proflen, m, avg, var, total);
+    }
+    free(sum);
+    rc = used;
+done:
+    agpu_free(d_x);
+    agpu_free(d_end);
+    agpu_free(d_prof);
+    agpu_free(d_sums);
+    if (stream)
+        agpu_stream_destroy(stream);
+    free(part_end);
+    free(recsums);
+    return rc;
+}

@@ -1,0 +1,136 @@
+"""The .cand consumer's CPU side: the oracle's restatement of prepfold's
+fold (oracle/o_fold.c) pinned by closed forms and by an independent
+point-by-point restatement of src/fold.c's buffer and dump rules, and the
+library's record reader and start-parameter conversion."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+A = pytest.importorskip("presto_amd.accel")
+
+
+def test_cand_fold_params_invert_the_average():
+    """The .cand holds average r and z; prepfold starts the fold from r0,
+    z0 with r = r0 + z0/2 + w/6 and z = z0 + w/2
+    (src/prepfold_pipeline.c:759-764)."""
+    T = 536.870912
+    r0, z0, w = 81234.375, -17.25, 42.0
+    r, z = r0 + z0 / 2 + w / 6, z0 + w / 2
+    f, fd, fdd = O.cand_fold_params(r, z, w, T)
+    assert f == pytest.approx(r0 / T, rel=1e-15)
+    assert fd == pytest.approx(z0 / T ** 2, rel=1e-13)
+    assert fdd == pytest.approx(w / T ** 3, rel=1e-15)
+    props = A.FourierProps()
+    props.r, props.z, props.w = r, z, w
+    assert A.cand_fold_params(props, T) == (f, fd, fdd)
+
+
+def test_read_rzw_cand_record(tmp_path):
+    """get_rzw_cand reads record candnum - 1 of 88-byte fourierprops."""
+    assert C.sizeof(A.FourierProps) == 88
+    recs = (A.FourierProps * 3)()
+    for k in range(3):
+        recs[k].r, recs[k].z, recs[k].w, recs[k].sig = 100.5 + k, 2.0 * k, -20.0 * k, 7.0 + k
+    path = tmp_path / "x_ACCEL_50.cand"
+    path.write_bytes(bytes(recs))
+    got = A.read_rzw_cand(path, 2)
+    assert bytes(got) == bytes(recs[1])
+    with pytest.raises(A.AccelError):
+        A.read_rzw_cand(path, 4)
+    with pytest.raises(A.AccelError):
+        A.read_rzw_cand(path, 0)
+
+
+def test_default_proflen():
+    """One bin per sample of the period, at most 64
+    (src/prepfold_pipeline.c:349-357)."""
+    assert A.fold_proflen(1.0 / 0.005, 1e-4) == 50
+    assert A.fold_proflen(123.4, 256e-6) == int(1 / 123.4 / 256e-6 + 0.5) == 32
+    assert A.fold_proflen(123.4, 64e-6) == 64
+    assert A.fold_proflen(1.0, 1e-3) == 64
+
+
+def test_constant_series_fills_every_bin_equally():
+    """A constant c drizzled at constant f: each completed turn puts
+    c / (f dt proflen) in every bin, so every bin of the summed profile
+    holds K c / (f dt proflen) for K completed turns."""
+    dt, f, c, nb = 1e-3, 9.7, 2.5, 32
+    x = np.full(16 * 1024, c, dtype=np.float32)
+    prof, st, n = O.fold_dat(x, dt, f, 0.0, 0.0, npart=4, proflen=nb)
+    K = math.floor(f * n * dt + 1e-9)
+    tot = prof.sum(0)
+    assert np.allclose(tot, K * c / (f * dt * nb), rtol=1e-9)
+    assert n == 16 * 1024
+    assert all(s["numdata"] == 4096 for s in st)
+    assert st[0]["data_avg"] == pytest.approx(c)
+
+
+def _fold_points(x, dt, f, fd, fdd, npart, nb):
+    """Independent restatement, point by point in Python floats: records of
+    1024 points, the phase added since the start tracked as a whole, each
+    piece of a point (split at bin edges and at whole turns of the phase
+    added) credited to the part whose end the turn's end does not exceed."""
+    rpp = len(x) // 1024 // npart
+    used = rpp * npart * 1024
+    prof = np.zeros((npart, nb))
+    ph = lambda t: t * (t * (t * fdd / 6 + fd / 2) + f)  # noqa: E731
+    proftime = 1024 * dt
+    # phase added at the end of each part
+    pend = []
+    for p in range(npart):
+        tlo = (p * rpp) * proftime + (rpp - 1) * proftime
+        pend.append(ph(tlo + 1024 * dt))
+    for i in range(used):
+        b = i // 1024
+        p, jj, l = b // rpp, b % rpp, i % 1024
+        tlo = (p * rpp) * proftime + jj * proftime
+        a, e = ph(tlo + l * dt), ph(tlo + (l + 1) * dt)
+        d = e - a
+        if d <= 1e-12:
+            continue
+        vpp = float(x[i]) / d
+        lo = a
+        while lo < e - 1e-15:
+            bin_lo = math.floor((lo - math.floor(lo)) * nb + 1e-12)
+            edge = math.floor(lo) + (bin_lo + 1) / nb
+            turn_end = math.floor(lo) + 1.0
+            hi = min(e, edge, turn_end)
+            tp = p
+            while tp < npart and pend[tp] < turn_end:
+                tp += 1
+            if tp < npart:
+                prof[tp, bin_lo % nb] += (hi - lo) * vpp
+            lo = hi
+    return prof
+
+
+@pytest.mark.parametrize("f,fd,fdd,npart,nb", [(37.3, 0.0, 0.0, 4, 24),
+                                               (5.21, 2e-3, 0.0, 3, 16),
+                                               (0.61, 1e-4, 3e-5, 2, 20)])
+def test_fold_matches_point_by_point_restatement(f, fd, fdd, npart, nb):
+    rng = np.random.default_rng(11)
+    dt = 1e-3
+    x = rng.standard_normal(12 * 1024).astype(np.float32)
+    want = _fold_points(x, dt, f, fd, fdd, npart, nb)
+    got, st, n = O.fold_dat(x, dt, f, fd, fdd, npart=npart, proflen=nb)
+    scale = np.abs(want).max()
+    assert np.max(np.abs(got - want)) < 1e-9 * scale
+
+
+def test_pulse_lands_at_its_phase():
+    """A narrow pulse train at phase 0.25 of a known f, fd folds into the
+    bin holding phase 0.25, and the profile stands far above noise."""
+    from presto_amd import synth
+    N, dt, f0, fd = 1 << 16, 1e-3, 3.7, 1e-4
+    t = np.arange(N) * dt
+    phase = f0 * t + 0.5 * fd * t * t
+    d = (phase - 0.25) - np.floor(phase - 0.25 + 0.5)
+    x = (np.exp(-0.5 * (d / 0.01) ** 2) + 0.05 * synth.make_series(N, dt, 5)).astype(np.float32)
+    prof, st, n = O.fold_dat(x, dt, f0, fd, 0.0, npart=8, proflen=40)
+    tot = prof.sum(0)
+    assert np.argmax(tot) == int(0.25 * 40)
+    assert st[3]["redchi"] > 100
