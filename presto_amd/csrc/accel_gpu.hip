@@ -600,6 +600,10 @@ __global__ __launch_bounds__(256) void k_plane_halo(float *plane, int nrows, int
 /* ------------------------------------------------------------------ */
 #define HS_TPB 256
 #define HS_WAVES (HS_TPB / 64)
+#define HS_RT AGPU_HS_RT
+#define HS_RT_ROW AGPU_HS_RT_ROW
+#define HS_RT_SUB AGPU_HS_RT_SUB
+#define HS_CT AGPU_HS_CT
 #define HS_CPT 4
 #define HS_CHUNK (HS_TPB * HS_CPT)
 
@@ -620,10 +624,13 @@ __device__ __forceinline__ void emit_hit(const agpu_harmsum_args &a, int b, int 
 }
 
 /* 64-float wave loads that cover one chunk's sub-harmonic span for fraction
- * k/H: the span is at most CHUNK k/H + 1 half-bins. */
+ * k/H (the span is at most CHUNK k/H + 1 half-bins), rounded up to a
+ * multiple of the workgroup's waves so that every wave issues the same
+ * loads without a bound check; the extra floats stay inside the tile row
+ * and its halo. */
 __host__ __device__ constexpr int hs_frac_wl(int k, int H)
 {
-    return (HS_CHUNK * k / H + 1 + 63) / 64;
+    return ((HS_CHUNK * k / H + 1 + 63) / 64 + HS_WAVES - 1) / HS_WAVES * HS_WAVES;
 }
 
 __host__ __device__ constexpr int hs_stage_wl(int S)
@@ -656,7 +663,7 @@ __host__ __device__ constexpr int hs_lds_wl(int nstages)
 }
 static_assert(hs_frac_wl(15, 16) * 64 <= AGPU_HALO && hs_frac_wl(31, 32) * 64 <= AGPU_HALO &&
                   hs_frac_wl(1, 2) * 64 <= AGPU_HALO,
-              "every sub-harmonic span fits in a tile-row halo");
+              "every padded sub-harmonic span fits in a tile-row halo");
 
 /* Index-rule variants of the harmonic sum:
  *   HS_STD   standard path, sub-harmonic half-bin rint((hb + rh_off) k / H)
@@ -673,7 +680,7 @@ static_assert(hs_frac_wl(15, 16) * 64 <= AGPU_HALO && hs_frac_wl(31, 32) * 64 <=
 /* rint (half-even) or, for HS_INMEM, (int)(x+0.5) of rh * k / 2^s.
  * 32-bit: the launcher checks that every rh * k fits. */
 template <int MODE>
-__device__ __forceinline__ int hs_round(int rh, int k, int s)
+__host__ __device__ __forceinline__ int hs_round(int rh, int k, int s)
 {
     const unsigned q = (unsigned) rh * (unsigned) k;
     const unsigned H = 1u << s;
@@ -682,27 +689,11 @@ __device__ __forceinline__ int hs_round(int rh, int k, int s)
     return (int) ((q + (H >> 1) - 1 + ((q >> s) & 1u)) >> s);
 }
 
-/* First half-bin of a chunk's span for fraction k / 2^s.  HS_FRAC starts
- * one half-bin below the grid rule's: its lookups lie within
- * (hb - 1/2, hb + 2) k / H, so the span [lo, lo + CHUNK k / H + 4) holds
- * them, inside the hs_frac_wl slots (whose 64-float rounding leaves at
- * least 31 spare half-bins). */
-template <int MODE>
-__device__ __forceinline__ int hs_span_lo(const agpu_harmsum_args &a, int rh_c0, int k, int s)
-{
-    if (MODE == HS_STD)
-        return hs_round<HS_STD>(rh_c0 + a.rh_off, k, s);
-    if (MODE == HS_INMEM)
-        return hs_round<HS_INMEM>(rh_c0, k, s);
-    const int lo = hs_round<HS_STD>(rh_c0, k, s) - 1;
-    return lo > 0 ? lo : 0;
-}
-
 /* per-thread staging registers of one fraction: its wave loads are dealt
  * round-robin to the HS_WAVES waves */
 __host__ __device__ constexpr int hs_frac_regs(int k, int H)
 {
-    return (hs_frac_wl(k, H) + HS_WAVES - 1) / HS_WAVES;
+    return hs_frac_wl(k, H) / HS_WAVES;
 }
 
 __host__ __device__ constexpr int hs_stage_regs(int S)
@@ -722,12 +713,13 @@ __host__ __device__ constexpr int hs_regs_upto(int S)   /* stages 1..S-1 */
 }
 
 /* Load every sub-harmonic span of stage S for this chunk into registers:
- * wave load i (64 consecutive half-bins) goes to wave i % HS_WAVES.  The row
- * base is wave-uniform and the span never leaves the tile-row plus its halo,
- * so every load is base + lane + immediate. */
-template <int S, int MODE>
-__device__ __forceinline__ void hs_load(const agpu_harmsum_args &a, float *v, int zi, int wi,
-                                        int nrows, int rh_c0, int wv, int lane)
+ * wave load i (64 consecutive half-bins) goes to wave i % HS_WAVES.  The
+ * span's plane offset is the chunk table's tile offset of its first
+ * half-bin plus the row table's offset of its (w, z) row, both loaded as
+ * scalars: every load is base + lane + immediate. */
+template <int S>
+__device__ __forceinline__ void hs_load(const agpu_harmsum_args &a, float *v, const long long *cb,
+                                        const unsigned *rt, int wv, int lane)
 {
     constexpr int H = 1 << S;
     constexpr int F0 = (H >> 1) - 1;   /* index of fraction 1/H */
@@ -735,17 +727,10 @@ __device__ __forceinline__ void hs_load(const agpu_harmsum_args &a, float *v, in
 #pragma unroll
     for (int k = 1; k < H; k += 2) {
         const int f = F0 + (k - 1) / 2;
-        const int zs = a.zmap[f * a.numz + zi];
-        const int ws = a.wmap[f * a.numw + wi];
-        const int lo = hs_span_lo<MODE>(a, rh_c0, k, S);
-        const int srow = ws * a.numz + zs;
-        /* the span starts in lo's tile-row and ends within its halo */
-        const float *b1 = a.plane + AGPU_PLANE_IDX(srow, lo, nrows) + wv * 64;
-        const int nwl = hs_frac_wl(k, H);
+        const float *b1 = a.plane + cb[1 + f] + rt[HS_RT_SUB + f] + wv * 64;
 #pragma unroll
         for (int u = 0; u < hs_frac_regs(k, H); u++)
-            if (wv + HS_WAVES * u < nwl)
-                v[r + u] = b1[u * (HS_WAVES * 64) + lane];
+            v[r + u] = b1[u * (HS_WAVES * 64) + lane];
         r += hs_frac_regs(k, H);
     }
 }
@@ -759,19 +744,19 @@ __device__ __forceinline__ void hs_store(float *lds, int base, const float *v, i
     for (int k = 1; k < H; k += 2) {
 #pragma unroll
         for (int u = 0; u < hs_frac_regs(k, H); u++)
-            if (wv + HS_WAVES * u < hs_frac_wl(k, H))
-                lds[(wl + wv + HS_WAVES * u) * 64 + lane] = v[r + u];
+            lds[(wl + wv + HS_WAVES * u) * 64 + lane] = v[r + u];
         r += hs_frac_regs(k, H);
         wl += hs_frac_wl(k, H);
     }
 }
 
 /* Accumulate stage S from LDS, in the reference's fraction order.  nr[j]
- * is the LDS row of norm^2 values of cell j's block. */
+ * is the LDS row of norm^2 values of cell j's block; clo[1 + f] the first
+ * half-bin of fraction f's span. */
 template <int S, int MODE>
 __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const float *lds,
                                             int base, const float *nr0, const float *nr1,
-                                            const float *nr2, const float *nr3, int rh_c0,
+                                            const float *nr2, const float *nr3, const int *clo,
                                             int rh_first, const double *rf, float *acc)
 {
     /* each normalised sub-harmonic power is rounded to FP32 before it is
@@ -785,7 +770,7 @@ __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const fl
 #pragma unroll
     for (int k = 1; k < H; k += 2) {
         const int f = F0 + (k - 1) / 2;
-        const int lo = hs_span_lo<MODE>(a, rh_c0, k, S);
+        const int lo = clo[1 + f];
         float nn[4];
         if (MODE != HS_INMEM) {
             nn[0] = nr0[1 + f];
@@ -822,31 +807,35 @@ __device__ __forceinline__ void hs_consume4(const agpu_harmsum_args &a, const fl
 }
 
 /* One workgroup = one (w,z) row x one chunk of 1024 fundamental half-bins,
- * 4 cells per thread (cell = chunk base + t + 256 j).  Workgroup ids are
- * dealt round-robin to the 8 XCDs and every row of chunk c runs on XCD
- * c % 8, so the sub-harmonic rows that the rows of a chunk share are read
- * once into that XCD's L2.  The fundamental values and every sub-harmonic
- * span of stages 1-4 are loaded with plain global loads, all in flight at
- * once.  The spans of stages 1-3 go to LDS; stage 4's wait in registers
- * and take the same LDS once stages 1-3 are summed, which halves the LDS
- * of a 16-harmonic workgroup (18.8 KB: 5-6 workgroups per CU instead of
- * 4).  A 32-harmonic search loads stage 5 in a third round.  Adding 256 to a fundamental half-bin adds an even
- * 256 k / 2^s to the rounded quotient, so one rounding per thread and
- * fraction serves all 4 cells. */
+ * 4 cells per thread (cell = chunk base + t + 256 j).  The grid is
+ * (8 rows' slots x groups of 8 chunks): blockIdx.x = 8 row + c, chunk =
+ * c_lo + 8 blockIdx.y + c.  Workgroups are dealt round-robin to the 8
+ * XCDs in linear order, so every row of a chunk runs on one XCD, one row
+ * after another, and the sub-harmonic rows that the rows of a chunk share
+ * are read once into that XCD's L2.  Row and chunk descriptors (plane
+ * offsets of every fraction's span) come from tables built with the
+ * search, so the per-workgroup scalar work is a few loads and adds.  The
+ * fundamental values and every sub-harmonic span of stages 1-4 are loaded
+ * with plain global loads, all in flight at once.  The spans of stages 1-3
+ * go to LDS; stage 4's wait in registers and take the same LDS once stages
+ * 1-3 are summed.  A 32-harmonic search loads stage 5 in a third round.
+ * Adding 256 to a fundamental half-bin adds an even 256 k / 2^s to the
+ * rounded quotient, so one rounding per thread and fraction serves all 4
+ * cells. */
 template <int NSTAGES, int MODE>
 __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_lo, int c_hi)
 {
     __shared__ float s_norm[3 * 32];   /* norm^2 of blocks b0..b0+2 x fraction */
     constexpr int NWL = hs_lds_wl(NSTAGES);
     __shared__ float lds[(NWL > 0 ? NWL : 1) * 64];
-    const int nrows = a.numz * a.numw;
-    const int bid = blockIdx.x;
-    const int xcd = bid & 7, local = bid >> 3;
-    const int m = local / nrows, row = local - m * nrows;
-    const int chunk = c_lo + xcd + 8 * m;
+    const int row = blockIdx.x >> 3;
+    const int chunk = c_lo + 8 * (int) blockIdx.y + (int) (blockIdx.x & 7);
     if (chunk >= c_hi)
         return;
-    const int wi = row / a.numz, zi = row - wi * a.numz;
+    const unsigned *rt = a.row_tab + (size_t) row * HS_RT;
+    const long long *cb = a.chunk_base + (size_t) chunk * HS_CT;
+    const int *clo = a.chunk_lo + (size_t) chunk * HS_CT;
+    const int wi = (int) rt[0], zi = (int) rt[1];
     const int ncell = a.nblocks * a.corr_uselen;
     const int rh_c0 = ((int) (a.rh0 / HS_CHUNK) + chunk) * HS_CHUNK;
     const int cell0 = rh_c0 - (int) a.rh0;   /* negative in the first chunk */
@@ -857,17 +846,17 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_l
     const int b0 = cell0 > 0 ? cell0 / cu : 0;
     const int e1 = (b0 + 1) * cu, e2 = (b0 + 2) * cu;
     const int nfr = a.nfrac + 1;
-    const float *prow = a.plane + AGPU_PLANE_IDX(row, rh_c0, nrows);
+    const float *prow = a.plane + cb[0] + rt[HS_RT_ROW];
     float fund[HS_CPT];
 #pragma unroll
     for (int j = 0; j < HS_CPT; j++)
         fund[j] = prow[t + HS_TPB * j];
     constexpr int NREG = hs_regs_upto(NSTAGES < 5 ? NSTAGES : 5);
     float v[NREG > 0 ? NREG : 1];
-    if (NSTAGES > 1) hs_load<1, MODE>(a, v + hs_regs_upto(1), zi, wi, nrows, rh_c0, wv, lane);
-    if (NSTAGES > 2) hs_load<2, MODE>(a, v + hs_regs_upto(2), zi, wi, nrows, rh_c0, wv, lane);
-    if (NSTAGES > 3) hs_load<3, MODE>(a, v + hs_regs_upto(3), zi, wi, nrows, rh_c0, wv, lane);
-    if (NSTAGES > 4) hs_load<4, MODE>(a, v + hs_regs_upto(4), zi, wi, nrows, rh_c0, wv, lane);
+    if (NSTAGES > 1) hs_load<1>(a, v + hs_regs_upto(1), cb, rt, wv, lane);
+    if (NSTAGES > 2) hs_load<2>(a, v + hs_regs_upto(2), cb, rt, wv, lane);
+    if (NSTAGES > 3) hs_load<3>(a, v + hs_regs_upto(3), cb, rt, wv, lane);
+    if (NSTAGES > 4) hs_load<4>(a, v + hs_regs_upto(4), cb, rt, wv, lane);
     if (MODE != HS_INMEM && t < 3 * nfr) {
         const int bb = min(b0 + t / nfr, a.nblocks - 1);
         s_norm[(t / nfr) * 32 + t % nfr] = a.nsq[(size_t) bb * nfr + t % nfr];
@@ -904,11 +893,18 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_l
         acc[j] = fund[j] * nn;
     }
 #define HS_EMIT(ST)                                                                    \
-    _Pragma("unroll") for (int j = 0; j < HS_CPT; j++) {                              \
-        const int c = cell0 + t + HS_TPB * j;                                          \
-        if (c >= 0 && c < ncell && acc[j] > a.powcut[ST]) {                            \
-            const int b = b0 + nb[j];                                                  \
-            emit_hit(a, b, ST, wi, zi, c - b * cu, acc[j]);                            \
+    {                                                                                  \
+        const float cut_ = a.powcut[ST];                                               \
+        const bool any_ = acc[0] > cut_ || acc[1] > cut_ || acc[2] > cut_ ||           \
+                          acc[3] > cut_;                                               \
+        if (__ballot(any_)) {                                                          \
+            _Pragma("unroll") for (int j = 0; j < HS_CPT; j++) {                      \
+                const int c = cell0 + t + HS_TPB * j;                                  \
+                if (c >= 0 && c < ncell && acc[j] > cut_) {                            \
+                    const int b = b0 + nb[j];                                          \
+                    emit_hit(a, b, ST, wi, zi, c - b * cu, acc[j]);                    \
+                }                                                                      \
+            }                                                                          \
         }                                                                              \
     }
     HS_EMIT(0)
@@ -918,7 +914,7 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_l
     const float *nrow2 = s_norm + nb[2] * 32;
     const float *nrow3 = s_norm + nb[3] * 32;
 #define HS_STAGE(ST)                                                                   \
-    hs_consume4<ST, MODE>(a, lds, hs_stage_base(ST), nrow0, nrow1, nrow2, nrow3, rh_c0, \
+    hs_consume4<ST, MODE>(a, lds, hs_stage_base(ST), nrow0, nrow1, nrow2, nrow3, clo,  \
                           rh_first, rf, acc);                                         \
     HS_EMIT(ST)
     if (NSTAGES > 1) { HS_STAGE(1) }
@@ -930,20 +926,35 @@ __global__ __launch_bounds__(HS_TPB) void k_harmsum(agpu_harmsum_args a, int c_l
         __syncthreads();
         hs_store<4>(lds, 0, v + hs_regs_upto(4), wv, lane);
         __syncthreads();
-        hs_consume4<4, MODE>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, rh_c0, rh_first, rf, acc);
+        hs_consume4<4, MODE>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, clo, rh_first, rf, acc);
         HS_EMIT(4)
     }
     if (NSTAGES > 5) {
         float v5[hs_stage_regs(5)];
-        hs_load<5, MODE>(a, v5, zi, wi, nrows, rh_c0, wv, lane);
+        hs_load<5>(a, v5, cb, rt, wv, lane);
         __syncthreads();   /* stages 1-4 are done with LDS */
         hs_store<5>(lds, 0, v5, wv, lane);
         __syncthreads();
-        hs_consume4<5, MODE>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, rh_c0, rh_first, rf, acc);
+        hs_consume4<5, MODE>(a, lds, 0, nrow0, nrow1, nrow2, nrow3, clo, rh_first, rf, acc);
         HS_EMIT(5)
     }
 #undef HS_STAGE
 #undef HS_EMIT
+}
+
+/* First half-bin of a chunk's span for fraction k / 2^s.  HS_FRAC starts
+ * one half-bin below the grid rule's: its lookups lie within
+ * (hb - 1/2, hb + 2) k / H, so the span [lo, lo + CHUNK k / H + 4) holds
+ * them, inside the hs_frac_wl slots (whose 64-float rounding leaves at
+ * least 31 spare half-bins). */
+static int hs_span_lo_host(int mode, int rh_off, int rh_c0, int k, int s)
+{
+    if (mode == HS_STD)
+        return hs_round<HS_STD>(rh_c0 + rh_off, k, s);
+    if (mode == HS_INMEM)
+        return hs_round<HS_INMEM>(rh_c0, k, s);
+    const int lo = hs_round<HS_STD>(rh_c0, k, s) - 1;
+    return lo > 0 ? lo : 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1342,7 +1353,7 @@ int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
         return 0;
     int tpg = (nt + groups - 1) / groups;
     long long nblk = (seg_hi - seg_lo) * groups;
-    if (nblk > 0x7fffffffLL)
+    if (nblk >= (1LL << 24))   /* a dispatch counts work-items in 32 bits */
         return fail("agpu_correlate: grid too large");
     hipLaunchKernelGGL(k_corr4096, dim3((unsigned) nblk), dim3(256), 0, (hipStream_t) stream,
                        (const float2 *) d_spec, numbins, lobin_file, (const float2 *) d_T, nt,
@@ -1358,14 +1369,20 @@ int agpu_plane_halo(float *d_plane, int nrows, long long ntiles, long long tile_
         tile_hi = ntiles;
     if (tile_lo < 0 || tile_lo > tile_hi)
         return fail("agpu_plane_halo: bad tile range");
-    long long nblk = (long long) nrows * (tile_hi - tile_lo);
-    if (nblk > 0x7fffffffLL)
-        return fail("agpu_plane_halo: grid too large");
-    if (nblk == 0)
+    if (nrows <= 0 || tile_lo == tile_hi)
         return 0;
-    hipLaunchKernelGGL(k_plane_halo, dim3((unsigned) nblk), dim3(256), 0, (hipStream_t) stream,
-                       d_plane, nrows, (int) ntiles, (int) tile_lo);
-    AGPU_CHECK(hipGetLastError());
+    /* launches of at most 2^24 - 1 workgroups: a dispatch counts work-items
+     * in 32 bits */
+    const long long per = ((1LL << 24) - 1) / nrows;
+    if (per < 1)
+        return fail("agpu_plane_halo: too many rows");
+    for (long long t0 = tile_lo; t0 < tile_hi; t0 += per) {
+        const long long t1 = t0 + per < tile_hi ? t0 + per : tile_hi;
+        const long long nblk = (long long) nrows * (t1 - t0);
+        hipLaunchKernelGGL(k_plane_halo, dim3((unsigned) nblk), dim3(256), 0,
+                           (hipStream_t) stream, d_plane, nrows, (int) ntiles, (int) t0);
+        AGPU_CHECK(hipGetLastError());
+    }
     return 0;
 }
 
@@ -1411,11 +1428,14 @@ int agpu_harmsum(const agpu_harmsum_args *a, long long c_lo, long long c_hi, voi
         return fail("agpu_harmsum: bad chunk range");
     if (c_lo == c_hi)
         return 0;
-    long long rows = (long long) a->numz * a->numw;
-    long long nblk = 8 * ((c_hi - c_lo + 7) / 8) * rows;
-    if (nblk > 0x7fffffffLL)
+    const long long rows = (long long) a->numz * a->numw;
+    if (!a->row_tab || !a->chunk_base || !a->chunk_lo)
+        return fail("agpu_harmsum: address tables missing");
+    /* grid (8 rows' slots, groups of 8 chunks); each dimension stays far
+     * below the dispatch packet's 32-bit work-item counts */
+    if (8 * rows * HS_TPB > 0xffffffffLL || (c_hi - c_lo + 7) / 8 > 0xffffLL)
         return fail("agpu_harmsum: grid too large");
-    dim3 grid((unsigned) nblk);
+    dim3 grid((unsigned) (8 * rows), (unsigned) ((c_hi - c_lo + 7) / 8));
     hipStream_t st = (hipStream_t) stream;
     const int cl = (int) c_lo, ch = (int) c_hi;
 #define HS_CASE(NS)                                                                         \
@@ -1440,6 +1460,57 @@ int agpu_harmsum(const agpu_harmsum_args *a, long long c_lo, long long c_hi, voi
     }
 #undef HS_CASE
     AGPU_CHECK(hipGetLastError());
+    return 0;
+}
+
+int agpu_harmsum_row_table(int numz, int numw, int nfrac, const int *zmap, const int *wmap,
+                           unsigned *out)
+{
+    const long long rows = (long long) numz * numw;
+    if (nfrac > AGPU_MAX_FRACS - 1 || AGPU_HS_RT_SUB + nfrac > AGPU_HS_RT)
+        return fail("agpu_harmsum_row_table: too many fractions");
+    if (rows * AGPU_ROWLEN > 0xffffffffLL)
+        return fail("agpu_harmsum_row_table: row offsets exceed 32 bits");
+    for (long long r = 0; r < rows; r++) {
+        const int wi = (int) (r / numz), zi = (int) (r % numz);
+        unsigned *o = out + r * AGPU_HS_RT;
+        memset(o, 0, sizeof(unsigned) * AGPU_HS_RT);
+        o[0] = (unsigned) wi;
+        o[1] = (unsigned) zi;
+        o[AGPU_HS_RT_ROW] = (unsigned) (r * AGPU_ROWLEN);
+        for (int f = 0; f < nfrac; f++) {
+            const long long srow = (long long) wmap[f * numw + wi] * numz + zmap[f * numz + zi];
+            o[AGPU_HS_RT_SUB + f] = (unsigned) (srow * AGPU_ROWLEN);
+        }
+    }
+    return 0;
+}
+
+int agpu_harmsum_chunk_tables(int mode, int rh_off, long long rh0, long long nchunks,
+                              int nstages, int nrows, long long *base, int *lo)
+{
+    if (mode < 0 || mode > 2 || nstages < 1 || nstages > AGPU_MAX_STAGES)
+        return fail("agpu_harmsum_chunk_tables: bad mode or stage count");
+    const long long tile_stride = (long long) nrows * AGPU_ROWLEN;
+    for (long long c = 0; c < nchunks; c++) {
+        long long *b = base + c * AGPU_HS_CT;
+        int *l = lo + c * AGPU_HS_CT;
+        memset(b, 0, sizeof(long long) * AGPU_HS_CT);
+        memset(l, 0, sizeof(int) * AGPU_HS_CT);
+        const long long rh_c0 = (rh0 / HS_CHUNK + c) * HS_CHUNK;
+        if (rh_c0 > 0x7fffffffLL)
+            return fail("agpu_harmsum_chunk_tables: half-bins beyond 2^31");
+        b[0] = (rh_c0 >> AGPU_TILE_LOG2) * tile_stride + (rh_c0 & (AGPU_TILE - 1));
+        l[0] = (int) rh_c0;
+        int f = 0;
+        for (int st = 1; st < nstages; st++)
+            for (int k = 1; k < (1 << st); k += 2, f++) {
+                const int x = hs_span_lo_host(mode, rh_off, (int) rh_c0, k, st);
+                l[1 + f] = x;
+                b[1 + f] = ((long long) x >> AGPU_TILE_LOG2) * tile_stride +
+                           (x & (AGPU_TILE - 1));
+            }
+    }
     return 0;
 }
 

@@ -29,17 +29,9 @@
 #include <unistd.h>
 #include "presto_accel.h"
 #include "accel_internal.h"
+#include "accel_rules.h"
 #include "agpu.h"
 
-#define ACCEL_NUMBETWEEN 2
-#define ACCEL_DR 0.5
-#define ACCEL_RDR 2
-#define ACCEL_DZ 2
-#define ACCEL_RDZ 0.5
-#define ACCEL_DW 20
-#define ACCEL_RDW 0.05
-#define ACCEL_CLOSEST_R 15.0
-#define NUMFINTBINS 16
 #define MAXRAMUSE 2000000000LL
 #define CORR_L 4096
 #define HIT_CAP0 (1u << 20)
@@ -79,55 +71,11 @@ static double now_ms(void)
     return 1e3 * (double) t.tv_sec + 1e-6 * (double) t.tv_nsec;
 }
 
-/* ---- index rules: src/accel_utils.c:36, 54-102 ---- */
-static int nearest_int(double x) { return (int) (x < 0 ? x - 0.5 : x + 0.5); }
-static double calc_required_r(double hf, double rfull) { return rint(ACCEL_RDR * rfull * hf) * ACCEL_DR; }
+/* ---- index rules (accel_rules.h) ---- */
 /* First plane bin of the block that starts at startr: the fundamental's
  * ffdot->rlo = floor(calc_required_r(1, startr)), src/accel_utils.c:1035-1037
  * (startr itself when it is whole). */
-static long long fund_rlo(double startr) { return (long long) floor(calc_required_r(1.0, startr)); }
-static int calc_required_z(double hf, double zfull) { return nearest_int(ACCEL_RDZ * zfull * hf) * ACCEL_DZ; }
-static int calc_required_w(double hf, double wfull) { return nearest_int(ACCEL_RDW * wfull * hf) * ACCEL_DW; }
-
-/* ---- kernel widths and FFT lengths: src/responses.c:27-84, src/corr_prep.c:3-35 ---- */
-static int z_resp_halfwidth(double z) { return (int) (0.5 * 1.1 * fabs(z)) + NUMFINTBINS; }
-
-static int w_resp_halfwidth(double z, double w)
-{
-    if (fabs(w) < 1.0e-7)
-        return z_resp_halfwidth(z);
-    double r0 = 0.5 * (w / 6.0 - z), r1 = 0.5 * (w / 6.0 + z);
-    double maxdev = fabs(r0) > fabs(r1) ? fabs(r0) : fabs(r1);
-    double u_ext = 0.5 - z / w;
-    if (u_ext > 0.0 && u_ext < 1.0) {
-        double r_ext = 0.5 * w * u_ext * u_ext + (z - w / 2.0) * u_ext + r0;
-        if (fabs(r_ext) > maxdev)
-            maxdev = fabs(r_ext);
-    }
-    return (int) (1.1 * maxdev) + NUMFINTBINS;
-}
-
-static long long next2(long long x)
-{
-    long long t = 1;
-    while (t < x)
-        t <<= 1;
-    return t;
-}
-
-static int next_good_fftlen(int n)
-{
-    static const int lens[17] = {128, 192, 256, 384, 512, 768, 1024, 1280, 2048,
-                                 4096, 5120, 7680, 10240, 12288, 15360, 16384, 25600};
-    if (n <= lens[0])
-        return lens[0];
-    if (n > lens[16])
-        return (int) next2(n);
-    int i = 0;
-    while (n > lens[i])
-        i++;
-    return lens[i];
-}
+static long long fund_rlo(double startr) { return (long long) floor(ar_required_r(1.0, startr)); }
 
 static int fftlen_from_kernwidth(int kw)
 {
@@ -136,17 +84,9 @@ static int fftlen_from_kernwidth(int kw)
     for (int i = 0; i < 9; i++)
         if (kw < lim[i])
             return len[i];
-    return (int) next2((long long) kw * 5);
+    return (int) ar_next2((long long) kw * 5);
 }
 
-static int calc_fftlen(int numharm, int harmnum, int zmax, int wmax, int corr_uselen)
-{
-    double hf = (double) harmnum / (double) numharm;
-    int bins_needed = (int) ceil(corr_uselen * hf) + 2;
-    int end_effects = 2 * ACCEL_NUMBETWEEN *
-        w_resp_halfwidth(calc_required_z(hf, zmax), calc_required_w(hf, wmax));
-    return next_good_fftlen(bins_needed + end_effects);
-}
 
 void accel_params_default(accel_params *p)
 {
@@ -250,7 +190,7 @@ int accel_geom_create(const accel_params *p, accel_geom *g)
         g->powcut[ii] = (float) accel_power_for_sigma(g->sigma, 1 << ii,
                                                       (double) g->numindep[ii]);
     }
-    g->maxkernlen = 2 * ACCEL_NUMBETWEEN * w_resp_halfwidth(g->zhi, g->whi);
+    g->maxkernlen = 2 * ACCEL_NUMBETWEEN * ar_w_halfwidth(g->zhi, g->whi);
     g->fftlen = fftlen_from_kernwidth(g->maxkernlen);
     if (g->fftlen < 2048)
         g->fftlen = 2048;
@@ -302,60 +242,10 @@ static void cset(cnode *n, float power, float sigma, int numharm, double r, doub
     n->c.w = w;
 }
 
+#define CN_R(n) ((n)->c.r)
+#define CN_SIGMA(n) ((n)->c.sigma)
 /* insert_new_accelcand, src/accel_utils.c:387-469 */
-static cnode *insert_cand(cnode *list, float power, float sigma, int numharm, double rr,
-                          double zz, double ww, int *added)
-{
-    cnode *tmp = list, *prev = NULL;
-    double prev_diff = ACCEL_CLOSEST_R + 1.0, next_diff;
-    *added = 0;
-    if (!list) {
-        *added = 1;
-        return cnew(power, sigma, numharm, rr, zz, ww);
-    }
-    while (tmp->next && tmp->c.r < rr) {
-        prev = tmp;
-        tmp = tmp->next;
-    }
-    next_diff = fabs(rr - tmp->c.r);
-    if (prev)
-        prev_diff = fabs(rr - prev->c.r);
-    if (prev_diff < ACCEL_CLOSEST_R) {
-        if (prev->c.sigma < sigma) {
-            cset(prev, power, sigma, numharm, rr, zz, ww);
-            *added = 1;
-        }
-        if (next_diff < ACCEL_CLOSEST_R && tmp->c.sigma < sigma) {
-            if (*added) {
-                prev->next = tmp->next;
-                free(tmp);
-            } else {
-                cset(tmp, power, sigma, numharm, rr, zz, ww);
-                *added = 1;
-            }
-        }
-    } else if (next_diff < ACCEL_CLOSEST_R) {
-        if (tmp->c.sigma < sigma) {
-            cset(tmp, power, sigma, numharm, rr, zz, ww);
-            *added = 1;
-        }
-    } else {
-        cnode *n = cnew(power, sigma, numharm, rr, zz, ww);
-        *added = 1;
-        if (!tmp->next && tmp->c.r < (rr - ACCEL_CLOSEST_R)) {
-            tmp->next = n;
-            return list;
-        }
-        if (prev) {
-            prev->next = n;
-            n->next = tmp;
-        } else {
-            n->next = list;
-            return n;
-        }
-    }
-    return list;
-}
+ACCEL_DEFINE_INSERT(insert_cand, cnode, CN_R, CN_SIGMA, cset, cnew, free)
 
 /* compare_accelcand_sigma, src/accel_utils.c:364-377 */
 static int cmp_sigma(const accel_cand *a, const accel_cand *b)
@@ -394,50 +284,7 @@ static cnode *sort_cands(cnode *l)
 }
 
 /* eliminate_harmonics, src/accel_utils.c:472-549 */
-static const double HARM_RATIOS[23][2] = {
-    {3, 2}, {5, 2}, {2, 3}, {4, 3}, {5, 3}, {3, 4}, {5, 4}, {2, 5}, {3, 5}, {4, 5},
-    {5, 6}, {2, 7}, {3, 7}, {4, 7}, {3, 8}, {5, 8}, {2, 9}, {3, 10}, {2, 11}, {3, 11},
-    {2, 13}, {3, 13}, {2, 15}};
-
-static int harmonically_related(double cr, double orr)
-{
-    const double tooclose = 1.5;
-    for (int ii = 1; ii <= 16; ii++)
-        if (fabs(cr / ii - orr) < tooclose)
-            return 1;
-    for (int ii = 1; ii <= 16; ii++)
-        if (fabs(cr * ii - orr) < tooclose)
-            return 1;
-    for (int k = 0; k < 23; k++)
-        if (fabs(cr * HARM_RATIOS[k][0] / HARM_RATIOS[k][1] - orr) < tooclose)
-            return 1;
-    return 0;
-}
-
-static cnode *eliminate_harmonics(cnode *cands, int *numcands)
-{
-    cnode *cur = cands;
-    if (!cur)
-        return cands;
-    while (cur->next) {
-        cnode *before = cur, *other = cur->next;
-        do {
-            if (harmonically_related(cur->c.r, other->c.r)) {
-                cnode *gone = other;
-                other = other->next;
-                before->next = other;
-                free(gone);
-                (*numcands)--;
-            } else {
-                before = other;
-                other = other->next;
-            }
-        } while (other);
-        if (cur->next)
-            cur = cur->next;
-    }
-    return cands;
-}
+ACCEL_DEFINE_ELIMINATE(eliminate_harmonics, cnode, CN_R, free)
 
 static void free_cands(cnode *l)
 {
@@ -474,6 +321,9 @@ struct accel_search {
     float *d_nsq;
     float *d_nsq_hb;      /* in-memory path: per-half-bin norm^2 */
     int *d_zmap, *d_wmap;
+    unsigned *d_rowtab;   /* harmonic-sum address tables (agpu.h) */
+    long long *d_cbase;
+    int *d_clo;
     agpu_hit *d_hits;
     unsigned *d_nhits;
     unsigned cap;
@@ -524,15 +374,15 @@ static int build_tables(accel_search *s, agpu_window **win_out, int **zmap_out,
         for (int k = 1; k < H; k += 2, f++) {
             double hf = (double) k / (double) H;
             for (int zi = 0; zi < numz; zi++)
-                zmap[f * numz + zi] = (calc_required_z(hf, g->zlo + zi * ACCEL_DZ) + zmax) / ACCEL_DZ;
+                zmap[f * numz + zi] = (ar_required_z(hf, g->zlo + zi * ACCEL_DZ) + zmax) / ACCEL_DZ;
             for (int wi = 0; wi < numw1; wi++)
                 wmap[f * numw1 + wi] =
-                    g->numw ? (calc_required_w(hf, g->wlo + wi * ACCEL_DW) + wmax) / ACCEL_DW : 0;
+                    g->numw ? (ar_required_w(hf, g->wlo + wi * ACCEL_DW) + wmax) / ACCEL_DW : 0;
         }
     }
     /* normalisation windows */
     agpu_window *win;
-    int fund_hw = w_resp_halfwidth(-g->zhi, -g->whi);
+    int fund_hw = ar_w_halfwidth(-g->zhi, -g->whi);
     int fund_nd = g->fftlen / ACCEL_NUMBETWEEN;
     s->maxn = fund_nd;
     if (g->inmem) {
@@ -560,9 +410,9 @@ static int build_tables(accel_search *s, agpu_window **win_out, int **zmap_out,
             int H = 1 << st;
             for (int k = 1; k < H; k += 2, f++) {
                 double hf = (double) k / (double) H;
-                int zmax_h = calc_required_z(hf, zmax), wmax_h = calc_required_w(hf, wmax);
-                hwf[f] = w_resp_halfwidth(-zmax_h, -wmax_h);
-                ndf[f] = calc_fftlen(H, k, zmax, wmax, g->corr_uselen) / ACCEL_NUMBETWEEN;
+                int zmax_h = ar_required_z(hf, zmax), wmax_h = ar_required_w(hf, wmax);
+                hwf[f] = ar_w_halfwidth(-zmax_h, -wmax_h);
+                ndf[f] = ar_calc_fftlen(H, k, zmax, wmax, g->corr_uselen) / ACCEL_NUMBETWEEN;
                 if (ndf[f] > s->maxn)
                     s->maxn = ndf[f];
             }
@@ -577,7 +427,7 @@ static int build_tables(accel_search *s, agpu_window **win_out, int **zmap_out,
                 int H = 1 << st;
                 for (int k = 1; k < H; k += 2, f++) {
                     double hf = (double) k / (double) H;
-                    long long rlo_h = (long long) floor(calc_required_r(hf, startr));
+                    long long rlo_h = (long long) floor(ar_required_r(hf, startr));
                     wb[1 + f].lobin = rlo_h - hwf[f];
                     wb[1 + f].numdata = ndf[f];
                 }
@@ -682,7 +532,7 @@ static int gen_templates(accel_search *s)
             int t = wi * g->numz + zi;
             zs[t] = (int) g->zlo + zi * ACCEL_DZ;
             ws[t] = g->numw ? (int) g->wlo + wi * ACCEL_DW : 0;
-            nk[t] = 2 * ACCEL_NUMBETWEEN * w_resp_halfwidth(zs[t], ws[t]);
+            nk[t] = 2 * ACCEL_NUMBETWEEN * ar_w_halfwidth(zs[t], ws[t]);
         }
     }
     int rc = agpu_gen_templates(zs, ws, nk, s->nt, s->L, s->d_T, s->stream);
@@ -822,8 +672,36 @@ static int search_init(accel_search *s, int device)
     GPU(agpu_h2d(s->d_zmap, zmap, sizeof(int) * (size_t) nf * g->numz, s->stream));
     GPU(agpu_h2d(s->d_wmap, wmap, sizeof(int) * (size_t) nf * s->numw1, s->stream));
     free(win);
+    /* harmonic-sum address tables: per (w, z) row and per chunk */
+    const long long rows = (long long) s->nt;
+    const long long ncell = (long long) g->nblocks * g->corr_uselen;
+    const long long rh0 = 2 * fund_rlo(g->rlo);
+    const long long nchunks = ncell > 0 ? (rh0 + ncell - 1) / 1024 - rh0 / 1024 + 1 : 1;
+    const int mode = g->inmem ? 1 : (s->d_bstart ? 2 : 0);
+    unsigned *rowtab = (unsigned *) malloc(sizeof(unsigned) * AGPU_HS_RT * (size_t) rows);
+    long long *cbase = (long long *) malloc(sizeof(long long) * AGPU_HS_CT * (size_t) nchunks);
+    int *clo = (int *) malloc(sizeof(int) * AGPU_HS_CT * (size_t) nchunks);
+    int rc = agpu_harmsum_row_table(g->numz, s->numw1, s->nfrac, zmap, wmap, rowtab) ||
+             agpu_harmsum_chunk_tables(mode, g->inmem ? 0 : s->rh_off, rh0, nchunks,
+                                       g->numharmstages, (int) rows, cbase, clo) ||
+             agpu_malloc((void **) &s->d_rowtab, sizeof(unsigned) * AGPU_HS_RT * (size_t) rows) ||
+             agpu_malloc((void **) &s->d_cbase,
+                         sizeof(long long) * AGPU_HS_CT * (size_t) nchunks) ||
+             agpu_malloc((void **) &s->d_clo, sizeof(int) * AGPU_HS_CT * (size_t) nchunks) ||
+             agpu_h2d(s->d_rowtab, rowtab, sizeof(unsigned) * AGPU_HS_RT * (size_t) rows,
+                      s->stream) ||
+             agpu_h2d(s->d_cbase, cbase, sizeof(long long) * AGPU_HS_CT * (size_t) nchunks,
+                      s->stream) ||
+             agpu_h2d(s->d_clo, clo, sizeof(int) * AGPU_HS_CT * (size_t) nchunks, s->stream);
+    free(rowtab);
+    free(cbase);
+    free(clo);
     free(zmap);
     free(wmap);
+    if (rc) {
+        set_err("harmonic-sum tables: %s", agpu_last_error());
+        return -1;
+    }
     if (bank_get(s))
         return -1;
     return 0;
@@ -859,6 +737,9 @@ void accel_search_destroy(accel_search *s)
     agpu_free(s->d_nsq_hb);
     agpu_free(s->d_zmap);
     agpu_free(s->d_wmap);
+    agpu_free(s->d_rowtab);
+    agpu_free(s->d_cbase);
+    agpu_free(s->d_clo);
     agpu_free(s->d_hits);
     agpu_free(s->d_nhits);
     agpu_free(s->d_bstart);
@@ -928,6 +809,9 @@ static void fill_harmsum_args(const accel_search *s, agpu_harmsum_args *a)
     a->nfrac = s->nfrac;
     a->zmap = s->d_zmap;
     a->wmap = s->d_wmap;
+    a->row_tab = s->d_rowtab;
+    a->chunk_base = s->d_cbase;
+    a->chunk_lo = s->d_clo;
     a->nsq = s->d_nsq;
     a->nsq_hb = s->d_nsq_hb;
     a->npre = s->npre;
@@ -1383,12 +1267,12 @@ long long accel_search_plane_cells(const accel_search *s)
  * (zin, win) with numharm harmonics: every template of harmonic n has
  * |z| <= |zin| n + 4 and |w| <= |win| n + 40, so its half width is at most
  * 1.1 (13/12 |w| + 3/2 |z|) + 16 (w_resp_halfwidth, src/responses.c:48-84)
- * and its chirp next2(6 hw + 200 + 2 hw_vol) points (src/responses.c:381). */
+ * and its chirp next2_to_n(6 hw + 200 + 2 hw_vol) points (src/responses.c:381). */
 static int rzw_npts_bound(int numharm, double zin, double win)
 {
     double zb = fabs(zin) * numharm + 4.0, wb = fabs(win) * numharm + 40.0;
     long long hw = (long long) (1.1 * (13.0 / 12.0 * wb + 1.5 * zb)) + 17;
-    return (int) next2(8 * hw + 200);
+    return (int) ar_next2(8 * hw + 200);
 }
 
 /* Device buffers of the refinement, kept by their owner between calls:

@@ -37,21 +37,11 @@
 #include <sys/stat.h>
 #include <unistd.h>
 #include "accel_surface.h"
+#include "accel_rules.h"
 #include "accel_internal.h"
 #include "agpu.h"
 
-/* include/accel.h:18-35, include/presto.h:99-107 */
-#define ACCEL_NUMBETWEEN 2
-#define ACCEL_DR 0.5
-#define ACCEL_RDR 2
-#define ACCEL_DZ 2
-#define ACCEL_RDZ 0.5
-#define ACCEL_DW 20
-#define ACCEL_RDW 0.05
-#define ACCEL_CLOSEST_R 15.0
-#define ACCEL_PADDING 2000
-#define DBLCORRECT 1e-14
-#define NUMFINTBINS 16
+#define ACCEL_PADDING 2000   /* include/accel.h:30 */
 #define SF_HIT_CAP0 (1u << 16)
 
 /* ---------------------------------------------------------------------- */
@@ -157,62 +147,9 @@ static int slist_length(const GSList *l)
 /* ---------------------------------------------------------------------- */
 /* index rules: src/accel_utils.c:36-102, src/responses.c:27-84            */
 
-static int nearest_int(double x) { return (int) (x < 0 ? x - 0.5 : x + 0.5); }
-static double required_r(double hf, double rfull) { return rint(ACCEL_RDR * rfull * hf) * ACCEL_DR; }
-static int required_z(double hf, double zfull) { return nearest_int(ACCEL_RDZ * zfull * hf) * ACCEL_DZ; }
-static int required_w(double hf, double wfull) { return nearest_int(ACCEL_RDW * wfull * hf) * ACCEL_DW; }
 static int index_r(double r, double lor) { return (int) ((r - lor) * ACCEL_RDR + DBLCORRECT); }
 static int index_z(double z, double loz) { return (int) ((z - loz) * ACCEL_RDZ + DBLCORRECT); }
 static int index_w(double w, double low) { return (int) ((w - low) * ACCEL_RDW + DBLCORRECT); }
-
-static int z_halfwidth(double z) { return (int) (0.5 * 1.1 * fabs(z)) + NUMFINTBINS; }
-
-static int w_halfwidth(double z, double w)
-{
-    if (fabs(w) < 1.0e-7)
-        return z_halfwidth(z);
-    double r0 = 0.5 * (w / 6.0 - z), r1 = 0.5 * (w / 6.0 + z);
-    double maxdev = fabs(r0) > fabs(r1) ? fabs(r0) : fabs(r1);
-    double u_ext = 0.5 - z / w;
-    if (u_ext > 0.0 && u_ext < 1.0) {
-        double r_ext = 0.5 * w * u_ext * u_ext + (z - w / 2.0) * u_ext + r0;
-        if (fabs(r_ext) > maxdev)
-            maxdev = fabs(r_ext);
-    }
-    return (int) (1.1 * maxdev) + NUMFINTBINS;
-}
-
-static long long next2(long long x)
-{
-    long long t = 1;
-    while (t < x)
-        t <<= 1;
-    return t;
-}
-
-/* src/corr_prep.c:3-35 */
-static int next_good_fftlen(int n)
-{
-    static const int lens[17] = {128, 192, 256, 384, 512, 768, 1024, 1280, 2048,
-                                 4096, 5120, 7680, 10240, 12288, 15360, 16384, 25600};
-    if (n <= lens[0])
-        return lens[0];
-    if (n > lens[16])
-        return (int) next2(n);
-    int i = 0;
-    while (n > lens[i])
-        i++;
-    return lens[i];
-}
-
-/* src/accel_utils.c:137-150 */
-static int calc_fftlen(int numharm, int harmnum, int zmax, int wmax, int corr_uselen)
-{
-    double hf = (double) harmnum / (double) numharm;
-    int bins_needed = (int) ceil(corr_uselen * hf) + 2;
-    int end_effects = 2 * ACCEL_NUMBETWEEN * w_halfwidth(required_z(hf, zmax), required_w(hf, wmax));
-    return next_good_fftlen(bins_needed + end_effects);
-}
 
 static int twon_to_index(int n)
 {
@@ -998,15 +935,15 @@ static int init_subharminfo(int numharm, int harmnum, int zmax, int wmax, subhar
     memset(shi, 0, sizeof(*shi));
     shi->numharm = numharm;
     shi->harmnum = harmnum;
-    shi->zmax = required_z(hf, zmax);
-    shi->wmax = required_w(hf, wmax);
+    shi->zmax = ar_required_z(hf, zmax);
+    shi->wmax = ar_required_w(hf, wmax);
     if (numharm > 1) {
         shi->rinds = (unsigned short *) malloc(sizeof(unsigned short) * obs->corr_uselen);
         shi->zinds = (unsigned short *) malloc(sizeof(unsigned short) * obs->corr_uselen);
     }
     const int fftlen = (numharm == 1 && harmnum == 1)
                            ? obs->fftlen
-                           : calc_fftlen(numharm, harmnum, zmax, wmax, obs->corr_uselen);
+                           : ar_calc_fftlen(numharm, harmnum, zmax, wmax, obs->corr_uselen);
     shi->numkern_zdim = (shi->zmax / ACCEL_DZ) * 2 + 1;
     shi->numkern_wdim = (shi->wmax / ACCEL_DW) * 2 + 1;
     shi->numkern = shi->numkern_zdim * shi->numkern_wdim;
@@ -1027,7 +964,7 @@ static int init_subharminfo(int numharm, int harmnum, int zmax, int wmax, subhar
             k->w = -shi->wmax + i * ACCEL_DW;
             k->fftlen = fftlen;
             k->numbetween = ACCEL_NUMBETWEEN;
-            k->kern_half_width = w_halfwidth((double) k->z, (double) k->w);
+            k->kern_half_width = ar_w_halfwidth((double) k->z, (double) k->w);
             k->numgoodbins = k->fftlen - 2 * k->numbetween * k->kern_half_width;
             k->data = NULL;
         }
@@ -1145,18 +1082,18 @@ ffdotpows *subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
         return NULL;
     }
     const double hf = (double) harmnum / (double) numharm;
-    const double drlo = required_r(hf, fullrlo), drhi = required_r(hf, fullrhi);
+    const double drlo = ar_required_r(hf, fullrlo), drhi = ar_required_r(hf, fullrhi);
     const long long rlo = (long long) floor(drlo);
-    const int zlo = required_z(hf, obs->zlo), wlo = required_w(hf, obs->wlo);
+    const int zlo = ar_required_z(hf, obs->zlo), wlo = ar_required_w(hf, obs->wlo);
     /* the lookup tables, src/accel_utils.c:1042-1053 */
     if (numharm > 1 && !obs->inmem) {
         for (int ii = 0; ii < obs->corr_uselen; ii++) {
             double rr = fullrlo + ii * ACCEL_DR;
-            shi->rinds[ii] = (unsigned short) index_r(required_r(hf, rr), (double) rlo);
+            shi->rinds[ii] = (unsigned short) index_r(ar_required_r(hf, rr), (double) rlo);
         }
         for (int ii = 0; ii < obs->numz; ii++) {
             double zz = obs->zlo + ii * ACCEL_DZ;
-            shi->zinds[ii] = (unsigned short) index_z((double) required_z(hf, zz), (double) zlo);
+            shi->zinds[ii] = (unsigned short) index_z((double) ar_required_z(hf, zz), (double) zlo);
         }
     }
     int numrs = (int) ((ceil(drhi) - floor(drlo)) * ACCEL_RDR + DBLCORRECT) + 1;
@@ -1318,15 +1255,15 @@ void add_ffdotpows(ffdotpows *fundamental, ffdotpows *subharmonic, int numharm, 
     /* every index the device will form, checked at its extremes (the
      * rules are monotonic in r, z and w); inside [0, 65535] the tables'
      * unsigned short storage changes nothing */
-    const int r0 = index_r(required_r(hf, a.sub_fullrlo), (double) a.sub_rlo);
-    const int r1 = index_r(required_r(hf, a.sub_fullrlo + (a.fnumrs - 1) * ACCEL_DR),
+    const int r0 = index_r(ar_required_r(hf, a.sub_fullrlo), (double) a.sub_rlo);
+    const int r1 = index_r(ar_required_r(hf, a.sub_fullrlo + (a.fnumrs - 1) * ACCEL_DR),
                            (double) a.sub_rlo);
-    const int z0 = index_z((double) required_z(hf, a.obs_zlo), (double) a.sub_zlo);
-    const int z1 = index_z((double) required_z(hf, a.obs_zlo + (a.fnumzs - 1) * ACCEL_DZ),
+    const int z0 = index_z((double) ar_required_z(hf, a.obs_zlo), (double) a.sub_zlo);
+    const int z1 = index_z((double) ar_required_z(hf, a.obs_zlo + (a.fnumzs - 1) * ACCEL_DZ),
                            (double) a.sub_zlo);
-    const int w0 = index_w((double) required_w(hf, (double) a.fund_wlo), (double) a.sub_wlo);
+    const int w0 = index_w((double) ar_required_w(hf, (double) a.fund_wlo), (double) a.sub_wlo);
     const int w1 = index_w(
-        (double) required_w(hf, (double) (a.fund_wlo + (a.fnumws - 1) * ACCEL_DW)),
+        (double) ar_required_w(hf, (double) (a.fund_wlo + (a.fnumws - 1) * ACCEL_DW)),
         (double) a.sub_wlo);
     if (r0 < 0 || r1 >= a.snumrs || z0 < 0 || z1 >= a.snumzs || w0 < 0 || w1 >= a.snumws) {
         sf_fail(-1, "add_ffdotpows: the sub-harmonic plane does not cover the fundamental");
@@ -1398,8 +1335,8 @@ void inmem_add_ffdotpows(ffdotpows *fundamental, accelobs *obs, int numharm, int
     }
     const int rmax = (int) ((ACCEL_RDR * rlo + numrs - 1) * hf + 0.5);
     const int rmin = (int) ((ACCEL_RDR * rlo) * hf + 0.5);
-    const int z0 = index_z((double) required_z(hf, (double) zlo), (double) zlo);
-    const int z1 = index_z((double) required_z(hf, (double) (zlo + (numzs - 1) * ACCEL_DZ)),
+    const int z0 = index_z((double) ar_required_z(hf, (double) zlo), (double) zlo);
+    const int z1 = index_z((double) ar_required_z(hf, (double) (zlo + (numzs - 1) * ACCEL_DZ)),
                            (double) zlo);
     if (rmin < 0 || rmax >= st->rlen || z0 < 0 || z1 >= obs->numz || fundamental->numws != 1) {
         sf_fail(-1, "inmem_add_ffdotpows: indices outside the full plane");
@@ -1468,61 +1405,21 @@ static void replace_data(GSList *n, float power, float sigma, int numharm, doubl
     n->data = create_accelcand(power, sigma, numharm, r, z, w);
 }
 
-/* insert_new_accelcand, src/accel_utils.c:387-469 */
-static GSList *insert_new_accelcand(GSList *list, float power, float sigma, int numharm,
-                                    double rr, double zz, double ww, int *added)
+static GSList *node_new_cand(float power, float sigma, int numharm, double r, double z,
+                            double w)
 {
-    GSList *tmp = list, *prev = NULL;
-    double prev_diff = ACCEL_CLOSEST_R + 1.0, next_diff;
-    *added = 0;
-    if (!list) {
-        *added = 1;
-        return node_new(create_accelcand(power, sigma, numharm, rr, zz, ww));
-    }
-    while (tmp->next && cand_r(tmp) < rr) {
-        prev = tmp;
-        tmp = tmp->next;
-    }
-    next_diff = fabs(rr - cand_r(tmp));
-    if (prev)
-        prev_diff = fabs(rr - cand_r(prev));
-    if (prev_diff < ACCEL_CLOSEST_R) {
-        if (cand_sigma(prev) < sigma) {
-            replace_data(prev, power, sigma, numharm, rr, zz, ww);
-            *added = 1;
-        }
-        if (next_diff < ACCEL_CLOSEST_R && cand_sigma(tmp) < sigma) {
-            if (*added) {
-                free_accelcand(tmp->data, NULL);
-                prev->next = tmp->next;
-                node_free(tmp);
-            } else {
-                replace_data(tmp, power, sigma, numharm, rr, zz, ww);
-                *added = 1;
-            }
-        }
-    } else if (next_diff < ACCEL_CLOSEST_R) {
-        if (cand_sigma(tmp) < sigma) {
-            replace_data(tmp, power, sigma, numharm, rr, zz, ww);
-            *added = 1;
-        }
-    } else {
-        GSList *n = node_new(create_accelcand(power, sigma, numharm, rr, zz, ww));
-        *added = 1;
-        if (!tmp->next && cand_r(tmp) < rr - ACCEL_CLOSEST_R) {
-            tmp->next = n;
-            return list;
-        }
-        if (prev) {
-            prev->next = n;
-            n->next = tmp;
-        } else {
-            n->next = list;
-            return n;
-        }
-    }
-    return list;
+    return node_new(create_accelcand(power, sigma, numharm, r, z, w));
 }
+
+static void node_drop(GSList *n)
+{
+    free_accelcand(n->data, NULL);
+    node_free(n);
+}
+
+/* insert_new_accelcand, src/accel_utils.c:387-469 */
+ACCEL_DEFINE_INSERT(insert_new_accelcand, GSList, cand_r, cand_sigma, replace_data, node_new_cand,
+                    node_drop)
 
 static int cmp_hit_idx(const void *a, const void *b)
 {
@@ -1634,52 +1531,15 @@ GSList *sort_accelcands(GSList *list)
     return head.next;
 }
 
+ACCEL_DEFINE_ELIMINATE(eliminate_list, GSList, cand_r, node_drop)
+
 /* eliminate_harmonics, src/accel_utils.c:472-549 */
 GSList *eliminate_harmonics(GSList *cands, int *numcands)
 {
-    static const double ratios[23][2] = {
-        {3, 2}, {5, 2}, {2, 3}, {4, 3}, {5, 3}, {3, 4}, {5, 4}, {2, 5},
-        {3, 5}, {4, 5}, {5, 6}, {2, 7}, {3, 7}, {4, 7}, {3, 8}, {5, 8},
-        {2, 9}, {3, 10}, {2, 11}, {3, 11}, {2, 13}, {3, 13}, {2, 15}};
-    const int maxharm = 16;
-    const double tooclose = 1.5;
-    int numremoved = 0;
-    if (!cands)
-        return cands;
-    GSList *cur = cands;
-    while (cur->next) {
-        const accelcand *c = (const accelcand *) cur->data;
-        GSList *prev = cur, *other = cur->next;
-        do {
-            const accelcand *o = (const accelcand *) other->data;
-            int remove = 0;
-            for (int i = 1; i <= maxharm && !remove; i++)
-                if (fabs(c->r / i - o->r) < tooclose)
-                    remove = 1;
-            for (int i = 1; i <= maxharm && !remove; i++)
-                if (fabs(c->r * i - o->r) < tooclose)
-                    remove = 1;
-            for (int i = 0; i < 23 && !remove; i++)
-                if (fabs(c->r * ratios[i][0] / ratios[i][1] - o->r) < tooclose)
-                    remove = 1;
-            if (remove) {
-                numremoved++;
-                GSList *gone = other;
-                other = other->next;
-                free_accelcand(gone->data, NULL);
-                prev->next = other;
-                node_free(gone);
-                *numcands = *numcands - 1;
-            } else {
-                prev = other;
-                other = other->next;
-            }
-        } while (other);
-        if (cur->next)
-            cur = cur->next;
-    }
-    if (numremoved)
-        printf("Removed %d likely harmonically related candidates.\n", numremoved);
+    const int before = *numcands;
+    cands = eliminate_list(cands, numcands);
+    if (before - *numcands)
+        printf("Removed %d likely harmonically related candidates.\n", before - *numcands);
     return cands;
 }
 

@@ -79,7 +79,28 @@ typedef struct {
     agpu_hit *hits;
     unsigned int *nhits;
     unsigned int cap;
+    /* address tables (agpu_harmsum_row_table / agpu_harmsum_chunk_tables) */
+    const unsigned *row_tab;       /* [numz numw][AGPU_HS_RT] */
+    const long long *chunk_base;   /* [nchunks][AGPU_HS_CT] */
+    const int *chunk_lo;           /* [nchunks][AGPU_HS_CT] */
 } agpu_harmsum_args;
+
+/* Row table of the harmonic sum, per (w, z) row: [0] w index, [1] z index,
+ * [AGPU_HS_RT_ROW] the row's offset in a tile (row x AGPU_ROWLEN floats),
+ * [AGPU_HS_RT_SUB + f] the offset of fraction f's sub-harmonic row.  Chunk
+ * tables, per chunk of 1024 fundamental half-bins: base[0] the plane offset
+ * (floats) of the chunk's first half-bin in row 0's tile, base[1 + f] that
+ * of fraction f's span start lo[1 + f] (its first half-bin). */
+#define AGPU_HS_RT 36
+#define AGPU_HS_RT_ROW 2
+#define AGPU_HS_RT_SUB 3
+#define AGPU_HS_CT 32
+int agpu_harmsum_row_table(int numz, int numw, int nfrac, const int *zmap, const int *wmap,
+                           unsigned *out);
+/* mode: 0 standard, 1 in-memory, 2 standard with per-cell FP64 lookups;
+ * rh_off as in agpu_harmsum_args */
+int agpu_harmsum_chunk_tables(int mode, int rh_off, long long rh0, long long nchunks,
+                              int nstages, int nrows, long long *base, int *lo);
 
 int agpu_init(int device);
 const char *agpu_last_error(void);
