@@ -57,21 +57,55 @@ def _stage(msg):
     print(f"bench: {msg} (peak rss {rss:.0f} MiB)", file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(fft, N, dt, max_blocks, threads):
     """The oracle (C restatement of the reference's search, templates of a
-    block in parallel over OpenMP threads as the reference's loop) on the
-    first max_blocks blocks of the same spectrum."""
+    block in parallel over OpenMP threads as the reference's loop at
+    src/accel_utils.c:1141-1193) on the first max_blocks blocks of the plain
+    workload's spectrum, and on the whole CPU-reference workload (2^20
+    samples, zmax 50, numharm 8, in-memory) with `threads` threads and with
+    one; with the host CPU model and the oracle's 2048-point FFT time."""
     from oracle import oracle as O
+    from presto_amd import accel as A, synth
     O.lib.o_set_threads(threads)
     o = O.make_obs(len(fft), N, dt, zmax=200, numharm=16, sigma=2.0)
     t0 = time.perf_counter()
     _, _, nb = O.accelsearch(o, fft, max_blocks=max_blocks)
     t = time.perf_counter() - t0
     cells = nb * o.corr_uselen * o.numz
+    x1 = synth.config1_series()
+    f1 = A.spectrum_from_series(x1, 64e-6)
+    o1 = O.make_obs(len(f1), len(x1), 64e-6, zmax=50, numharm=8, sigma=2.0)
+    c1_cells = None
+    c1 = {}
+    for nt in (threads, 1):
+        O.lib.o_set_threads(nt)
+        t1 = time.perf_counter()
+        _, cands, nb1 = O.accelsearch(o1, f1)
+        el = time.perf_counter() - t1
+        c1_cells = nb1 * o1.corr_uselen * o1.numz
+        c1[f"threads_{nt}"] = {"seconds": el, "cells_per_s": c1_cells / el,
+                               "dm_trials_per_s": 1.0 / el, "candidates": len(cands)}
+    O.lib.o_set_threads(threads)
     return dict(value=cells / t, unit="ffdot cells/s", cores=threads, kind="port",
-                sample=f"first {nb} of the workload's {o.numz}-template blocks "
+                sample=f"first {nb} of the plain workload's {o.numz}-template blocks "
                        f"(standard path, all 16 harmonic stages), {t:.1f} s on "
-                       f"{threads} OpenMP threads")
+                       f"{threads} OpenMP threads",
+                cpu_model=_cpu_model(), host_cpus=os.cpu_count(),
+                fft2048_ns=O.lib.o_time_cfft_ns(2048, 101),
+                cpu_reference_workload={
+                    "workload": "2^20 samples, dt 64 us, one pulsar at 123.456 Hz (z 20), "
+                                "zmax 50, numharm 8, sigma 2, in-memory, whole search",
+                    "cells": c1_cells, **c1})
 
 
 def batched_section(ntrials, threads):
@@ -159,7 +193,9 @@ def serving_section(nreq, rate, threads, budget_gib, workers):
     arrive as a Poisson stream and go to the request loop (accel_server),
     which admits them by device memory and runs them concurrently; each is
     taken through spectrum preparation on the GPU, search and refinement.
-    Latency = arrival to result."""
+    Latency = arrival to result.  Series are generated by a thread pool a
+    bounded number of requests ahead of their arrival."""
+    import collections
     import threading
     from concurrent.futures import ThreadPoolExecutor
     from presto_amd import accel as A, synth
@@ -170,17 +206,20 @@ def serving_section(nreq, rate, threads, budget_gib, workers):
         i, t, N, dt, zmax, wmax, nh, seed, ps = q
         return synth.make_series_f32(N, dt, seed, ps)
 
-    with ThreadPoolExecutor(max(1, threads)) as ex:
-        series = list(ex.map(gen, reqs))
     params = [A.make_params(q[2] // 2, q[2], q[3], zmax=q[4], numharm=q[6], sigma=2.0,
                             wmax=(q[5] if q[5] else None)) for q in reqs]
-    cells = 0
+    cells = []
     for p in params:
         g = A.geometry(p)
-        cells += g.nblocks * g.corr_uselen * g.numz * max(g.numw, 1)
-    _stage(f"serving workload: {nreq} requests prepared")
+        cells.append(g.nblocks * g.corr_uselen * g.numz * max(g.numw, 1))
+    _stage(f"serving workload: {nreq} requests drawn")
     lat, svc, status = [None] * nreq, [None] * nreq, [None] * nreq
-    with A.Server(budget_bytes=int(budget_gib) << 30, workers=workers) as sv:
+    done_at, timing = [None] * nreq, [None] * nreq
+    ahead = 96
+    with ThreadPoolExecutor(max(1, threads)) as ex, \
+            A.Server(budget_bytes=int(budget_gib) << 30, workers=workers) as sv:
+        pending = collections.deque(ex.submit(gen, q) for q in reqs[:ahead])
+        nxt = min(ahead, nreq)
         tickets = [None] * nreq
         ready = threading.Semaphore(0)
 
@@ -188,32 +227,54 @@ def serving_section(nreq, rate, threads, budget_gib, workers):
             for i in range(nreq):
                 ready.acquire()
                 status[i], lat[i], svc[i] = sv.wait(tickets[i])
+                done_at[i] = time.perf_counter()
+                timing[i] = sv.timing(tickets[i])
+                if (i + 1) % 500 == 0:
+                    _stage(f"serving workload: {i + 1}/{nreq} requests done")
 
         th = threading.Thread(target=waiter)
         th.start()
         t0 = time.perf_counter()
+        late = 0
         for i, q in enumerate(reqs):
+            x = pending.popleft().result()
+            if nxt < nreq:
+                pending.append(ex.submit(gen, reqs[nxt]))
+                nxt += 1
             dtw = t0 + q[1] - time.perf_counter()
             if dtw > 0:
                 time.sleep(dtw)
-            tickets[i] = sv.submit(params[i], series[i])
+            elif dtw < -0.05:
+                late += 1
+            tickets[i] = sv.submit(params[i], x)
             ready.release()
         th.join()
         wall = time.perf_counter() - t0
     ok = [i for i in range(nreq) if status[i] == 0]
     lat_ok = [lat[i] for i in ok]
+    # the sustained rate: completions over the second half of the run
+    half = sorted(done_at[i] for i in ok)
+    mid = len(half) // 2
+    sustained = (len(half) - mid - 1) / (half[-1] - half[mid]) if len(half) > 4 else None
+    phases = {k: float(np.mean([timing[i][k] for i in ok]))
+              for k in ("queued", "create", "prep", "search", "refine", "service")}
+    lat2 = [lat[i] for i in ok[len(ok) // 2:]]
     _stage(f"serving workload: {len(ok)}/{nreq} served in {wall:.2f} s")
     return {"workload": f"{nreq} requests, Poisson arrivals at {rate:g}/s, lengths 2^20..2^24 "
-                        "(log-uniform, powers of two), zmax {0,50,200}, wmax {0,100}, "
-                        "numharm {8,16}, 20% with a pulsar; GPU spectrum preparation, search "
-                        "and refinement per request",
+                        "(log-uniform, nearest even length with factors 2,3,5,7), zmax "
+                        "{0,50,200}, wmax {0,100}, numharm {8,16}, 20% with a pulsar; GPU "
+                        "spectrum preparation, search and refinement per request",
             "requests": nreq, "served": len(ok), "workers": workers,
             "hbm_budget_gib": budget_gib,
-            "dm_trials_per_s": len(ok) / wall, "cells_per_s": cells / wall,
-            "offered_rate_per_s": rate,
+            "dm_trials_per_s": len(ok) / wall, "sustained_dm_trials_per_s": sustained,
+            "cells_per_s": sum(cells[i] for i in ok) / wall,
+            "offered_rate_per_s": rate, "submitted_late": late,
             "p50_ms": float(np.percentile(lat_ok, 50)) if ok else None,
             "p99_ms": float(np.percentile(lat_ok, 99)) if ok else None,
-            "service_p50_ms": float(np.percentile([svc[i] for i in ok], 50)) if ok else None}
+            "p50_ms_second_half": float(np.percentile(lat2, 50)) if lat2 else None,
+            "p99_ms_second_half": float(np.percentile(lat2, 99)) if lat2 else None,
+            "service_p50_ms": float(np.percentile([svc[i] for i in ok], 50)) if ok else None,
+            "mean_ms": phases}
 
 
 def dedisp_section(numdms=64):
@@ -366,7 +427,7 @@ def main():
                     help="blocks of the oracle CPU baseline sample (0: skip)")
     ap.add_argument("--batch-trials", type=int, default=256,
                     help="spectra of the batched DM-trial section (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=120,
+    ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests of the serving section (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-workers", type=int, default=4)

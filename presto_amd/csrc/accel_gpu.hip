@@ -628,6 +628,11 @@ __device__ __forceinline__ void emit_hit(const agpu_harmsum_args &a, int b, int 
  * multiple of the workgroup's waves so that every wave issues the same
  * loads without a bound check; the extra floats stay inside the tile row
  * and its halo. */
+__host__ __device__ constexpr int hs_frac_wl_span(int k, int H)
+{
+    return (HS_CHUNK * k / H + 1 + 63) / 64;
+}
+
 __host__ __device__ constexpr int hs_frac_wl(int k, int H)
 {
     return ((HS_CHUNK * k / H + 1 + 63) / 64 + HS_WAVES - 1) / HS_WAVES * HS_WAVES;
@@ -727,10 +732,15 @@ __device__ __forceinline__ void hs_load(const agpu_harmsum_args &a, float *v, co
 #pragma unroll
     for (int k = 1; k < H; k += 2) {
         const int f = F0 + (k - 1) / 2;
-        const float *b1 = a.plane + cb[1 + f] + rt[HS_RT_SUB + f] + wv * 64;
+        const float *b1 = a.plane + cb[1 + f] + rt[HS_RT_SUB + f];
+        /* wave loads past the span's own (the padding to a multiple of the
+         * waves) repeat its last one: cache hits, no extra HBM traffic */
+        const int real = hs_frac_wl_span(k, H);
 #pragma unroll
-        for (int u = 0; u < hs_frac_regs(k, H); u++)
-            v[r + u] = b1[u * (HS_WAVES * 64) + lane];
+        for (int u = 0; u < hs_frac_regs(k, H); u++) {
+            const int wl = min(wv + HS_WAVES * u, real - 1);
+            v[r + u] = b1[wl * 64 + lane];
+        }
         r += hs_frac_regs(k, H);
     }
 }
@@ -1049,7 +1059,7 @@ int agpu_arena_release(void)
         snprintf(g_err, sizeof(g_err), "agpu_arena_release: %d blocks still allocated", g_nused);
         rc = -2;
     } else if (g_arena) {
-        hipFree(g_arena);
+        (void) hipFree(g_arena);
         g_arena = NULL;
         g_arena_len = g_arena_used = 0;
         g_nfree = 0;
@@ -1216,12 +1226,30 @@ int agpu_event_elapsed(float *ms, void *start, void *stop)
         }                                                                         \
     } while (0)
 
-int agpu_gen_templates(const int *zs, const int *ws, const int *numkern, int nt, int L,
-                       void *d_T, void *stream)
+/* The taps of every template rounded to FP32, as the reference stores
+ * gen_z_response / gen_w_response (src/responses.c:159-418): row t at
+ * R + t*stride, zero past its numkern taps. */
+__global__ void k_resp_round(const double2 *resp, const int *nks, int nkmax, int stride,
+                             float2 *R)
+{
+    const int t = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= stride)
+        return;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (k < nks[t] && k < nkmax) {
+        const double2 d = resp[(size_t) t * nkmax + k];
+        v = make_float2((float) d.x, (float) d.y);
+    }
+    R[(size_t) t * stride + k] = v;
+}
+
+/* The response stage shared by the template bank (d_T, length L) and the
+ * FP32 tap table (d_R, row stride Rstride); either may be NULL. */
+static int gen_responses(const int *zs, const int *ws, const int *numkern, int nt, int L,
+                         void *d_T, int Rstride, void *d_R, void *stream)
 {
     hipStream_t st = (hipStream_t) stream;
-    if (nt <= 0 || L <= 0 || nt > 65535)
-        return fail("agpu_gen_templates: bad template count or length");
     int rc = -1;
     int nkmax = 0, nw = 0, nptsmax = 0;
     /* scratch comes from agpu_malloc: inside a request loop's arena no
@@ -1235,7 +1263,8 @@ int agpu_gen_templates(const int *zs, const int *ws, const int *numkern, int nt,
         goto cleanup;
     }
     for (int t = 0; t < nt; t++) {
-        if (numkern[t] <= 0 || numkern[t] % 4 || numkern[t] > L) {
+        if (numkern[t] <= 0 || numkern[t] % 4 || (d_T && numkern[t] > L) ||
+            (d_R && numkern[t] > Rstride)) {
             snprintf(g_err, sizeof(g_err),
                      "agpu_gen_templates: kernel width must be a positive multiple of 4 <= L");
             goto cleanup;
@@ -1281,10 +1310,18 @@ int agpu_gen_templates(const int *zs, const int *ws, const int *numkern, int nt,
                                d_w, d_nk, d_np, d_tl, nkmax, d_resp);
             AGPU_TRY(hipGetLastError());
         }
-        dim3 g2((L + 255) / 256, nt);
-        hipLaunchKernelGGL(k_template_dft, g2, dim3(256), 0, st, d_resp, d_nk, nkmax, L,
-                           (float2 *) d_T);
-        AGPU_TRY(hipGetLastError());
+        if (d_T) {
+            dim3 g2((L + 255) / 256, nt);
+            hipLaunchKernelGGL(k_template_dft, g2, dim3(256), 0, st, d_resp, d_nk, nkmax, L,
+                               (float2 *) d_T);
+            AGPU_TRY(hipGetLastError());
+        }
+        if (d_R) {
+            dim3 g3((Rstride + 255) / 256, nt);
+            hipLaunchKernelGGL(k_resp_round, g3, dim3(256), 0, st, d_resp, d_nk, nkmax, Rstride,
+                               (float2 *) d_R);
+            AGPU_TRY(hipGetLastError());
+        }
     }
     rc = 0;
 cleanup:
@@ -1302,6 +1339,22 @@ cleanup:
     free(npts);
     free(tlist);
     return rc;
+}
+
+int agpu_gen_templates(const int *zs, const int *ws, const int *numkern, int nt, int L,
+                       void *d_T, void *stream)
+{
+    if (nt <= 0 || L <= 0 || nt > 65535 || !d_T)
+        return fail("agpu_gen_templates: bad template count or length");
+    return gen_responses(zs, ws, numkern, nt, L, d_T, 0, NULL, stream);
+}
+
+int agpu_gen_response_taps(const int *zs, const int *ws, const int *numkern, int nt,
+                           int stride, void *d_R, void *stream)
+{
+    if (nt <= 0 || stride <= 0 || nt > 65535 || !d_R)
+        return fail("agpu_gen_response_taps: bad template count or stride");
+    return gen_responses(zs, ws, numkern, nt, 0, NULL, stride, d_R, stream);
 }
 
 int agpu_block_norms(const void *d_spec, long long numbins, long long lobin_file,

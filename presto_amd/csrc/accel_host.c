@@ -1373,7 +1373,7 @@ static int refine_device(refine_ws *w, const void *d_spec, long long numdata, in
         if (chunk > fit)
             chunk = fit > 0 ? fit : 1;
     }
-    if (ws_reserve(w, win ? agpu_refine_rzw_scratch(chunk, cap) : 0))
+    if (ws_reserve(w, win ? agpu_refine_rzw_scratch(chunk, cap) : agpu_refine_rz_scratch(chunk)))
         return -1;
     int nev[REFINE_CHUNK];
     for (int c0 = 0; c0 < n; c0 += chunk) {
@@ -1407,9 +1407,10 @@ static int refine_device(refine_ws *w, const void *d_spec, long long numdata, in
         a.scratch = w->d_scr;
         a.npts_cap = cap;
         a.raw = raw;
+        a.maxharm = maxh;
         if (agpu_memset(w->d_der, 0, sizeof(double) * 7 * mhx, stream) ||
             agpu_memset(w->d_pow, 0, sizeof(double) * mhx, stream) ||
-            (win ? agpu_refine_rzw(&a, stream, maxh) : agpu_refine_rz(&a, stream)) ||
+            (win ? agpu_refine_rzw(&a, stream) : agpu_refine_rz(&a, stream)) ||
             agpu_d2h(rout + c0, w->d_rout, md, stream) ||
             agpu_d2h(zout + c0, w->d_zout, md, stream) ||
             (wout && agpu_d2h(wout + c0, w->d_wout, md, stream)) ||
@@ -1423,7 +1424,9 @@ static int refine_device(refine_ws *w, const void *d_spec, long long numdata, in
         }
         for (int i = 0; i < m; i++)
             if (nev[i] < 0) {
-                set_err("refinement: candidate %d exceeded the chirp bound", c0 + i);
+                set_err(win ? "refinement: candidate %d exceeded the chirp bound"
+                            : "refinement: candidate %d: its workgroups' barrier wait ran out",
+                        c0 + i);
                 return -1;
             }
     }

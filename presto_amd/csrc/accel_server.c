@@ -35,6 +35,7 @@ typedef struct {
     int status;
     double t_submit, t_start, t_done;
     double ms_evict, ms_create, ms_work;   /* inside the service time */
+    double ms_prep, ms_search, ms_refine;  /* inside ms_work */
     int ncands;
     accel_cand *cands;
     accel_fourierprops *props;  /* when optimized */
@@ -114,10 +115,19 @@ static int take_handle(accel_server *sv, const accel_params *p, long long bytes,
 static void run_request(accel_server *sv, srv_req *r, accel_search *s)
 {
     r->status = -1;
-    if (accel_search_set_series(s, r->series, r->n) || accel_search_run(s)) {
+    double t0 = now_s();
+    if (accel_search_set_series(s, r->series, r->n)) {
         snprintf(r->err, sizeof(r->err), "%s", accel_last_error());
         return;
     }
+    double t1 = now_s();
+    r->ms_prep = 1e3 * (t1 - t0);
+    if (accel_search_run(s)) {
+        snprintf(r->err, sizeof(r->err), "%s", accel_last_error());
+        return;
+    }
+    double t2 = now_s();
+    r->ms_search = 1e3 * (t2 - t1);
     int n = accel_search_ncands(s);
     r->ncands = n;
     r->cands = (accel_cand *) malloc(sizeof(accel_cand) * (n ? n : 1));
@@ -129,6 +139,7 @@ static void run_request(accel_server *sv, srv_req *r, accel_search *s)
             return;
         }
     }
+    r->ms_refine = 1e3 * (now_s() - t2);
     r->status = 0;
     (void) sv;
 }
@@ -209,6 +220,9 @@ static void *worker(void *arg)
         r = &sv->req[ti];
         r->ms_create = 1e3 * (t1 - t0);
         r->ms_work = 1e3 * (t2 - t1);
+        r->ms_prep = local.ms_prep;
+        r->ms_search = local.ms_search;
+        r->ms_refine = local.ms_refine;
         r->status = local.status;
         r->ncands = local.ncands;
         r->cands = local.cands;
@@ -290,18 +304,21 @@ int accel_server_wait(accel_server *sv, int ticket, double *latency_ms, double *
     return st;
 }
 
-int accel_server_timing(accel_server *sv, int ticket, double *ms6)
+int accel_server_timing(accel_server *sv, int ticket, double *ms9)
 {
     pthread_mutex_lock(&sv->mu);
     int rc = -1;
     if (ticket >= 0 && ticket < sv->nreq && sv->req[ticket].state == 2) {
         srv_req *r = &sv->req[ticket];
-        ms6[0] = 1e3 * (r->t_start - r->t_submit);   /* queued */
-        ms6[1] = r->ms_evict;                         /* evicting cached handles */
-        ms6[2] = r->ms_create;                        /* creating the handle */
-        ms6[3] = r->ms_work;                          /* preparation, search, refinement */
-        ms6[4] = 1e3 * (r->t_done - r->t_start);      /* service */
-        ms6[5] = (double) r->bytes;
+        ms9[0] = 1e3 * (r->t_start - r->t_submit);   /* queued */
+        ms9[1] = r->ms_evict;                         /* evicting cached handles */
+        ms9[2] = r->ms_create;                        /* creating the handle */
+        ms9[3] = r->ms_prep;                          /* upload, real FFT, de-reddening */
+        ms9[4] = r->ms_search;                        /* search through the candidate list */
+        ms9[5] = r->ms_refine;                        /* refinement and properties */
+        ms9[6] = r->ms_work;                          /* the three above */
+        ms9[7] = 1e3 * (r->t_done - r->t_start);      /* service */
+        ms9[8] = (double) r->bytes;
         rc = 0;
     }
     pthread_mutex_unlock(&sv->mu);

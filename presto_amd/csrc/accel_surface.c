@@ -170,6 +170,8 @@ typedef struct sf_buf {
     struct sf_buf *next;
 } sf_buf;
 
+#define SF_MAXTAPS 64
+
 typedef struct sf_state {
     const accelobs *obs;
     accel_search *s;
@@ -181,6 +183,12 @@ typedef struct sf_state {
     float *d_nsq;
     void *d_wdata;            /* -locpow: the normalised window */
     size_t wcap;
+    struct {                  /* -locpow with w rows: FP32 taps per harmonic */
+        int numharm, harmnum, stride;
+        void *d_R;
+        int *d_nk;
+    } taps[SF_MAXTAPS];
+    int ntaps;
     agpu_sf_hit *d_hits;
     unsigned *d_count;
     unsigned cap;
@@ -222,6 +230,10 @@ static void state_release(sf_state *st)
         agpu_free(st->d_nsq);
     if (st->d_wdata)
         agpu_free(st->d_wdata);
+    for (int i = 0; i < st->ntaps; i++) {
+        agpu_free(st->taps[i].d_R);
+        agpu_free(st->taps[i].d_nk);
+    }
     if (st->d_hits)
         agpu_free(st->d_hits);
     if (st->d_count)
@@ -635,10 +647,6 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
         obs->dat_input = strcmp(suffix, "fft") != 0;
         input_shorts = strcmp(suffix, "sdat") == 0;
         free(suffix);
-    }
-    if (cmd->locpowP && !cmd->photonP && cmd->wmaxP) {
-        sf_fail(1, "the -locpow normalisation is built for -wmax 0 searches only");
-        return;
     }
     obs->use_harmonic_polishing = !cmd->noharmpolishP;   /* src/accel_utils.c:1490-1493 */
     if (accel_readinf(idata, obs->rootfilenm)) {
@@ -1060,6 +1068,62 @@ void free_subharminfos(accelobs *obs, subharminfo **shis)
 /* ---------------------------------------------------------------------- */
 /* planes                                                                  */
 
+/* The FP32 response taps of every (w, z) row of one harmonic's kernels
+ * (gen_w_response / gen_z_response of init_kernel, src/accel_utils.c:153-169),
+ * made on the device once per harmonic and kept with the search: the
+ * -locpow planes of jerk searches correlate each window with them
+ * directly.  Returns the cache slot or -1. */
+static int locpow_taps(sf_state *st, const subharminfo *shi)
+{
+    for (int i = 0; i < st->ntaps; i++)
+        if (st->taps[i].numharm == shi->numharm && st->taps[i].harmnum == shi->harmnum)
+            return i;
+    if (st->ntaps == SF_MAXTAPS) {
+        agpu_set_error("too many harmonics for the -locpow tap tables");
+        return -1;
+    }
+    const int nt = shi->numkern;
+    int *zs = (int *) malloc(sizeof(int) * nt), *ws = (int *) malloc(sizeof(int) * nt);
+    int *nk = (int *) malloc(sizeof(int) * nt);
+    int stride = 64, rc = -1;
+    void *d_R = NULL;
+    int *d_nk = NULL;
+    if (!zs || !ws || !nk) {
+        agpu_set_error("out of host memory");
+        goto out;
+    }
+    for (int i = 0; i < shi->numkern_wdim; i++)
+        for (int j = 0; j < shi->numkern_zdim; j++) {
+            const kernel *k = &shi->kern[i][j];
+            const int t = i * shi->numkern_zdim + j;
+            zs[t] = k->z;
+            ws[t] = k->w;
+            nk[t] = 2 * ACCEL_NUMBETWEEN * k->kern_half_width;
+            while (stride < nk[t])
+                stride += 64;
+        }
+    if (agpu_malloc(&d_R, sizeof(fcomplex) * (size_t) nt * stride) ||
+        agpu_malloc((void **) &d_nk, sizeof(int) * (size_t) nt) ||
+        agpu_h2d(d_nk, nk, sizeof(int) * (size_t) nt, st->v.stream) ||
+        agpu_gen_response_taps(zs, ws, nk, nt, stride, d_R, st->v.stream))
+        goto out;
+    st->taps[st->ntaps].numharm = shi->numharm;
+    st->taps[st->ntaps].harmnum = shi->harmnum;
+    st->taps[st->ntaps].stride = stride;
+    st->taps[st->ntaps].d_R = d_R;
+    st->taps[st->ntaps].d_nk = d_nk;
+    d_R = NULL;
+    d_nk = NULL;
+    rc = st->ntaps++;
+out:
+    agpu_free(d_R);
+    agpu_free(d_nk);
+    free(zs);
+    free(ws);
+    free(nk);
+    return rc;
+}
+
 fcomplex *get_fourier_amplitudes(long long lobin, int numbins, accelobs *obs)
 {
     fcomplex *out = (fcomplex *) calloc((size_t) (numbins > 0 ? numbins : 1), sizeof(fcomplex));
@@ -1145,13 +1209,20 @@ ffdotpows *subharm_fderivs_vol(int numharm, int harmnum, double fullrlo, double 
             }
             st->wcap = wb;
         }
-        if (numws != 1 ||
-            agpu_sf_locpow_norm(st->v.d_spec, st->v.numbins, st->v.lobin, lobin, numdata,
-                                st->d_wdata, st->v.stream) ||
-            agpu_sf_locpow_plane(st->d_wdata, numdata, binoffset, zlo, numzs, numrs, f->d_pow,
-                                 st->v.stream))
-            sf_fail(-1, "subharm_fderivs_vol: %s",
-                    numws != 1 ? "-locpow with w rows" : agpu_last_error());
+        int bad = agpu_sf_locpow_norm(st->v.d_spec, st->v.numbins, st->v.lobin, lobin, numdata,
+                                      st->d_wdata, st->v.stream);
+        if (!bad && numws == 1) {
+            bad = agpu_sf_locpow_plane(st->d_wdata, numdata, binoffset, zlo, numzs, numrs,
+                                       f->d_pow, st->v.stream);
+        } else if (!bad) {
+            const int t = locpow_taps(st, shi);
+            bad = t < 0 ||
+                  agpu_sf_locpow_plane_tab(st->d_wdata, numdata, binoffset, numzs * numws, numrs,
+                                           st->taps[t].d_R, st->taps[t].d_nk, st->taps[t].stride,
+                                           f->d_pow, st->v.stream);
+        }
+        if (bad)
+            sf_fail(-1, "subharm_fderivs_vol: %s", agpu_last_error());
         return &f->pub;
     }
     if (st->obs->nph > 0.0) {   /* -photon, src/accel_utils.c:1080-1086 */

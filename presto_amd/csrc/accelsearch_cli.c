@@ -12,10 +12,13 @@
  * -photon), optional birdie zapping; then the search, the FP64 refinement
  * and the outputs <root>_ACCEL_<zmax>[_JERK_<wmax>] (text, .inf appended),
  * .cand and, for an .fft input, .txtcand.  The local-power normalisation
- * (-locpow) needs a correlation per window, which the accel.h surface
- * (create_accelobs and the reference's driver) does; this command refuses it.  -otheropt selects the reference's transposed-plane variants,
- * which form the same sums: it changes nothing here.  -ncpus is accepted
- * and ignored (the work runs on the GPU).
+ * (-locpow) scales every bin of a window by its own local power, so the
+ * plane of the search handle (raw powers times one norm per window) does
+ * not apply: with -locpow the command runs the reference's block loop
+ * over the accel.h surface instead (run_surface), which normalises and
+ * correlates each window on the GPU.  -otheropt selects the reference's
+ * transposed-plane variants, which form the same sums: it changes nothing
+ * here.  -ncpus is accepted and ignored (the work runs on the GPU).
  */
 #include <limits.h>
 #include <math.h>
@@ -24,6 +27,7 @@
 #include <string.h>
 #include <time.h>
 #include "presto_accel.h"
+#include "accel_surface.h"
 
 static void usage(void)
 {
@@ -116,13 +120,122 @@ static void *read_file(const char *path, size_t elem, long long *count)
     return buf;
 }
 
+/* The reference's driver (src/accelsearch.c:82-290) over the accel.h
+ * surface: the path of the -locpow normalisation. */
+static int run_surface(Cmdline *cmd)
+{
+    accelobs obs;
+    infodata idata;
+    create_accelobs(&obs, &idata, cmd, 1);
+    const double rstep = obs.corr_uselen * 0.5;
+    /* birdies zapped in the in-memory spectrum (src/accelsearch.c:86-111) */
+    if (cmd->zaplistP && !obs.mmap_file && obs.fft) {
+        int nb = accel_read_birdies(cmd->zaplist, obs.T, cmd->baryv, NULL, NULL, 0);
+        if (nb < 0) {
+            fprintf(stderr, "accelsearch: cannot read zaplist '%s'\n", cmd->zaplist);
+            return 1;
+        }
+        double *lo = (double *) malloc(sizeof(double) * (nb ? nb : 1));
+        double *hi = (double *) malloc(sizeof(double) * (nb ? nb : 1));
+        accel_read_birdies(cmd->zaplist, obs.T, cmd->baryv, lo, hi, nb);
+        printf("Zapping them using a barycentric velocity of %.5gc.\n\n", cmd->baryv);
+        if (accel_zap_birdies(obs.fft, obs.numbins, (double) obs.N, lo, hi, nb)) {
+            fprintf(stderr, "accelsearch: zapping failed\n");
+            return 1;
+        }
+        free(lo);
+        free(hi);
+    }
+    printf("Searching with up to %d harmonics summed:\n", 1 << (obs.numharmstages - 1));
+    printf("  f = %.1f to %.1f Hz\n", obs.rlo / obs.T, obs.rhi / obs.T);
+    printf("  r = %.1f to %.1f Fourier bins\n", obs.rlo, obs.rhi);
+    printf("  z = %.1f to %.1f Fourier bins drifted\n", obs.zlo, obs.zhi);
+    if (obs.numw)
+        printf("  w = %.1f to %.1f Fourier-derivative bins drifted\n", obs.wlo, obs.whi);
+    subharminfo **shis = create_subharminfos(&obs);
+    GSList *cands = NULL;
+    if (obs.inmem) {
+        /* the plane below rlo, for the lowest blocks' sub-harmonics */
+        for (double startr = 8.0; startr < obs.rlo; startr += rstep) {
+            ffdotpows *f = subharm_fderivs_vol(1, 1, startr, startr + rstep - 0.5, &shis[0][0],
+                                               &obs);
+            fund_to_ffdotplane(f, &obs);
+            free_ffdotpows(f);
+        }
+    }
+    for (double startr = obs.rlo; startr + rstep < obs.highestbin; startr += rstep) {
+        const double lastr = startr + rstep - 0.5;
+        ffdotpows *fund = subharm_fderivs_vol(1, 1, startr, lastr, &shis[0][0], &obs);
+        cands = search_ffdotpows(fund, 1, &obs, cands);
+        if (obs.numharmstages > 1) {
+            if (obs.inmem)
+                fund_to_ffdotplane(fund, &obs);
+            for (int stage = 1; stage < obs.numharmstages; stage++) {
+                const int h = 1 << stage;
+                for (int harm = 1; harm < h; harm += 2) {
+                    if (obs.inmem) {
+                        inmem_add_ffdotpows(fund, &obs, h, harm);
+                    } else {
+                        ffdotpows *sub = subharm_fderivs_vol(h, harm, startr, lastr,
+                                                             &shis[stage][harm - 1], &obs);
+                        add_ffdotpows(fund, sub, h, harm);
+                        free_ffdotpows(sub);
+                    }
+                }
+                cands = search_ffdotpows(fund, h, &obs, cands);
+            }
+        }
+        free_ffdotpows(fund);
+    }
+    free_subharminfos(&obs, shis);
+    printf("\nDone searching.  Now optimizing each candidate.\n\n");
+    int numcands = 0;
+    for (GSList *l = cands; l; l = l->next)
+        numcands++;
+    if (numcands) {
+        cands = sort_accelcands(cands);
+        if (cmd->numharm > 1 && !cmd->noharmremoveP)
+            cands = eliminate_harmonics(cands, &numcands);
+        for (GSList *l = cands; l; l = l->next)
+            optimize_accelcand((accelcand *) l->data, &obs);
+        fourierprops *props = (fourierprops *) malloc(sizeof(fourierprops) * numcands);
+        int i = 0;
+        for (GSList *l = cands; l; l = l->next, i++) {
+            const accelcand *c = (const accelcand *) l->data;
+            /* the fundamental's r and z from the harmonic sum, errors from
+             * the grid (src/accelsearch.c:263-275) */
+            accel_calc_props(&c->derivs[0], c->r, c->z, c->w, &props[i]);
+            props[i].rerr = 0.5f / c->numharm;
+            props[i].zerr = 2.0f / c->numharm;
+            props[i].werr = 20.0f / c->numharm;
+        }
+        output_fundamentals(props, cands, &obs, &idata);
+        output_harmonics(cands, &obs, &idata);
+        FILE *f = fopen(obs.candnm, "wb");
+        if (!f || fwrite(props, sizeof(fourierprops), (size_t) numcands, f) != (size_t) numcands) {
+            fprintf(stderr, "accelsearch: cannot write '%s'\n", obs.candnm);
+            return 1;
+        }
+        fclose(f);
+        free(props);
+        printf("Wrote %d candidates to '%s' and '%s'.\n\n", numcands, obs.accelnm, obs.candnm);
+    } else {
+        printf("No candidates above sigma = %.2f were found.\n\n", obs.sigma);
+    }
+    for (GSList *l = cands; l; l = l->next)
+        free_accelcand(l->data, NULL);
+    accel_gslist_free(cands);
+    free_accelobs(&obs);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     accel_params p;
     accel_params_default(&p);
     const char *infile = NULL, *zaplist = NULL;
     double baryv = 0.0;
-    int device = 0, photon = 0;
+    int device = 0, photon = 0, locpow = 0, median = 0, otheropt = 0;
     p.floP = 1;
     p.fhiP = 1;
     for (int i = 1; i < argc; i++) {
@@ -166,11 +279,11 @@ int main(int argc, char **argv)
         } else if (!strcmp(a, "-inmem"))
             p.inmem = 1;
         else if (!strcmp(a, "-median"))
-            ;
+            median = 1;
         else if (!strcmp(a, "-photon"))
             photon = 1;
         else if (!strcmp(a, "-otheropt"))
-            ;   /* the transposed-plane variants form the same sums
+            otheropt = 1;   /* the transposed-plane variants form the same sums
                  * (src/accel_utils.c:1236-1251, 1284-1309, 1364-1409) */
         else if (!strcmp(a, "-zaplist"))
             zaplist = NEXT();
@@ -186,11 +299,9 @@ int main(int argc, char **argv)
             device = atoi(NEXT());
         else if (!strcmp(a, "-noharmpolish"))
             p.noharmpolish = 1;
-        else if (!strcmp(a, "-locpow")) {
-            fprintf(stderr, "accelsearch: option %s runs through the accel.h surface "
-                            "(create_accelobs), not this command\n", a);
-            return 1;
-        } else if (a[0] == '-') {
+        else if (!strcmp(a, "-locpow"))
+            locpow = 1;
+        else if (a[0] == '-') {
             fprintf(stderr, "accelsearch: unknown option %s\n", a);
             usage();
             return 1;
@@ -201,6 +312,48 @@ int main(int argc, char **argv)
     if (!infile) {
         usage();
         return 1;
+    }
+    if (locpow) {
+        /* the reference's Cmdline, as its generated parser fills it
+         * (flo and fhi carry defaults, so they are always given) */
+        Cmdline cmd;
+        memset(&cmd, 0, sizeof(cmd));
+        char *args[1] = {(char *) infile};
+        cmd.ncpusP = 1;
+        cmd.ncpus = 1;
+        cmd.lobinP = p.lobin != 0;
+        cmd.lobin = (int) p.lobin;
+        cmd.numharmP = 1;
+        cmd.numharm = p.numharm;
+        cmd.zmaxP = 1;
+        cmd.zmax = p.zmax;
+        cmd.wmaxP = p.wmaxP;
+        cmd.wmax = p.wmax;
+        cmd.sigmaP = 1;
+        cmd.sigma = p.sigma;
+        cmd.rloP = p.rloP;
+        cmd.rlo = p.rlo;
+        cmd.rhiP = p.rhiP;
+        cmd.rhi = p.rhi;
+        cmd.floP = 1;
+        cmd.flo = p.flo;
+        cmd.fhiP = 1;
+        cmd.fhi = p.fhi;
+        cmd.inmemP = p.inmem;
+        cmd.photonP = photon;
+        cmd.medianP = median;
+        cmd.locpowP = 1;
+        cmd.zaplistP = zaplist != NULL;
+        cmd.zaplist = (char *) zaplist;
+        cmd.baryvP = baryv != 0.0;
+        cmd.baryv = baryv;
+        cmd.otheroptP = otheropt;
+        cmd.noharmpolishP = p.noharmpolish;
+        cmd.noharmremoveP = p.noharmremove;
+        cmd.argc = 1;
+        cmd.argv = args;
+        accel_surface_set_device(device);
+        return run_surface(&cmd);
     }
     /* root name and input type (src/accel_utils.c:1459-1488) */
     const char *dot = strrchr(infile, '.');

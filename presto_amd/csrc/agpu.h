@@ -132,6 +132,11 @@ int agpu_event_elapsed(float *ms, void *start, void *stop);
  * entries. */
 int agpu_gen_templates(const int *zs, const int *ws, const int *numkern, int nt, int L,
                        void *d_T, void *stream);
+/* The same responses as taps, rounded to FP32 as the reference stores
+ * them: R[t][k] for k < numkern[t] (centre at numkern[t]/2), zero up to the
+ * row stride. */
+int agpu_gen_response_taps(const int *zs, const int *ws, const int *numkern, int nt,
+                           int stride, void *d_R, void *stream);
 
 /* Exact block medians of |x|^2 (FP32 powers) and the reference's
  * normalisation: norm = 1/sqrt(median/ln 2) (FP64), nsq = (float)(norm^2).
@@ -191,6 +196,11 @@ int agpu_sf_locpow_norm(const void *d_spec, long long numbins, long long lobin_f
  * bins (numdata float2), z rows zlo + 2 i (surface_gpu.hip) */
 int agpu_sf_locpow_plane(const void *d_wdata, int numdata, int binoffset, int zlo, int numzs,
                          int numrs, float *d_out, void *stream);
+/* -locpow block plane [nrows][numrs] of a jerk window: row r correlates
+ * with d_nk[r] FP32 taps at d_R + r*stride (surface_gpu.hip) */
+int agpu_sf_locpow_plane_tab(const void *d_wdata, int numdata, int binoffset, int nrows,
+                             int numrs, const void *d_R, const int *d_nk, int stride,
+                             float *d_out, void *stream);
 
 /* Fused overlap-save correlation: raw |corr|^2 plane rows for templates
  * [0, nt) (row = template), half-bins [0, hb_end), segment length L = 4096,
@@ -259,16 +269,22 @@ typedef struct {
      * grid of zin +- DZ, win +- DW, raw power, derivatives with the start's
      * local power, powers = the grid maximum); numharm must be 1 */
     int raw;
+    /* the largest numharm of the candidates (workgroups per candidate) */
+    int maxharm;
 } agpu_refine_args;
 
-/* max_rz_arr_harmonics per candidate */
+/* max_rz_arr_harmonics per candidate, one workgroup per harmonic (a
+ * cooperative launch); scratch must hold agpu_refine_rz_scratch(ncand)
+ * bytes.  A candidate whose workgroups could not keep in step gets
+ * rout = NaN, nevals = -1. */
+size_t agpu_refine_rz_scratch(int ncand);
 int agpu_refine_rz(const agpu_refine_args *args, void *stream);
 /* max_rzw_arr_harmonics per candidate, spread over many workgroups per
  * candidate; maxharm: the largest numharm of the candidates.  scratch must
  * hold agpu_refine_rzw_scratch(ncand, npts_cap) bytes.  A candidate whose
  * chirp length exceeds npts_cap gets rout = NaN, nevals = -1. */
 size_t agpu_refine_rzw_scratch(int ncand, int npts_cap);
-int agpu_refine_rzw(const agpu_refine_args *args, void *stream, int maxharm);
+int agpu_refine_rzw(const agpu_refine_args *args, void *stream);
 
 /* Subband de-dispersion (dedisp_gpu.hip).  raw: [nspec][numchan] FP32,
  * ascending frequency; ds: [nsub][nds]; out: [numdms][numout].  maxdelay /

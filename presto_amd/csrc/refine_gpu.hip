@@ -1,24 +1,28 @@
 /*
  * refine_gpu.hip -- FP64 candidate refinement on the GPU: the work of
- * optimize_accelcand (src/accel_utils.c:552-626) for the default
- * harmonic-polishing path, one workgroup per candidate.
+ * optimize_accelcand (src/accel_utils.c:552-626), spread over many
+ * workgroups per candidate.
  *
- *   k_refine_rz : max_rz_arr_harmonics (src/maximize_rz.c:171-228): local
- *       powers, a two-pass Nelder-Mead simplex over (r, z/4) maximising the
- *       locally normalised summed power of all harmonics, then per
- *       harmonic the power and the 5-point derivatives (get_localpower3d
- *       and get_derivs3d, src/characteristics.c:79-192).
+ *   k_refine_rz_mw : max_rz_arr_harmonics (src/maximize_rz.c:171-228):
+ *       local powers, a two-pass Nelder-Mead simplex over (r, z/4)
+ *       maximising the locally normalised summed power of all harmonics,
+ *       then per harmonic the power and the 5-point derivatives
+ *       (get_localpower3d and get_derivs3d, src/characteristics.c:79-192);
+ *       one workgroup per harmonic, in lockstep.
+ *   k_rzw_* : max_rzw_arr_harmonics (src/maximize_rzw.c:84-191), a grid
+ *       search, as a sequence of kernels per harmonic.
  *
  * Every function value is a set of Fourier interpolations (rz_interp,
  * src/rzinterp.c:144-237): the response taps are generated on the fly in
  * FP64 (Fresnel integrals), rounded to FP32 as the reference stores them,
- * multiplied by the FP32 spectrum and summed in FP64 over the workgroup
- * with a fixed-order tree, so every thread holds the same value and the
- * simplex runs in lockstep, deterministically.
+ * multiplied by the FP32 spectrum and summed in FP64 over a workgroup with
+ * a fixed-order tree, and the harmonics' terms are summed in harmonic
+ * order, so the simplex runs deterministically.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <pthread.h>
 #include "agpu.h"
 #include "dev_fresnel.h"
 
@@ -156,26 +160,102 @@ __device__ static void derivs_wg(const float2 *spec, long long numdata, double r
     out7[6] = locpow;
 }
 
-struct rf_ctx {
-    const float2 *spec;
-    long long numdata;
-    const float *locpow;   /* LDS, numharm entries */
-    int numharm, kernhw;
-    double2 *red;
-    int nevals;
+/* ---- max_rz_arr_harmonics over one workgroup per harmonic ----
+ *
+ * The simplex is run by H workgroups per candidate in lockstep: at every
+ * function evaluation workgroup k forms harmonic k's locally normalised
+ * power, stores it in the candidate's group record and waits at the
+ * group's barrier; then every workgroup sums the H terms in harmonic
+ * order, so all of them hold the same value and take the same simplex
+ * step.  The local powers before and the
+ * powers and derivatives after the simplex are per harmonic too.  The
+ * launcher keeps every workgroup of a launch resident at once; the
+ * barrier's wait is bounded (RZ_SPIN_LIMIT polls), and a
+ * group whose wait runs out is marked bad and stops. */
+#define RZ_SPIN_LIMIT (1LL << 24)
+
+static pthread_mutex_t g_rz_launch_mu = PTHREAD_MUTEX_INITIALIZER;
+
+struct rz_group {
+    unsigned count;              /* barrier arrivals, monotonic */
+    int bad;
+    double term[2][RF_MAXH];     /* one evaluation's terms, by parity */
 };
 
-/* power_call_rz_harmonics_gsl (src/maximize_rz.c:43-63): minus the summed
- * locally normalised power at (r, z = zs * 4) over all harmonics */
-__device__ static double rf_fun(rf_ctx &c, double r, double zs)
+struct rz_mw {
+    rz_group *g;
+    int k, H;                    /* this workgroup's harmonic, the candidate's count */
+    unsigned arrivals;           /* barriers passed by this workgroup */
+    int bad;
+    float lp;                    /* this harmonic's local power */
+};
+
+/* every thread of the workgroup calls it */
+__device__ static void rz_group_barrier(rz_mw &m, int *s_flag)
+{
+    __syncthreads();
+    m.arrivals++;
+    if (threadIdx.x == 0) {
+        int bad = __hip_atomic_load(&m.g->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!bad) {
+            __hip_atomic_fetch_add(&m.g->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned target = m.arrivals * (unsigned) m.H;
+            long long spins = 0;
+            while (__hip_atomic_load(&m.g->count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+                   target) {
+                if (__hip_atomic_load(&m.g->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                    ++spins > RZ_SPIN_LIMIT) {
+                    __hip_atomic_store(&m.g->bad, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    bad = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        *s_flag = bad;
+    }
+    __syncthreads();
+    m.bad = *s_flag;
+}
+
+struct rf_ctx_mw {
+    const float2 *spec;
+    long long numdata;
+    int kernhw;
+    double2 *red;
+    int nevals;
+    rz_mw *m;
+    int *s_flag;
+    double *s_tot;
+};
+
+/* power_call_rz_harmonics_gsl (src/maximize_rz.c:43-63) with harmonic k
+ * formed by workgroup k */
+__device__ static double rf_fun_mw(rf_ctx_mw &c, double r, double zs)
 {
     c.nevals++;
-    double tot = 0.0;
-    for (int k = 0; k < c.numharm; k++) {
-        const int n = k + 1;
-        double2 a = rz_interp_wg(c.spec, c.numdata, r * n, zs * 4.0 * n, c.kernhw, c.red);
-        tot += (a.x * a.x + a.y * a.y) / c.locpow[k];
+    rz_mw &m = *c.m;
+    if (m.bad)
+        return 0.0;
+    const int n = m.k + 1;
+    double2 a = rz_interp_wg(c.spec, c.numdata, r * n, zs * 4.0 * n, c.kernhw, c.red);
+    const double term = (a.x * a.x + a.y * a.y) / m.lp;
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&m.g->term[c.nevals & 1][m.k], term, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    rz_group_barrier(m, c.s_flag);
+    if (m.bad)
+        return 0.0;
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int k = 0; k < m.H; k++)
+            tot += __hip_atomic_load(&m.g->term[c.nevals & 1][k], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        *c.s_tot = tot;
     }
+    __syncthreads();
+    const double tot = *c.s_tot;
+    __syncthreads();
     if (!isfinite(tot))
         return 0.0;
     return -tot;
@@ -185,18 +265,19 @@ __device__ static double rf_fun(rf_ctx &c, double r, double zs)
  * contract the worst corner, else shrink towards the best; incremental
  * centre and squared rms size; stop when the size drops below tol or after
  * maxiter iterations (src/maximize_rz.c:65-100). */
-__device__ static void rf_simplex(rf_ctx &c, double *x, double step, double tol, int maxiter)
+__device__ static void rf_simplex_mw(rf_ctx_mw &c, double *x, double step, double tol,
+                                     int maxiter)
 {
     const double P = 3.0;
     double X[3][2], Y[3], C[2], S2;
     X[0][0] = x[0];
     X[0][1] = x[1];
-    Y[0] = rf_fun(c, x[0], x[1]);
+    Y[0] = rf_fun_mw(c, x[0], x[1]);
     for (int i = 0; i < 2; i++) {
         double xt0 = x[0] + (i == 0 ? step : 0.0), xt1 = x[1] + (i == 1 ? step : 0.0);
         X[i + 1][0] = xt0;
         X[i + 1][1] = xt1;
-        Y[i + 1] = rf_fun(c, xt0, xt1);
+        Y[i + 1] = rf_fun_mw(c, xt0, xt1);
     }
     auto center_size = [&]() {
         for (int j = 0; j < 2; j++)
@@ -240,14 +321,13 @@ __device__ static void rf_simplex(rf_ctx &c, double *x, double step, double tol,
                 ds_hi = v;
             }
         }
-        /* corner moves: xc = a*centre + b*x_hi, a = (1-k) P/(P-1), b = (P k - 1)/(P-1) */
-        const double ar = 3.0, br = -2.0;      /* k = -1: reflection */
+        const double ar = 3.0, br = -2.0;
         double r0 = ar * C[0] + br * X[hi][0], r1 = ar * C[1] + br * X[hi][1];
-        double val = rf_fun(c, r0, r1);
+        double val = rf_fun_mw(c, r0, r1);
         if (isfinite(val) && val < Y[lo]) {
-            const double ae = 4.5, be = -3.5;  /* k = -2: expansion */
+            const double ae = 4.5, be = -3.5;
             double e0 = ae * C[0] + be * X[hi][0], e1 = ae * C[1] + be * X[hi][1];
-            double val2 = rf_fun(c, e0, e1);
+            double val2 = rf_fun_mw(c, e0, e1);
             if (isfinite(val2) && val2 < Y[lo])
                 update(hi, e0, e1, val2);
             else
@@ -255,9 +335,9 @@ __device__ static void rf_simplex(rf_ctx &c, double *x, double step, double tol,
         } else if (!isfinite(val) || val > ds_hi) {
             if (isfinite(val) && val <= Y[hi])
                 update(hi, r0, r1, val);
-            const double ac = 0.75, bc = 0.25; /* k = 1/2: contraction */
+            const double ac = 0.75, bc = 0.25;
             double k0 = ac * C[0] + bc * X[hi][0], k1 = ac * C[1] + bc * X[hi][1];
-            double val2 = rf_fun(c, k0, k1);
+            double val2 = rf_fun_mw(c, k0, k1);
             if (isfinite(val2) && val2 <= Y[hi]) {
                 update(hi, k0, k1, val2);
             } else {
@@ -266,7 +346,7 @@ __device__ static void rf_simplex(rf_ctx &c, double *x, double step, double tol,
                         continue;
                     X[i][0] = 0.5 * (X[i][0] + X[lo][0]);
                     X[i][1] = 0.5 * (X[i][1] + X[lo][1]);
-                    Y[i] = rf_fun(c, X[i][0], X[i][1]);
+                    Y[i] = rf_fun_mw(c, X[i][0], X[i][1]);
                 }
                 center_size();
             }
@@ -280,52 +360,62 @@ __device__ static void rf_simplex(rf_ctx &c, double *x, double step, double tol,
         if (!(S2 > 0))
             center_size();
         size = sqrt(S2);
-    } while (!(size < tol) && it < maxiter);
+    } while (!(size < tol) && it < maxiter && !c.m->bad);
     x[0] = X[lo][0];
     x[1] = X[lo][1];
 }
 
-__global__ __launch_bounds__(RF_TPB) void k_refine_rz(agpu_refine_args a)
+/* blockIdx.x = candidate * hmax + harmonic index; a.scratch: one rz_group
+ * per candidate, zeroed by the launcher */
+__global__ __launch_bounds__(RF_TPB) void k_refine_rz_mw(agpu_refine_args a, int hmax)
 {
     __shared__ double2 red[RF_TPB];
-    __shared__ float s_lp[RF_MAXH];
-    const int ci = blockIdx.x;
+    __shared__ int s_flag;
+    __shared__ double s_tot;
+    const int ci = blockIdx.x / hmax, k = blockIdx.x % hmax;
     const int H = a.numharm[ci];
+    if (k >= H)
+        return;
     const float2 *spec = (const float2 *) a.spec;
     const double rin = a.rin[ci], zin = a.zin[ci];
-    for (int k = 0; k < H; k++) {
-        const int n = k + 1;
+    rz_mw m;
+    m.g = (rz_group *) a.scratch + ci;
+    m.k = k;
+    m.H = H;
+    m.arrivals = 0;
+    m.bad = 0;
+    const int n = k + 1;
+    {
         double lp = localpower_wg(spec, a.numdata, rin * n, zin * n, red);
-        if (threadIdx.x == 0)
-            s_lp[k] = a.raw ? 1.0f : (float) lp;
+        m.lp = a.raw ? 1.0f : (float) lp;
     }
-    __syncthreads();
-    rf_ctx c{spec, a.numdata, s_lp, H, 0, red, 0};
+    rf_ctx_mw c{spec, a.numdata, 0, red, 0, &m, &s_flag, &s_tot};
     double x[2] = {rin, zin / 4.0};
     c.kernhw = zr_halfwidth(fabs(zin * H) + 4.0, 0);
-    rf_simplex(c, x, 0.4, 1.0e-4, 5000);
+    rf_simplex_mw(c, x, 0.4, 1.0e-4, 5000);
     c.kernhw = zr_halfwidth(fabs(x[1] * H) + 4.0, 1);
-    rf_simplex(c, x, 0.01, 1.0e-7, 5000);
+    rf_simplex_mw(c, x, 0.01, 1.0e-7, 5000);
     const double rout = x[0], zout = x[1] * 4.0;
     __syncthreads();
-    for (int k = 0; k < H; k++) {
-        const int n = k + 1;
-        const float lp = (float) localpower_wg(spec, a.numdata, rout * n, zout * n, red);
-        double2 amp = rz_interp_wg(spec, a.numdata, rout * n, zout / 4.0 * n * 4.0, c.kernhw,
-                                   red);
-        double d7[7];
-        derivs_wg(spec, a.numdata, rout * n, zout * n, lp, d7, red);
-        if (threadIdx.x == 0) {
-            a.powers[(size_t) ci * RF_MAXH + k] = amp.x * amp.x + amp.y * amp.y;
-            for (int j = 0; j < 7; j++)
-                a.derivs[((size_t) ci * RF_MAXH + k) * 7 + j] = d7[j];
+    const float lp = (float) localpower_wg(spec, a.numdata, rout * n, zout * n, red);
+    double2 amp = rz_interp_wg(spec, a.numdata, rout * n, zout / 4.0 * n * 4.0, c.kernhw, red);
+    double d7[7];
+    derivs_wg(spec, a.numdata, rout * n, zout * n, lp, d7, red);
+    if (threadIdx.x == 0) {
+        a.powers[(size_t) ci * RF_MAXH + k] = amp.x * amp.x + amp.y * amp.y;
+        for (int j = 0; j < 7; j++)
+            a.derivs[((size_t) ci * RF_MAXH + k) * 7 + j] = d7[j];
+        if (k == 0) {
+            a.rout[ci] = m.bad ? NAN : rout;
+            a.zout[ci] = zout;
+            a.nevals[ci] = m.bad ? -1 : c.nevals;
         }
     }
-    if (threadIdx.x == 0) {
-        a.rout[ci] = rout;
-        a.zout[ci] = zout;
-        a.nevals[ci] = c.nevals;
-    }
+}
+
+size_t agpu_refine_rz_scratch(int ncand)
+{
+    return sizeof(rz_group) * (size_t) (ncand > 0 ? ncand : 1);
 }
 
 int agpu_refine_rz(const agpu_refine_args *args, void *stream)
@@ -336,9 +426,63 @@ int agpu_refine_rz(const agpu_refine_args *args, void *stream)
         agpu_set_error("agpu_refine_rz: too many candidates");
         return -2;
     }
-    hipLaunchKernelGGL(k_refine_rz, dim3(args->ncand), dim3(RF_TPB), 0, (hipStream_t) stream,
-                       *args);
-    RF_CHECK(hipGetLastError());
+    if (!args->scratch) {
+        agpu_set_error("agpu_refine_rz: no group scratch (agpu_refine_rz_scratch bytes)");
+        return -2;
+    }
+    hipStream_t st = (hipStream_t) stream;
+    int hmax = args->maxharm;
+    if (hmax < 1 || hmax > RF_MAXH) {
+        agpu_set_error("agpu_refine_rz: harmonic bound out of range");
+        return -2;
+    }
+    /* The workgroups of a candidate wait for each other, so every
+     * workgroup of a launch must be resident at once: a launch holds at
+     * most as many workgroups as the device keeps resident, and launches of
+     * this kernel are serialised across the process (lock held until the
+     * launch completes).  Other kernels never wait, so they only delay a
+     * workgroup's start. */
+    int dev = 0, ncu = 0, per_cu = 0;
+    RF_CHECK(hipGetDevice(&dev));
+    RF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    RF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_refine_rz_mw, RF_TPB, 0));
+    const long long resident = (long long) ncu * per_cu;
+    const int chunk = (int) (resident / hmax);
+    if (chunk < 1) {
+        agpu_set_error("agpu_refine_rz: a candidate's workgroups do not fit on the device");
+        return -2;
+    }
+    pthread_mutex_lock(&g_rz_launch_mu);
+    hipError_t e = hipMemsetAsync(args->scratch, 0, agpu_refine_rz_scratch(args->ncand), st);
+    for (int c0 = 0; c0 < args->ncand && e == hipSuccess; c0 += chunk) {
+        const int m = args->ncand - c0 < chunk ? args->ncand - c0 : chunk;
+        agpu_refine_args a = *args;
+        a.ncand = m;
+        a.numharm += c0;
+        a.rin += c0;
+        a.zin += c0;
+        if (a.win)
+            a.win += c0;
+        a.rout += c0;
+        a.zout += c0;
+        if (a.wout)
+            a.wout += c0;
+        a.derivs += (size_t) c0 * RF_MAXH * 7;
+        a.powers += (size_t) c0 * RF_MAXH;
+        a.nevals += c0;
+        a.scratch = (void *) ((rz_group *) args->scratch + c0);
+        hipLaunchKernelGGL(k_refine_rz_mw, dim3(m * hmax), dim3(RF_TPB), 0, st, a, hmax);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+    }
+    pthread_mutex_unlock(&g_rz_launch_mu);
+    if (e != hipSuccess) {
+        char m_[256];
+        snprintf(m_, sizeof(m_), "agpu_refine_rz: %s", hipGetErrorString(e));
+        agpu_set_error(m_);
+        return -1;
+    }
     return 0;
 }
 
@@ -1092,8 +1236,9 @@ size_t agpu_refine_rzw_scratch(int ncand, int npts_cap)
     return rzw_layout_of(ncand, npts_cap).total;
 }
 
-int agpu_refine_rzw(const agpu_refine_args *args, void *stream, int maxharm)
+int agpu_refine_rzw(const agpu_refine_args *args, void *stream)
 {
+    const int maxharm = args->maxharm;
     if (args->ncand <= 0)
         return 0;
     if (args->ncand > 1 << 16 || args->npts_cap <= 0 || maxharm < 1 || maxharm > RF_MAXH) {

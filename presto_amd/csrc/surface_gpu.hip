@@ -305,6 +305,40 @@ __global__ __launch_bounds__(SF_TPB) void k_sf_locpow_plane(const float2 *wdata,
     out[(long long) zi * numrs + kk] = fr * fr + fi * fi;
 }
 
+/* The same plane for a window with w rows (jerk searches): row
+ * r = w index * numzs + z index correlates with its own nks[r] FP32 taps,
+ * R + r*stride (agpu_gen_response_taps), staged in LDS. */
+__global__ __launch_bounds__(SF_TPB) void k_sf_locpow_plane_tab(const float2 *wdata, int numdata,
+                                                                int binoffset, int numrs,
+                                                                const float2 *R, const int *nks,
+                                                                int stride, float *out)
+{
+    extern __shared__ float2 taps[];
+    const int row = blockIdx.y;
+    const int nk = nks[row];
+    for (int m = threadIdx.x; m < nk; m += SF_TPB)
+        taps[m] = R[(size_t) row * stride + m];
+    __syncthreads();
+    const int kk = blockIdx.x * SF_TPB + threadIdx.x;
+    if (kk >= numrs)
+        return;
+    const int j = kk + 2 * binoffset;
+    double ar = 0.0, ai = 0.0;
+    int m0 = -nk / 2;
+    if ((j + m0) & 1)
+        m0++;
+    for (int m = m0; m < nk / 2; m += 2) {
+        const int i = (j + m) >> 1;
+        if (i < 0 || i >= numdata)
+            continue;
+        const float2 d = wdata[i], q = taps[nk / 2 + m];
+        ar += (double) d.x * q.x + (double) d.y * q.y;
+        ai += (double) d.y * q.x - (double) d.x * q.y;
+    }
+    const float fr = (float) ar, fi = (float) ai;
+    out[(long long) row * numrs + kk] = fr * fr + fi * fi;
+}
+
 int agpu_sf_locpow_norm(const void *d_spec, long long numbins, long long lobin_file,
                         long long lobin, int numdata, void *d_wdata, void *stream)
 {
@@ -332,6 +366,24 @@ int agpu_sf_locpow_plane(const void *d_wdata, int numdata, int binoffset, int zl
     dim3 grid((unsigned) ((numrs + SF_TPB - 1) / SF_TPB), (unsigned) numzs);
     hipLaunchKernelGGL(k_sf_locpow_plane, grid, dim3(SF_TPB), lds, (hipStream_t) stream,
                        (const float2 *) d_wdata, numdata, binoffset, zlo, numrs, d_out);
+    SF_CHECK(hipGetLastError());
+    return 0;
+}
+
+int agpu_sf_locpow_plane_tab(const void *d_wdata, int numdata, int binoffset, int nrows,
+                             int numrs, const void *d_R, const int *d_nk, int stride,
+                             float *d_out, void *stream)
+{
+    if (numdata <= 0 || nrows <= 0 || nrows > 65535 || numrs <= 0 || binoffset < 0 ||
+        stride <= 0)
+        return agpu_set_error("agpu_sf_locpow_plane_tab: bad geometry"), -1;
+    const size_t lds = sizeof(float2) * (size_t) stride;
+    if (lds > 64 * 1024)
+        return agpu_set_error("agpu_sf_locpow_plane_tab: response too long for LDS"), -1;
+    dim3 grid((unsigned) ((numrs + SF_TPB - 1) / SF_TPB), (unsigned) nrows);
+    hipLaunchKernelGGL(k_sf_locpow_plane_tab, grid, dim3(SF_TPB), lds, (hipStream_t) stream,
+                       (const float2 *) d_wdata, numdata, binoffset, numrs, (const float2 *) d_R,
+                       d_nk, stride, d_out);
     SF_CHECK(hipGetLastError());
     return 0;
 }

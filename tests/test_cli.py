@@ -35,8 +35,10 @@ def test_cli_rejects_bad_arguments(tmp_path):
     assert os.path.exists(CLI)
     r = _run(["-bogus", "x.dat"], tmp_path)
     assert r.returncode == 1 and "unknown option" in r.stderr
+    # -locpow runs the reference's block loop over the accel.h surface,
+    # whose create_accelobs reads the .inf first
     r = _run(["-locpow", "x.dat"], tmp_path)
-    assert r.returncode == 1 and "accel.h surface" in r.stderr
+    assert r.returncode != 0 and "Error opening file 'x.inf'" in r.stdout
     # the ranges of clig/accelsearch_cmd.cli:13-44, with the generated
     # parser's messages (src/accelsearch_cmd.c:246-335)
     for args, msg in [(["-zmax", "1300"], "`-zmax' greater than max=1200"),

@@ -143,6 +143,43 @@ def test_fft_input_cli_equals_surface(tmp_path, photon, nopolish):
         assert open(tmp_path / n, "rb").read() == surf[n], n
 
 
+@pytest.mark.parametrize("inmem", [True, False])
+def test_locpow_cli_equals_surface(tmp_path, inmem):
+    """accelsearch -locpow runs the reference's block loop over the
+    surface inside the command (src/accelsearch.c:82-290): its .txtcand,
+    ACCEL text and .cand bytes equal the Python-driven loop's, on the
+    in-memory path (2^17 samples) and the standard one (2^23, 4990-5000 Hz)."""
+    import subprocess
+    if inmem:
+        N, zmax, band = 1 << 17, 20, []
+        pul = [dict(f0=123.456, z=6.0, duty=0.05, amp=0.3),
+               dict(f0=377.7, z=-10.0, duty=0.08, amp=0.25)]
+    else:
+        N, zmax, band = 1 << 23, 200, ["-flo", "4990", "-fhi", "5000"]
+        pul = [dict(f0=4995.37, z=30.0, duty=0.2, amp=0.08)]
+    dt = 64e-6
+    print(f"case: .fft input, CLI -locpow vs surface, N={N}, zmax {zmax}, numharm 4, "
+          f"inmem {inmem}", flush=True)
+    x = synth.make_series_f32(N, dt, 78, pul)
+    A.realfft_gpu(x).tofile(tmp_path / "lpc.fft")
+    S.write_inf(str(tmp_path / "lpc"), N, dt)
+    kw = {} if inmem else dict(flo=4990.0, fhi=5000.0)
+    cmd = S.make_cmdline(str(tmp_path / "lpc.fft"), zmax=zmax, numharm=4, locpow=True, **kw)
+    info, cands = S.drive(str(tmp_path / "lpc.fft"), cmd)
+    assert info["inmem"] == int(inmem) and len(cands) >= 1
+    names = (f"lpc_ACCEL_{zmax}", f"lpc_ACCEL_{zmax}.cand", f"lpc_ACCEL_{zmax}.txtcand")
+    surf = {n: open(tmp_path / n, "rb").read() for n in names}
+    for n in names:
+        os.remove(tmp_path / n)
+    cli = os.path.join(os.path.dirname(A.__file__), "accelsearch")
+    r = subprocess.run([cli, "-zmax", str(zmax), "-numharm", "4", "-locpow"] + band +
+                       ["lpc.fft"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Done searching" in r.stdout
+    for n in names:
+        assert open(tmp_path / n, "rb").read() == surf[n], n
+
+
 def test_jerk_run_equals_handle(tmp_path):
     print("case: surface jerk run, 2^16 samples, zmax 20, wmax 60, numharm 4", flush=True)
     N, dt = 1 << 16, 64e-6
@@ -248,19 +285,53 @@ def test_block_planes_tables_and_adds(tmp_path, wmax):
 
 
 def test_refusals_return_errors(tmp_path):
-    print("case: surface refusals (locpow with wmax, bad suffix)", flush=True)
+    print("case: surface refusals (bad suffix)", flush=True)
     N, dt = 1 << 14, 64e-6
     x = synth.make_series_f32(N, dt, 75)
     root = _write_dat(tmp_path, "ref", x, dt)
     obs, idata = S.AccelObs(), S.InfoData()
-    cmd = S.make_cmdline(root + ".dat", wmax=20, locpow=True)
-    S.lib.create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), 1)
-    with pytest.raises(A.AccelError, match="locpow"):
-        S._check()
     cmd = S.make_cmdline(root + ".txt")
     S.lib.create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), 1)
     with pytest.raises(A.AccelError, match="must be an"):
         S._check()
+
+
+def test_locpow_jerk_run_equals_oracle(tmp_path):
+    """-locpow with -wmax: each window is normalised bin by bin and
+    correlated with every (z, w) row's FP32 response taps (jerk rows from
+    the FP32-sampled chirp); the candidate list of the reference's block
+    loop through the surface matches the oracle's, powers to 1e-4 relative,
+    only candidates within 0.05 sigma of the threshold may differ."""
+    N, dt = 1 << 16, 64e-6
+    kw = dict(zmax=20, wmax=60, numharm=4)
+    print(f"case: surface -locpow jerk run, N={N}, {kw}", flush=True)
+    sigma = 3.0
+    x = synth.make_series_f32(N, dt, 79, [dict(f0=211.3, z=6.0, w=40.0, duty=0.05, amp=0.5)])
+    root = _write_dat(tmp_path, "lpj", x, dt)
+    cmd = S.make_cmdline(root + ".dat", sigma=sigma, locpow=True, **kw)
+    info, cands = S.drive(root + ".dat", cmd, optimize=False, outputs=False)
+    assert info["numw"] > 1
+    with A.Search(A.make_params(N // 2, N, dt, **kw)) as s:
+        s.set_series(x)
+        spec = s.spectrum()
+    obs = O.make_obs(len(spec), N, dt, sigma=sigma, locpow=True, **kw)
+    _, oc, _ = O.accelsearch(obs, spec)
+    assert len(oc) > 0 and len(cands) > 0
+    ours = {(c["numharm"], round(c["r"], 6), round(c["z"], 6), round(c["w"], 6)): c
+            for c in cands}
+    ref = {(int(c[2]), round(c[3], 6), round(c[4], 6), round(c[5], 6)): c for c in oc}
+    matched = 0
+    for k, c in ours.items():
+        if k in ref:
+            assert abs(c["power"] - ref[k][0]) <= 1e-4 * ref[k][0], k
+            matched += 1
+        else:
+            assert c["sigma"] < sigma + 0.05, k
+    for k, c in ref.items():
+        if k not in ours:
+            assert c[1] < sigma + 0.05, k
+    assert matched > 0
+    print(f"{len(cands)} candidates, {matched} matched in the oracle's list", flush=True)
 
 
 @pytest.mark.parametrize("inmem", [True, False])
