@@ -25,9 +25,9 @@
  *   - The spectrum is uploaded to the GPU by create_subharminfos(), after
  *     the caller's zapping (src/accelsearch.c:86-111).
  *   - -photon normalises every block by 1/sqrt(nph), as the reference
- *     does.  -locpow is built for searches without -wmax: each
- *     subharm_fderivs_vol() normalises its window bin by bin on the GPU and
- *     correlates it directly; with -wmax it is refused.
+ *     does.  With -locpow each subharm_fderivs_vol() normalises its window
+ *     bin by bin on the GPU and correlates it directly with every row's
+ *     FP32 response taps (z rows and, with -wmax, the jerk rows).
  *   - Errors print the reference's message and call exit() with its code,
  *     as the reference does; accel_surface_set_exit(0) makes them return
  *     instead, with accel_last_error() set and accel_surface_failed() true.
