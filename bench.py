@@ -373,6 +373,7 @@ def jerk_section(n):
             x = synth.make_series_f32(N, dt, seed, pulsars)
             s.set_series(x)
             if k == 0:
+                tuned = s.tune()          # measured template-group size
                 s.run()                   # warm
             t0 = time.perf_counter()
             s.run()
@@ -413,6 +414,8 @@ def jerk_section(n):
                           "device_total": avg(tims, "total")},
             "host_ms": {k: avg(hosts, k) for k in hosts[0]},
             "hits_mean": float(np.mean(nhits)), "candidates_mean": float(np.mean(ncands)),
+            "corr_tuning": {"groups_kept": tuned[0],
+                            "timed_groups_ms": tuned[1]},
             "raw_plane_gb": plane_gb, "optimize_ms": 1e3 * float(np.mean(t_opt)),
             "dm_trials_per_s_through_refinement": n / float(np.sum(t_search) + np.sum(t_opt)),
             "signals_found": found, "first": best}
@@ -459,6 +462,7 @@ def main():
     _stage("search handle created")
     g = s.geom
     s.set_spectrum(fft)
+    tuned = s.tune()     # measured template-group size of the correlation
     for _ in range(args.warmup):
         s.run()
     lat, tim, host = [], [], []
@@ -548,6 +552,8 @@ def main():
                      "frac": hs_bytes / (hs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": measured_traffic("k_harmsum<5, 0>"),
                      "bytes_per_cell": hs_bytes_per_cell},
+        "corr_tuning": {"groups_kept": tuned[0],
+                        "timed_groups_ms": tuned[1]},
         "roofline_correlation": {"bound": "valu", "kernel": "k_corr4096",
                                  "achieved_gbs": corr_bytes / (corr_ms * 1e-3) / 1e9,
                                  "hbm_frac": corr_bytes / (corr_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -167,6 +167,16 @@ int accel_search_host_timings(const accel_search *s, double *ms4);
  * per segment, template groups, segments. */
 int accel_search_corr_shape(const accel_search *s, int *L, int *uselen, int *groups,
                             long long *nseg);
+/* Measured choice of the correlation's template-group size, the
+ * counterpart of the reference's FFTW_MEASURE planning of its correlation
+ * FFTs (src/accel_utils.c:1133-1135, src/fftcalls.c:130-138): times the
+ * correlation kernel over the current spectrum for each group count that
+ * is a multiple of 8 giving 4 to 64 templates per group (at most ncand of
+ * them, nearest the default first), keeps the fastest for this handle and
+ * returns how many were timed; groups[i] and ms[i] (best of 3 rounds)
+ * describe each, *best the one kept.  Every group count writes the same
+ * raw plane values, so results do not change. */
+int accel_search_tune(accel_search *s, int ncand, int *groups, float *ms, int *best);
 /* Raw plane half-bins written per run (templates x half-bins). */
 long long accel_search_plane_cells(const accel_search *s);
 

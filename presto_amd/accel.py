@@ -85,6 +85,7 @@ _sig("accel_search_plane_row", _i, _p, _i, _i, _ll, _i, _p)
 _sig("accel_search_nwindows", _i, _p)
 _sig("accel_search_norms", _i, _p, _p, _i)
 _sig("accel_search_timings", _i, _p, _p)
+_sig("accel_search_tune", _i, _p, _i, _p, _p, _p)
 _sig("accel_search_corr_shape", _i, _p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
      C.POINTER(_ll))
 _sig("accel_search_plane_cells", _ll, _p)
@@ -137,6 +138,7 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_collect", "accel_search_ncands", "accel_search_cands",
             "accel_search_nhits", "accel_search_hits", "accel_search_plane_row",
             "accel_search_nwindows", "accel_search_norms", "accel_search_timings",
+            "accel_search_tune",
             "accel_search_host_timings",
             "accel_search_corr_shape", "accel_search_plane_cells",
             "accel_search_windows", "accel_deredden", "accel_prepare_dat_spectrum",
@@ -319,6 +321,17 @@ class Search:
         lib.accel_search_host_timings(self.h, ms.ctypes.data)
         return dict(launch=float(ms[0]), wait=float(ms[1]), copy=float(ms[2]),
                     replay=float(ms[3]))
+
+    def tune(self, ncand=8):
+        """Measured template-group size of the correlation (accel_search_tune):
+        returns (groups kept, [(groups, ms), ...])."""
+        g = np.zeros(ncand, dtype=np.int32)
+        ms = np.zeros(ncand, dtype=np.float32)
+        best = _i()
+        n = lib.accel_search_tune(self.h, ncand, g.ctypes.data, ms.ctypes.data, C.byref(best))
+        if n < 0:
+            raise _err()
+        return best.value, [(int(g[i]), float(ms[i])) for i in range(n)]
 
     def corr_shape(self):
         L, u, g, ns = _i(), _i(), _i(), _ll()

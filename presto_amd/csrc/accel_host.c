@@ -1256,6 +1256,69 @@ int accel_search_corr_shape(const accel_search *s, int *L, int *uselen, int *gro
     return 0;
 }
 
+int accel_search_tune(accel_search *s, int ncand, int *groups, float *ms, int *best)
+{
+    if (!s || ncand <= 0 || !groups || !ms || !best) {
+        set_err("accel_search_tune: bad arguments");
+        return -1;
+    }
+    /* group counts (multiples of 8) whose group sizes lie in [4, 64],
+     * ordered by distance from the default */
+    int cand[64], nc = 0;
+    for (int G = 8; G <= 8 * ((s->nt + 31) / 32) && nc < 64; G += 8) {
+        const int tpg = (s->nt + G - 1) / G;
+        if (tpg > 64 || tpg < 4 || (long long) G * s->nseg >= (1LL << 24))
+            continue;
+        if (nc && (s->nt + cand[nc - 1] - 1) / cand[nc - 1] == tpg)
+            continue;   /* same group size as a smaller count */
+        cand[nc++] = G;
+    }
+    if (nc == 0)
+        cand[nc++] = s->groups;
+    for (int i = 1; i < nc; i++)   /* insertion sort by |G - default| */
+        for (int k = i; k > 0 && abs(cand[k] - s->groups) < abs(cand[k - 1] - s->groups); k--) {
+            const int t = cand[k];
+            cand[k] = cand[k - 1];
+            cand[k - 1] = t;
+        }
+    if (nc > ncand)
+        nc = ncand;
+    /* the whole spectrum (a sample of segments favours small groups: its
+     * last round of workgroups weighs more), one warm-up launch, then
+     * three rounds over the candidates, best of each */
+    const accel_geom *g = &s->g;
+    for (int i = 0; i < nc; i++)
+        ms[i] = 0.0f;
+    for (int rep = -1; rep < 3; rep++) {
+        for (int i = 0; i < nc; i++) {
+            float t = 0.0f;
+            GPU(agpu_event_record(s->ev[1], s->stream));
+            GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw,
+                               s->uselen, s->hb_end, s->d_plane, s->ntiles, cand[i], 0, -1,
+                               s->stream));
+            GPU(agpu_event_record(s->ev[2], s->stream));
+            GPU(agpu_stream_sync(s->stream));
+            GPU(agpu_event_elapsed(&t, s->ev[1], s->ev[2]));
+            if (rep < 0)
+                break;   /* warm-up */
+            if (rep == 0 || t < ms[i])
+                ms[i] = t;
+        }
+    }
+    int kept = cand[0];
+    float kept_ms = ms[0];
+    for (int i = 0; i < nc; i++) {
+        groups[i] = cand[i];
+        if (ms[i] < kept_ms) {
+            kept = cand[i];
+            kept_ms = ms[i];
+        }
+    }
+    s->groups = kept;
+    *best = kept;
+    return nc;
+}
+
 long long accel_search_plane_cells(const accel_search *s)
 {
     return (long long) s->nt * s->hb_end;
