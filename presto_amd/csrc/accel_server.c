@@ -4,7 +4,12 @@
  * Requests (a time series plus accelsearch parameters) are queued as they
  * arrive.  Worker threads admit them in arrival order whenever the device
  * memory they need (accel_search_footprint) fits under the budget next to
- * the requests already running; each running request owns a search handle
+ * the requests already running.  While the oldest request waits for
+ * memory, a later one may start beside it (backfill) if it fits now and
+ * the requests started that way, together with the oldest, fit the budget:
+ * the oldest then waits only for the requests that were running when it
+ * reached the head of the queue, and small requests do not queue behind a
+ * large one.  Each running request owns a search handle
  * and its HIP stream, so requests overlap on the device: one request's
  * upload, FFT and de-reddening, host replay and refinement run beside the
  * others' correlation and harmonic sums.  Idle handles stay cached (their
@@ -24,6 +29,7 @@
 #include "agpu.h"
 
 #define SRV_MAXW 64
+#define SRV_SCAN 256   /* queued requests looked at for a backfill */
 
 typedef struct {
     accel_params p;
@@ -32,6 +38,7 @@ typedef struct {
     int optimize;
     long long bytes;
     int state;                  /* 0 queued, 1 running, 2 done */
+    int backfill;               /* started while an older request waited */
     int status;
     double t_submit, t_start, t_done;
     double ms_evict, ms_create, ms_work;   /* inside the service time */
@@ -60,6 +67,7 @@ struct accel_server {
     int shutdown;
     srv_req *req;
     int nreq, capreq, head;     /* queue = tickets head .. nreq-1 in state 0 */
+    long long backfill_bytes;   /* footprints of running backfilled requests */
     srv_handle *hd;
     int nhd, caphd;
     long long next_id;
@@ -158,18 +166,40 @@ static void *worker(void *arg)
             pthread_cond_wait(&sv->cv, &sv->mu);
             continue;
         }
-        const int ti = sv->head;
+        int ti = sv->head;
         double ms_evict = 0.0;
         int hi = take_handle(sv, &sv->req[ti].p, sv->req[ti].bytes, &ms_evict);
         sv->req[ti].ms_evict += ms_evict;
         if (hi == -1) {
-            pthread_cond_wait(&sv->cv, &sv->mu);
-            continue;
+            /* the oldest request waits for memory: a later one that fits
+             * now may start if it and the other backfilled requests leave
+             * room for the oldest */
+            const long long need = sv->req[sv->head].bytes;
+            ti = -1;
+            for (int k = sv->head + 1; k < sv->nreq && k <= sv->head + SRV_SCAN; k++) {
+                srv_req *q = &sv->req[k];
+                if (q->state != 0 || sv->backfill_bytes + q->bytes + need > sv->budget)
+                    continue;
+                double me = 0.0;
+                hi = take_handle(sv, &q->p, q->bytes, &me);
+                q->ms_evict += me;
+                if (hi >= 0) {
+                    ti = k;
+                    break;
+                }
+            }
+            if (ti < 0) {
+                pthread_cond_wait(&sv->cv, &sv->mu);
+                continue;
+            }
+            sv->req[ti].backfill = 1;
+            sv->backfill_bytes += sv->req[ti].bytes;
         }
         srv_req *r = &sv->req[ti];
         r->state = 1;
         r->t_start = now_s();
-        sv->head++;
+        if (ti == sv->head)
+            sv->head++;
         if (hi == -2) {
             snprintf(r->err, sizeof(r->err), "request needs %lld bytes, budget %lld", r->bytes,
                      sv->budget);
@@ -230,6 +260,8 @@ static void *worker(void *arg)
         memcpy(r->err, local.err, sizeof(r->err));
         free(r->series);
         r->series = NULL;
+        if (r->backfill)
+            sv->backfill_bytes -= r->bytes;
         r->t_done = now_s();
         r->state = 2;
         pthread_cond_broadcast(&sv->cv);

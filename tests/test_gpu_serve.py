@@ -88,3 +88,37 @@ def test_request_array_growth_while_workers_run():
             ca, pa = alone[k]
             assert np.array_equal(sv.candidates(t), ca)
             assert [bytes(v) for v in sv.props(t)] == [bytes(v) for v in pa]
+
+
+def test_small_request_backfills_beside_a_waiting_large_one():
+    """Two large requests that cannot share the budget and a small one
+    behind them: the small one starts beside the first while the second
+    waits for memory (backfill), and its results are those it has alone."""
+    N, dt = 1 << 21, 64e-6
+    big = A.make_params(N // 2, N, dt, zmax=200, numharm=16, sigma=2.0)
+    xa = synth.make_series(N, dt, 931, [dict(f0=211.7, z=12.0, duty=0.05, amp=0.05)])
+    xh = synth.make_series(N, dt, 932, [])
+    n, sdt = 1 << 15, 1e-3
+    small = A.make_params(n // 2, n, sdt, zmax=0, numharm=2, sigma=2.0)
+    xs = synth.make_series(n, sdt, 933, [dict(f0=41.3, z=0.0, duty=0.06, amp=0.4)])
+    fb, fs = A.footprint(big), A.footprint(small)
+    budget = fb + fs + (64 << 20)
+    print(f"case: backfill, big {fb >> 20} MiB x 2, small {fs >> 20} MiB, budget "
+          f"{budget >> 20} MiB", flush=True)
+    assert 2 * fb > budget
+    with A.Server(budget_bytes=budget, workers=4) as sv:
+        ta = sv.submit(big, xa)
+        th = sv.submit(big, xh)
+        ts = sv.submit(small, xs)
+        for t in (ta, th, ts):
+            assert sv.wait(t)[0] == 0
+        qh, qs = sv.timing(th)["queued"], sv.timing(ts)["queued"]
+        cs, ps = sv.candidates(ts), sv.props(ts)
+        assert sv.memory() <= budget
+    print(f"queued: second large {qh:.1f} ms, small {qs:.1f} ms", flush=True)
+    assert qs < qh
+    with A.Server(budget_bytes=budget, workers=1) as sv:
+        t = sv.submit(small, xs)
+        assert sv.wait(t)[0] == 0
+        assert np.array_equal(sv.candidates(t), cs)
+        assert [bytes(v) for v in sv.props(t)] == [bytes(v) for v in ps]
