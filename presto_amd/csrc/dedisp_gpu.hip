@@ -224,8 +224,13 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_dms(const float *ds, long long nd
  * subband it loads the ds run that the group's offsets reach,
  * [u0 + lo_s, u0 + DD_DMT + hi_s), once (coalesced), and every DM of the
  * group reads its shifted window from LDS: one global read per sample and
- * subband instead of one per DM. */
+ * subband instead of one per DM.  Subbands are staged DD_SB at a time
+ * (the group's sums wait in registers between passes), so a workgroup
+ * holds DD_SB runs of LDS and six workgroups share a CU; the sums still
+ * add the subbands in ascending order. */
 #define DD_DMT (2 * DD_TPB)
+#define DD_SB 8
+#define DD_COLS 4   /* loads per thread and run: width <= DD_COLS * DD_TPB */
 __global__ __launch_bounds__(DD_TPB) void k_dd_dms_lds(const float *ds, long long nds, int nsub,
                                                        const int *offsets, int numdms,
                                                        long long nwrite, long long numout,
@@ -233,7 +238,7 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_dms_lds(const float *ds, long lon
                                                        float *out)
 {
 #pragma clang fp contract(off)
-    extern __shared__ float lds[];   /* [nsub][width] */
+    extern __shared__ float lds[];   /* [DD_SB][width] */
     __shared__ int lo[256];
     const long long bid = blockIdx.x;
     const int xcd = (int) (bid & 7);
@@ -242,49 +247,71 @@ __global__ __launch_bounds__(DD_TPB) void k_dd_dms_lds(const float *ds, long lon
     const long long tile = (local / ngroups) * 8 + xcd;
     if (tile >= ntiles)
         return;
-    const int d0 = g * AGPU_DD_GROUP, d1 = min(d0 + AGPU_DD_GROUP, numdms);
+    const int d0 = g * AGPU_DD_GROUP, nd = min(AGPU_DD_GROUP, numdms - d0);
     const long long u0 = tile * DD_DMT;
     for (int si = threadIdx.x; si < nsub; si += DD_TPB) {
         int m = offsets[(long long) d0 * nsub + si];
-        for (int d = d0 + 1; d < d1; d++)
+        for (int d = d0 + 1; d < d0 + nd; d++)
             m = min(m, offsets[(long long) d * nsub + si]);
         lo[si] = m;
     }
-    __syncthreads();
-    /* DD_UNROLL independent loads in flight per thread before their LDS
-     * stores */
-    const int n = nsub * width;
-    for (int i0 = threadIdx.x; i0 < n; i0 += DD_TPB * DD_UNROLL) {
-        float val[DD_UNROLL];
+    const int x0 = threadIdx.x, x1 = threadIdx.x + DD_TPB;
+    float a0[AGPU_DD_GROUP], a1[AGPU_DD_GROUP];
 #pragma unroll
-        for (int u = 0; u < DD_UNROLL; u++) {
-            const int i = i0 + u * DD_TPB;
-            const int si = i / width;
-            const long long k = i < n ? u0 + lo[si] + (i - si * width) : nds;
-            val[u] = k < nds ? ds[(long long) si * nds + k] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < DD_UNROLL; u++) {
-            const int i = i0 + u * DD_TPB;
-            if (i < n)
-                lds[i] = val[u];
-        }
+    for (int d = 0; d < AGPU_DD_GROUP; d++) {
+        a0[d] = -0.0f;
+        a1[d] = -0.0f;
     }
     __syncthreads();
-    const int x0 = threadIdx.x, x1 = threadIdx.x + DD_TPB;
-    const bool w0 = u0 + x0 < nwrite, w1 = u0 + x1 < nwrite;
-    for (int d = d0; d < d1; d++) {
-        const int *off = offsets + (long long) d * nsub;
-        float a0 = -0.0f, a1 = -0.0f;
-        for (int si = 0; si < nsub; si++) {
-            const int b = si * width + (off[si] - lo[si]);
-            a0 = a0 + lds[b + x0];
-            a1 = a1 + lds[b + x1];
+    for (int s0 = 0; s0 < nsub; s0 += DD_SB) {
+        const int nr = min(DD_SB, nsub - s0);
+        /* half a pass of loads in flight before their LDS stores */
+#pragma unroll
+        for (int h = 0; h < DD_SB; h += DD_SB / 2) {
+            float val[DD_SB / 2][DD_COLS];
+#pragma unroll
+            for (int r = 0; r < DD_SB / 2; r++) {
+#pragma unroll
+                for (int c = 0; c < DD_COLS; c++) {
+                    const int col = threadIdx.x + c * DD_TPB;
+                    const long long k =
+                        h + r < nr && col < width ? u0 + lo[s0 + h + r] + col : nds;
+                    val[r][c] = k < nds ? ds[(long long) (s0 + h + r) * nds + k] : 0.0f;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < DD_SB / 2; r++) {
+#pragma unroll
+                for (int c = 0; c < DD_COLS; c++) {
+                    const int col = threadIdx.x + c * DD_TPB;
+                    if (h + r < nr && col < width)
+                        lds[(h + r) * width + col] = val[r][c];
+                }
+            }
         }
-        if (w0)
-            out[(long long) d * numout + u0 + x0] = a0;
-        if (w1)
-            out[(long long) d * numout + u0 + x1] = a1;
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < AGPU_DD_GROUP; d++) {
+            if (d < nd) {
+                const int *off = offsets + (long long) (d0 + d) * nsub + s0;
+                for (int r = 0; r < nr; r++) {
+                    const int b = r * width + (off[r] - lo[s0 + r]);
+                    a0[d] = a0[d] + lds[b + x0];
+                    a1[d] = a1[d] + lds[b + x1];
+                }
+            }
+        }
+        __syncthreads();   /* the next pass overwrites the runs */
+    }
+    const bool w0 = u0 + x0 < nwrite, w1 = u0 + x1 < nwrite;
+#pragma unroll
+    for (int d = 0; d < AGPU_DD_GROUP; d++) {
+        if (d < nd) {
+            if (w0)
+                out[(long long) (d0 + d) * numout + u0 + x0] = a0[d];
+            if (w1)
+                out[(long long) (d0 + d) * numout + u0 + x1] = a1[d];
+        }
     }
 }
 
@@ -369,8 +396,9 @@ int agpu_dedisp_dms(const float *d_ds, long long nds, int nsub, const int *d_off
     /* LDS path when every group's runs fit: width = DD_DMT + the largest
      * spread of one subband's offsets over AGPU_DD_GROUP consecutive DMs */
     const int width = DD_DMT + span;
-    const long long ldsb = (long long) nsub * width * (long long) sizeof(float);
-    if (span >= 0 && span <= maxoff && nsub <= 256 && ldsb <= 64 * 1024) {
+    const long long ldsb = (long long) (nsub < DD_SB ? nsub : DD_SB) * width *
+                           (long long) sizeof(float);
+    if (span >= 0 && span <= maxoff && nsub <= 256 && width <= DD_COLS * DD_TPB) {
         const int ngroups = (numdms + AGPU_DD_GROUP - 1) / AGPU_DD_GROUP;
         const long long ntl = (nwrite + DD_DMT - 1) / DD_DMT;
         const long long nblk2 = ((ntl + 7) / 8) * 8 * ngroups;
