@@ -288,6 +288,13 @@ typedef struct {
  * (src/maximize_rz.c:171) or, for jerk searches, max_rzw_arr_harmonics
  * (src/maximize_rzw.c:84). */
 int accel_search_optimize(accel_search *s);
+/* No counterpart in the reference.  The refinement runs a candidate's
+ * harmonics as workgroups that wait for each other; a wait is bounded and
+ * a candidate whose wait runs out is refined again on its own.  This sets
+ * the first attempt's bound to 2^log2_polls polls (0..24, default 24) for
+ * the process, so that tests can make the first attempt fail and check the
+ * retry's results; nothing else should lower it. */
+void accel_refine_set_first_wait(int log2_polls);
 /* Refined harmonics of candidate i (after accel_search_optimize): the
  * accelcand fields hirs, hizs, hiws, pows and derivs
  * (include/accel.h:83-95).  Returns the harmonic count. */

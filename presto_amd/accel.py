@@ -86,6 +86,7 @@ _sig("accel_search_nwindows", _i, _p)
 _sig("accel_search_norms", _i, _p, _p, _i)
 _sig("accel_search_timings", _i, _p, _p)
 _sig("accel_search_tune", _i, _p, _i, _p, _p, _p)
+_sig("accel_refine_set_first_wait", None, _i)
 _sig("accel_search_corr_shape", _i, _p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
      C.POINTER(_ll))
 _sig("accel_search_plane_cells", _ll, _p)
@@ -138,7 +139,7 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_search_collect", "accel_search_ncands", "accel_search_cands",
             "accel_search_nhits", "accel_search_hits", "accel_search_plane_row",
             "accel_search_nwindows", "accel_search_norms", "accel_search_timings",
-            "accel_search_tune",
+            "accel_search_tune", "accel_refine_set_first_wait",
             "accel_search_host_timings",
             "accel_search_corr_shape", "accel_search_plane_cells",
             "accel_search_windows", "accel_deredden", "accel_prepare_dat_spectrum",

@@ -1556,6 +1556,8 @@ static int refine_unpolished(accel_search *s, int n, const int *nh, const double
     return rc;
 }
 
+void accel_refine_set_first_wait(int log2_polls) { agpu_refine_set_first_wait(log2_polls); }
+
 int accel_search_optimize(accel_search *s)
 {
     const accel_geom *g = &s->g;

@@ -244,7 +244,9 @@ int agpu_realfft(const float *d_x, long long n, void *d_out, void *d_work, void 
  * PRESTO's packing to n samples; d_X may be d_out; the same d_work size. */
 int agpu_realfft_inverse(const void *d_X, long long n, float *d_out, void *d_work, void *stream);
 
-/* Candidate refinement (harmonic polishing), one workgroup per candidate.
+/* Candidate refinement (harmonic polishing) on the device: max_rz_arr_harmonics
+ * with one workgroup per harmonic of a candidate, max_rzw_arr_harmonics as a
+ * sequence of kernels over many workgroups.
  * spec: the resident spectrum (numdata complex FP32 bins, bin 0 = lobin);
  * rin = candidate r - lobin; per candidate up to AGPU_REFINE_MAXH
  * harmonics; derivs[c][h][7] = pow, phs, dpow, dphs, d2pow, d2phs, locpow;
@@ -260,7 +262,8 @@ typedef struct {
     double *derivs;
     double *powers;
     int *nevals;
-    /* jerk refinement only: agpu_refine_rzw_scratch(ncand, npts_cap) bytes */
+    /* agpu_refine_rz_scratch(ncand) bytes (the candidates' group records)
+     * or, for jerk refinement, agpu_refine_rzw_scratch(ncand, npts_cap) */
     void *scratch;
     int npts_cap;
     /* 1: one harmonic per candidate without polishing (-noharmpolish):
@@ -279,6 +282,10 @@ typedef struct {
  * rout = NaN, nevals = -1. */
 size_t agpu_refine_rz_scratch(int ncand);
 int agpu_refine_rz(const agpu_refine_args *args, void *stream);
+/* The first attempt's barrier wait of agpu_refine_rz, 2^log2_polls polls
+ * (0..24, default 24); candidates whose wait runs out are run again with
+ * the full bound.  Lowered only to test that retry. */
+void agpu_refine_set_first_wait(int log2_polls);
 /* max_rzw_arr_harmonics per candidate, spread over many workgroups per
  * candidate; maxharm: the largest numharm of the candidates.  scratch must
  * hold agpu_refine_rzw_scratch(ncand, npts_cap) bytes.  A candidate whose

@@ -167,12 +167,16 @@ __device__ static void derivs_wg(const float2 *spec, long long numdata, double r
  * power, stores it in the candidate's group record and waits at the
  * group's barrier; then every workgroup sums the H terms in harmonic
  * order, so all of them hold the same value and take the same simplex
- * step.  The local powers before and the
- * powers and derivatives after the simplex are per harmonic too.  The
- * launcher keeps every workgroup of a launch resident at once; the
- * barrier's wait is bounded (RZ_SPIN_LIMIT polls), and a
- * group whose wait runs out is marked bad and stops. */
+ * step.  The local powers before and the powers and derivatives after
+ * the simplex are per harmonic too.  The launcher sizes each launch so
+ * that all its workgroups can be resident at once; the barrier's wait is
+ * bounded (RZ_SPIN_LIMIT polls), a group whose wait runs out is marked bad
+ * and stops, and the launcher runs that candidate again on its own. */
 #define RZ_SPIN_LIMIT (1LL << 24)
+
+/* log2 of the first attempt's wait bound; lowered only by tests of the
+ * retry path (agpu_refine_set_first_wait) */
+static int g_rz_first_log2 = 24;
 
 static pthread_mutex_t g_rz_launch_mu = PTHREAD_MUTEX_INITIALIZER;
 
@@ -187,6 +191,7 @@ struct rz_mw {
     int k, H;                    /* this workgroup's harmonic, the candidate's count */
     unsigned arrivals;           /* barriers passed by this workgroup */
     int bad;
+    long long spin_limit;        /* polls before the wait gives up */
     float lp;                    /* this harmonic's local power */
 };
 
@@ -204,7 +209,7 @@ __device__ static void rz_group_barrier(rz_mw &m, int *s_flag)
             while (__hip_atomic_load(&m.g->count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
                    target) {
                 if (__hip_atomic_load(&m.g->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-                    ++spins > RZ_SPIN_LIMIT) {
+                    ++spins > m.spin_limit) {
                     __hip_atomic_store(&m.g->bad, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     bad = 1;
                     break;
@@ -367,7 +372,8 @@ __device__ static void rf_simplex_mw(rf_ctx_mw &c, double *x, double step, doubl
 
 /* blockIdx.x = candidate * hmax + harmonic index; a.scratch: one rz_group
  * per candidate, zeroed by the launcher */
-__global__ __launch_bounds__(RF_TPB) void k_refine_rz_mw(agpu_refine_args a, int hmax)
+__global__ __launch_bounds__(RF_TPB) void k_refine_rz_mw(agpu_refine_args a, int hmax,
+                                                         long long spin_limit)
 {
     __shared__ double2 red[RF_TPB];
     __shared__ int s_flag;
@@ -384,6 +390,7 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rz_mw(agpu_refine_args a, int
     m.H = H;
     m.arrivals = 0;
     m.bad = 0;
+    m.spin_limit = spin_limit;
     const int n = k + 1;
     {
         double lp = localpower_wg(spec, a.numdata, rin * n, zin * n, red);
@@ -416,6 +423,14 @@ __global__ __launch_bounds__(RF_TPB) void k_refine_rz_mw(agpu_refine_args a, int
 size_t agpu_refine_rz_scratch(int ncand)
 {
     return sizeof(rz_group) * (size_t) (ncand > 0 ? ncand : 1);
+}
+
+static hipError_t rz_launch(const agpu_refine_args *args, int c0, int m, int hmax,
+                            long long spin_limit, hipStream_t st);
+
+void agpu_refine_set_first_wait(int log2_polls)
+{
+    g_rz_first_log2 = log2_polls < 0 ? 0 : (log2_polls > 24 ? 24 : log2_polls);
 }
 
 int agpu_refine_rz(const agpu_refine_args *args, void *stream)
@@ -454,28 +469,34 @@ int agpu_refine_rz(const agpu_refine_args *args, void *stream)
     }
     pthread_mutex_lock(&g_rz_launch_mu);
     hipError_t e = hipMemsetAsync(args->scratch, 0, agpu_refine_rz_scratch(args->ncand), st);
+    int *nev = NULL;
     for (int c0 = 0; c0 < args->ncand && e == hipSuccess; c0 += chunk) {
         const int m = args->ncand - c0 < chunk ? args->ncand - c0 : chunk;
-        agpu_refine_args a = *args;
-        a.ncand = m;
-        a.numharm += c0;
-        a.rin += c0;
-        a.zin += c0;
-        if (a.win)
-            a.win += c0;
-        a.rout += c0;
-        a.zout += c0;
-        if (a.wout)
-            a.wout += c0;
-        a.derivs += (size_t) c0 * RF_MAXH * 7;
-        a.powers += (size_t) c0 * RF_MAXH;
-        a.nevals += c0;
-        a.scratch = (void *) ((rz_group *) args->scratch + c0);
-        hipLaunchKernelGGL(k_refine_rz_mw, dim3(m * hmax), dim3(RF_TPB), 0, st, a, hmax);
-        e = hipGetLastError();
-        if (e == hipSuccess)
-            e = hipStreamSynchronize(st);
+        e = rz_launch(args, c0, m, hmax, 1LL << g_rz_first_log2, st);
     }
+    /* a candidate whose group's wait ran out (its workgroups could not all
+     * be resident, e.g. under another process's kernels) runs again on its
+     * own, up to three times; one that still fails is reported by its
+     * nevals of -1 */
+    if (e == hipSuccess && !(nev = (int *) malloc(sizeof(int) * (size_t) args->ncand)))
+        e = hipErrorOutOfMemory;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(nev, args->nevals, sizeof(int) * (size_t) args->ncand,
+                           hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    for (int c = 0; c < args->ncand && e == hipSuccess; c++)
+        for (int attempt = 0; attempt < 3 && nev[c] < 0 && e == hipSuccess; attempt++) {
+            e = hipMemsetAsync((rz_group *) args->scratch + c, 0, sizeof(rz_group), st);
+            if (e == hipSuccess)
+                e = rz_launch(args, c, 1, hmax, RZ_SPIN_LIMIT, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(nev + c, args->nevals + c, sizeof(int),
+                                   hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(st);
+        }
+    free(nev);
     pthread_mutex_unlock(&g_rz_launch_mu);
     if (e != hipSuccess) {
         char m_[256];
@@ -484,6 +505,33 @@ int agpu_refine_rz(const agpu_refine_args *args, void *stream)
         return -1;
     }
     return 0;
+}
+
+/* One launch of k_refine_rz_mw over candidates [c0, c0 + m), waited for. */
+static hipError_t rz_launch(const agpu_refine_args *args, int c0, int m, int hmax,
+                            long long spin_limit, hipStream_t st)
+{
+    agpu_refine_args a = *args;
+    a.ncand = m;
+    a.numharm += c0;
+    a.rin += c0;
+    a.zin += c0;
+    if (a.win)
+        a.win += c0;
+    a.rout += c0;
+    a.zout += c0;
+    if (a.wout)
+        a.wout += c0;
+    a.derivs += (size_t) c0 * RF_MAXH * 7;
+    a.powers += (size_t) c0 * RF_MAXH;
+    a.nevals += c0;
+    a.scratch = (void *) ((rz_group *) args->scratch + c0);
+    hipLaunchKernelGGL(k_refine_rz_mw, dim3(m * hmax), dim3(RF_TPB), 0, st, a, hmax,
+                       spin_limit);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    return e;
 }
 
 /* ------------------------------------------------------------------ */

@@ -378,3 +378,31 @@ def test_unpolished_rzw_matches_oracle():
                 assert abs(pw[k] - opw) <= 2e-5 * opw
                 assert abs(der[k].locpow - ord_.locpow) <= 2e-5 * ord_.locpow
                 assert abs(der[k].pow - ord_.pow) <= 2e-5 * ord_.pow
+
+
+def test_refinement_retry_after_a_wait_runs_out():
+    """A candidate's harmonics are refined by workgroups that wait for each
+    other at every simplex step.  With the first attempt's wait bound set to
+    one poll the groups give up, and each candidate is refined again on its
+    own with the full bound: the refined properties equal a normal run's
+    bit for bit."""
+    _say("case: refinement retry, N=2^16 zmax 100 numharm 8, first wait 1 poll")
+    dt = 64e-6
+    x = synth.make_series(1 << 16, dt, 58, [dict(f0=61.3, z=9.0, duty=0.05, amp=0.35),
+                                            dict(f0=377.7, z=-7.0, duty=0.08, amp=0.3)])
+    fft = A.spectrum_from_series(x, dt)
+    p = A.make_params(len(fft), len(x), dt, zmax=100, numharm=8, sigma=2.0)
+    with A.Search(p) as s:
+        s.set_spectrum(fft)
+        s.run()
+        c = s.candidates()
+        assert len(c) >= 2 and max(c["numharm"]) > 1
+        s.optimize()
+        want = [bytes(v) for v in s.fund_props()]
+        A.lib.accel_refine_set_first_wait(0)
+        try:
+            s.optimize()
+            got = [bytes(v) for v in s.fund_props()]
+        finally:
+            A.lib.accel_refine_set_first_wait(24)
+    assert got == want
