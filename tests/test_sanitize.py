@@ -39,3 +39,27 @@ def test_host_code_under_asan_ubsan(tmp_path):
     assert "sanitized host checks passed" in r.stdout
     assert "runtime error" not in r.stderr and "Sanitizer" not in r.stderr
     assert (scratch / "sifted.txt").stat().st_size > 0
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
+@pytest.mark.parametrize("sanitizer", ["thread", "address,undefined"])
+def test_request_loop_under_sanitizer(tmp_path, sanitizer):
+    """The request loop (accel_server.c) with test doubles of the search
+    handle (tests/sanitize/server_driver.c): 200 mixed requests through 4
+    workers within a tight budget, a request above the budget, and a small
+    request backfilled beside a large one that waits for memory; under
+    ThreadSanitizer (data races on the queue, the handle cache and the
+    request array) and under AddressSanitizer/UBSan."""
+    exe = tmp_path / "server_driver"
+    cmd = ["gcc", "-O1", "-g", "-std=gnu11", f"-fsanitize={sanitizer}",
+           "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-I", CSRC,
+           "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "sanitize", "server_driver.c"),
+           os.path.join(CSRC, "accel_server.c"), "-lpthread", "-lm", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1", ASAN_OPTIONS="detect_leaks=1")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, (r.stdout + r.stderr)[-4000:]
+    assert "request loop checks passed" in r.stdout
+    assert "ThreadSanitizer" not in r.stderr and "runtime error" not in r.stderr
