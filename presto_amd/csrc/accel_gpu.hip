@@ -520,13 +520,17 @@ __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long 
                                                   long long lobin_file, const float2 *T,
                                                   int nt, int hw, int uselen, long long hb_end,
                                                   float *plane, int groups, int tpg,
-                                                  long long seg0)
+                                                  long long seg0, long long nsr)
 {
     __shared__ float2 lds[FFT_LDS];
     __shared__ float2 tw1l[256];
     const int j = threadIdx.x;
-    const int g = blockIdx.x % groups;
-    const long long seg = seg0 + blockIdx.x / groups;
+    /* XCD x (block ids x, x + 8, ...) takes the template groups x, x + 8,
+     * ... one after another, each over all nsr segments, so the workgroups
+     * resident on an XCD share one group's templates in its L2 (a jerk
+     * bank is ~200 MB; with 8 groups this is one group per XCD) */
+    const int g = (int) (blockIdx.x & 7) + 8 * (int) ((blockIdx.x >> 3) / nsr);
+    const long long seg = seg0 + (long long) (blockIdx.x >> 3) % nsr;
     const long long seg_hb = seg * (long long) uselen;
     const long long lobin = seg_hb / 2 - hw;
     const int t0 = g * tpg;
@@ -1393,8 +1397,8 @@ int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
         return fail("agpu_correlate: only L = 4096 is built");
     if (uselen <= 0 || uselen % 2 || uselen > L - 4 * hw + 1)
         return fail("agpu_correlate: uselen must be even and <= L - 4*hw + 1");
-    if (groups <= 0)
-        return fail("agpu_correlate: groups must be positive");
+    if (groups <= 0 || groups % 8)
+        return fail("agpu_correlate: groups must be a positive multiple of 8");
     if (hb_end > ntiles * AGPU_TILE)
         return fail("agpu_correlate: plane tiles do not cover the half-bin range");
     long long nseg = (hb_end + uselen - 1) / uselen;
@@ -1410,7 +1414,7 @@ int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
         return fail("agpu_correlate: grid too large");
     hipLaunchKernelGGL(k_corr4096, dim3((unsigned) nblk), dim3(256), 0, (hipStream_t) stream,
                        (const float2 *) d_spec, numbins, lobin_file, (const float2 *) d_T, nt,
-                       hw, uselen, hb_end, d_plane, groups, tpg, seg_lo);
+                       hw, uselen, hb_end, d_plane, groups, tpg, seg_lo, seg_hi - seg_lo);
     AGPU_CHECK(hipGetLastError());
     return 0;
 }
