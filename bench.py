@@ -410,7 +410,7 @@ def jerk_section(n):
             "search_ms_max": 1e3 * float(np.max(t_search)),
             "cells_per_s": cells / float(np.mean(t_search)),
             "kernel_ms": {"correlation": avg(tims, "corr"), "harmonic_sum": avg(tims, "harmsum"),
-                          "block_norms": avg(tims, "norms"), "plane_halo": avg(tims, "halo"),
+                          "block_norms": avg(tims, "norms"), "corr_to_harmsum_gap": avg(tims, "halo"),
                           "device_total": avg(tims, "total")},
             "host_ms": {k: avg(hosts, k) for k in hosts[0]},
             "hits_mean": float(np.mean(nhits)), "candidates_mean": float(np.mean(ncands)),
@@ -542,7 +542,7 @@ def main():
         "p50_ms": 1e3 * float(np.percentile(lat, 50)),
         "p99_ms": 1e3 * float(np.percentile(lat, 99)),
         "kernel_ms": {"block_norms": avg("norms"), "correlation": corr_ms,
-                      "plane_halo": avg("halo"), "harmonic_sum": hs_ms,
+                      "corr_to_harmsum_gap": avg("halo"), "harmonic_sum": hs_ms,
                       "device_total": avg("total")},
         "host_ms": {k: float(np.mean([h[k] for h in host])) for k in host[0]},
         "candidates": ncand,

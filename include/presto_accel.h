@@ -156,8 +156,11 @@ int accel_search_plane_row(const accel_search *s, int wi, int zi, long long hb0,
                            float *out);
 int accel_search_nwindows(const accel_search *s);
 int accel_search_norms(const accel_search *s, double *out, int n);
-/* Milliseconds of the last run (HIP events on the search's stream): block
- * norms, correlation, plane halo fill, harmonic sum, total. */
+/* Milliseconds of the last run (HIP events): block norms (on a second
+ * stream, beside the correlation), correlation, the gap between the
+ * correlation and the harmonic sum (the plane's tile halos are written by
+ * the correlation, so about 0), harmonic sum (including any wait for the
+ * norms), total. */
 int accel_search_timings(const accel_search *s, float *ms5);
 /* Host wall-clock milliseconds of the last run: enqueueing the kernels,
  * waiting for them, copying the hits back, replaying the hits into the

@@ -577,26 +577,16 @@ __global__ __launch_bounds__(256) void k_corr4096(const float2 *spec, long long 
         for (int r = 0; r < 16; r++) {
             const int q = j + 256 * r - 2 * hw;
             const long long hb = seg_hb + q;
-            if (q >= 0 && q < uselen && hb < hb_end)
-                plane[AGPU_PLANE_IDX(t, hb, nt)] =
-                    __builtin_fmaf(v[r].x, v[r].x, v[r].y * v[r].y);
+            if (q >= 0 && q < uselen && hb < hb_end) {
+                const float p = __builtin_fmaf(v[r].x, v[r].x, v[r].y * v[r].y);
+                plane[AGPU_PLANE_IDX(t, hb, nt)] = p;
+                /* the first AGPU_HALO half-bins of a tile are repeated in
+                 * the previous tile's halo */
+                if ((hb & (AGPU_TILE - 1)) < AGPU_HALO && hb >= AGPU_TILE)
+                    plane[AGPU_PLANE_IDX(t, hb - AGPU_TILE, nt) + AGPU_TILE] = p;
+            }
         }
     }
-}
-
-/* Tile-row halos: row r of tile t repeats the first AGPU_HALO half-bins of
- * row r of tile t+1.  One workgroup per (tile, row), float4 copies. */
-__global__ __launch_bounds__(256) void k_plane_halo(float *plane, int nrows, int ntiles,
-                                                    int tile0)
-{
-    const int row = blockIdx.x % nrows;
-    const int tile = tile0 + blockIdx.x / nrows;
-    if (tile + 1 >= ntiles)
-        return;
-    const float4 *src = (const float4 *) (plane + AGPU_PLANE_IDX(row, (long long) (tile + 1) * AGPU_TILE, nrows));
-    float4 *dst = (float4 *) (plane + AGPU_PLANE_IDX(row, (long long) tile * AGPU_TILE, nrows) + AGPU_TILE);
-    for (int i = threadIdx.x; i < AGPU_HALO / 4; i += 256)
-        dst[i] = src[i];
 }
 
 /* ------------------------------------------------------------------ */
@@ -1180,6 +1170,12 @@ int agpu_stream_create(void **s)
     return 0;
 }
 
+int agpu_stream_wait_event(void *s, void *e)
+{
+    AGPU_CHECK(hipStreamWaitEvent((hipStream_t) s, (hipEvent_t) e, 0));
+    return 0;
+}
+
 int agpu_stream_destroy(void *s)
 {
     AGPU_CHECK(hipStreamDestroy((hipStream_t) s));
@@ -1416,30 +1412,6 @@ int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
                        (const float2 *) d_spec, numbins, lobin_file, (const float2 *) d_T, nt,
                        hw, uselen, hb_end, d_plane, groups, tpg, seg_lo, seg_hi - seg_lo);
     AGPU_CHECK(hipGetLastError());
-    return 0;
-}
-
-int agpu_plane_halo(float *d_plane, int nrows, long long ntiles, long long tile_lo,
-                    long long tile_hi, void *stream)
-{
-    if (tile_hi < 0 || tile_hi > ntiles)
-        tile_hi = ntiles;
-    if (tile_lo < 0 || tile_lo > tile_hi)
-        return fail("agpu_plane_halo: bad tile range");
-    if (nrows <= 0 || tile_lo == tile_hi)
-        return 0;
-    /* launches of at most 2^24 - 1 workgroups: a dispatch counts work-items
-     * in 32 bits */
-    const long long per = ((1LL << 24) - 1) / nrows;
-    if (per < 1)
-        return fail("agpu_plane_halo: too many rows");
-    for (long long t0 = tile_lo; t0 < tile_hi; t0 += per) {
-        const long long t1 = t0 + per < tile_hi ? t0 + per : tile_hi;
-        const long long nblk = (long long) nrows * (t1 - t0);
-        hipLaunchKernelGGL(k_plane_halo, dim3((unsigned) nblk), dim3(256), 0,
-                           (hipStream_t) stream, d_plane, nrows, (int) ntiles, (int) t0);
-        AGPU_CHECK(hipGetLastError());
-    }
     return 0;
 }
 

@@ -311,7 +311,9 @@ struct accel_search {
     int npre;             /* in-memory pre-population blocks */
     int nwin, maxn;
     void *stream;
+    void *stream2;                    /* the block norms, beside the correlation */
     void *ev[5];
+    void *ev_norms;
     void *d_spec;
     int own_spec;
     void *d_T;
@@ -626,8 +628,10 @@ static int search_init(accel_search *s, int device)
     accel_geom *g = &s->g;
     GPU(agpu_init(device));
     GPU(agpu_stream_create(&s->stream));
+    GPU(agpu_stream_create(&s->stream2));
     for (int i = 0; i < 5; i++)
         GPU(agpu_event_create(&s->ev[i]));
+    GPU(agpu_event_create(&s->ev_norms));
     s->numw1 = g->numw ? g->numw : 1;
     s->nt = s->numw1 * g->numz;
     s->hw = g->maxkernlen / (2 * ACCEL_NUMBETWEEN);
@@ -749,8 +753,12 @@ void accel_search_destroy(accel_search *s)
     for (int i = 0; i < 5; i++)
         if (s->ev[i])
             agpu_event_destroy(s->ev[i]);
+    if (s->ev_norms)
+        agpu_event_destroy(s->ev_norms);
     if (s->stream)
         agpu_stream_destroy(s->stream);
+    if (s->stream2)
+        agpu_stream_destroy(s->stream2);
     free(s->hits);
     free_cands(s->cands);
     free(s->opt_nh);
@@ -831,18 +839,24 @@ int accel_search_launch(accel_search *s)
     const double t0 = now_ms();
     GPU(agpu_memset(s->d_nhits, 0, sizeof(unsigned), s->stream));
     GPU(agpu_event_record(s->ev[0], s->stream));
+    /* the block norms (latency-bound, few workgroups) run on a second
+     * stream beside the correlation (VALU-bound); the harmonic sum waits
+     * for both */
+    GPU(agpu_stream_wait_event(s->stream2, s->ev[0]));
     if (!(g->nph > 0.0))   /* -photon: the constant norms set at creation */
         GPU(agpu_block_norms(s->d_spec, g->numbins, g->lobin, s->d_win, s->nwin, s->maxn,
-                             s->d_norm, s->d_nsq, s->stream));
+                             s->d_norm, s->d_nsq, s->stream2));
     if (g->inmem)
         GPU(agpu_inmem_nsq_table(s->d_nsq, s->npre, g->nblocks, 2 * fund_rlo(g->rlo),
-                                 g->corr_uselen, s->hb_end, s->d_nsq_hb, s->stream));
+                                 g->corr_uselen, s->hb_end, s->d_nsq_hb, s->stream2));
+    GPU(agpu_event_record(s->ev_norms, s->stream2));
     GPU(agpu_event_record(s->ev[1], s->stream));
     GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw, s->uselen,
                        s->hb_end, s->d_plane, s->ntiles, s->groups, 0, -1, s->stream));
     GPU(agpu_event_record(s->ev[2], s->stream));
-    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, 0, -1, s->stream));
+    /* the tile halos are written by the correlation itself */
     GPU(agpu_event_record(s->ev[3], s->stream));
+    GPU(agpu_stream_wait_event(s->stream, s->ev_norms));
     fill_harmsum_args(s, &a);
     GPU(agpu_harmsum(&a, 0, -1, s->stream));
     GPU(agpu_event_record(s->ev[4], s->stream));
@@ -1126,7 +1140,8 @@ int accel_search_collect(accel_search *s)
             return -1;
         }
     }
-    for (int i = 0; i < 4; i++)
+    GPU(agpu_event_elapsed(&s->ms[0], s->ev[0], s->ev_norms));
+    for (int i = 1; i < 4; i++)
         GPU(agpu_event_elapsed(&s->ms[i], s->ev[i], s->ev[i + 1]));
     GPU(agpu_event_elapsed(&s->ms[4], s->ev[0], s->ev[4]));
     double t1 = now_ms();
@@ -1667,7 +1682,6 @@ int accel_search_plane_only(accel_search *s)
     const accel_geom *g = &s->g;
     GPU(agpu_correlate(s->d_spec, g->numbins, g->lobin, s->d_T, s->nt, s->L, s->hw, s->uselen,
                        s->hb_end, s->d_plane, s->ntiles, s->groups, 0, -1, s->stream));
-    GPU(agpu_plane_halo(s->d_plane, s->nt, s->ntiles, 0, -1, s->stream));
     GPU(agpu_stream_sync(s->stream));
     return 0;
 }

@@ -36,7 +36,7 @@ typedef struct {
  * Element (row, hb) lives at
  *   ((hb / AGPU_TILE) * nrows + row) * AGPU_ROWLEN + hb % AGPU_TILE
  * (and, for hb % AGPU_TILE < AGPU_HALO, also in the previous tile's halo,
- * filled by agpu_plane_halo after the correlation).
+ * which the correlation writes with the element).
  * All rows of a tile share a few 2 MiB pages, which keeps the harmonic-sum
  * gathers (16 rows at a time) inside the translation caches, and the halo
  * lets every sub-harmonic span (at most 1088 half-bins) be read from one
@@ -120,6 +120,8 @@ int agpu_memset(void *p, int v, size_t bytes, void *stream);
 int agpu_stream_create(void **s);
 int agpu_stream_destroy(void *s);
 int agpu_stream_sync(void *s);
+/* work queued on stream s after this waits for event e */
+int agpu_stream_wait_event(void *s, void *e);
 int agpu_event_create(void **e);
 int agpu_event_destroy(void *e);
 int agpu_event_record(void *e, void *stream);
@@ -204,18 +206,12 @@ int agpu_sf_locpow_plane_tab(const void *d_wdata, int numdata, int binoffset, in
 
 /* Fused overlap-save correlation: raw |corr|^2 plane rows for templates
  * [0, nt) (row = template), half-bins [0, hb_end), segment length L = 4096,
- * written in the tiled layout above.  Only segments [seg_lo, seg_hi)
+ * written in the tiled layout above, halos included.  Only segments [seg_lo, seg_hi)
  * (segment s: half-bins [s uselen, (s+1) uselen); seg_hi -1: to the end). */
 int agpu_correlate(const void *d_spec, long long numbins, long long lobin_file,
                    const void *d_T, int nt, int L, int hw, int uselen, long long hb_end,
                    float *d_plane, long long ntiles, int groups, long long seg_lo,
                    long long seg_hi, void *stream);
-
-/* Fills the tile-row halos of tiles [tile_lo, tile_hi) (-1: to the end) of
- * a plane written by agpu_correlate; tile t needs half-bins up to
- * (t+1) AGPU_TILE + AGPU_HALO computed. */
-int agpu_plane_halo(float *d_plane, int nrows, long long ntiles, long long tile_lo,
-                    long long tile_hi, void *stream);
 
 /* Harmonic sums of fundamental chunks [c_lo, c_hi) (-1: to the end); chunk
  * c covers the 1024 half-bins from agpu_harmsum_chunk_hb(a, c). */
