@@ -623,12 +623,15 @@ static int photon_norms(accel_search *s)
     return 0;
 }
 
-static int search_init(accel_search *s, int device)
+static int search_init(accel_search *s, int device, int nstreams)
 {
     accel_geom *g = &s->g;
     GPU(agpu_init(device));
     GPU(agpu_stream_create(&s->stream));
-    GPU(agpu_stream_create(&s->stream2));
+    if (nstreams > 1)
+        GPU(agpu_stream_create(&s->stream2));
+    else
+        s->stream2 = s->stream;
     for (int i = 0; i < 5; i++)
         GPU(agpu_event_create(&s->ev[i]));
     GPU(agpu_event_create(&s->ev_norms));
@@ -713,9 +716,14 @@ static int search_init(accel_search *s, int device)
 
 accel_search *accel_search_create(const accel_params *p, int device)
 {
+    return accel_search_create_streams(p, device, 2);
+}
+
+accel_search *accel_search_create_streams(const accel_params *p, int device, int nstreams)
+{
     accel_search *s = (accel_search *) calloc(1, sizeof(accel_search));
     s->p = *p;
-    if (accel_geom_create(p, &s->g) || search_init(s, device)) {
+    if (accel_geom_create(p, &s->g) || search_init(s, device, nstreams)) {
         char keep[sizeof(g_err)];
         memcpy(keep, g_err, sizeof(keep));
         accel_search_destroy(s);
@@ -731,6 +739,8 @@ void accel_search_destroy(accel_search *s)
         return;
     if (s->stream)
         agpu_stream_sync(s->stream);
+    if (s->stream2 && s->stream2 != s->stream)
+        agpu_stream_sync(s->stream2);
     if (s->own_spec)
         agpu_free(s->d_spec);
     bank_put(s->d_T);
@@ -757,7 +767,7 @@ void accel_search_destroy(accel_search *s)
         agpu_event_destroy(s->ev_norms);
     if (s->stream)
         agpu_stream_destroy(s->stream);
-    if (s->stream2)
+    if (s->stream2 && s->stream2 != s->stream)
         agpu_stream_destroy(s->stream2);
     free(s->hits);
     free_cands(s->cands);
@@ -842,7 +852,9 @@ int accel_search_launch(accel_search *s)
     /* the block norms (latency-bound, few workgroups) run on a second
      * stream beside the correlation (VALU-bound); the harmonic sum waits
      * for both */
-    GPU(agpu_stream_wait_event(s->stream2, s->ev[0]));
+    const int two = s->stream2 != s->stream;
+    if (two)
+        GPU(agpu_stream_wait_event(s->stream2, s->ev[0]));
     if (!(g->nph > 0.0))   /* -photon: the constant norms set at creation */
         GPU(agpu_block_norms(s->d_spec, g->numbins, g->lobin, s->d_win, s->nwin, s->maxn,
                              s->d_norm, s->d_nsq, s->stream2));
@@ -856,7 +868,8 @@ int accel_search_launch(accel_search *s)
     GPU(agpu_event_record(s->ev[2], s->stream));
     /* the tile halos are written by the correlation itself */
     GPU(agpu_event_record(s->ev[3], s->stream));
-    GPU(agpu_stream_wait_event(s->stream, s->ev_norms));
+    if (two)
+        GPU(agpu_stream_wait_event(s->stream, s->ev_norms));
     fill_harmsum_args(s, &a);
     GPU(agpu_harmsum(&a, 0, -1, s->stream));
     GPU(agpu_event_record(s->ev[4], s->stream));

@@ -26,6 +26,12 @@ typedef struct {
 } accel_dev_view;
 
 ACCEL_HIDDEN void accel_search_dev_view(const accel_search *s, accel_dev_view *v);
+/* accel_search_create with nstreams = 2 (the block norms on a second
+ * stream beside the correlation) or 1 (everything on one stream: for the
+ * request loop, whose concurrent requests already overlap, and where more
+ * streams per handle would share the device's few hardware queues). */
+ACCEL_HIDDEN accel_search *accel_search_create_streams(const accel_params *p, int device,
+                                                       int nstreams);
 /* The raw plane alone (correlation and halos over every half-bin), then
  * waits for it. */
 ACCEL_HIDDEN int accel_search_plane_only(accel_search *s);

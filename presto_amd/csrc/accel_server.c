@@ -27,6 +27,7 @@
 #include <time.h>
 #include "presto_accel.h"
 #include "agpu.h"
+#include "accel_internal.h"
 
 #define SRV_MAXW 64
 #define SRV_SCAN 256   /* queued requests looked at for a backfill */
@@ -220,7 +221,7 @@ static void *worker(void *arg)
         agpu_init(sv->device);
         double t0 = now_s();
         if (!s)
-            s = accel_search_create(&p, sv->device);
+            s = accel_search_create_streams(&p, sv->device, 1);
         double t1 = now_s();
         if (s)
             run_request(sv, &local, s);

@@ -68,6 +68,12 @@ accel_search *accel_search_create(const accel_params *p, int device)
     return s;
 }
 
+accel_search *accel_search_create_streams(const accel_params *p, int device, int nstreams)
+{
+    (void) nstreams;
+    return accel_search_create(p, device);
+}
+
 void accel_search_destroy(accel_search *s)
 {
     pthread_mutex_lock(&g_mu);
