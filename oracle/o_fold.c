@@ -134,3 +134,122 @@ long long o_fold_dat(const float *series, long long n, double dt, double f, doub
     free(buffer);
     return done;
 }
+
+/* ---- prepfold's period / period-derivative search, one subband ----
+ * src/prepfold_pipeline.c:1633-1915 with nsub 1, no p-dotdot search;
+ * src/fold.c:696-754 (combine_profs); src/characteristics.c:562-575
+ * (chisqr); src/prepfold_utils.c:7-73 (the phase-delay conversions). */
+
+static double o_fdot2phasedelay(double fdot, double time)
+{
+    double r = fdot * time * time / 2.0;
+    return r == -0 ? 0.0 : r;
+}
+
+static double o_phasedelay2fdot(double phasedelay, double time)
+{
+    return time == 0.0 ? 0.0 : 2.0 * phasedelay / (time * time);
+}
+
+static double o_switch_pfdot(double pf, double pfdot)
+{
+    if (pf == 0.0)
+        return 0.0;
+    double r = -pfdot / (pf * pf);
+    return r == -0 ? 0.0 : r;
+}
+
+static double o_switch_pfdotdot(double pf, double pfdot, double pfdotdot)
+{
+    if (pf == 0.0 || pfdotdot == 0.0)
+        return 0.0;
+    double r = 2.0 * pfdot * pfdot / (pf * pf * pf) - pfdotdot / (pf * pf);
+    return r == -0 ? 0.0 : r;
+}
+
+/* combine_profs: each profile shifted by its delay rounded to whole bins
+ * and summed in part order; the parts' profile statistics summed; the
+ * reduced chi-squared of the sum */
+static double o_combine(const double *profs, int npart, int proflen, const double *delays,
+                        double prof_avg, double prof_var, double *out)
+{
+    for (int k = 0; k < proflen; k++)
+        out[k] = 0.0;
+    int index = 0;
+    for (int ii = 0; ii < npart; ii++) {
+        double ld = fmod(-delays[ii], proflen);
+        if (ld < 0.0)
+            ld += proflen;
+        const int offset = (int) (ld + 0.5);
+        int jj = 0, kk = proflen - offset;
+        for (; jj < offset; jj++, kk++, index++)
+            out[kk] += profs[index];
+        for (kk = 0; jj < proflen; jj++, kk++, index++)
+            out[kk] += profs[index];
+    }
+    double chi = 0.0;
+    for (int k = 0; k < proflen; k++) {
+        const double d = out[k] - prof_avg;
+        chi += d * d;
+    }
+    return chi / prof_var / (proflen - 1.0);
+}
+
+int o_fold_search(const double *profs, const o_foldstats *parts, int npart, int proflen,
+                  double f, double fd, double fdd, double T, const double *parttimes,
+                  int npfact, int pstep, int pdstep, float *ppdot, double *best)
+{
+    const int nt = 2 * npfact * proflen + 1;
+    double prof_avg = 0.0, prof_var = 0.0;
+    for (int ii = 0; ii < npart; ii++) {
+        prof_avg += parts[ii].prof_avg;
+        prof_var += parts[ii].prof_var;
+    }
+    double *fdots = (double *) malloc(sizeof(double) * nt);
+    double *periods = (double *) malloc(sizeof(double) * nt);
+    double *pdots = (double *) malloc(sizeof(double) * nt);
+    double *pd_delays = (double *) malloc(sizeof(double) * npart);
+    double *delays = (double *) malloc(sizeof(double) * npart);
+    double *cur = (double *) malloc(sizeof(double) * proflen);
+    for (int i = 0; i < nt; i++) {
+        const int tot = i - (nt - 1) / 2;
+        double d = (double) (tot * pstep) / proflen;
+        periods[i] = 1.0 / (f + d / T);
+        d = (double) (tot * pdstep) / proflen;
+        fdots[i] = o_phasedelay2fdot(d, T);
+        pdots[i] = o_switch_pfdot(f, fd + fdots[i]);
+    }
+    int bip = 0, bipd = 0;
+    double bestredchi = 0.0;
+    for (int ipd = 0; ipd < nt; ipd++) {
+        for (int ii = 0; ii < npart; ii++)
+            pd_delays[ii] = 0.0 + o_fdot2phasedelay(fdots[ipd], parttimes[ii]) * proflen;
+        for (int ip = 0; ip < nt; ip++) {
+            const int totpdelay = pstep * (ip - (nt - 1) / 2);
+            for (int ii = 0; ii < npart; ii++)
+                delays[ii] = pd_delays[ii] + (double) (ii * totpdelay) / npart;
+            const double rc = o_combine(profs, npart, proflen, delays, prof_avg, prof_var, cur);
+            if (ppdot)
+                ppdot[(size_t) ipd * nt + ip] = (float) rc;
+            if (rc > bestredchi) {
+                bestredchi = rc;
+                bip = ip;
+                bipd = ipd;
+            }
+        }
+    }
+    /* best[]: ip, ipd, redchi, p, pd, pdd */
+    best[0] = bip;
+    best[1] = bipd;
+    best[2] = bestredchi;
+    best[3] = periods[bip];
+    best[4] = pdots[bipd];
+    best[5] = o_switch_pfdotdot(1.0 / periods[bip], fd + fdots[bipd], fdd);
+    free(fdots);
+    free(periods);
+    free(pdots);
+    free(pd_delays);
+    free(delays);
+    free(cur);
+    return nt;
+}

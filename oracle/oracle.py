@@ -492,6 +492,31 @@ _sig("o_fold_dat", _ll, C.POINTER(_f), _ll, _d, _d, _d, _d, _i, _i, C.POINTER(_d
      C.POINTER(OFoldStats))
 
 
+_sig("o_fold_search", _i, C.POINTER(_d), C.POINTER(OFoldStats), _i, _i, _d, _d, _d, _d,
+     C.POINTER(_d), _i, _i, _i, C.POINTER(_f), C.POINTER(_d))
+
+
+def fold_search(profs, stats, f, fd, fdd, T, parttimes, npfact=2, pstep=2, pdstep=4):
+    """prepfold's p / p-dot search over the sub-integrations: (ppdot
+    [numtrials][numtrials] of reduced chi-squared, dict of the best
+    trial)."""
+    profs = np.ascontiguousarray(profs, dtype=np.float64)
+    npart, proflen = profs.shape
+    st = (OFoldStats * npart)()
+    for s_, d in zip(st, stats):
+        for k, _ in OFoldStats._fields_:
+            setattr(s_, k, d[k])
+    nt = 2 * npfact * proflen + 1
+    pp = np.zeros((nt, nt), dtype=np.float32)
+    best = np.zeros(6)
+    pt = np.ascontiguousarray(parttimes, dtype=np.float64)
+    lib.o_fold_search(profs.ctypes.data_as(C.POINTER(_d)), st, npart, proflen, f, fd, fdd, T,
+                      pt.ctypes.data_as(C.POINTER(_d)), npfact, pstep, pdstep,
+                      pp.ctypes.data_as(C.POINTER(_f)), best.ctypes.data_as(C.POINTER(_d)))
+    return pp, dict(ip=int(best[0]), ipd=int(best[1]), redchi=best[2], p=best[3], pd=best[4],
+                    pdd=best[5])
+
+
 def cand_fold_params(r, z, w, T):
     """prepfold's start f, fdot, fdotdot of a candidate's average r, z, w."""
     f, fd, fdd = _d(), _d(), _d()

@@ -164,7 +164,8 @@ EXPORTED = ["accel_params_default", "accel_geom_create", "accel_search_create",
             "accel_sift_remove_dm_problems", "accel_sift_remove_harmonics",
             "accel_sift_sort_sigma", "accel_sift_extend", "accel_sift_count", "accel_sift_get",
             "accel_sift_write", "accel_sift_write_report", "accel_sift_write_summary",
-            "accel_read_rzw_cand", "accel_cand_fold_params", "accel_fold_proflen", "accel_fold"]
+            "accel_read_rzw_cand", "accel_cand_fold_params", "accel_fold_proflen", "accel_fold",
+            "accel_fold_search"]
 
 
 class AccelError(RuntimeError):
@@ -792,6 +793,16 @@ _sig("accel_fold", _ll, _p, _ll, _d, _d, _d, _d, _i, _i, _p, C.POINTER(FoldStats
      C.POINTER(FoldStats), _i)
 
 
+class FoldBest(C.Structure):
+    """accel_fold_best: the best trial of prepfold's p / p-dot search."""
+    _fields_ = [("numtrials", _i), ("ip", _i), ("ipd", _i), ("p", _d), ("pd", _d), ("pdd", _d),
+                ("stats", FoldStats)]
+
+
+_sig("accel_fold_search", _i, _p, C.POINTER(FoldStats), _i, _i, _d, _d, _d, _d, _p, _i, _i, _i,
+     _i, _p, _p, C.POINTER(FoldBest), _i)
+
+
 def read_rzw_cand(candfile, candnum):
     """Record candnum (from 1) of a .cand file (get_rzw_cand)."""
     out = FourierProps()
@@ -825,3 +836,26 @@ def fold(series, dt, f, fd=0.0, fdd=0.0, npart=64, proflen=None, device=0):
     if n < 0:
         raise _err()
     return prof, list(parts), total, int(n)
+
+
+def fold_search(rawfolds, parts, f, fd, fdd, T, parttimes, npfact=2, pstep=2, pdstep=4,
+                nosearch=False, device=0):
+    """prepfold's period / p-dot search over the sub-integrations of a fold
+    on the GPU: (ppdot [numtrials][numtrials] float32 reduced chi-squared,
+    indexed [ipd][ip], or None with nosearch; best profile; FoldBest)."""
+    prof = np.ascontiguousarray(rawfolds, dtype=np.float64)
+    npart, proflen = prof.shape
+    pt = np.ascontiguousarray(parttimes, dtype=np.float64)
+    if len(pt) != npart or len(parts) != npart:
+        raise ValueError("one part time and one FoldStats per sub-integration")
+    st = (FoldStats * npart)(*parts)
+    nt = 2 * npfact * proflen + 1
+    pp = None if nosearch else np.zeros((nt, nt), dtype=np.float32)
+    bestprof = np.zeros(proflen, dtype=np.float64)
+    best = FoldBest()
+    if lib.accel_fold_search(prof.ctypes.data, st, npart, proflen, f, fd, fdd, T, pt.ctypes.data,
+                             npfact, pstep, pdstep, int(bool(nosearch)),
+                             None if pp is None else pp.ctypes.data, bestprof.ctypes.data,
+                             C.byref(best), device):
+        raise _err()
+    return pp, bestprof, best

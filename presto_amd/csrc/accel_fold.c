@@ -4,8 +4,8 @@
  * src/get_candidates.c:27-37).  A candidate's record is read, its average
  * r, z, w become the start f, fdot, fdotdot of the fold, and the series is
  * folded on the GPU (fold_gpu.hip) into npart sub-integration profiles
- * with fold()'s statistics.  The period / period-derivative search that
- * prepfold runs afterwards is not built.
+ * with fold()'s statistics; prepfold's period / period-derivative search
+ * over those sub-integrations then runs on the GPU (accel_fold_search).
  */
 #include <math.h>
 #include <stdio.h>
@@ -177,5 +177,181 @@ done:
         agpu_stream_destroy(stream);
     free(part_end);
     free(recsums);
+    return rc;
+}
+
+/* src/prepfold_utils.c:7-73 */
+static double pfdot_of(double pf, double pfdot)
+{
+    if (pf == 0.0)
+        return 0.0;
+    const double r = -pfdot / (pf * pf);
+    return r == -0 ? 0.0 : r;
+}
+
+static double pfdotdot_of(double pf, double pfdot, double pfdotdot)
+{
+    if (pf == 0.0 || pfdotdot == 0.0)
+        return 0.0;
+    const double r = 2.0 * pfdot * pfdot / (pf * pf * pf) - pfdotdot / (pf * pf);
+    return r == -0 ? 0.0 : r;
+}
+
+static double fdot_of_delay(double phasedelay, double time)
+{
+    return time == 0.0 ? 0.0 : 2.0 * phasedelay / (time * time);
+}
+
+/* combine_profs for one trial (src/fold.c:696-754): the best profile */
+static void combine_trial(const double *profs, int npart, int proflen, const double *parttimes,
+                          double fdot, int tot, double *out)
+{
+    for (int k = 0; k < proflen; k++)
+        out[k] = 0.0;
+    for (int ii = 0; ii < npart; ii++) {
+        const double t = parttimes[ii];
+        double pd = fdot * t * t / 2.0;
+        if (pd == -0)
+            pd = 0.0;
+        const double delay = (0.0 + pd * proflen) + (double) (ii * tot) / npart;
+        double ld = fmod(-delay, (double) proflen);
+        if (ld < 0.0)
+            ld += proflen;
+        const int off = (int) (ld + 0.5);
+        for (int jj = 0; jj < proflen; jj++) {
+            int kk = jj - off;
+            if (kk < 0)
+                kk += proflen;
+            out[kk] += profs[(size_t) ii * proflen + jj];
+        }
+    }
+}
+
+int accel_fold_search(const double *rawfolds, const accel_foldstats *parts, int npart,
+                      int proflen, double f, double fd, double fdd, double T,
+                      const double *parttimes, int npfact, int pstep, int pdstep, int nosearch,
+                      float *ppdot, double *bestprof, accel_fold_best *best, int device)
+{
+    if (npart < 1 || proflen < 2 || npfact < 1 || pstep < 1 || pdstep < 1 || !(T > 0.0) ||
+        !(f > 0.0) || !best) {
+        accel_set_error("fold search: bad arguments");
+        return -1;
+    }
+    /* without a search the reference keeps the plane it shows small
+     * (src/prepfold_pipeline.c:1648-1654) */
+    const int nt = nosearch && proflen > 100 ? 201 : 2 * npfact * proflen + 1;
+    if (nt > 65535) {
+        accel_set_error("fold search: %d trials per axis is too many", nt);
+        return -1;
+    }
+    /* src/prepfold_pipeline.c:1696-1705 */
+    double *periods = (double *) malloc(sizeof(double) * nt);
+    double *fdots = (double *) malloc(sizeof(double) * nt);
+    double *pp = (double *) malloc(sizeof(double) * (size_t) nt * nt);
+    double *prof = (double *) malloc(sizeof(double) * proflen);
+    if (!periods || !fdots || !pp || !prof) {
+        free(periods);
+        free(fdots);
+        free(pp);
+        free(prof);
+        accel_set_error("fold search: out of host memory");
+        return -1;
+    }
+    for (int i = 0; i < nt; i++) {
+        const int tot = i - (nt - 1) / 2;
+        double d = (double) (tot * pstep) / proflen;
+        periods[i] = 1.0 / (f + d / T);
+        d = (double) (tot * pdstep) / proflen;
+        fdots[i] = fdot_of_delay(d, T);
+    }
+    accel_foldstats st;
+    memset(&st, 0, sizeof(st));
+    for (int ii = 0; ii < npart; ii++) {
+        st.numdata += parts[ii].numdata;
+        st.data_avg += parts[ii].data_avg;
+        st.data_var += parts[ii].data_var;
+        st.prof_avg += parts[ii].prof_avg;
+        st.prof_var += parts[ii].prof_var;
+    }
+    st.data_avg /= npart;
+    st.data_var /= npart;
+    st.numprof = proflen;
+    int bip = (nt - 1) / 2, bipd = (nt - 1) / 2, rc = -1;
+    void *stream = NULL, *d_prof = NULL, *d_fd = NULL, *d_pt = NULL, *d_pp = NULL;
+    if (!nosearch) {
+        const size_t pbytes = sizeof(double) * (size_t) npart * proflen;
+        if (agpu_init(device) || agpu_stream_create(&stream) || agpu_malloc(&d_prof, pbytes) ||
+            agpu_malloc(&d_fd, sizeof(double) * nt) ||
+            agpu_malloc(&d_pt, sizeof(double) * npart) ||
+            agpu_malloc(&d_pp, sizeof(double) * (size_t) nt * nt) ||
+            agpu_h2d(d_prof, rawfolds, pbytes, stream) ||
+            agpu_h2d(d_fd, fdots, sizeof(double) * nt, stream) ||
+            agpu_h2d(d_pt, parttimes, sizeof(double) * npart, stream)) {
+            accel_set_error("fold search: %s", agpu_last_error());
+            goto done;
+        }
+        agpu_fold_search_args a;
+        a.profs = (const double *) d_prof;
+        a.fdots = (const double *) d_fd;
+        a.parttimes = (const double *) d_pt;
+        a.prof_avg = st.prof_avg;
+        a.prof_var = st.prof_var;
+        a.npart = npart;
+        a.proflen = proflen;
+        a.numtrials = nt;
+        a.pstep = pstep;
+        a.redchi = (double *) d_pp;
+        if (agpu_fold_search(&a, stream) ||
+            agpu_d2h(pp, d_pp, sizeof(double) * (size_t) nt * nt, stream) ||
+            agpu_stream_sync(stream)) {
+            accel_set_error("fold search: %s", agpu_last_error());
+            goto done;
+        }
+        /* the first maximum in (p-dot, period) order: the reference's
+         * strictly-greater test over its loops, from a redchi of 0 */
+        double bestrc = 0.0;
+        bip = bipd = 0;
+        for (int ipd = 0; ipd < nt; ipd++)
+            for (int ip = 0; ip < nt; ip++)
+                if (pp[(size_t) ipd * nt + ip] > bestrc) {
+                    bestrc = pp[(size_t) ipd * nt + ip];
+                    bip = ip;
+                    bipd = ipd;
+                }
+        if (ppdot)   /* a float plane, as the reference keeps it */
+            for (size_t i = 0; i < (size_t) nt * nt; i++)
+                ppdot[i] = (float) pp[i];
+    }
+    /* the best profile and its statistics (the redchi in FP64, as
+     * combine_profs leaves it) */
+    combine_trial(rawfolds, npart, proflen, parttimes, fdots[bipd], pstep * (bip - (nt - 1) / 2),
+                  prof);
+    double chi = 0.0;
+    for (int k = 0; k < proflen; k++) {
+        const double d = prof[k] - st.prof_avg;
+        chi += d * d;
+    }
+    st.redchi = chi / st.prof_var / (proflen - 1.0);
+    if (bestprof)
+        memcpy(bestprof, prof, sizeof(double) * proflen);
+    best->numtrials = nt;
+    best->ip = bip;
+    best->ipd = bipd;
+    best->p = periods[bip];
+    best->pd = pfdot_of(f, fd + fdots[bipd]);
+    best->pdd = pfdotdot_of(1.0 / periods[bip], fd + fdots[bipd], fdd);
+    best->stats = st;
+    rc = 0;
+done:
+    agpu_free(d_prof);
+    agpu_free(d_fd);
+    agpu_free(d_pt);
+    agpu_free(d_pp);
+    if (stream)
+        agpu_stream_destroy(stream);
+    free(periods);
+    free(fdots);
+    free(pp);
+    free(prof);
     return rc;
 }

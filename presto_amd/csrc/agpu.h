@@ -328,6 +328,17 @@ typedef struct {
     double *recsums;
 } agpu_fold_args;
 int agpu_fold(const agpu_fold_args *args, void *stream);
+/* prepfold's p / p-dot search over one subband's sub-integrations:
+ * redchi[ipd][ip] = reduced chi-squared of trial (ip, ipd) of numtrials x
+ * numtrials; profs npart x proflen, fdots and parttimes are device
+ * arrays; prof_avg, prof_var the parts' sums. */
+typedef struct {
+    const double *profs, *fdots, *parttimes;
+    double prof_avg, prof_var;
+    int npart, proflen, numtrials, pstep;
+    double *redchi;
+} agpu_fold_search_args;
+int agpu_fold_search(const agpu_fold_search_args *args, void *stream);
 
 #ifdef __cplusplus
 }

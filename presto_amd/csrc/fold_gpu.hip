@@ -156,3 +156,74 @@ extern "C" int agpu_fold(const agpu_fold_args *args, void *stream)
     FD_CHECK(hipGetLastError());
     return 0;
 }
+
+/* prepfold's period / period-derivative search over the sub-integrations
+ * of one subband (src/prepfold_pipeline.c:1696-1868; combine_profs,
+ * src/fold.c:696-754; chisqr, src/characteristics.c:562-575).  One
+ * workgroup per trial (ip, ipd): part ii is delayed by
+ * fdot2phasedelay(fdots[ipd], parttimes[ii]) proflen + ii tot / npart bins
+ * (tot = pstep (ip - centre)), the delay rounded to whole bins as
+ * combine_profs does; thread k sums bin k of the shifted profiles in part
+ * order, and the reduced chi-squared is summed over the bins in order by
+ * one thread, so every trial's value is the reference's operation for
+ * operation. */
+__global__ __launch_bounds__(256) void k_fold_search(agpu_fold_search_args a)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double fs_lds[];
+    double *prof = fs_lds;                     /* proflen */
+    int *offs = (int *) (fs_lds + a.proflen);  /* npart */
+    const int ip = blockIdx.x, ipd = blockIdx.y, nt = a.numtrials;
+    const int L = a.proflen;
+    const int tot = a.pstep * (ip - (nt - 1) / 2);
+    for (int ii = threadIdx.x; ii < a.npart; ii += blockDim.x) {
+        const double t = a.parttimes[ii];
+        double pd = a.fdots[ipd] * t * t / 2.0;   /* fdot2phasedelay */
+        if (pd == -0)
+            pd = 0.0;
+        const double delay = (0.0 + pd * L) + (double) (ii * tot) / a.npart;
+        double ld = fmod(-delay, (double) L);
+        if (ld < 0.0)
+            ld += L;
+        offs[ii] = (int) (ld + 0.5);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        double s = 0.0;
+        for (int ii = 0; ii < a.npart; ii++) {
+            int j = k + offs[ii];
+            if (j >= L)
+                j -= L;
+            s += a.profs[(size_t) ii * L + j];
+        }
+        prof[k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double chi = 0.0;
+        for (int k = 0; k < L; k++) {
+            const double d = prof[k] - a.prof_avg;
+            chi += d * d;
+        }
+        a.redchi[(size_t) ipd * nt + ip] = chi / a.prof_var / (L - 1.0);
+    }
+}
+
+extern "C" int agpu_fold_search(const agpu_fold_search_args *args, void *stream)
+{
+    if (args->npart < 1 || args->proflen < 2 || args->numtrials < 1 ||
+        args->numtrials > 65535) {
+        agpu_set_error("agpu_fold_search: bad sizes");
+        return -1;
+    }
+    const size_t lds = sizeof(double) * (size_t) args->proflen +
+                       sizeof(int) * (size_t) args->npart;
+    if (lds > 64 * 1024) {
+        agpu_set_error("agpu_fold_search: profile and parts too large for LDS");
+        return -1;
+    }
+    hipLaunchKernelGGL(k_fold_search, dim3((unsigned) args->numtrials, (unsigned) args->numtrials),
+                       dim3(64), lds, (hipStream_t) stream, *args);
+    FD_CHECK(hipGetLastError());
+    return 0;
+}

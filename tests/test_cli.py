@@ -59,6 +59,24 @@ def test_cli_rejects_bad_arguments(tmp_path):
     assert r.returncode == 1 and "cannot read N and dt" in r.stderr
 
 
+def test_accelfold_rejects_bad_arguments(tmp_path):
+    """accelfold keeps prepfold's option ranges (clig/prepfold_cmd.cli:55-67)
+    and refuses a file that is not an ACCEL candidate file."""
+    fold = os.path.join(os.path.dirname(A.__file__), "accelfold")
+    base = ["-accelfile", "x_ACCEL_50.cand", "-accelcand", "1", "x.dat"]
+    for args in (["-npfact", "11"], ["-npfact", "0"], ["-pstep", "11"], ["-pdstep", "21"],
+                 ["-pdstep", "0"], ["-npart", "5000"]):
+        r = subprocess.run([fold] + args + base, cwd=tmp_path, capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 1 and "must be" in r.stderr, (args, r.stderr)
+    r = subprocess.run([fold, "-bogus"] + base, cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "usage" in r.stderr
+    r = subprocess.run([fold, "-accelfile", "x.cand", "-accelcand", "1", "x.dat"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "not an ACCEL candidate file" in r.stderr
+
+
 @pytest.mark.gpu
 def test_gpu_realfft_matches_fp64():
     for n in (1 << 10, 1 << 16, 1 << 20):

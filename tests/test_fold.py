@@ -134,3 +134,122 @@ def test_pulse_lands_at_its_phase():
     tot = prof.sum(0)
     assert np.argmax(tot) == int(0.25 * 40)
     assert st[3]["redchi"] > 100
+
+
+# ---- prepfold's p / p-dot search (o_fold_search) ----
+
+def _parts(profs, prof_var=1.0):
+    """Per-part statistics as prepfold keeps them: prof_avg the part's mean
+    bin, prof_var a given variance per part."""
+    return [dict(numdata=1000.0, data_avg=0.0, data_var=1.0, numprof=float(profs.shape[1]),
+                 prof_avg=float(p.mean()), prof_var=prof_var, redchi=0.0) for p in profs]
+
+
+def _redchi(prof, avg, var):
+    return float(np.sum((prof - avg) ** 2) / var / (len(prof) - 1))
+
+
+def test_fold_search_centre_trial_is_the_plain_sum():
+    """The centre trial moves no part: its value is the reduced chi-squared
+    of the summed parts against the summed averages and variances."""
+    rng = np.random.default_rng(3)
+    npart, L, npfact = 6, 24, 1
+    profs = rng.standard_normal((npart, L)) + 5.0
+    st = _parts(profs, prof_var=1.7)
+    T = 100.0
+    pp, best = O.fold_search(profs, st, 9.5, 1e-4, 0.0, T, np.arange(npart) * T / npart,
+                             npfact=npfact)
+    nt = 2 * npfact * L + 1
+    assert pp.shape == (nt, nt)
+    c = (nt - 1) // 2
+    want = _redchi(profs.sum(0), sum(s["prof_avg"] for s in st), 1.7 * npart)
+    assert pp[c, c] == np.float32(want)
+
+
+def _spike_parts(npart, L, pos, h=50.0, seed=0):
+    rng = np.random.default_rng(seed)
+    profs = 0.01 * rng.standard_normal((npart, L))
+    for ii in range(npart):
+        profs[ii, pos[ii] % L] += h
+    return profs
+
+
+def test_fold_search_realigns_a_period_drift():
+    """Folding at f a pulsar of frequency f + df moves its pulse by
+    -df t L bins at time t; the trial period 1/(f + df) realigns the parts,
+    so it holds the plane's maximum, every spike in one bin.  (8 parts of
+    32 bins: no other trial of the plane gives the same whole-bin shifts.)"""
+    npart, L, pstep, f, T = 8, 32, 2, 5.0, 200.0
+    s = 1                                          # +1 bin per part
+    df = -s * npart / (L * T)
+    pos = [3 + s * ii for ii in range(npart)]
+    profs = _spike_parts(npart, L, pos)
+    st = _parts(profs)
+    pp, best = O.fold_search(profs, st, f, 0.0, 0.0, T, np.arange(npart) * T / npart,
+                             npfact=2, pstep=pstep)
+    nt = pp.shape[0]
+    c = (nt - 1) // 2
+    ip = c - s * npart // pstep                     # the trial at f + df
+    assert pp[c, ip] == pp.max()
+    assert (best["ipd"], best["ip"]) == np.unravel_index(np.argmax(pp), pp.shape)
+    aligned = np.zeros(L)
+    for ii in range(npart):
+        aligned += np.roll(profs[ii], -(pos[ii] - pos[0]))
+    want = _redchi(aligned, sum(x["prof_avg"] for x in st), npart * 1.0)
+    assert pp[c, ip] == pytest.approx(want, rel=1e-6)
+    assert best["ip"] == ip and best["ipd"] == c
+    assert best["p"] == pytest.approx(1.0 / (f + df), rel=1e-14)
+    assert best["pd"] == 0.0
+
+
+def test_fold_search_realigns_a_pdot_drift():
+    """An f-dot error dfd moves the pulse by -dfd t^2 / 2 L bins: spikes at
+    b0 - ii^2 over parts at t = ii T / npart are realigned by the p-dot
+    trial with dfd = 2 npart^2 / (L T^2), whose p-dot is -(fd + dfd) / f^2."""
+    npart, L, pdstep, f, fd, T = 8, 32, 4, 5.0, 1e-6, 200.0
+    pos = [10 - ii * ii for ii in range(npart)]
+    profs = _spike_parts(npart, L, pos, seed=1)
+    st = _parts(profs)
+    pp, best = O.fold_search(profs, st, f, fd, 0.0, T, np.arange(npart) * T / npart,
+                             npfact=2, pdstep=pdstep)
+    nt = pp.shape[0]
+    c = (nt - 1) // 2
+    dfd = 2.0 * npart ** 2 / (L * T ** 2)
+    ipd = c + npart ** 2 // pdstep
+    assert pp[ipd, c] == pp.max()
+    assert best["ip"] == c and best["ipd"] == ipd
+    assert best["pd"] == pytest.approx(-(fd + dfd) / f ** 2, rel=1e-12)
+    assert best["p"] == pytest.approx(1.0 / f, rel=1e-15)
+
+
+def test_fold_search_common_rotation_changes_nothing():
+    """Rotating every part by the same number of bins rotates every trial's
+    summed profile, which leaves each reduced chi-squared unchanged."""
+    rng = np.random.default_rng(8)
+    npart, L = 5, 20
+    profs = rng.standard_normal((npart, L)) ** 2
+    st = _parts(profs)
+    args = (7.0, 2e-5, 0.0, 300.0, np.arange(npart) * 60.0)
+    pp, best = O.fold_search(profs, st, *args, npfact=1)
+    pp2, best2 = O.fold_search(np.roll(profs, 7, axis=1), st, *args, npfact=1)
+    assert np.allclose(pp, pp2, rtol=1e-5, atol=0)
+    assert (best["ip"], best["ipd"]) == (best2["ip"], best2["ipd"])
+
+
+def test_fold_search_nosearch_keeps_the_fold():
+    """-nosearch: the fold's own parameters, the plain sum of the parts
+    (host arithmetic only), and a 201-trial plane once proflen > 100
+    (src/prepfold_pipeline.c:1648-1654)."""
+    rng = np.random.default_rng(4)
+    for L, nt in ((40, 161), (128, 201)):
+        profs = rng.standard_normal((6, L)) + 3.0
+        st = [A.FoldStats(1000.0, 3.0, 1.0, float(L), float(p.mean()), 2.0, 0.0) for p in profs]
+        pp, prof, best = A.fold_search(profs, st, 11.0, 2e-6, 0.0, 90.0, np.arange(6) * 15.0,
+                                       nosearch=True)
+        assert pp is None and best.numtrials == nt
+        assert best.ip == best.ipd == (nt - 1) // 2
+        assert np.array_equal(prof, profs.sum(0))
+        assert best.p == 1.0 / 11.0 and best.pd == pytest.approx(-2e-6 / 121.0, rel=1e-15)
+        want = _redchi(profs.sum(0), sum(float(p.mean()) for p in profs), 12.0)
+        assert best.stats.redchi == pytest.approx(want, rel=1e-13)
+        assert best.stats.prof_var == 12.0 and best.stats.data_avg == 3.0

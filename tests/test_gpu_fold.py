@@ -1,8 +1,10 @@
-"""The .cand consumer on the GPU (accel_fold, the accelfold command): the
-GPU fold against the oracle's restatement of prepfold's fold (profiles
-within 1e-9 of the largest bin: FP64 phases here, long double in the
-oracle as in the reference; statistics within 1e-12), and the end-to-end
-path accelsearch -> .cand -> fold on the sifting fixtures' pulsar."""
+"""The .cand consumer on the GPU (accel_fold, accel_fold_search, the
+accelfold command): the GPU fold against the oracle's restatement of
+prepfold's fold (profiles within 1e-9 of the largest bin: FP64 phases here,
+long double in the oracle as in the reference; statistics within 1e-12);
+the p / p-dot search's plane bit-equal to the oracle's, with the same best
+trial; and the end-to-end path accelsearch -> .cand -> fold -> search on
+the sifting fixtures' pulsar."""
 import os
 import subprocess
 import sys
@@ -100,9 +102,80 @@ def test_cand_file_to_fold_confirms_the_pulsar(tmp_path):
     _, _, off, _ = A.fold(x, F.DT, 97.13, 0.0, 0.0)
     _say(f"off-frequency reduced chi-squared {off.redchi:.2f}")
     assert off.redchi < 2.0
+    # prepfold's search of the fold: the same best trial through the library
+    # and the command, and a better profile than the candidate's own
+    rpp = n // 1024 // 64
+    pt = np.arange(64) * rpp * 1024 * F.DT
+    _, best_prof, best = A.fold_search(prof, parts, f, fd, fdd, n * F.DT, pt)
+    _say(f"searched {best.numtrials}^2 trials: best p {best.p:.12f} s, "
+         f"reduced chi-squared {best.stats.redchi:.2f}")
+    assert best.stats.redchi >= total.redchi * 0.99
+    assert abs(1.0 / best.p - 123.4) < 2.0 / T or abs(1.0 / best.p - 2 * 123.4) < 2.0 / T
     fold_cli = os.path.join(os.path.dirname(A.__file__), "accelfold")
     r = subprocess.run([fold_cli, "-accelfile", cand, "-accelcand", "1", str(root) + ".dat"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    line = [s for s in r.stdout.splitlines() if s.startswith("Reduced chi-squared")][0]
+    out = r.stdout.splitlines()
+    line = [s for s in out if s.startswith("Fold reduced chi-squared")][0]
     assert float(line.split("=")[1]) == pytest.approx(total.redchi, rel=1e-4)
+    line = [s for s in out if s.startswith("Reduced chi-squared")][0]
+    assert float(line.split("=")[1]) == pytest.approx(best.stats.redchi, rel=1e-4)
+    line = [s for s in out if s.startswith("Best p =")][0]
+    assert float(line.split()[3]) == pytest.approx(best.p, rel=1e-13)
+    r = subprocess.run([fold_cli, "-nosearch", "-accelfile", cand, "-accelcand", "1",
+                        str(root) + ".dat"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    line = [s for s in r.stdout.splitlines() if s.startswith("Best p =")][0]
+    assert float(line.split()[3]) == pytest.approx(1.0 / f, rel=1e-13)
+
+
+def _parttimes(N, dt, npart):
+    """prepfold's sub-integration start times: whole reads of 1024 points
+    (src/prepfold_pipeline.c:1515)."""
+    return np.arange(npart) * ((N // 1024) // npart) * 1024 * dt
+
+
+@pytest.mark.parametrize("N,dt,f,off,npart,nb,npfact,pstep,pdstep", [
+    (1 << 20, 64e-6, 123.456, 1.3, 64, 64, 2, 2, 4),     # prepfold's defaults
+    (1 << 18, 1e-3, 7.31, -0.7, 7, 37, 3, 1, 2),         # ragged parts, other steps
+    (1 << 18, 64e-6, 33.3, 0.4, 16, 200, 1, 2, 4),       # profile longer than the workgroup
+    (1 << 16, 1e-3, 2.5, 0.0, 1, 16, 1, 2, 4),           # one part: every trial the same
+])
+def test_gpu_fold_search_equals_oracle(N, dt, f, off, npart, nb, npfact, pstep, pdstep):
+    """A pulsar folded `off` Fourier bins away from its frequency: the
+    GPU's plane of reduced chi-squared equals the oracle's bit for bit, and
+    the best trial, its period, p-dot, profile and statistics agree."""
+    _say(f"case: fold search N={N} f={f} off={off} npart={npart} proflen={nb} "
+         f"npfact={npfact} pstep={pstep} pdstep={pdstep}")
+    x = synth.make_series(N, dt, 23, [dict(f0=f, z=0.0, duty=0.08, amp=0.3)])
+    ff = f + off / (N * dt)
+    prof, parts, total, n = A.fold(x, dt, ff, 0.0, 0.0, npart=npart, proflen=nb)
+    T = n * dt                     # the span folded, as prepfold takes it
+    pt = _parttimes(N, dt, npart)
+    pp, best_prof, best = A.fold_search(prof, parts, ff, 0.0, 0.0, T, pt, npfact=npfact,
+                                        pstep=pstep, pdstep=pdstep)
+    st = [{k: getattr(p, k) for k, _ in A.FoldStats._fields_} for p in parts]
+    want, wb = O.fold_search(prof, st, ff, 0.0, 0.0, T, pt, npfact=npfact, pstep=pstep,
+                             pdstep=pdstep)
+    nt = 2 * npfact * nb + 1
+    assert best.numtrials == nt and pp.shape == (nt, nt)
+    bad = int(np.count_nonzero(pp.view(np.uint32) != want.view(np.uint32)))
+    _say(f"plane {nt}x{nt}: {bad} trials differ; best (ip, ipd) = ({best.ip}, {best.ipd}) "
+         f"oracle ({wb['ip']}, {wb['ipd']}); redchi {best.stats.redchi:.3f} "
+         f"(plain fold {total.redchi:.3f})")
+    assert bad == 0
+    assert (best.ip, best.ipd) == (wb["ip"], wb["ipd"])
+    assert best.stats.redchi == pytest.approx(wb["redchi"], rel=1e-13)
+    assert best.p == wb["p"] and best.pd == wb["pd"] and best.pdd == wb["pdd"]
+    assert best.stats.redchi >= total.redchi * (1 - 1e-12)
+    assert np.sum(best_prof) == pytest.approx(prof.sum(), rel=1e-12)
+    if npart == 1:
+        # one part: every trial is the same sum, and the first one wins
+        assert (best.ip, best.ipd) == (0, 0) and np.all(pp == pp[0, 0])
+    if off != 0.0:
+        # the search finds the pulse the off-frequency fold smeared; where
+        # it keeps the fold's p-dot, its period is the pulsar's to within
+        # one trial step (pstep / proflen turns over T)
+        assert best.stats.redchi > 3.0 * total.redchi
+        if best.pd == 0.0:
+            assert abs(1.0 / best.p - f) <= 1.5 * pstep / (nb * T)
