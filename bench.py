@@ -20,6 +20,9 @@ plane), correlation 4 bytes per raw-plane cell written plus the spectrum
 and template reads.  `traffic` is the measured HBM bytes per launch from
 the committed rocprofv3 counter summary profiles/plain_2e23_counters.md
 (FETCH_SIZE x 2 + WRITE_SIZE, the scale of a plain copy kernel).
+`hbm_read_ceiling` is what a streaming-read kernel (k_read_probe, 16-byte
+loads over 4 GiB, the best of several grids) reaches on the same GPU in the
+same run; the harmonic sums' rate is also given as a fraction of it.
 """
 import argparse
 import json
@@ -441,6 +444,9 @@ def main():
                     help="DM trials of the filterbank section (0 skips it)")
     ap.add_argument("--jerk", type=int, default=64,
                     help="spectra of the full-size jerk section (0: skip)")
+    ap.add_argument("--read-probe-gib", type=int, default=4,
+                    help="buffer of the streaming-read probe that measures the read "
+                         "ceiling (0: skip)")
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
                     help="OpenMP threads of the CPU baseline")
@@ -493,6 +499,13 @@ def main():
             t_out.append(time.perf_counter() - ts)
     s.close()
     _stage("refinement and outputs timed")
+    ceiling = None
+    if args.read_probe_gib > 0:
+        from presto_amd import profiling
+        gbs, grid, pms = profiling.read_probe(args.read_probe_gib << 30)
+        ceiling = {"kernel": "k_read_probe", "gbs": gbs, "grid": grid,
+                   "bytes": args.read_probe_gib << 30, "ms_per_read": pms}
+        _stage(f"read probe: {gbs:.0f} GB/s")
     opt_ms = 1e3 * float(np.median(t_opt))
     out_ms = 1e3 * float(np.median(t_out))
 
@@ -551,7 +564,10 @@ def main():
                      "unit": "GB/s",
                      "frac": hs_bytes / (hs_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": measured_traffic("k_harmsum<5, 0>"),
-                     "bytes_per_cell": hs_bytes_per_cell},
+                     "bytes_per_cell": hs_bytes_per_cell,
+                     "frac_of_read_ceiling": (hs_bytes / (hs_ms * 1e-3) / 1e9 / ceiling["gbs"]
+                                              if ceiling else None)},
+        "hbm_read_ceiling": ceiling,
         "corr_tuning": {"groups_kept": tuned[0],
                         "timed_groups_ms": tuned[1]},
         "roofline_correlation": {"bound": "valu", "kernel": "k_corr4096",

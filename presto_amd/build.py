@@ -75,6 +75,13 @@ def build(verbose=False):
         run(["gcc", "-o", out + ".tmp", obj, "-L", HERE, "-lpresto_accel",
              "-Wl,-rpath,$ORIGIN", "-lm"])
         os.replace(out + ".tmp", out)
+    # the streaming-read probe (measurement only; nothing in the search
+    # links it)
+    obj = os.path.join(BUILD, "bw_probe.o")
+    run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "bw_probe.hip"), "-o", obj])
+    out = os.path.join(HERE, "libpresto_probe.so")
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp", obj])
+    os.replace(out + ".tmp", out)
     if verbose:
         print("built", LIB)
     return LIB

@@ -6,6 +6,12 @@ python -m presto_amd.profiling copy NBYTES
     the counter reading for a known number of bytes read and written: the
     scale factor that turns those counters into HBM bytes.
 
+python -m presto_amd.profiling read NBYTES
+    The streaming-read probe (k_read_probe, libpresto_probe.so) over a
+    zero-filled buffer of NBYTES at grids of 1024 ... 1048576 workgroups:
+    prints the best read rate.  Under `rocprofv3 --pmc FETCH_SIZE` it is a
+    known read without writes.
+
 python -m presto_amd.profiling summarize DB [DB ...]
     Per-kernel averages from rocprofv3 result databases (*_results.db):
     dispatch count, mean duration (us) and the mean of every collected
@@ -13,6 +19,7 @@ python -m presto_amd.profiling summarize DB [DB ...]
 """
 import collections
 import ctypes as C
+import os
 import sqlite3
 import sys
 
@@ -30,6 +37,19 @@ def copy(nbytes, reps=5):
     hip.hipFree(src)
     hip.hipFree(dst)
     print(f"copied {nbytes} bytes x {reps}", flush=True)
+
+
+def read_probe(nbytes, reps=5):
+    """(best read GB/s, its grid, ms per read) of the streaming-read probe."""
+    lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpresto_probe.so"))
+    lib.probe_read_bandwidth.restype = C.c_int
+    lib.probe_read_bandwidth.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    gbs, grid, ms = C.c_double(), C.c_int(), C.c_double()
+    if lib.probe_read_bandwidth(int(nbytes), int(reps), C.byref(gbs), C.byref(grid),
+                                C.byref(ms)):
+        raise RuntimeError("read probe failed")
+    return gbs.value, grid.value, ms.value
 
 
 def _short(name):
@@ -66,6 +86,10 @@ def summarize(db):
 def main(argv):
     if len(argv) >= 2 and argv[0] == "copy":
         copy(int(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "read":
+        gbs, grid, ms = read_probe(int(argv[1]))
+        print(f"read {int(argv[1])} bytes: {gbs:.1f} GB/s at {grid} workgroups "
+              f"({ms:.3f} ms per read)", flush=True)
     elif len(argv) >= 2 and argv[0] == "summarize":
         for db in argv[1:]:
             print(f"# {db}")
