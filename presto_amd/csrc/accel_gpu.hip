@@ -285,6 +285,87 @@ __global__ __launch_bounds__(256) void k_block_norms(const float2 *spec, long lo
     }
 }
 
+/* Windows of at most NW_KPL * 64 bins: one wavefront per window, four
+ * windows per workgroup, and no workgroup barrier.  Each lane keeps
+ * NW_KPL power bit patterns in registers (bins past the window hold
+ * 0xffffffff, above every power); the k-th smallest pattern is built
+ * bit by bit from the top: a bit is kept when fewer than k + 1 patterns
+ * lie below the value with it set, the counts coming from wave ballots.
+ * The selected patterns, hence the medians and norms, are those of
+ * k_block_norms. */
+#define NW_KPL 16
+
+__device__ static unsigned wave_select(const unsigned *key, unsigned k)
+{
+    unsigned v = 0;
+    for (int bit = 31; bit >= 0; bit--) {
+        const unsigned c = v | (1u << bit);
+        unsigned below = 0;
+#pragma unroll
+        for (int j = 0; j < NW_KPL; j++)
+            below += (unsigned) __popcll(__ballot(key[j] < c));
+        if (below <= k)
+            v = c;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_block_norms_wave(const float2 *spec, long long numbins,
+                                                          long long lobin_file,
+                                                          const agpu_window *win, int nwin,
+                                                          double *norm, float *nsq)
+{
+#pragma clang fp contract(off)
+    const int wid = (int) blockIdx.x * 4 + (int) (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid >= nwin)
+        return;
+    const agpu_window w = win[wid];
+    const long long lo = w.lobin - lobin_file;
+    const int n = w.numdata;
+    unsigned key[NW_KPL];
+#pragma unroll
+    for (int j = 0; j < NW_KPL; j++) {
+        const int i = lane + 64 * j;
+        unsigned kv = 0xffffffffu;
+        if (i < n) {
+            const long long b = lo + i;
+            float p = 0.0f;
+            if (b >= 0 && b < numbins) {
+                const float2 v = spec[b];
+                p = v.x * v.x + v.y * v.y;
+            }
+            kv = __float_as_uint(p);
+        }
+        key[j] = kv;
+    }
+    const unsigned lhs = (unsigned) (n - 1) / 2, rhs = (unsigned) n / 2;
+    const unsigned ka = wave_select(key, lhs);
+    float med = __uint_as_float(ka);
+    if (lhs != rhs) {
+        /* the (lhs+1)-th pattern: ka again if more than lhs + 1 patterns
+         * are <= ka, else the smallest pattern above ka */
+        unsigned le = 0, mn = 0xffffffffu;
+#pragma unroll
+        for (int j = 0; j < NW_KPL; j++) {
+            le += (unsigned) __popcll(__ballot(key[j] <= ka));
+            if (key[j] > ka && key[j] < mn)
+                mn = key[j];
+        }
+        for (int d = 32; d >= 1; d >>= 1) {
+            const unsigned o = __shfl_xor(mn, d, 64);
+            mn = o < mn ? o : mn;
+        }
+        const unsigned kb = le >= rhs + 1 ? ka : mn;
+        med = (float) (0.5 * ((double) __uint_as_float(ka) + (double) __uint_as_float(kb)));
+    }
+    if (lane == 0) {
+        const double nv = 1.0 / sqrt((double) med / 0.69314718055994530942);
+        norm[wid] = nv;
+        nsq[wid] = (float) (nv * nv);
+    }
+}
+
 /* One window given by value (the accel.h surface normalises one block
  * per subharm_fderivs_vol call). */
 __global__ __launch_bounds__(256) void k_block_norm1(const float2 *spec, long long numbins,
@@ -1365,6 +1446,13 @@ int agpu_block_norms(const void *d_spec, long long numbins, long long lobin_file
         return 0;
     if (maxn <= 0 || maxn > NORM_MAXN)
         return fail("agpu_block_norms: window longer than the LDS selection buffer");
+    if (maxn <= NW_KPL * 64) {
+        hipLaunchKernelGGL(k_block_norms_wave, dim3((unsigned) ((nwin + 3) / 4)), dim3(256), 0,
+                           (hipStream_t) stream, (const float2 *) d_spec, numbins, lobin_file,
+                           d_win, nwin, d_norm, d_nsq);
+        AGPU_CHECK(hipGetLastError());
+        return 0;
+    }
     hipLaunchKernelGGL(k_block_norms, dim3(nwin), dim3(256), sizeof(unsigned) * maxn,
                        (hipStream_t) stream, (const float2 *) d_spec, numbins, lobin_file,
                        d_win, d_norm, d_nsq);
