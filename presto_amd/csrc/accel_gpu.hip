@@ -310,19 +310,12 @@ __device__ static unsigned wave_select(const unsigned *key, unsigned k)
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_block_norms_wave(const float2 *spec, long long numbins,
-                                                          long long lobin_file,
-                                                          const agpu_window *win, int nwin,
-                                                          double *norm, float *nsq)
+/* The exact median of window [lo, lo+n) (n <= NW_KPL * 64) by one
+ * wavefront; every lane returns it. */
+__device__ static float wave_block_median(const float2 *spec, long long numbins, long long lo,
+                                          int n, int lane)
 {
 #pragma clang fp contract(off)
-    const int wid = (int) blockIdx.x * 4 + (int) (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (wid >= nwin)
-        return;
-    const agpu_window w = win[wid];
-    const long long lo = w.lobin - lobin_file;
-    const int n = w.numdata;
     unsigned key[NW_KPL];
 #pragma unroll
     for (int j = 0; j < NW_KPL; j++) {
@@ -341,24 +334,36 @@ __global__ __launch_bounds__(256) void k_block_norms_wave(const float2 *spec, lo
     }
     const unsigned lhs = (unsigned) (n - 1) / 2, rhs = (unsigned) n / 2;
     const unsigned ka = wave_select(key, lhs);
-    float med = __uint_as_float(ka);
-    if (lhs != rhs) {
-        /* the (lhs+1)-th pattern: ka again if more than lhs + 1 patterns
-         * are <= ka, else the smallest pattern above ka */
-        unsigned le = 0, mn = 0xffffffffu;
+    if (lhs == rhs)
+        return __uint_as_float(ka);
+    /* the (lhs+1)-th pattern: ka again if more than lhs + 1 patterns are
+     * <= ka, else the smallest pattern above ka */
+    unsigned le = 0, mn = 0xffffffffu;
 #pragma unroll
-        for (int j = 0; j < NW_KPL; j++) {
-            le += (unsigned) __popcll(__ballot(key[j] <= ka));
-            if (key[j] > ka && key[j] < mn)
-                mn = key[j];
-        }
-        for (int d = 32; d >= 1; d >>= 1) {
-            const unsigned o = __shfl_xor(mn, d, 64);
-            mn = o < mn ? o : mn;
-        }
-        const unsigned kb = le >= rhs + 1 ? ka : mn;
-        med = (float) (0.5 * ((double) __uint_as_float(ka) + (double) __uint_as_float(kb)));
+    for (int j = 0; j < NW_KPL; j++) {
+        le += (unsigned) __popcll(__ballot(key[j] <= ka));
+        if (key[j] > ka && key[j] < mn)
+            mn = key[j];
     }
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned o = __shfl_xor(mn, d, 64);
+        mn = o < mn ? o : mn;
+    }
+    const unsigned kb = le >= rhs + 1 ? ka : mn;
+    return (float) (0.5 * ((double) __uint_as_float(ka) + (double) __uint_as_float(kb)));
+}
+
+__global__ __launch_bounds__(256) void k_block_norms_wave(const float2 *spec, long long numbins,
+                                                          long long lobin_file,
+                                                          const agpu_window *win, int nwin,
+                                                          double *norm, float *nsq)
+{
+    const int wid = (int) blockIdx.x * 4 + (int) (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid >= nwin)
+        return;
+    const agpu_window w = win[wid];
+    const float med = wave_block_median(spec, numbins, w.lobin - lobin_file, w.numdata, lane);
     if (lane == 0) {
         const double nv = 1.0 / sqrt((double) med / 0.69314718055994530942);
         norm[wid] = nv;
@@ -395,6 +400,22 @@ __global__ __launch_bounds__(256) void k_dered_means(const float2 *spec, long lo
     const float med = block_median(spec, numbins, w.lobin, w.numdata, keys, hist, sh);
     if (threadIdx.x == 0)
         mean[blockIdx.x] = (float) ((double) med / 0.69314718055994530942);
+}
+
+/* The same block means, one wavefront per block (blocks of at most
+ * NW_KPL * 64 bins, as the de-reddening's always are). */
+__global__ __launch_bounds__(256) void k_dered_means_wave(const float2 *spec, long long numbins,
+                                                          const agpu_window *blk, int nblk,
+                                                          float *mean)
+{
+    const int wid = (int) blockIdx.x * 4 + (int) (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wid >= nblk)
+        return;
+    const agpu_window w = blk[wid];
+    const float med = wave_block_median(spec, numbins, w.lobin, w.numdata, lane);
+    if (lane == 0)
+        mean[wid] = (float) ((double) med / 0.69314718055994530942);
 }
 
 /* invsqrtf of src/misc_utils.c:291-302: one Newton step from the
@@ -459,8 +480,12 @@ int agpu_deredden(void *d_spec, long long numbins, const agpu_window *d_blk, int
     hipStream_t st = (hipStream_t) stream;
     if (nblk <= 0 || maxn <= 0 || maxn > NORM_MAXN)
         return fail("agpu_deredden: bad block table");
-    hipLaunchKernelGGL(k_dered_means, dim3(nblk), dim3(256), sizeof(unsigned) * maxn, st,
-                       (const float2 *) d_spec, numbins, d_blk, d_mean);
+    if (maxn <= NW_KPL * 64)
+        hipLaunchKernelGGL(k_dered_means_wave, dim3((unsigned) ((nblk + 3) / 4)), dim3(256), 0,
+                           st, (const float2 *) d_spec, numbins, d_blk, nblk, d_mean);
+    else
+        hipLaunchKernelGGL(k_dered_means, dim3(nblk), dim3(256), sizeof(unsigned) * maxn, st,
+                           (const float2 *) d_spec, numbins, d_blk, d_mean);
     AGPU_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_dered_apply, dim3((unsigned) ((numbins + 255) / 256)), dim3(256), 0, st,
                        (float2 *) d_spec, numbins, d_blk, (const float *) d_mean, nblk);

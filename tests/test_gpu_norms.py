@@ -47,3 +47,23 @@ def test_every_window_norm_equals_oracle(N, zmax, numharm, quantize, inmem):
     assert lo.min() < 0 or inmem
     assert (nd.max() > 1024) == (zmax == 600)
     assert bad == 0
+
+
+@pytest.mark.parametrize("N,dt", [(1 << 21, 64e-6), (3 * 5 * 7 * 2 ** 13, 1e-4)])
+def test_device_deredden_equals_oracle(N, dt):
+    """The de-reddening of a time series on the device (block medians one
+    wavefront per block, then the interpolated scaling) equals the
+    oracle's restatement of src/misc_utils.c:912-1067 bit for bit."""
+    _say(f"case: device deredden N={N} dt={dt}")
+    x = synth.make_series_f32(N, dt, 43, [dict(f0=77.7, z=5.0, duty=0.05, amp=0.2)])
+    p = A.make_params(N // 2, N, dt, zmax=20, numharm=4, sigma=2.0)
+    with A.Search(p) as s:
+        s.set_series(x)
+        spec = s.spectrum()
+    ref, _ = O.deredden(A.realfft_gpu(x), 6, 200, 0.5 / dt, N * dt)
+    diff = np.flatnonzero(spec.view(np.uint64) != ref.view(np.uint64))
+    _say(f"{len(spec)} bins: {len(diff)} differ, at {diff[:8].tolist()}")
+    # bin 0 becomes (1, 1) after the de-reddening unless -photon
+    # (src/accel_utils.c:1621-1624)
+    assert spec[0] == np.complex64(1.0 + 1.0j) and ref[0] == np.complex64(1.0 + 0.0j)
+    assert diff.tolist() == [0]
