@@ -482,6 +482,7 @@ def main():
     wall = time.perf_counter() - t0
     _stage(f"{args.steps} timed steps done")
     ncand = len(s.candidates())
+    nhits = len(s.hits())
     shape = s.corr_shape()
     plane_cells = s.plane_cells()
     # the rest of a DM trial: FP64 refinement of every candidate on the GPU
@@ -559,6 +560,7 @@ def main():
                       "device_total": avg("total")},
         "host_ms": {k: float(np.mean([h[k] for h in host])) for k in host[0]},
         "candidates": ncand,
+        "hits": nhits,
         "roofline": {"bound": "hbm", "kernel": "k_harmsum<5,0>",
                      "achieved": hs_bytes / (hs_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",

@@ -1004,7 +1004,7 @@ static void *sig_worker(void *arg)
 }
 
 #define SIG_MAX_THREADS 16
-#define SIG_PER_THREAD 2048
+#define SIG_PER_THREAD 128   /* ~1 us per sigma: threads pay off above a few hundred hits */
 
 static void hit_sigmas(const accel_hit *h, long long n, const accel_geom *g, float *sig)
 {
