@@ -5,8 +5,10 @@
  *   k_resp_z / k_template_dft : FP64 z-response templates, FFT'd to the
  *       correlation length in FP64 and kept resident as FP32
  *       (src/accel_utils.c:153-170, src/responses.c:228-316).
- *   k_block_norms : exact block medians (radix select in LDS) and the
- *       block-median normalisation of src/accel_utils.c:1087-1099.
+ *   k_block_norms_wave / k_block_norms : exact block medians (one
+ *       wavefront per window by ballot counts for windows of <= 1024
+ *       bins, a workgroup radix select in LDS above) and the block-median
+ *       normalisation of src/accel_utils.c:1087-1099.
  *   k_corr4096 : fused overlap-save correlation -- spread spectrum
  *       segment -> Stockham radix-16 FFT in LDS -> multiply by each
  *       template -> inverse FFT -> |.|^2 -> raw plane row
