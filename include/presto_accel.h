@@ -598,13 +598,13 @@ long long accel_fold(const float *series, long long n, double dt, double f, doub
 /* The best trial of prepfold's period / period-derivative search. */
 typedef struct {
     int numtrials;              /* per axis: 2 npfact proflen + 1 */
-    int ip, ipd;                /* the best trial's period and p-dot indices */
+    int ip, ipd, ipdd;          /* the best trial's period, p-dot, p-dotdot indices */
     double p, pd, pdd;          /* its topocentric period, p-dot, p-dotdot */
     accel_foldstats stats;      /* the combined profile's statistics */
 } accel_fold_best;
 /* prepfold's search of the folded sub-integrations for the best period
  * and period derivative (src/prepfold_pipeline.c:1633-1915, one subband,
- * no p-dotdot search; combine_profs, src/fold.c:696-754) on the GPU:
+ * p-dotdot search with searchpdd; combine_profs, src/fold.c:696-754) on the GPU:
  * numtrials = 2 npfact proflen + 1 periods and as many p-dots around the
  * fold's f, fd (fdd kept), spaced so that the last part moves by pstep
  * (periods) or pdstep (p-dots) bins; every trial delays part ii by its
@@ -613,11 +613,15 @@ typedef struct {
  * the folded span in seconds.  ppdot (numtrials x numtrials floats,
  * [ipd][ip]) and bestprof (proflen) may be NULL.  With nosearch the centre
  * trial (the fold's own parameters) is returned (and numtrials is 201
- * when proflen > 100, as the reference sizes its plane).  Returns 0 or -1. */
+ * when proflen > 100, as the reference sizes its plane).  With searchpdd
+ * (prepfold -searchpdd) as many p-dotdots are searched as well, the
+ * outermost loop, and ppdot holds numtrials^3 floats [ipdd][ipd][ip] (the
+ * reference keeps no plane then).  Returns 0 or -1. */
 int accel_fold_search(const double *rawfolds, const accel_foldstats *parts, int npart,
                       int proflen, double f, double fd, double fdd, double T,
                       const double *parttimes, int npfact, int pstep, int pdstep, int nosearch,
-                      float *ppdot, double *bestprof, accel_fold_best *best, int device);
+                      int searchpdd, float *ppdot, double *bestprof, accel_fold_best *best,
+                      int device);
 
 #ifdef __cplusplus
 }

@@ -146,6 +146,17 @@ static double o_fdot2phasedelay(double fdot, double time)
     return r == -0 ? 0.0 : r;
 }
 
+static double o_fdotdot2phasedelay(double fdotdot, double time)
+{
+    double r = fdotdot * time * time * time / 6.0;
+    return r == -0 ? 0.0 : r;
+}
+
+static double o_phasedelay2fdotdot(double phasedelay, double time)
+{
+    return time == 0.0 ? 0.0 : 6.0 * phasedelay / (time * time * time);
+}
+
 static double o_phasedelay2fdot(double phasedelay, double time)
 {
     return time == 0.0 ? 0.0 : 2.0 * phasedelay / (time * time);
@@ -197,7 +208,7 @@ static double o_combine(const double *profs, int npart, int proflen, const doubl
 
 int o_fold_search(const double *profs, const o_foldstats *parts, int npart, int proflen,
                   double f, double fd, double fdd, double T, const double *parttimes,
-                  int npfact, int pstep, int pdstep, float *ppdot, double *best)
+                  int npfact, int pstep, int pdstep, int searchpdd, float *ppdot, double *best)
 {
     const int nt = 2 * npfact * proflen + 1;
     double prof_avg = 0.0, prof_var = 0.0;
@@ -208,6 +219,8 @@ int o_fold_search(const double *profs, const o_foldstats *parts, int npart, int 
     double *fdots = (double *) malloc(sizeof(double) * nt);
     double *periods = (double *) malloc(sizeof(double) * nt);
     double *pdots = (double *) malloc(sizeof(double) * nt);
+    double *fdotdots = (double *) malloc(sizeof(double) * nt);
+    double *pdd_delays = (double *) malloc(sizeof(double) * npart);
     double *pd_delays = (double *) malloc(sizeof(double) * npart);
     double *delays = (double *) malloc(sizeof(double) * npart);
     double *cur = (double *) malloc(sizeof(double) * proflen);
@@ -218,37 +231,54 @@ int o_fold_search(const double *profs, const o_foldstats *parts, int npart, int 
         d = (double) (tot * pdstep) / proflen;
         fdots[i] = o_phasedelay2fdot(d, T);
         pdots[i] = o_switch_pfdot(f, fd + fdots[i]);
+        fdotdots[i] = o_phasedelay2fdotdot(d, T);
     }
-    int bip = 0, bipd = 0;
+    /* src/prepfold_pipeline.c:1814-1868 for one subband: the p-dotdot
+     * loop outermost when searched, one pass with no p-dotdot delay
+     * otherwise; the first strictly larger redchi wins */
+    const int npdd = searchpdd ? nt : 1;
+    int bip = 0, bipd = 0, bipdd = 0;
     double bestredchi = 0.0;
-    for (int ipd = 0; ipd < nt; ipd++) {
+    for (int ipdd = 0; ipdd < npdd; ipdd++) {
         for (int ii = 0; ii < npart; ii++)
-            pd_delays[ii] = 0.0 + o_fdot2phasedelay(fdots[ipd], parttimes[ii]) * proflen;
-        for (int ip = 0; ip < nt; ip++) {
-            const int totpdelay = pstep * (ip - (nt - 1) / 2);
+            pdd_delays[ii] =
+                searchpdd ? o_fdotdot2phasedelay(fdotdots[ipdd], parttimes[ii]) * proflen : 0.0;
+        for (int ipd = 0; ipd < nt; ipd++) {
             for (int ii = 0; ii < npart; ii++)
-                delays[ii] = pd_delays[ii] + (double) (ii * totpdelay) / npart;
-            const double rc = o_combine(profs, npart, proflen, delays, prof_avg, prof_var, cur);
-            if (ppdot)
-                ppdot[(size_t) ipd * nt + ip] = (float) rc;
-            if (rc > bestredchi) {
-                bestredchi = rc;
-                bip = ip;
-                bipd = ipd;
+                pd_delays[ii] =
+                    pdd_delays[ii] + o_fdot2phasedelay(fdots[ipd], parttimes[ii]) * proflen;
+            for (int ip = 0; ip < nt; ip++) {
+                const int totpdelay = pstep * (ip - (nt - 1) / 2);
+                for (int ii = 0; ii < npart; ii++)
+                    delays[ii] = pd_delays[ii] + (double) (ii * totpdelay) / npart;
+                const double rc =
+                    o_combine(profs, npart, proflen, delays, prof_avg, prof_var, cur);
+                if (ppdot)
+                    ppdot[((size_t) ipdd * nt + ipd) * nt + ip] = (float) rc;
+                if (rc > bestredchi) {
+                    bestredchi = rc;
+                    bip = ip;
+                    bipd = ipd;
+                    bipdd = ipdd;
+                }
             }
         }
     }
-    /* best[]: ip, ipd, redchi, p, pd, pdd */
+    /* best[]: ip, ipd, redchi, p, pd, pdd, ipdd (src/prepfold_pipeline.c:1886-1897) */
     best[0] = bip;
     best[1] = bipd;
     best[2] = bestredchi;
     best[3] = periods[bip];
     best[4] = pdots[bipd];
-    best[5] = o_switch_pfdotdot(1.0 / periods[bip], fd + fdots[bipd], fdd);
+    best[5] = o_switch_pfdotdot(1.0 / periods[bip], fd + fdots[bipd],
+                                searchpdd ? fdd + fdotdots[bipdd] : fdd);
+    best[6] = bipdd;
     free(fdots);
     free(periods);
     free(pdots);
     free(pd_delays);
+    free(fdotdots);
+    free(pdd_delays);
     free(delays);
     free(cur);
     return nt;

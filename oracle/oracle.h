@@ -234,12 +234,14 @@ void o_cand_fold_params(double r, double z, double w, double T, double *f, doubl
                         double *fdd);
 long long o_fold_dat(const float *series, long long n, double dt, double f, double fd,
                      double fdd, int npart, int proflen, double *rawfolds, o_foldstats *stats);
-/* prepfold's p / p-dot search over the sub-integrations (one subband, no
- * p-dotdot search): ppdot[ipd][ip] = reduced chi-squared of each trial
- * (numtrials = 2 npfact proflen + 1, returned); best = {ip, ipd, redchi,
- * p, pd, pdd} of the first maximum in (ipd, ip) order. */
+/* prepfold's p / p-dot (and with searchpdd p-dotdot) search over the
+ * sub-integrations of one subband: ppdot[ipdd][ipd][ip] = reduced
+ * chi-squared of each trial (numtrials = 2 npfact proflen + 1, returned;
+ * one ipdd plane without searchpdd); best = {ip, ipd, redchi, p, pd, pdd,
+ * ipdd} of the first maximum in (ipdd, ipd, ip) order. */
 int o_fold_search(const double *profs, const o_foldstats *parts, int npart, int proflen,
                   double f, double fd, double fdd, double T, const double *parttimes,
-                  int npfact, int pstep, int pdstep, float *ppdot, double *best);
+                  int npfact, int pstep, int pdstep, int searchpdd, float *ppdot,
+                  double *best);
 
 #endif

@@ -493,13 +493,14 @@ _sig("o_fold_dat", _ll, C.POINTER(_f), _ll, _d, _d, _d, _d, _i, _i, C.POINTER(_d
 
 
 _sig("o_fold_search", _i, C.POINTER(_d), C.POINTER(OFoldStats), _i, _i, _d, _d, _d, _d,
-     C.POINTER(_d), _i, _i, _i, C.POINTER(_f), C.POINTER(_d))
+     C.POINTER(_d), _i, _i, _i, _i, C.POINTER(_f), C.POINTER(_d))
 
 
-def fold_search(profs, stats, f, fd, fdd, T, parttimes, npfact=2, pstep=2, pdstep=4):
+def fold_search(profs, stats, f, fd, fdd, T, parttimes, npfact=2, pstep=2, pdstep=4,
+                searchpdd=False):
     """prepfold's p / p-dot search over the sub-integrations: (ppdot
-    [numtrials][numtrials] of reduced chi-squared, dict of the best
-    trial)."""
+    [numtrials][numtrials] of reduced chi-squared -- [numtrials]^3, indexed
+    [ipdd][ipd][ip], with searchpdd -- and a dict of the best trial)."""
     profs = np.ascontiguousarray(profs, dtype=np.float64)
     npart, proflen = profs.shape
     st = (OFoldStats * npart)()
@@ -507,14 +508,14 @@ def fold_search(profs, stats, f, fd, fdd, T, parttimes, npfact=2, pstep=2, pdste
         for k, _ in OFoldStats._fields_:
             setattr(s_, k, d[k])
     nt = 2 * npfact * proflen + 1
-    pp = np.zeros((nt, nt), dtype=np.float32)
-    best = np.zeros(6)
+    pp = np.zeros((nt, nt, nt) if searchpdd else (nt, nt), dtype=np.float32)
+    best = np.zeros(7)
     pt = np.ascontiguousarray(parttimes, dtype=np.float64)
     lib.o_fold_search(profs.ctypes.data_as(C.POINTER(_d)), st, npart, proflen, f, fd, fdd, T,
-                      pt.ctypes.data_as(C.POINTER(_d)), npfact, pstep, pdstep,
+                      pt.ctypes.data_as(C.POINTER(_d)), npfact, pstep, pdstep, int(searchpdd),
                       pp.ctypes.data_as(C.POINTER(_f)), best.ctypes.data_as(C.POINTER(_d)))
     return pp, dict(ip=int(best[0]), ipd=int(best[1]), redchi=best[2], p=best[3], pd=best[4],
-                    pdd=best[5])
+                    pdd=best[5], ipdd=int(best[6]))
 
 
 def cand_fold_params(r, z, w, T):

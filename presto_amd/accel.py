@@ -795,12 +795,12 @@ _sig("accel_fold", _ll, _p, _ll, _d, _d, _d, _d, _i, _i, _p, C.POINTER(FoldStats
 
 class FoldBest(C.Structure):
     """accel_fold_best: the best trial of prepfold's p / p-dot search."""
-    _fields_ = [("numtrials", _i), ("ip", _i), ("ipd", _i), ("p", _d), ("pd", _d), ("pdd", _d),
-                ("stats", FoldStats)]
+    _fields_ = [("numtrials", _i), ("ip", _i), ("ipd", _i), ("ipdd", _i), ("p", _d), ("pd", _d),
+                ("pdd", _d), ("stats", FoldStats)]
 
 
 _sig("accel_fold_search", _i, _p, C.POINTER(FoldStats), _i, _i, _d, _d, _d, _d, _p, _i, _i, _i,
-     _i, _p, _p, C.POINTER(FoldBest), _i)
+     _i, _i, _p, _p, C.POINTER(FoldBest), _i)
 
 
 def read_rzw_cand(candfile, candnum):
@@ -839,10 +839,12 @@ def fold(series, dt, f, fd=0.0, fdd=0.0, npart=64, proflen=None, device=0):
 
 
 def fold_search(rawfolds, parts, f, fd, fdd, T, parttimes, npfact=2, pstep=2, pdstep=4,
-                nosearch=False, device=0):
-    """prepfold's period / p-dot search over the sub-integrations of a fold
-    on the GPU: (ppdot [numtrials][numtrials] float32 reduced chi-squared,
-    indexed [ipd][ip], or None with nosearch; best profile; FoldBest)."""
+                nosearch=False, searchpdd=False, device=0):
+    """prepfold's period / p-dot (and with searchpdd p-dotdot) search over
+    the sub-integrations of a fold on the GPU: (ppdot [numtrials][numtrials]
+    float32 reduced chi-squared indexed [ipd][ip] -- [numtrials]^3,
+    [ipdd][ipd][ip], with searchpdd -- or None with nosearch; best profile;
+    FoldBest)."""
     prof = np.ascontiguousarray(rawfolds, dtype=np.float64)
     npart, proflen = prof.shape
     pt = np.ascontiguousarray(parttimes, dtype=np.float64)
@@ -850,11 +852,12 @@ def fold_search(rawfolds, parts, f, fd, fdd, T, parttimes, npfact=2, pstep=2, pd
         raise ValueError("one part time and one FoldStats per sub-integration")
     st = (FoldStats * npart)(*parts)
     nt = 2 * npfact * proflen + 1
-    pp = None if nosearch else np.zeros((nt, nt), dtype=np.float32)
+    pp = None if nosearch else np.zeros((nt, nt, nt) if searchpdd else (nt, nt),
+                                        dtype=np.float32)
     bestprof = np.zeros(proflen, dtype=np.float64)
     best = FoldBest()
     if lib.accel_fold_search(prof.ctypes.data, st, npart, proflen, f, fd, fdd, T, pt.ctypes.data,
-                             npfact, pstep, pdstep, int(bool(nosearch)),
+                             npfact, pstep, pdstep, int(bool(nosearch)), int(bool(searchpdd)),
                              None if pp is None else pp.ctypes.data, bestprof.ctypes.data,
                              C.byref(best), device):
         raise _err()

@@ -204,16 +204,23 @@ static double fdot_of_delay(double phasedelay, double time)
 
 /* combine_profs for one trial (src/fold.c:696-754): the best profile */
 static void combine_trial(const double *profs, int npart, int proflen, const double *parttimes,
-                          double fdot, int tot, double *out)
+                          double fdotdot, int searchpdd, double fdot, int tot, double *out)
 {
     for (int k = 0; k < proflen; k++)
         out[k] = 0.0;
     for (int ii = 0; ii < npart; ii++) {
         const double t = parttimes[ii];
+        double pdd_delay = 0.0;
+        if (searchpdd) {
+            double pdd = fdotdot * t * t * t / 6.0;
+            if (pdd == -0)
+                pdd = 0.0;
+            pdd_delay = pdd * proflen;
+        }
         double pd = fdot * t * t / 2.0;
         if (pd == -0)
             pd = 0.0;
-        const double delay = (0.0 + pd * proflen) + (double) (ii * tot) / npart;
+        const double delay = (pdd_delay + pd * proflen) + (double) (ii * tot) / npart;
         double ld = fmod(-delay, (double) proflen);
         if (ld < 0.0)
             ld += proflen;
@@ -230,7 +237,8 @@ static void combine_trial(const double *profs, int npart, int proflen, const dou
 int accel_fold_search(const double *rawfolds, const accel_foldstats *parts, int npart,
                       int proflen, double f, double fd, double fdd, double T,
                       const double *parttimes, int npfact, int pstep, int pdstep, int nosearch,
-                      float *ppdot, double *bestprof, accel_fold_best *best, int device)
+                      int searchpdd, float *ppdot, double *bestprof, accel_fold_best *best,
+                      int device)
 {
     if (npart < 1 || proflen < 2 || npfact < 1 || pstep < 1 || pdstep < 1 || !(T > 0.0) ||
         !(f > 0.0) || !best) {
@@ -247,11 +255,15 @@ int accel_fold_search(const double *rawfolds, const accel_foldstats *parts, int 
     /* src/prepfold_pipeline.c:1696-1705 */
     double *periods = (double *) malloc(sizeof(double) * nt);
     double *fdots = (double *) malloc(sizeof(double) * nt);
-    double *pp = (double *) malloc(sizeof(double) * (size_t) nt * nt);
+    const int npdd = searchpdd && !nosearch ? nt : 1;
+    const size_t ncube = (size_t) npdd * nt * nt;
+    double *fdotdots = (double *) malloc(sizeof(double) * nt);
+    double *pp = (double *) malloc(sizeof(double) * ncube);
     double *prof = (double *) malloc(sizeof(double) * proflen);
-    if (!periods || !fdots || !pp || !prof) {
+    if (!periods || !fdots || !fdotdots || !pp || !prof) {
         free(periods);
         free(fdots);
+        free(fdotdots);
         free(pp);
         free(prof);
         accel_set_error("fold search: out of host memory");
@@ -263,6 +275,7 @@ int accel_fold_search(const double *rawfolds, const accel_foldstats *parts, int 
         periods[i] = 1.0 / (f + d / T);
         d = (double) (tot * pdstep) / proflen;
         fdots[i] = fdot_of_delay(d, T);
+        fdotdots[i] = 6.0 * d / (T * T * T);   /* phasedelay2fdotdot, T > 0 */
     }
     accel_foldstats st;
     memset(&st, 0, sizeof(st));
@@ -276,16 +289,19 @@ int accel_fold_search(const double *rawfolds, const accel_foldstats *parts, int 
     st.data_avg /= npart;
     st.data_var /= npart;
     st.numprof = proflen;
-    int bip = (nt - 1) / 2, bipd = (nt - 1) / 2, rc = -1;
+    int bip = (nt - 1) / 2, bipd = (nt - 1) / 2, bipdd = searchpdd ? (nt - 1) / 2 : 0, rc = -1;
     void *stream = NULL, *d_prof = NULL, *d_fd = NULL, *d_pt = NULL, *d_pp = NULL;
+    void *d_fdd = NULL;
     if (!nosearch) {
         const size_t pbytes = sizeof(double) * (size_t) npart * proflen;
         if (agpu_init(device) || agpu_stream_create(&stream) || agpu_malloc(&d_prof, pbytes) ||
             agpu_malloc(&d_fd, sizeof(double) * nt) ||
             agpu_malloc(&d_pt, sizeof(double) * npart) ||
-            agpu_malloc(&d_pp, sizeof(double) * (size_t) nt * nt) ||
+            agpu_malloc(&d_fdd, sizeof(double) * nt) ||
+            agpu_malloc(&d_pp, sizeof(double) * ncube) ||
             agpu_h2d(d_prof, rawfolds, pbytes, stream) ||
             agpu_h2d(d_fd, fdots, sizeof(double) * nt, stream) ||
+            agpu_h2d(d_fdd, fdotdots, sizeof(double) * nt, stream) ||
             agpu_h2d(d_pt, parttimes, sizeof(double) * npart, stream)) {
             accel_set_error("fold search: %s", agpu_last_error());
             goto done;
@@ -293,6 +309,7 @@ int accel_fold_search(const double *rawfolds, const accel_foldstats *parts, int 
         agpu_fold_search_args a;
         a.profs = (const double *) d_prof;
         a.fdots = (const double *) d_fd;
+        a.fdotdots = searchpdd ? (const double *) d_fdd : NULL;
         a.parttimes = (const double *) d_pt;
         a.prof_avg = st.prof_avg;
         a.prof_var = st.prof_var;
@@ -300,32 +317,38 @@ int accel_fold_search(const double *rawfolds, const accel_foldstats *parts, int 
         a.proflen = proflen;
         a.numtrials = nt;
         a.pstep = pstep;
-        a.redchi = (double *) d_pp;
-        if (agpu_fold_search(&a, stream) ||
-            agpu_d2h(pp, d_pp, sizeof(double) * (size_t) nt * nt, stream) ||
-            agpu_stream_sync(stream)) {
+        for (int ipdd = 0; ipdd < npdd; ipdd++) {
+            a.ipdd = ipdd;
+            a.redchi = (double *) d_pp + (size_t) ipdd * nt * nt;
+            if (agpu_fold_search(&a, stream)) {
+                accel_set_error("fold search: %s", agpu_last_error());
+                goto done;
+            }
+        }
+        if (agpu_d2h(pp, d_pp, sizeof(double) * ncube, stream) || agpu_stream_sync(stream)) {
             accel_set_error("fold search: %s", agpu_last_error());
             goto done;
         }
-        /* the first maximum in (p-dot, period) order: the reference's
-         * strictly-greater test over its loops, from a redchi of 0 */
+        /* the first maximum in (p-dotdot, p-dot, period) order: the
+         * reference's strictly-greater test over its loops, from a redchi
+         * of 0 */
         double bestrc = 0.0;
-        bip = bipd = 0;
-        for (int ipd = 0; ipd < nt; ipd++)
-            for (int ip = 0; ip < nt; ip++)
-                if (pp[(size_t) ipd * nt + ip] > bestrc) {
-                    bestrc = pp[(size_t) ipd * nt + ip];
-                    bip = ip;
-                    bipd = ipd;
-                }
-        if (ppdot)   /* a float plane, as the reference keeps it */
-            for (size_t i = 0; i < (size_t) nt * nt; i++)
+        bip = bipd = bipdd = 0;
+        for (size_t i = 0; i < ncube; i++)
+            if (pp[i] > bestrc) {
+                bestrc = pp[i];
+                bip = (int) (i % nt);
+                bipd = (int) (i / nt % nt);
+                bipdd = (int) (i / ((size_t) nt * nt));
+            }
+        if (ppdot)   /* float, as the reference keeps its plane */
+            for (size_t i = 0; i < ncube; i++)
                 ppdot[i] = (float) pp[i];
     }
     /* the best profile and its statistics (the redchi in FP64, as
      * combine_profs leaves it) */
-    combine_trial(rawfolds, npart, proflen, parttimes, fdots[bipd], pstep * (bip - (nt - 1) / 2),
-                  prof);
+    combine_trial(rawfolds, npart, proflen, parttimes, searchpdd ? fdotdots[bipdd] : 0.0,
+                  searchpdd, fdots[bipd], pstep * (bip - (nt - 1) / 2), prof);
     double chi = 0.0;
     for (int k = 0; k < proflen; k++) {
         const double d = prof[k] - st.prof_avg;
@@ -339,7 +362,9 @@ int accel_fold_search(const double *rawfolds, const accel_foldstats *parts, int 
     best->ipd = bipd;
     best->p = periods[bip];
     best->pd = pfdot_of(f, fd + fdots[bipd]);
-    best->pdd = pfdotdot_of(1.0 / periods[bip], fd + fdots[bipd], fdd);
+    best->ipdd = bipdd;
+    best->pdd = pfdotdot_of(1.0 / periods[bip], fd + fdots[bipd],
+                            searchpdd ? fdd + fdotdots[bipdd] : fdd);
     best->stats = st;
     rc = 0;
 done:
@@ -347,10 +372,12 @@ done:
     agpu_free(d_fd);
     agpu_free(d_pt);
     agpu_free(d_pp);
+    agpu_free(d_fdd);
     if (stream)
         agpu_stream_destroy(stream);
     free(periods);
     free(fdots);
+    free(fdotdots);
     free(pp);
     free(prof);
     return rc;

@@ -2,20 +2,20 @@
  * accelfold -- prepfold -accelcand for a .dat time series, on the GPU:
  *
  *     accelfold [-n proflen] [-npart n] [-npfact n] [-pstep n] [-pdstep n]
- *               [-nosearch] -accelfile X_ACCEL_200.cand -accelcand k X.dat
+ *               [-nosearch] [-searchpdd] -accelfile X_ACCEL_200.cand -accelcand k X.dat
  *
  * Reads record k of the .cand file (src/get_candidates.c:27-37) and the
  * .inf of its root (the part before "_ACCEL", src/prepfold_pipeline.c:
  * 745-756), turns the candidate's average r, z, w into the start f, fdot,
  * fdotdot (src/prepfold_pipeline.c:757-767), folds the series as prepfold
  * does (accel_fold) and, unless -nosearch, searches the sub-integrations
- * for the best period and p-dot (accel_fold_search,
+ * for the best period and p-dot, and with -searchpdd p-dotdot
+ * (accel_fold_search,
  * src/prepfold_pipeline.c:1633-1915 for one subband).  Prints the folding
  * parameters, the best trial, its profile and reduced chi-squared.
  * prepfold's option names and defaults are kept
- * (clig/prepfold_cmd.cli:45,55-67); its DM and p-dotdot
- * searches, plots and .pfd output are not built, and only the options
- * above are accepted.
+ * (clig/prepfold_cmd.cli:45,55-67); its DM search, plots and .pfd output
+ * are not built, and only the options above are accepted.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -26,7 +26,7 @@
 static void usage(void)
 {
     fprintf(stderr, "usage: accelfold [-n proflen] [-npart n] [-npfact n] [-pstep n] "
-                    "[-pdstep n] [-nosearch] -accelfile file.cand -accelcand k file.dat\n");
+                    "[-pdstep n] [-nosearch] [-searchpdd] -accelfile file.cand -accelcand k file.dat\n");
     exit(1);
 }
 
@@ -34,6 +34,7 @@ int main(int argc, char **argv)
 {
     const char *candfile = NULL, *datfile = NULL;
     int candnum = 0, proflen = 0, npart = 64, npfact = 2, pstep = 2, pdstep = 4, nosearch = 0;
+    int searchpdd = 0;
     for (int i = 1; i < argc; i++) {
         if (!strcmp(argv[i], "-accelfile") && i + 1 < argc)
             candfile = argv[++i];
@@ -51,6 +52,8 @@ int main(int argc, char **argv)
             pdstep = atoi(argv[++i]);
         else if (!strcmp(argv[i], "-nosearch"))
             nosearch = 1;
+        else if (!strcmp(argv[i], "-searchpdd"))
+            searchpdd = 1;
         else if (argv[i][0] == '-')
             usage();
         else
@@ -123,7 +126,8 @@ int main(int argc, char **argv)
         parttimes[p] = (double) (p * rpp) * 1024.0 * idata.dt;
     accel_fold_best best;
     if (accel_fold_search(prof, parts, npart, proflen, f, fd, fdd, used * idata.dt, parttimes,
-                          npfact, pstep, pdstep, nosearch, NULL, best_prof, &best, 0)) {
+                          npfact, pstep, pdstep, nosearch, searchpdd, NULL, best_prof, &best,
+                          0)) {
         fprintf(stderr, "accelfold: %s\n", accel_last_error());
         return 1;
     }
@@ -134,10 +138,11 @@ int main(int argc, char **argv)
     printf("  p = %.12g s    pd = %.6g s/s\n", 1.0 / f, -fd / (f * f));
     printf("Fold reduced chi-squared = %.4f\n", total.redchi);
     if (!nosearch)
-        printf("Searched %d periods x %d p-dots (npfact %d, pstep %d, pdstep %d)\n",
-               best.numtrials, best.numtrials, npfact, pstep, pdstep);
-    printf("Best p = %.15g s   pd = %.8g s/s   pdd = %.8g s/s^2  (trial %d, %d)\n", best.p,
-           best.pd, best.pdd, best.ip, best.ipd);
+        printf("Searched %d periods x %d p-dots%s (npfact %d, pstep %d, pdstep %d)\n",
+               best.numtrials, best.numtrials, searchpdd ? " x p-dotdots" : "", npfact, pstep,
+               pdstep);
+    printf("Best p = %.15g s   pd = %.8g s/s   pdd = %.8g s/s^2  (trial %d, %d, %d)\n", best.p,
+           best.pd, best.pdd, best.ip, best.ipd, best.ipdd);
     printf("Profile (%s):\n", nosearch ? "sum of the parts" : "best trial");
     for (int k = 0; k < proflen; k++)
         printf("  %4d  %.6f\n", k, best_prof[k]);

@@ -333,10 +333,11 @@ int agpu_fold(const agpu_fold_args *args, void *stream);
  * numtrials; profs npart x proflen, fdots and parttimes are device
  * arrays; prof_avg, prof_var the parts' sums. */
 typedef struct {
-    const double *profs, *fdots, *parttimes;
+    const double *profs, *fdots, *fdotdots, *parttimes;   /* fdotdots: NULL, no p-dotdot */
     double prof_avg, prof_var;
     int npart, proflen, numtrials, pstep;
-    double *redchi;
+    int ipdd;             /* the p-dotdot trial of this launch (with fdotdots) */
+    double *redchi;       /* numtrials x numtrials, [ipd][ip], of trial ipdd */
 } agpu_fold_search_args;
 int agpu_fold_search(const agpu_fold_search_args *args, void *stream);
 

@@ -160,8 +160,10 @@ extern "C" int agpu_fold(const agpu_fold_args *args, void *stream)
 /* prepfold's period / period-derivative search over the sub-integrations
  * of one subband (src/prepfold_pipeline.c:1696-1868; combine_profs,
  * src/fold.c:696-754; chisqr, src/characteristics.c:562-575).  One
- * workgroup per trial (ip, ipd): part ii is delayed by
- * fdot2phasedelay(fdots[ipd], parttimes[ii]) proflen + ii tot / npart bins
+ * workgroup per trial (ip, ipd) of the launch's p-dotdot trial: part ii
+ * is delayed by fdotdot2phasedelay(fdotdots[ipdd], parttimes[ii]) proflen
+ * (0 without a p-dotdot search) + fdot2phasedelay(fdots[ipd],
+ * parttimes[ii]) proflen + ii tot / npart bins
  * (tot = pstep (ip - centre)), the delay rounded to whole bins as
  * combine_profs does; thread k sums bin k of the shifted profiles in part
  * order, and the reduced chi-squared is summed over the bins in order by
@@ -178,10 +180,17 @@ __global__ __launch_bounds__(256) void k_fold_search(agpu_fold_search_args a)
     const int tot = a.pstep * (ip - (nt - 1) / 2);
     for (int ii = threadIdx.x; ii < a.npart; ii += blockDim.x) {
         const double t = a.parttimes[ii];
+        double pdd_delay = 0.0;
+        if (a.fdotdots) {
+            double pdd = a.fdotdots[a.ipdd] * t * t * t / 6.0;   /* fdotdot2phasedelay */
+            if (pdd == -0)
+                pdd = 0.0;
+            pdd_delay = pdd * L;
+        }
         double pd = a.fdots[ipd] * t * t / 2.0;   /* fdot2phasedelay */
         if (pd == -0)
             pd = 0.0;
-        const double delay = (0.0 + pd * L) + (double) (ii * tot) / a.npart;
+        const double delay = (pdd_delay + pd * L) + (double) (ii * tot) / a.npart;
         double ld = fmod(-delay, (double) L);
         if (ld < 0.0)
             ld += L;

@@ -253,3 +253,28 @@ def test_fold_search_nosearch_keeps_the_fold():
         want = _redchi(profs.sum(0), sum(float(p.mean()) for p in profs), 12.0)
         assert best.stats.redchi == pytest.approx(want, rel=1e-13)
         assert best.stats.prof_var == 12.0 and best.stats.data_avg == 3.0
+
+
+def test_fold_search_realigns_a_pdotdot_drift():
+    """-searchpdd: an f-dotdot error moves the pulse by -dfdd t^3 / 6 L
+    bins; spikes at b0 - ii^3 over parts at t = ii T / npart are realigned
+    by the p-dotdot trial of delay ii^3 (tot pdstep = npart^3), whose cube
+    entry is the maximum; the opposite trial is not."""
+    npart, L, pdstep, f, T = 4, 32, 4, 5.0, 200.0
+    pos = [20 - ii ** 3 for ii in range(npart)]
+    profs = _spike_parts(npart, L, pos, seed=2)
+    st = _parts(profs)
+    pp, best = O.fold_search(profs, st, f, 0.0, 0.0, T, np.arange(npart) * T / npart,
+                             npfact=1, pdstep=pdstep, searchpdd=True)
+    nt = pp.shape[0]
+    c = (nt - 1) // 2
+    assert pp.shape == (nt, nt, nt)
+    k = npart ** 3 // pdstep
+    assert pp[c + k, c, c] == pp.max() > pp[c - k, c, c]
+    first = np.unravel_index(np.argmax(pp), pp.shape)
+    assert (best["ipdd"], best["ipd"], best["ip"]) == first
+    # without the p-dotdot loop the search is the p / p-dot plane of the
+    # centre p-dotdot trial
+    plane, b2 = O.fold_search(profs, st, f, 0.0, 0.0, T, np.arange(npart) * T / npart,
+                              npfact=1, pdstep=pdstep)
+    assert np.array_equal(plane, pp[c]) and b2["ipdd"] == 0

@@ -179,3 +179,29 @@ def test_gpu_fold_search_equals_oracle(N, dt, f, off, npart, nb, npfact, pstep, 
         assert best.stats.redchi > 3.0 * total.redchi
         if best.pd == 0.0:
             assert abs(1.0 / best.p - f) <= 1.5 * pstep / (nb * T)
+
+
+@pytest.mark.parametrize("npart,nb,fdd", [(6, 16, 2e-7), (5, 11, -1e-6)])
+def test_gpu_fold_search_pdotdot_equals_oracle(npart, nb, fdd):
+    """-searchpdd: the GPU's cube of reduced chi-squared over (p-dotdot,
+    p-dot, period) equals the oracle's bit for bit, with the same best
+    trial and p-dotdot."""
+    N, dt, f = 1 << 18, 1e-3, 3.3
+    _say(f"case: fold search -searchpdd N={N} f={f} fdd={fdd} npart={npart} proflen={nb}")
+    x = synth.make_series(N, dt, 29, [dict(f0=f, z=0.0, duty=0.08, amp=0.3)])
+    prof, parts, total, n = A.fold(x, dt, f, 0.0, fdd, npart=npart, proflen=nb)
+    T = n * dt
+    pt = _parttimes(N, dt, npart)
+    pp, best_prof, best = A.fold_search(prof, parts, f, 0.0, fdd, T, pt, npfact=1,
+                                        searchpdd=True)
+    st = [{k: getattr(p, k) for k, _ in A.FoldStats._fields_} for p in parts]
+    want, wb = O.fold_search(prof, st, f, 0.0, fdd, T, pt, npfact=1, searchpdd=True)
+    nt = 2 * nb + 1
+    assert pp.shape == want.shape == (nt, nt, nt)
+    bad = int(np.count_nonzero(pp.view(np.uint32) != want.view(np.uint32)))
+    _say(f"cube {nt}^3: {bad} trials differ; best (ip, ipd, ipdd) = "
+         f"({best.ip}, {best.ipd}, {best.ipdd}) oracle ({wb['ip']}, {wb['ipd']}, {wb['ipdd']})")
+    assert bad == 0
+    assert (best.ip, best.ipd, best.ipdd) == (wb["ip"], wb["ipd"], wb["ipdd"])
+    assert best.p == wb["p"] and best.pd == wb["pd"] and best.pdd == wb["pdd"]
+    assert best.stats.redchi == pytest.approx(wb["redchi"], rel=1e-13)
